@@ -1,0 +1,1190 @@
+// shesolve.hip -- MI355X (gfx950) SHE linear solver behind the C ABI in include/shesolve.h.
+//
+// Replaces the ViennaCL CPU path of the reference solver call
+// (src/solvers/viennacl/serial_linear_solver.hpp:99-156: CSR copy, ilu0_precond,
+// bicgstab; src/solvers/viennacl/parallel_linear_solver.hpp:50-127: block ILU0).
+//
+// Kernels (all HBM-bound sparse / vector work; no dense contraction, so no MFMA):
+//   - CSR SpMV: LPR lanes per row (a power of two dividing the 64-wide wavefront),
+//     coalesced streaming of values and column indices, shuffle row reduction,
+//     fixed persistent grid so the fused dot products reduce to a fixed partial count;
+//   - ILU(0) factorisation and forward/backward sweeps, level-scheduled: rows of a
+//     level are independent, one launch per level; the factorisation keeps the
+//     working row in LDS;
+//   - fused BiCGStab vector updates with per-block partial dot products reduced by a
+//     one-block finalise kernel that also computes the recurrence scalars on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "shesolve.h"
+
+namespace {
+
+thread_local std::string g_err;
+void set_err(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+#define HIPCHK(expr)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return -10;                                                                    \
+    }                                                                                \
+  } while (0)
+
+constexpr int kBlock = 256;      // threads per block for streaming kernels
+constexpr int kGrid = 2048;      // persistent grid (8 workgroups per CU on 256 CUs)
+constexpr int kMaxRowLDS = 512;  // longest row the LDS factorisation accepts
+
+// ---------------------------------------------------------------- device scalars
+struct Scalars {
+  double rho, alpha, omega, beta;
+  double r0v, ts, tt, rr, r0r, ss;
+  double bnorm, res;
+  int breakdown, pad;
+};
+
+// ---------------------------------------------------------------- reductions
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+// Block-wide sum of NV values; result valid in thread 0.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV]) {
+  __shared__ double red[NV][kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double s = wave_sum(v[q]);
+    if (lane == 0) red[q][wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0;
+      for (int w = 0; w < kBlock / 64; ++w) s += red[q][w];
+      v[q] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- SpMV
+// y = A x            (mode 0)
+// y = b - A x        (mode 1)
+// plus optional fused dots of y with d0 and d1 (per-block partials).
+template <int LPR, typename TV, typename T, typename TA>
+__global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                 const TV *__restrict__ v, const T *__restrict__ x,
+                                                 T *__restrict__ y, const T *__restrict__ b,
+                                                 const T *__restrict__ d0, const T *__restrict__ d1,
+                                                 double *__restrict__ part) {
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = gtid & (LPR - 1);
+  const int groups = gridDim.x * blockDim.x / LPR;
+  double acc0 = 0.0, acc1 = 0.0;
+  // all lanes of an LPR group share `row`, so groups leave the loop together
+  for (int row = gtid / LPR; row < n; row += groups) {
+    TA s = 0;
+    const int beg = rp[row], end = rp[row + 1];
+    for (int k = beg + lane; k < end; k += LPR) s += (TA)v[k] * (TA)x[ci[k]];
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
+    if (lane == 0) {
+      T out = b ? (T)(b[row] - s) : (T)s;
+      y[row] = out;
+      if (d0) acc0 += (double)d0[row] * (double)out;
+      if (d1) acc1 += (double)d1[row] * (double)out;
+    }
+  }
+  if (part) {
+    double vv[2] = {acc0, acc1};
+    block_sum<2>(vv);
+    if (threadIdx.x == 0) {
+      part[blockIdx.x] = vv[0];
+      part[kGrid + blockIdx.x] = vv[1];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- vector kernels
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_copy_dot(int n, const T *__restrict__ a, T *__restrict__ out,
+                                                     double *__restrict__ part) {
+  // out = a; partial (a, a)
+  double s = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    T v = a[i];
+    out[i] = v;
+    s += (double)v * (double)v;
+  }
+  double vv[1] = {s};
+  block_sum<1>(vv);
+  if (threadIdx.x == 0) part[blockIdx.x] = vv[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_update_p(int n, T *__restrict__ p, const T *__restrict__ r,
+                                                     const T *__restrict__ v, const Scalars *__restrict__ sc) {
+  const T beta = (T)sc->beta, omega = (T)sc->omega;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_update_s(int n, T *__restrict__ s, const T *__restrict__ r,
+                                                     const T *__restrict__ v, const Scalars *__restrict__ sc) {
+  const T alpha = (T)sc->alpha;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s[i] = r[i] - alpha * v[i];
+}
+
+// x += alpha*ph + omega*sh ; r = s - omega*t ; partials (r,r), (r0,r)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_update_xr(int n, T *__restrict__ x, T *__restrict__ r,
+                                                      const T *__restrict__ ph, const T *__restrict__ sh,
+                                                      const T *__restrict__ s, const T *__restrict__ t,
+                                                      const T *__restrict__ r0, const Scalars *__restrict__ sc,
+                                                      double *__restrict__ part) {
+  const T alpha = (T)sc->alpha, omega = (T)sc->omega;
+  double a0 = 0, a1 = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x[i] += alpha * ph[i] + omega * sh[i];
+    T ri = s[i] - omega * t[i];
+    r[i] = ri;
+    a0 += (double)ri * (double)ri;
+    a1 += (double)r0[i] * (double)ri;
+  }
+  double vv[2] = {a0, a1};
+  block_sum<2>(vv);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = vv[0];
+    part[kGrid + blockIdx.x] = vv[1];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(int64_t n, const int *__restrict__ pos, const double *__restrict__ src,
+                                                   double *__restrict__ dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[pos[i]];
+}
+
+// FP64 iterative-refinement helpers
+__global__ __launch_bounds__(kBlock) void k_cast_scale(int n, const double *__restrict__ a, float *__restrict__ out,
+                                                       const double *__restrict__ nrm) {
+  const double s = nrm[0] > 0 ? 1.0 / nrm[0] : 1.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = (float)(a[i] * s);
+}
+__global__ __launch_bounds__(kBlock) void k_axpy_f32(int n, double *__restrict__ x, const float *__restrict__ d,
+                                                     const double *__restrict__ nrm) {
+  const double s = nrm[0];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += (double)d[i] * s;
+}
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kBlock) void k_cast(int64_t n, const TI *__restrict__ a, TO *__restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (TO)a[i];
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill(int64_t n, T *__restrict__ a, T v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) a[i] = v;
+}
+
+// ---------------------------------------------------------------- finalise kernels (1 block)
+__device__ inline double reduce_part(const double *part, int count) {
+  __shared__ double red[kBlock / 64];
+  double s = 0;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) s += part[i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double t = 0;
+  for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+// op: 0 = init (rr -> bnorm/res, rho=alpha=omega=1, r0r=rr)
+//     1 = after v = A ph: alpha = rho / r0v
+//     2 = after t = A sh: omega = ts/tt
+//     3 = after x/r update: res, beta for next iteration
+//     4 = store norm of partial sum (sqrt) into slot
+__global__ __launch_bounds__(kBlock) void k_finalize(int op, const double *__restrict__ part, Scalars *sc,
+                                                     double *__restrict__ out_norm) {
+  if (op == 0) {
+    double rr = reduce_part(part, kGrid);
+    if (threadIdx.x == 0) {
+      sc->rr = rr;
+      sc->bnorm = sqrt(rr);
+      sc->res = 1.0;
+      sc->rho = sc->alpha = sc->omega = 1.0;
+      sc->r0r = rr;
+      sc->beta = 0.0;
+      sc->breakdown = 0;
+      // first direction: p = r (beta = 0 with p = v = 0)
+      sc->beta = (sc->r0r / sc->rho) * (sc->alpha / sc->omega);
+      sc->rho = sc->r0r;
+    }
+  } else if (op == 1) {
+    double r0v = reduce_part(part, kGrid);
+    if (threadIdx.x == 0) {
+      sc->r0v = r0v;
+      if (r0v == 0.0 || !isfinite(r0v)) { sc->breakdown = 1; sc->alpha = 0.0; }
+      else sc->alpha = sc->rho / r0v;
+    }
+  } else if (op == 2) {
+    double ts = reduce_part(part, kGrid);
+    double tt = reduce_part(part + kGrid, kGrid);
+    if (threadIdx.x == 0) {
+      sc->ts = ts;
+      sc->tt = tt;
+      sc->omega = tt > 0.0 ? ts / tt : 0.0;
+    }
+  } else if (op == 3) {
+    double rr = reduce_part(part, kGrid);
+    double r0r = reduce_part(part + kGrid, kGrid);
+    if (threadIdx.x == 0) {
+      sc->rr = rr;
+      sc->r0r = r0r;
+      sc->res = sc->bnorm > 0 ? sqrt(rr) / sc->bnorm : 0.0;
+      if (sc->omega == 0.0 || r0r == 0.0 || !isfinite(rr)) {
+        sc->breakdown = 1;
+      } else {
+        sc->beta = (r0r / sc->rho) * (sc->alpha / sc->omega);
+        sc->rho = r0r;
+      }
+    }
+  } else if (op == 4) {
+    double s = reduce_part(part, kGrid);
+    if (threadIdx.x == 0) out_norm[0] = sqrt(s);
+  }
+}
+
+// ---------------------------------------------------------------- level-scheduled ILU(0)
+// Forward sweep (unit lower triangle): out[row] = rhs[row] - sum_{k<diag} L[k] out[col[k]]
+template <int LPR, typename TV, typename T>
+__global__ __launch_bounds__(kBlock) void k_lsolve(int cnt, const int *__restrict__ rows, const int *__restrict__ rp,
+                                                   const int *__restrict__ ci, const int *__restrict__ dg,
+                                                   const TV *__restrict__ fv, const T *__restrict__ rhs,
+                                                   T *__restrict__ out) {
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = gtid / LPR, lane = gtid & (LPR - 1);
+  if (g >= cnt) return;
+  const int row = rows[g];
+  T s = 0;
+  for (int k = rp[row] + lane; k < dg[row]; k += LPR) s += (T)fv[k] * out[ci[k]];
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
+  if (lane == 0) out[row] = rhs[row] - s;
+}
+// Backward sweep (upper triangle with diagonal), in place on out.
+template <int LPR, typename TV, typename T>
+__global__ __launch_bounds__(kBlock) void k_usolve(int cnt, const int *__restrict__ rows, const int *__restrict__ rp,
+                                                   const int *__restrict__ ci, const int *__restrict__ dg,
+                                                   const TV *__restrict__ fv, T *__restrict__ out) {
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = gtid / LPR, lane = gtid & (LPR - 1);
+  if (g >= cnt) return;
+  const int row = rows[g];
+  const int d = dg[row];
+  T s = 0;
+  for (int k = d + 1 + lane; k < rp[row + 1]; k += LPR) s += (T)fv[k] * out[ci[k]];
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
+  if (lane == 0) out[row] = (out[row] - s) / (T)fv[d];
+}
+
+// ILU(0) of the rows of one level; one 64-lane block per row, row kept in LDS.
+// Row-wise IKJ order (Saad, Iterative Methods, 2nd ed., Alg. 10.4).
+__global__ __launch_bounds__(64) void k_ilu0_level(int cnt, const int *__restrict__ rows, const int *__restrict__ rp,
+                                                   const int *__restrict__ ci, const int *__restrict__ dg,
+                                                   double *__restrict__ fv) {
+  __shared__ int cols[kMaxRowLDS];
+  __shared__ double vals[kMaxRowLDS];
+  if ((int)blockIdx.x >= cnt) return;
+  const int row = rows[blockIdx.x];
+  const int beg = rp[row], len = rp[row + 1] - beg, nl = dg[row] - beg;
+  for (int t = threadIdx.x; t < len; t += 64) {
+    cols[t] = ci[beg + t];
+    vals[t] = fv[beg + t];
+  }
+  __syncthreads();
+  for (int t = 0; t < nl; ++t) {
+    const int kc = cols[t];
+    const double l = vals[t] / fv[dg[kc]];
+    const int ub = dg[kc] + 1, ue = rp[kc + 1];
+    for (int q = ub + threadIdx.x; q < ue; q += 64) {
+      const int j = ci[q];
+      int lo = t + 1, hi = len;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cols[mid] < j) lo = mid + 1; else hi = mid;
+      }
+      if (lo < len && cols[lo] == j) vals[lo] -= l * fv[q];
+    }
+    if (threadIdx.x == 0) vals[t] = l;
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < len; t += 64) fv[beg + t] = vals[t];
+}
+
+// ---------------------------------------------------------------- host side
+inline int grid_for(int64_t items, int per_block) {
+  int64_t g = (items + per_block - 1) / per_block;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, 1 << 30));
+}
+
+struct Levels {
+  std::vector<int> ptr;   // level boundaries into rows
+  int *d_rows = nullptr;  // rows ordered by level (ascending within a level)
+  int count() const { return (int)ptr.size() - 1; }
+};
+
+}  // namespace
+
+struct shesolve_ctx {
+  int device = 0;
+  shesolve_params prm{};
+  hipStream_t stream = nullptr;
+  // matrix
+  int n = 0;
+  int64_t nnz = 0;
+  int *d_rp = nullptr, *d_ci = nullptr;
+  double *d_v64 = nullptr;
+  float *d_v32 = nullptr;
+  std::vector<int> h_rp, h_ci;  // pattern copy for analysis
+  // factor (pattern = A or its block-diagonal part)
+  bool factor_is_A = true;
+  int64_t fnnz = 0;
+  int *d_frp = nullptr, *d_fci = nullptr, *d_fdg = nullptr, *d_fpos = nullptr;
+  double *d_f64 = nullptr;
+  float *d_f32 = nullptr;
+  std::vector<int> h_frp, h_fci, h_fdg;
+  std::vector<int> blk_of_row;
+  Levels lvL, lvU;
+  int lpr = 8, flpr = 8;
+  // vectors (FP64 set and FP32 set)
+  double *v64[9] = {};
+  float *v32[9] = {};
+  double *d_part = nullptr;
+  Scalars *d_sc = nullptr;
+  double *d_norm = nullptr;
+  int vec_n = 0;
+  double last_setup_ms = 0;
+  // profiling
+  bool prof = false;
+  double ktime[4] = {0, 0, 0, 0};
+  int64_t kcount[4] = {0, 0, 0, 0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
+int pick_lpr(double avg) {
+  if (avg <= 3) return 2;
+  if (avg <= 6) return 4;
+  if (avg <= 14) return 8;
+  if (avg <= 28) return 16;
+  if (avg <= 56) return 32;
+  return 64;
+}
+
+// timing helper: brackets a launch with events when profiling is on
+struct KTimer {
+  shesolve_ctx *c;
+  int kind;
+  KTimer(shesolve_ctx *c_, int k) : c(c_), kind(k) {
+    if (c->prof) hipEventRecord(c->ev0, c->stream);
+  }
+  ~KTimer() {
+    if (c->prof) {
+      hipEventRecord(c->ev1, c->stream);
+      hipEventSynchronize(c->ev1);
+      float ms = 0;
+      hipEventElapsedTime(&ms, c->ev0, c->ev1);
+      c->ktime[kind] += ms * 1e3;
+      c->kcount[kind] += 1;
+    }
+  }
+};
+
+template <typename TV, typename T, typename TA>
+void launch_spmv(shesolve_ctx *c, int lpr, const TV *v, const T *x, T *y, const T *b, const T *d0, const T *d1,
+                 double *part) {
+  KTimer kt(c, 0);
+  const int n = c->n;
+  int g = std::min<int64_t>(kGrid, grid_for((int64_t)n * lpr, kBlock));
+  if (part) g = kGrid;  // fixed partial count: every block writes its partial
+#define SPMV_CASE(L)                                                                                        \
+  case L:                                                                                                   \
+    hipLaunchKernelGGL((k_spmv<L, TV, T, TA>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rp, c->d_ci, v, \
+                       x, y, b, d0, d1, part);                                                              \
+    break;
+  switch (lpr) {
+    SPMV_CASE(2) SPMV_CASE(4) SPMV_CASE(8) SPMV_CASE(16) SPMV_CASE(32) SPMV_CASE(64)
+    default:
+      hipLaunchKernelGGL((k_spmv<8, TV, T, TA>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rp, c->d_ci, v, x, y,
+                         b, d0, d1, part);
+  }
+#undef SPMV_CASE
+}
+
+template <typename TV, typename T>
+void launch_precond(shesolve_ctx *c, const TV *fv, const T *in, T *out) {
+  const int n = c->n;
+  if (c->prm.precond == SHESOLVE_PRECOND_NONE || !c->d_frp) {
+    hipMemcpyAsync(out, in, sizeof(T) * n, hipMemcpyDeviceToDevice, c->stream);
+    return;
+  }
+  const int L = c->flpr;
+  for (int l = 0; l < c->lvL.count(); ++l) {
+    KTimer kt(c, 1);
+    int cnt = c->lvL.ptr[l + 1] - c->lvL.ptr[l];
+    const int *rows = c->lvL.d_rows + c->lvL.ptr[l];
+    int g = grid_for((int64_t)cnt * L, kBlock);
+#define LS_CASE(LL)                                                                                         \
+  case LL:                                                                                                  \
+    hipLaunchKernelGGL((k_lsolve<LL, TV, T>), dim3(g), dim3(kBlock), 0, c->stream, cnt, rows, c->d_frp,     \
+                       c->d_fci, c->d_fdg, fv, in, out);                                                    \
+    break;
+    switch (L) { LS_CASE(2) LS_CASE(4) LS_CASE(8) LS_CASE(16) LS_CASE(32) default: LS_CASE(64) }
+#undef LS_CASE
+  }
+  for (int l = 0; l < c->lvU.count(); ++l) {
+    KTimer kt(c, 2);
+    int cnt = c->lvU.ptr[l + 1] - c->lvU.ptr[l];
+    const int *rows = c->lvU.d_rows + c->lvU.ptr[l];
+    int g = grid_for((int64_t)cnt * L, kBlock);
+#define US_CASE(LL)                                                                                         \
+  case LL:                                                                                                  \
+    hipLaunchKernelGGL((k_usolve<LL, TV, T>), dim3(g), dim3(kBlock), 0, c->stream, cnt, rows, c->d_frp,     \
+                       c->d_fci, c->d_fdg, fv, out);                                                        \
+    break;
+    switch (L) { US_CASE(2) US_CASE(4) US_CASE(8) US_CASE(16) US_CASE(32) default: US_CASE(64) }
+#undef US_CASE
+  }
+}
+
+int ensure_vectors(shesolve_ctx *c) {
+  if (c->vec_n == c->n && c->v64[0]) return 0;
+  for (auto &p : c->v64) dfree(p);
+  for (auto &p : c->v32) dfree(p);
+  size_t n = (size_t)std::max(c->n, 1);
+  for (auto &p : c->v64) HIPCHK(hipMalloc(&p, n * sizeof(double)));
+  for (auto &p : c->v32) HIPCHK(hipMalloc(&p, n * sizeof(float)));
+  if (!c->d_part) HIPCHK(hipMalloc(&c->d_part, sizeof(double) * 2 * kGrid));
+  if (!c->d_sc) HIPCHK(hipMalloc(&c->d_sc, sizeof(Scalars)));
+  if (!c->d_norm) HIPCHK(hipMalloc(&c->d_norm, sizeof(double) * 4));
+  c->vec_n = c->n;
+  return 0;
+}
+
+void free_matrix(shesolve_ctx *c) {
+  dfree(c->d_rp); dfree(c->d_ci); dfree(c->d_v64); dfree(c->d_v32);
+  if (!c->factor_is_A) { dfree(c->d_frp); dfree(c->d_fci); }
+  c->d_frp = c->d_fci = nullptr;
+  dfree(c->d_fdg); dfree(c->d_fpos); dfree(c->d_f64); dfree(c->d_f32);
+  dfree(c->lvL.d_rows); dfree(c->lvU.d_rows);
+  c->lvL.ptr.clear(); c->lvU.ptr.clear();
+}
+
+// Level sets of the lower (forward) and upper (backward) triangles of the factor pattern.
+void compute_levels(const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<int> &dg, int n,
+                    std::vector<int> &Lptr, std::vector<int> &Lrows, std::vector<int> &Uptr, std::vector<int> &Urows) {
+  std::vector<int> lev(n);
+  int maxl = 0;
+  for (int i = 0; i < n; ++i) {
+    int l = 0;
+    for (int k = rp[i]; k < dg[i]; ++k) l = std::max(l, lev[ci[k]] + 1);
+    lev[i] = l;
+    maxl = std::max(maxl, l);
+  }
+  auto bucket = [&](int nl, std::vector<int> &ptr, std::vector<int> &rows) {
+    ptr.assign(nl + 2, 0);
+    for (int i = 0; i < n; ++i) ptr[lev[i] + 1]++;
+    for (int l = 0; l <= nl; ++l) ptr[l + 1] += ptr[l];
+    rows.assign(n, 0);
+    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < n; ++i) rows[pos[lev[i]]++] = i;
+    ptr.resize(nl + 2);
+  };
+  bucket(maxl, Lptr, Lrows);
+  maxl = 0;
+  for (int i = n - 1; i >= 0; --i) {
+    int l = 0;
+    for (int k = dg[i] + 1; k < rp[i + 1]; ++k) l = std::max(l, lev[ci[k]] + 1);
+    lev[i] = l;
+    maxl = std::max(maxl, l);
+  }
+  bucket(maxl, Uptr, Urows);
+}
+
+int analyse_and_factor(shesolve_ctx *c) {
+  const int n = c->n;
+  if (n <= 0) return 0;
+  // factor pattern
+  if (c->prm.precond == SHESOLVE_PRECOND_NONE) return 0;
+  if (c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILU0 && !c->blk_of_row.empty()) {
+    c->factor_is_A = false;
+    c->h_frp.assign(n + 1, 0);
+    c->h_fci.clear();
+    c->h_fci.reserve(c->h_ci.size());
+    for (int i = 0; i < n; ++i) {
+      for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k)
+        if (c->blk_of_row[c->h_ci[k]] == c->blk_of_row[i]) c->h_fci.push_back(c->h_ci[k]);
+      c->h_frp[i + 1] = (int)c->h_fci.size();
+    }
+  } else {
+    c->factor_is_A = true;
+    c->h_frp = c->h_rp;
+    c->h_fci = c->h_ci;
+  }
+  c->fnnz = c->h_frp[n];
+  c->h_fdg.assign(n, -1);
+  int maxlen = 0;
+  for (int i = 0; i < n; ++i) {
+    maxlen = std::max(maxlen, c->h_frp[i + 1] - c->h_frp[i]);
+    for (int k = c->h_frp[i]; k < c->h_frp[i + 1]; ++k)
+      if (c->h_fci[k] == i) { c->h_fdg[i] = k; break; }
+    if (c->h_fdg[i] < 0) {
+      set_err("ILU0: row %d has no diagonal entry", i);
+      return 2;
+    }
+  }
+  if (maxlen > kMaxRowLDS) {
+    set_err("ILU0: row length %d exceeds %d", maxlen, kMaxRowLDS);
+    return 2;
+  }
+  std::vector<int> Lptr, Lrows, Uptr, Urows;
+  compute_levels(c->h_frp, c->h_fci, c->h_fdg, n, Lptr, Lrows, Uptr, Urows);
+  c->lvL.ptr = Lptr;
+  c->lvU.ptr = Uptr;
+  HIPCHK(hipMalloc(&c->lvL.d_rows, sizeof(int) * std::max(n, 1)));
+  HIPCHK(hipMalloc(&c->lvU.d_rows, sizeof(int) * std::max(n, 1)));
+  HIPCHK(hipMemcpyAsync(c->lvL.d_rows, Lrows.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->lvU.d_rows, Urows.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+  if (c->factor_is_A) {
+    c->d_frp = c->d_rp;
+    c->d_fci = c->d_ci;
+  } else {
+    HIPCHK(hipMalloc(&c->d_frp, sizeof(int) * (n + 1)));
+    HIPCHK(hipMalloc(&c->d_fci, sizeof(int) * std::max<int64_t>(c->fnnz, 1)));
+    HIPCHK(hipMemcpyAsync(c->d_frp, c->h_frp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_fci, c->h_fci.data(), sizeof(int) * c->fnnz, hipMemcpyHostToDevice, c->stream));
+    std::vector<int> pos;
+    pos.reserve(c->fnnz);
+    for (int i = 0; i < n; ++i)
+      for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k)
+        if (c->blk_of_row[c->h_ci[k]] == c->blk_of_row[i]) pos.push_back(k);
+    HIPCHK(hipMalloc(&c->d_fpos, sizeof(int) * std::max<int64_t>(c->fnnz, 1)));
+    HIPCHK(hipMemcpyAsync(c->d_fpos, pos.data(), sizeof(int) * c->fnnz, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  HIPCHK(hipMalloc(&c->d_fdg, sizeof(int) * std::max(n, 1)));
+  HIPCHK(hipMemcpyAsync(c->d_fdg, c->h_fdg.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMalloc(&c->d_f64, sizeof(double) * std::max<int64_t>(c->fnnz, 1)));
+  HIPCHK(hipMalloc(&c->d_f32, sizeof(float) * std::max<int64_t>(c->fnnz, 1)));
+  c->flpr = pick_lpr((double)c->fnnz / std::max(n, 1) / 2.0);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// Copies A's values into the factor pattern (gather for block ILU) and factorises.
+int refactor(shesolve_ctx *c) {
+  if (c->prm.precond == SHESOLVE_PRECOND_NONE || !c->d_frp) return 0;
+  if (c->factor_is_A) {
+    HIPCHK(hipMemcpyAsync(c->d_f64, c->d_v64, sizeof(double) * c->nnz, hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(c->fnnz, kBlock * 4)), dim3(kBlock), 0, c->stream, c->fnnz, c->d_fpos,
+                       c->d_v64, c->d_f64);
+  }
+  for (int l = 0; l < c->lvL.count(); ++l) {
+    int cnt = c->lvL.ptr[l + 1] - c->lvL.ptr[l];
+    hipLaunchKernelGGL(k_ilu0_level, dim3(cnt), dim3(64), 0, c->stream, cnt, c->lvL.d_rows + c->lvL.ptr[l],
+                       c->d_frp, c->d_fci, c->d_fdg, c->d_f64);
+  }
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->fnnz, kBlock * 4)), dim3(kBlock), 0, c->stream,
+                     c->fnnz, c->d_f64, c->d_f32);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int upload_values(shesolve_ctx *c) {
+  hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->nnz, kBlock * 4)), dim3(kBlock), 0, c->stream, c->nnz,
+                     c->d_v64, c->d_v32);
+  HIPCHK(hipGetLastError());
+  return refactor(c);
+}
+
+// One BiCGStab solve (right preconditioned) on the device, precision T.
+// b, x device vectors of precision T.  Returns iterations; status in *st.
+// confirm_true: on convergence of the recurrence residual, confirm with the
+// explicit residual (FP64 solves).  FP32 inner solves of the refinement loop
+// skip it: the outer FP64 residual is the acceptance test there.
+template <typename TV, typename T>
+int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, double tol, int max_iters,
+                 int *st, double *res_out, bool confirm_true = true) {
+  const int n = c->n;
+  T **V;
+  if constexpr (sizeof(T) == 8) V = (T **)c->v64; else V = (T **)c->v32;
+  T *r = V[0], *r0 = V[1], *p = V[2], *v = V[3], *ph = V[4], *s = V[5], *sh = V[6], *t = V[7];
+  double *part = c->d_part;
+  Scalars hs{};
+  const int vg = kGrid;
+  // r = b - A x ; r0 = r ; (r,r)
+  launch_spmv<TV, T, T>(c, c->lpr, Av, x, r, b, nullptr, nullptr, nullptr);
+  {
+    KTimer kt(c, 3);
+    hipLaunchKernelGGL((k_copy_dot<T>), dim3(vg), dim3(kBlock), 0, c->stream, n, r, r0, part);
+  }
+  // bnorm comes from b, not from the initial residual
+  hipLaunchKernelGGL((k_fill<T>), dim3(vg), dim3(kBlock), 0, c->stream, (int64_t)n, p, (T)0);
+  hipLaunchKernelGGL((k_fill<T>), dim3(vg), dim3(kBlock), 0, c->stream, (int64_t)n, v, (T)0);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 0, part, c->d_sc, c->d_norm);
+  // |b|
+  hipLaunchKernelGGL((k_copy_dot<T>), dim3(vg), dim3(kBlock), 0, c->stream, n, b, s, part);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, part, c->d_sc, c->d_norm);
+  double bn = 0;
+  HIPCHK(hipMemcpyAsync(&hs, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&bn, c->d_norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double r_init = hs.bnorm;  // ||r_0||
+  // store ||b|| as the normaliser
+  {
+    Scalars tmp = hs;
+    tmp.bnorm = bn;
+    tmp.res = bn > 0 ? r_init / bn : 0.0;
+    HIPCHK(hipMemcpyAsync(c->d_sc, &tmp, sizeof(Scalars), hipMemcpyHostToDevice, c->stream));
+    hs = tmp;
+  }
+  *st = 1;
+  if (bn == 0.0) {
+    hipLaunchKernelGGL((k_fill<T>), dim3(vg), dim3(kBlock), 0, c->stream, (int64_t)n, x, (T)0);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *st = 0;
+    *res_out = 0.0;
+    return 0;
+  }
+  if (hs.res <= tol) {
+    *st = 0;
+    *res_out = hs.res;
+    return 0;
+  }
+  int it = 0;
+  while (it < max_iters) {
+    ++it;
+    {
+      KTimer kt(c, 3);
+      hipLaunchKernelGGL((k_update_p<T>), dim3(vg), dim3(kBlock), 0, c->stream, n, p, r, v, c->d_sc);
+    }
+    launch_precond<TV, T>(c, Mv, p, ph);
+    launch_spmv<TV, T, T>(c, c->lpr, Av, ph, v, nullptr, r0, nullptr, part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 1, part, c->d_sc, c->d_norm);
+    {
+      KTimer kt(c, 3);
+      hipLaunchKernelGGL((k_update_s<T>), dim3(vg), dim3(kBlock), 0, c->stream, n, s, r, v, c->d_sc);
+    }
+    launch_precond<TV, T>(c, Mv, s, sh);
+    launch_spmv<TV, T, T>(c, c->lpr, Av, sh, t, nullptr, s, t, part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 2, part, c->d_sc, c->d_norm);
+    {
+      KTimer kt(c, 3);
+      hipLaunchKernelGGL((k_update_xr<T>), dim3(vg), dim3(kBlock), 0, c->stream, n, x, r, ph, sh, s, t, r0, c->d_sc,
+                         part);
+    }
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 3, part, c->d_sc, c->d_norm);
+    HIPCHK(hipMemcpyAsync(&hs, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!std::isfinite(hs.res)) {
+      *st = 2;
+      break;
+    }
+    if (hs.res <= tol && !confirm_true) {
+      *st = 0;
+      break;
+    }
+    if (hs.res <= tol) {
+      // confirm with the explicit residual r = b - A x
+      launch_spmv<TV, T, T>(c, c->lpr, Av, x, r, b, nullptr, nullptr, nullptr);
+      hipLaunchKernelGGL((k_copy_dot<T>), dim3(vg), dim3(kBlock), 0, c->stream, n, r, r0, part);
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, part, c->d_sc, c->d_norm);
+      double rn = 0;
+      HIPCHK(hipMemcpyAsync(&rn, c->d_norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      hs.res = rn / bn;
+      if (hs.res <= tol) {
+        *st = 0;
+        break;
+      }
+      hs.breakdown = 1;  // restart from the true residual (r0 = r already)
+    }
+    if (hs.breakdown) {
+      // restart: r0 = r, p = v = 0, rho = (r0,r), alpha = omega = 1
+      hipLaunchKernelGGL((k_copy_dot<T>), dim3(vg), dim3(kBlock), 0, c->stream, n, r, r0, part);
+      hipLaunchKernelGGL((k_fill<T>), dim3(vg), dim3(kBlock), 0, c->stream, (int64_t)n, p, (T)0);
+      hipLaunchKernelGGL((k_fill<T>), dim3(vg), dim3(kBlock), 0, c->stream, (int64_t)n, v, (T)0);
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 0, part, c->d_sc, c->d_norm);
+      Scalars tmp{};
+      HIPCHK(hipMemcpyAsync(&tmp, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      tmp.bnorm = bn;
+      tmp.res = std::sqrt(tmp.rr) / bn;
+      HIPCHK(hipMemcpyAsync(c->d_sc, &tmp, sizeof(Scalars), hipMemcpyHostToDevice, c->stream));
+      if (tmp.rr == 0.0) { *st = 0; hs.res = 0; break; }
+    }
+  }
+  *res_out = hs.res;
+  return it;
+}
+
+// FP64 residual norm ||b - A x|| into d_norm[slot] and r64 = b - A x
+int residual64(shesolve_ctx *c, const double *b, const double *x, double *r, double *out) {
+  launch_spmv<double, double, double>(c, c->lpr, c->d_v64, x, r, b, nullptr, nullptr, nullptr);
+  hipLaunchKernelGGL((k_copy_dot<double>), dim3(kGrid), dim3(kBlock), 0, c->stream, c->n, r, r, c->d_part);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, c->d_part, c->d_sc, c->d_norm);
+  HIPCHK(hipMemcpyAsync(out, c->d_norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int solve_device(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_report *rep) {
+  auto t0 = std::chrono::steady_clock::now();
+  shesolve_report r{};
+  if (!c->d_rp) {
+    set_err("no matrix set");
+    return -1;
+  }
+  if (int e = ensure_vectors(c)) return e;
+  const int n = c->n;
+  if (!warm) hipLaunchKernelGGL((k_fill<double>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)n, x, 0.0);
+  if (c->prm.precision == SHESOLVE_FP64) {
+    int st = 1;
+    double res = 0;
+    int it = bicgstab_dev<double, double>(c, c->d_v64, c->d_f64, b, x, c->prm.tol, c->prm.max_iters, &st, &res);
+    if (it < 0) return it;
+    r.iterations = it;
+    r.status = st;
+    r.rel_residual = res;
+  } else {
+    // FP32 BiCGStab inside FP64 iterative refinement
+    double *r64 = c->v64[8];
+    float *r32 = c->v32[8];
+    double bn = 0;
+    // ||b||
+    hipLaunchKernelGGL((k_copy_dot<double>), dim3(kGrid), dim3(kBlock), 0, c->stream, n, b, c->v64[0], c->d_part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, c->d_part, c->d_sc, c->d_norm);
+    HIPCHK(hipMemcpyAsync(&bn, c->d_norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (bn == 0.0) {
+      hipLaunchKernelGGL((k_fill<double>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)n, x, 0.0);
+      HIPCHK(hipStreamSynchronize(c->stream));
+      r.status = 0;
+    } else {
+      r.status = 1;
+      for (int k = 0; k <= c->prm.max_ir_steps; ++k) {
+        double rn = 0;
+        if (int e = residual64(c, b, x, r64, &rn)) return e;
+        r.rel_residual = rn / bn;
+        if (r.rel_residual <= c->prm.tol) {
+          r.status = 0;
+          break;
+        }
+        if (k == c->prm.max_ir_steps || r.iterations >= c->prm.max_iters) break;
+        r.ir_steps++;
+        // r32 = r64 / ||r64|| ; d = 0 ; inner solve ; x += ||r|| * d
+        hipLaunchKernelGGL(k_cast_scale, dim3(kGrid), dim3(kBlock), 0, c->stream, n, r64, r32, c->d_norm);
+        // the correction d (n floats) lives in the FP64 slot 7, idle during the FP32 solve
+        float *dd = reinterpret_cast<float *>(c->v64[7]);
+        hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)n, dd, 0.0f);
+        int st = 1;
+        double ires = 0;
+        // keep ||r|| on the host: the inner solve reuses d_norm
+        int it = bicgstab_dev<float, float>(c, c->d_v32, c->d_f32, r32, dd, c->prm.inner_tol,
+                                            c->prm.max_iters - r.iterations, &st, &ires, false);
+        if (it < 0) return it;
+        r.iterations += it;
+        HIPCHK(hipMemcpyAsync(c->d_norm, &rn, sizeof(double), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_axpy_f32, dim3(kGrid), dim3(kBlock), 0, c->stream, n, x, dd, c->d_norm);
+        if (st == 2 && it <= 1) {
+          r.status = 2;
+          break;
+        }
+      }
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  r.setup_ms = c->last_setup_ms;
+  r.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (rep) *rep = r;
+  return 0;
+}
+
+int set_pattern_host(shesolve_ctx *c, int64_t n, int64_t nnz, const int64_t *rp, const int32_t *ci, int nblocks,
+                     const int64_t *ranges) {
+  if (n <= 0 || n > 0x7fffffff) { set_err("n out of range"); return 2; }
+  if (nnz < 0 || nnz > 0x7fffffff) { set_err("nnz must fit a 32-bit index"); return 3; }
+  if (rp[0] != 0 || rp[n] != nnz) { set_err("row_ptr inconsistent with nnz"); return 4; }
+  c->h_rp.resize(n + 1);
+  for (int64_t i = 0; i <= n; ++i) c->h_rp[i] = (int)rp[i];
+  c->h_ci.assign(ci, ci + nnz);
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
+      if (ci[k] < 0 || ci[k] >= n || (k > rp[i] && ci[k] <= ci[k - 1])) {
+        set_err("column indices must be in range and strictly ascending per row (row %lld)", (long long)i);
+        return 5;
+      }
+    }
+  c->blk_of_row.clear();
+  if (c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILU0) {
+    if (nblocks > 0 && !ranges) { set_err("block ranges missing"); return 7; }
+    c->blk_of_row.assign(n, -1);
+    for (int bk = 0; bk < nblocks; ++bk)
+      for (int64_t i = std::max<int64_t>(0, ranges[2 * bk]); i < std::min<int64_t>(n, ranges[2 * bk + 1]); ++i)
+        if (c->blk_of_row[i] < 0) c->blk_of_row[i] = bk;
+    for (int64_t i = 0; i < n; ++i)
+      if (c->blk_of_row[i] < 0) c->blk_of_row[i] = nblocks;
+  }
+  c->n = (int)n;
+  c->nnz = nnz;
+  return 0;
+}
+
+int set_matrix_common(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, const int64_t *rp,
+                      const int32_t *ci, const double *val, int nblocks, const int64_t *ranges) {
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(hipSetDevice(c->device));
+  free_matrix(c);
+  std::vector<int64_t> hrp;
+  std::vector<int32_t> hci;
+  if (device_src) {
+    hrp.resize(n + 1);
+    HIPCHK(hipMemcpy(hrp.data(), rp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+    nnz = hrp[n];
+    hci.resize(nnz);
+    HIPCHK(hipMemcpy(hci.data(), ci, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    rp = hrp.data();
+    ci = hci.data();
+  }
+  if (int e = set_pattern_host(c, n, nnz, rp, ci, nblocks, ranges)) return e;
+  HIPCHK(hipMalloc(&c->d_rp, sizeof(int) * (n + 1)));
+  HIPCHK(hipMalloc(&c->d_ci, sizeof(int) * std::max<int64_t>(nnz, 1)));
+  HIPCHK(hipMalloc(&c->d_v64, sizeof(double) * std::max<int64_t>(nnz, 1)));
+  HIPCHK(hipMalloc(&c->d_v32, sizeof(float) * std::max<int64_t>(nnz, 1)));
+  HIPCHK(hipMemcpyAsync(c->d_rp, c->h_rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_ci, c->h_ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_v64, val, sizeof(double) * nnz, device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                        c->stream));
+  c->lpr = c->prm.lanes_per_row > 0 ? c->prm.lanes_per_row : pick_lpr((double)nnz / n);
+  if (int e = analyse_and_factor(c)) return e;
+  if (int e = upload_values(c)) return e;
+  if (int e = ensure_vectors(c)) return e;
+  c->last_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+void shesolve_default_params(shesolve_params *p) {
+  if (!p) return;
+  p->tol = 1e-10;
+  p->max_iters = 1000;
+  p->precond = SHESOLVE_PRECOND_ILU0;
+  p->precision = SHESOLVE_FP64;
+  p->inner_tol = 1e-5;
+  p->max_ir_steps = 30;
+  p->lanes_per_row = 0;
+}
+
+const char *shesolve_last_error(void) { return g_err.c_str(); }
+
+static int check_params(const shesolve_params *p) {
+  if (!(p->tol > 0)) { set_err("tol must be positive"); return 1; }
+  if (p->max_iters <= 0) { set_err("max_iters must be positive"); return 1; }
+  if (p->precond != SHESOLVE_PRECOND_NONE && p->precond != SHESOLVE_PRECOND_ILU0 &&
+      p->precond != SHESOLVE_PRECOND_BLOCK_ILU0) { set_err("unknown preconditioner %d", p->precond); return 1; }
+  if (p->precision != SHESOLVE_FP64 && p->precision != SHESOLVE_FP32_IR) { set_err("unknown precision"); return 1; }
+  if (p->lanes_per_row != 0 && (p->lanes_per_row < 2 || p->lanes_per_row > 64 ||
+                                (p->lanes_per_row & (p->lanes_per_row - 1)))) {
+    set_err("lanes_per_row must be 0 or a power of two in [2, 64]");
+    return 1;
+  }
+  return 0;
+}
+
+int shesolve_create(shesolve_handle *h, int device, const shesolve_params *p) {
+  if (!h) { set_err("handle pointer is NULL"); return 1; }
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) { set_err("no HIP device available"); return -2; }
+  if (device < 0 || device >= cnt) { set_err("device %d out of range", device); return 2; }
+  shesolve_params prm;
+  shesolve_default_params(&prm);
+  if (p) {
+    if (check_params(p)) return 3;
+    prm = *p;
+  }
+  HIPCHK(hipSetDevice(device));
+  shesolve_ctx *c = new shesolve_ctx();
+  c->device = device;
+  c->prm = prm;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    set_err("stream/event creation failed");
+    delete c;
+    return -10;
+  }
+  *h = c;
+  return 0;
+}
+
+int shesolve_destroy(shesolve_handle h) {
+  if (!h) return 1;
+  hipSetDevice(h->device);
+  hipStreamSynchronize(h->stream);
+  free_matrix(h);
+  for (auto &p : h->v64) dfree(p);
+  for (auto &p : h->v32) dfree(p);
+  dfree(h->d_part);
+  dfree(h->d_sc);
+  dfree(h->d_norm);
+  hipEventDestroy(h->ev0);
+  hipEventDestroy(h->ev1);
+  hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+int shesolve_set_params(shesolve_handle h, const shesolve_params *p) {
+  if (!h) return 1;
+  if (!p) return 2;
+  if (check_params(p)) return 2;
+  bool refactor_needed = h->d_rp && p->precond != h->prm.precond;
+  h->prm = *p;
+  if (h->d_rp && p->lanes_per_row > 0) h->lpr = p->lanes_per_row;
+  if (refactor_needed) {
+    set_err("changing the preconditioner needs a new shesolve_set_matrix");
+    return -3;
+  }
+  return 0;
+}
+
+int shesolve_set_matrix(shesolve_handle h, int64_t n, int64_t nnz, const int64_t *row_ptr, const int32_t *col,
+                        const double *val, int nblocks, const int64_t *block_ranges) {
+  if (!h) return 1;
+  if (!row_ptr) return 4;
+  if (!col && nnz > 0) return 5;
+  if (!val && nnz > 0) return 6;
+  return set_matrix_common(h, false, n, nnz, row_ptr, col, val, nblocks, block_ranges);
+}
+
+int shesolve_set_matrix_device(shesolve_handle h, int64_t n, int64_t nnz, const int64_t *d_row_ptr,
+                               const int32_t *d_col, const double *d_val, int nblocks, const int64_t *block_ranges) {
+  if (!h) return 1;
+  if (!d_row_ptr) return 4;
+  if (!d_col) return 5;
+  if (!d_val) return 6;
+  return set_matrix_common(h, true, n, nnz, d_row_ptr, d_col, d_val, nblocks, block_ranges);
+}
+
+int shesolve_update_values(shesolve_handle h, const double *val) {
+  if (!h) return 1;
+  if (!val) return 2;
+  if (!h->d_v64) { set_err("no matrix set"); return -1; }
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipMemcpyAsync(h->d_v64, val, sizeof(double) * h->nnz, hipMemcpyHostToDevice, h->stream));
+  int e = upload_values(h);
+  h->last_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return e;
+}
+
+int shesolve_update_values_device(shesolve_handle h, const double *d_val) {
+  if (!h) return 1;
+  if (!d_val) return 2;
+  if (!h->d_v64) { set_err("no matrix set"); return -1; }
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipMemcpyAsync(h->d_v64, d_val, sizeof(double) * h->nnz, hipMemcpyDeviceToDevice, h->stream));
+  int e = upload_values(h);
+  h->last_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return e;
+}
+
+int shesolve_solve(shesolve_handle h, const double *b, double *x, int warm_start, shesolve_report *report) {
+  if (!h) return 1;
+  if (!b) return 2;
+  if (!x) return 3;
+  if (!h->d_rp) { set_err("no matrix set"); return -1; }
+  HIPCHK(hipSetDevice(h->device));
+  for (int i = 0; i < h->n; ++i)
+    if (!std::isfinite(b[i])) {
+      // check_vector_for_valid_entries (util/checks.hpp:110-130) throws on NaN/Inf
+      set_err("right-hand side holds a non-finite value at %d", i);
+      return 2;
+    }
+  double *db = nullptr, *dx = nullptr;
+  HIPCHK(hipMalloc(&db, sizeof(double) * h->n));
+  HIPCHK(hipMalloc(&dx, sizeof(double) * h->n));
+  HIPCHK(hipMemcpyAsync(db, b, sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
+  if (warm_start) HIPCHK(hipMemcpyAsync(dx, x, sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
+  int e = solve_device(h, db, dx, warm_start, report);
+  if (e == 0) {
+    hipMemcpyAsync(x, dx, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) { set_err("copy back failed"); e = -10; }
+  }
+  hipFree(db);
+  hipFree(dx);
+  return e;
+}
+
+int shesolve_solve_device(shesolve_handle h, const double *d_b, double *d_x, int warm_start, shesolve_report *report) {
+  if (!h) return 1;
+  if (!d_b) return 2;
+  if (!d_x) return 3;
+  HIPCHK(hipSetDevice(h->device));
+  return solve_device(h, d_b, d_x, warm_start, report);
+}
+
+int shesolve_spmv(shesolve_handle h, const double *x, double *y) {
+  if (!h) return 1;
+  if (!x) return 2;
+  if (!y) return 3;
+  if (!h->d_rp) { set_err("no matrix set"); return -1; }
+  HIPCHK(hipSetDevice(h->device));
+  if (int e = ensure_vectors(h)) return e;
+  HIPCHK(hipMemcpyAsync(h->v64[0], x, sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
+  launch_spmv<double, double, double>(h, h->lpr, h->d_v64, h->v64[0], h->v64[1], nullptr, nullptr, nullptr, nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(y, h->v64[1], sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int shesolve_spmv_f32(shesolve_handle h, const float *x, float *y) {
+  if (!h) return 1;
+  if (!x) return 2;
+  if (!y) return 3;
+  if (!h->d_rp) { set_err("no matrix set"); return -1; }
+  HIPCHK(hipSetDevice(h->device));
+  if (int e = ensure_vectors(h)) return e;
+  HIPCHK(hipMemcpyAsync(h->v32[0], x, sizeof(float) * h->n, hipMemcpyHostToDevice, h->stream));
+  launch_spmv<float, float, float>(h, h->lpr, h->d_v32, h->v32[0], h->v32[1], nullptr, nullptr, nullptr, nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(y, h->v32[1], sizeof(float) * h->n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int shesolve_precond_apply(shesolve_handle h, const double *rr, double *z) {
+  if (!h) return 1;
+  if (!rr) return 2;
+  if (!z) return 3;
+  if (!h->d_rp) { set_err("no matrix set"); return -1; }
+  HIPCHK(hipSetDevice(h->device));
+  if (int e = ensure_vectors(h)) return e;
+  HIPCHK(hipMemcpyAsync(h->v64[0], rr, sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
+  launch_precond<double, double>(h, h->d_f64, h->v64[0], h->v64[1]);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(z, h->v64[1], sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int64_t shesolve_factor_nnz(shesolve_handle h) { return h ? h->fnnz : -1; }
+
+int shesolve_get_factors(shesolve_handle h, int64_t *row_ptr, int32_t *col, double *val) {
+  if (!h) return 1;
+  if (!row_ptr) return 2;
+  if (!col) return 3;
+  if (!val) return 4;
+  if (!h->d_f64) { set_err("no factorisation"); return -1; }
+  HIPCHK(hipSetDevice(h->device));
+  for (int i = 0; i <= h->n; ++i) row_ptr[i] = h->h_frp[i];
+  std::memcpy(col, h->h_fci.data(), sizeof(int32_t) * h->fnnz);
+  HIPCHK(hipMemcpyAsync(val, h->d_f64, sizeof(double) * h->fnnz, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int shesolve_levels(shesolve_handle h, int64_t *out2) {
+  if (!h) return 1;
+  if (!out2) return 2;
+  out2[0] = h->lvL.count();
+  out2[1] = h->lvU.count();
+  return 0;
+}
+
+int shesolve_set_profiling(shesolve_handle h, int on) {
+  if (!h) return 1;
+  h->prof = on != 0;
+  for (int k = 0; k < 4; ++k) { h->ktime[k] = 0; h->kcount[k] = 0; }
+  return 0;
+}
+
+int shesolve_kernel_stats(shesolve_handle h, int kind, double *avg_us, int64_t *launches) {
+  if (!h) return 1;
+  if (kind < 0 || kind > 3) return 2;
+  if (avg_us) *avg_us = h->kcount[kind] ? h->ktime[kind] / h->kcount[kind] : 0.0;
+  if (launches) *launches = h->kcount[kind];
+  return 0;
+}
+
+int shesolve_time_spmv(shesolve_handle h, int fp32, int reps, double *avg_us) {
+  if (!h) return 1;
+  if (reps <= 0) return 3;
+  if (!avg_us) return 4;
+  if (!h->d_rp) { set_err("no matrix set"); return -1; }
+  HIPCHK(hipSetDevice(h->device));
+  if (int e = ensure_vectors(h)) return e;
+  bool was = h->prof;
+  h->prof = false;
+  if (fp32) {
+    hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, h->stream, (int64_t)h->n, h->v32[0], 1.0f);
+    launch_spmv<float, float, float>(h, h->lpr, h->d_v32, h->v32[0], h->v32[1], nullptr, nullptr, nullptr, nullptr);
+  } else {
+    hipLaunchKernelGGL((k_fill<double>), dim3(kGrid), dim3(kBlock), 0, h->stream, (int64_t)h->n, h->v64[0], 1.0);
+    launch_spmv<double, double, double>(h, h->lpr, h->d_v64, h->v64[0], h->v64[1], nullptr, nullptr, nullptr,
+                                        nullptr);
+  }
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  for (int r = 0; r < reps; ++r) {
+    if (fp32)
+      launch_spmv<float, float, float>(h, h->lpr, h->d_v32, h->v32[0], h->v32[1], nullptr, nullptr, nullptr, nullptr);
+    else
+      launch_spmv<double, double, double>(h, h->lpr, h->d_v64, h->v64[0], h->v64[1], nullptr, nullptr, nullptr,
+                                          nullptr);
+  }
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipEventSynchronize(h->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  *avg_us = ms * 1e3 / reps;
+  h->prof = was;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,212 @@
+"""Python front end of libshesolve.so (the MI355X SHE linear solver).
+
+The library is the product; this module only binds its C ABI
+(include/shesolve.h) with ctypes for tests, benchmarks and scripting.  It
+never falls back to a CPU path: if the library is missing it raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBPATH = os.path.join(PKG, "libshesolve.so")
+
+PRECOND_NONE, PRECOND_ILU0, PRECOND_BLOCK_ILU0 = 0, 1, 3
+FP64, FP32_IR = 0, 1
+
+# symbols declared in include/shesolve.h
+EXPORTS = (
+    "shesolve_default_params", "shesolve_last_error", "shesolve_create", "shesolve_destroy",
+    "shesolve_set_params", "shesolve_set_matrix", "shesolve_set_matrix_device", "shesolve_update_values",
+    "shesolve_update_values_device", "shesolve_solve", "shesolve_solve_device", "shesolve_spmv",
+    "shesolve_spmv_f32", "shesolve_precond_apply", "shesolve_factor_nnz", "shesolve_get_factors",
+    "shesolve_levels", "shesolve_set_profiling", "shesolve_kernel_stats", "shesolve_time_spmv",
+)
+
+
+class Params(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iters", C.c_int), ("precond", C.c_int), ("precision", C.c_int),
+                ("inner_tol", C.c_double), ("max_ir_steps", C.c_int), ("lanes_per_row", C.c_int)]
+
+
+class Report(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("ir_steps", C.c_int), ("rel_residual", C.c_double),
+                ("setup_ms", C.c_double), ("solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SheSolveError(RuntimeError):
+    pass
+
+
+_lib = None
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Loads libshesolve.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise SheSolveError(f"{LIBPATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(LIBPATH)
+    L.shesolve_default_params.argtypes = [C.POINTER(Params)]
+    L.shesolve_last_error.restype = C.c_char_p
+    L.shesolve_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(Params)]
+    L.shesolve_destroy.argtypes = [C.c_void_p]
+    L.shesolve_set_params.argtypes = [C.c_void_p, C.POINTER(Params)]
+    L.shesolve_set_matrix.argtypes = [C.c_void_p, C.c_int64, C.c_int64, _i64p, _i32p, _f64p, C.c_int, C.c_void_p]
+    L.shesolve_set_matrix_device.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_int, C.c_void_p]
+    L.shesolve_update_values.argtypes = [C.c_void_p, _f64p]
+    L.shesolve_update_values_device.argtypes = [C.c_void_p, C.c_void_p]
+    L.shesolve_solve.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int, C.POINTER(Report)]
+    L.shesolve_solve_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(Report)]
+    L.shesolve_spmv.argtypes = [C.c_void_p, _f64p, _f64p]
+    L.shesolve_spmv_f32.argtypes = [C.c_void_p, _f32p, _f32p]
+    L.shesolve_precond_apply.argtypes = [C.c_void_p, _f64p, _f64p]
+    L.shesolve_factor_nnz.restype = C.c_int64
+    L.shesolve_factor_nnz.argtypes = [C.c_void_p]
+    L.shesolve_get_factors.argtypes = [C.c_void_p, _i64p, _i32p, _f64p]
+    L.shesolve_levels.argtypes = [C.c_void_p, _i64p]
+    L.shesolve_set_profiling.argtypes = [C.c_void_p, C.c_int]
+    L.shesolve_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.shesolve_time_spmv.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    for name in EXPORTS:
+        if name not in ("shesolve_last_error", "shesolve_factor_nnz", "shesolve_default_params"):
+            getattr(L, name).restype = C.c_int
+    L.shesolve_default_params.restype = None
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise SheSolveError(f"{what} failed ({rc}): {lib().shesolve_last_error().decode()}")
+
+
+def default_params(**kw):
+    p = Params()
+    lib().shesolve_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Solver:
+    """One solver handle on one HIP device."""
+
+    def __init__(self, device=0, **params):
+        self.params = default_params(**params)
+        h = C.c_void_p()
+        _check(lib().shesolve_create(C.byref(h), device, C.byref(self.params)), "shesolve_create")
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().shesolve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        _check(lib().shesolve_set_params(self._h, C.byref(self.params)), "shesolve_set_params")
+
+    def set_matrix(self, row_ptr, col, val, block_ranges=None):
+        row_ptr = np.ascontiguousarray(row_ptr, np.int64)
+        col = np.ascontiguousarray(col, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        n = len(row_ptr) - 1
+        rr = None
+        nb = 0
+        if block_ranges is not None:
+            rr = np.ascontiguousarray(np.asarray(block_ranges, np.int64).reshape(-1))
+            nb = len(rr) // 2
+        _check(lib().shesolve_set_matrix(self._h, n, int(row_ptr[-1]), row_ptr, col, val, nb,
+                                         rr.ctypes.data if rr is not None else None), "shesolve_set_matrix")
+        self.n = n
+
+    def set_matrix_device(self, n, nnz, d_row_ptr, d_col, d_val, block_ranges=None):
+        rr = None
+        nb = 0
+        if block_ranges is not None:
+            rr = np.ascontiguousarray(np.asarray(block_ranges, np.int64).reshape(-1))
+            nb = len(rr) // 2
+        _check(lib().shesolve_set_matrix_device(self._h, n, nnz, d_row_ptr, d_col, d_val, nb,
+                                                rr.ctypes.data if rr is not None else None),
+               "shesolve_set_matrix_device")
+        self.n = n
+
+    def update_values(self, val):
+        _check(lib().shesolve_update_values(self._h, np.ascontiguousarray(val, np.float64)), "shesolve_update_values")
+
+    def update_values_device(self, d_val):
+        _check(lib().shesolve_update_values_device(self._h, d_val), "shesolve_update_values_device")
+
+    def solve(self, b, x0=None):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.n) if x0 is None else np.array(x0, np.float64, copy=True)
+        rep = Report()
+        _check(lib().shesolve_solve(self._h, b, x, 0 if x0 is None else 1, C.byref(rep)), "shesolve_solve")
+        return x, rep.as_dict()
+
+    def solve_device(self, d_b, d_x, warm_start=False):
+        rep = Report()
+        _check(lib().shesolve_solve_device(self._h, d_b, d_x, 1 if warm_start else 0, C.byref(rep)),
+               "shesolve_solve_device")
+        return rep.as_dict()
+
+    def spmv(self, x):
+        y = np.zeros(self.n)
+        _check(lib().shesolve_spmv(self._h, np.ascontiguousarray(x, np.float64), y), "shesolve_spmv")
+        return y
+
+    def spmv_f32(self, x):
+        y = np.zeros(self.n, np.float32)
+        _check(lib().shesolve_spmv_f32(self._h, np.ascontiguousarray(x, np.float32), y), "shesolve_spmv_f32")
+        return y
+
+    def precond_apply(self, r):
+        z = np.zeros(self.n)
+        _check(lib().shesolve_precond_apply(self._h, np.ascontiguousarray(r, np.float64), z), "shesolve_precond_apply")
+        return z
+
+    def factors(self):
+        nnz = int(lib().shesolve_factor_nnz(self._h))
+        rp = np.zeros(self.n + 1, np.int64)
+        ci = np.zeros(max(nnz, 1), np.int32)
+        v = np.zeros(max(nnz, 1))
+        _check(lib().shesolve_get_factors(self._h, rp, ci, v), "shesolve_get_factors")
+        return rp, ci[:nnz], v[:nnz]
+
+    def levels(self):
+        o = np.zeros(2, np.int64)
+        _check(lib().shesolve_levels(self._h, o), "shesolve_levels")
+        return int(o[0]), int(o[1])
+
+    def set_profiling(self, on=True):
+        _check(lib().shesolve_set_profiling(self._h, 1 if on else 0), "shesolve_set_profiling")
+
+    def kernel_stats(self, kind):
+        a = C.c_double()
+        n = C.c_int64()
+        _check(lib().shesolve_kernel_stats(self._h, kind, C.byref(a), C.byref(n)), "shesolve_kernel_stats")
+        return a.value, n.value
+
+    def time_spmv(self, fp32=False, reps=20):
+        a = C.c_double()
+        _check(lib().shesolve_time_spmv(self._h, 1 if fp32 else 0, reps, C.byref(a)), "shesolve_time_spmv")
+        return a.value

@@ -75,8 +75,9 @@ def built_in_potential(T, dn, dp):
     net = dn - dp
     naux, paux = _aux(T, 0), _aux(T, 1)
     x = np.sqrt(net * net + 4.0 * naux * paux)
-    pos = T * np.log((x + net) * 0.5 / naux) * KB / Q
-    neg = -T * np.log((x - net) * 0.5 / paux) * KB / Q
+    with np.errstate(divide="ignore", invalid="ignore"):
+        pos = T * np.log((x + net) * 0.5 / naux) * KB / Q
+        neg = -T * np.log((x - net) * 0.5 / paux) * KB / Q
     out = np.where(net >= 0.0, pos, neg)
     return np.where((dn <= 0) & (dp <= 0), 0.0, out)
 
@@ -125,6 +126,26 @@ def mosfet_2d(width_nm=200.0, height_nm=100.0, vx=161, vy=81, pocket_w_nm=50.0,
     contact[0, :] = 1
     return Device(2, nx, ny, xv, yv, dn, dp, psi, contact.ravel().astype(np.int32), T,
                   name=f"mosfet2d_{vx}x{vy}")
+
+
+def nin_2d(width_nm=100.0, height_nm=40.0, vx=41, vy=17, nplus_frac=0.2, n_plus=1e24, n_low=1e21,
+           bias=0.3, T=300.0):
+    """Two-dimensional n+ n n+ structure (a diode extruded in y); contacts are the
+    left and right cell columns (0 V and ``bias``)."""
+    W, Hh = width_nm * 1e-9, height_nm * 1e-9
+    xv = _vertices(0.0, W, vx)
+    yv = _vertices(0.0, Hh, vy)
+    nx, ny = vx - 1, vy - 1
+    xc = 0.5 * (xv[:-1] + xv[1:])
+    yc = 0.5 * (yv[:-1] + yv[1:])
+    X, Y = np.meshgrid(xc, yc)
+    dn = np.where((X < nplus_frac * W) | (X > (1.0 - nplus_frac) * W), n_plus, n_low).ravel()
+    dp = np.zeros(nx * ny)
+    psi = built_in_potential(T, dn, dp) + bias * ((X - xc[0]) / (xc[-1] - xc[0])).ravel()
+    contact = np.zeros((ny, nx), dtype=np.int32)
+    contact[:, 0] = 1
+    contact[:, nx - 1] = 1
+    return Device(2, nx, ny, xv, yv, dn, dp, psi, contact.ravel().astype(np.int32), T, name=f"nin2d_{vx}x{vy}")
 
 
 def cfg1():
