@@ -1,0 +1,92 @@
+/*
+ * shefill.h -- C ABI of the MI355X SHE coefficient fill (libshesolve.so).
+ *
+ * Builds, for a structured 1D line or 2D quadrilateral device, the system that
+ * the reference hands to its linear solver:
+ *
+ *   she::assemble(...)  viennashe/she/assemble_all.hpp:46-424 (streaming,
+ *                       scattering, contact boundary terms)
+ *   she::solve(...)     viennashe/she/linear_solver.hpp:96-189 up to the
+ *                       solvers::solve call: odd elimination (she/eliminate.hpp:80-293),
+ *                       unknown scaling (she/rescaling.hpp:47-107), row
+ *                       normalisation (math/linalg_util.hpp:200-282) and the
+ *                       per-energy block ranges (she/linear_solver.hpp:42-83)
+ *
+ * with the energy grid, expansion orders, boundary values and unknown mapping
+ * of she/setup_energies.hpp:61-153, she/expansion_order.hpp:60-364,
+ * she/boundary_conditions.hpp:57-136 and mapping.hpp:97-329.
+ *
+ * The host computes the integer structure (energies, orders, mapping,
+ * sparsity); the device computes every value in FP64 and keeps the reduced
+ * CSR system resident, ready for shesolve_set_matrix_device.  Error
+ * conventions as in shesolve.h (0 ok, >0 bad argument position, <0 internal).
+ */
+#ifndef SHEFILL_H
+#define SHEFILL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct shefill_ctx *shefill_handle;
+
+typedef struct {
+  int dim;                 /* 1 or 2 */
+  int nx, ny;              /* cells per direction (ny = 1 in 1D); cell id = j*nx + i */
+  const double *xv;        /* nx+1 vertex x coordinates [m] */
+  const double *yv;        /* ny+1 vertex y coordinates [m] (2D; ignored in 1D) */
+  const double *doping_n;  /* donor doping per cell [m^-3] */
+  const double *doping_p;  /* acceptor doping per cell [m^-3] */
+  const double *potential; /* electrostatic potential per cell [V] */
+  const int32_t *contact;  /* 1 where the cell carries a contact potential */
+  double temperature;      /* lattice temperature [K] */
+} shefill_device;
+
+typedef struct {
+  int carrier;             /* 0 electrons, 1 holes */
+  int L_max;               /* expansion order: 1, 3, 5 or 7 */
+  double energy_spacing;   /* [J] */
+  double min_kin_range;    /* [J] */
+  double max_kin_range;    /* [J] */
+  double tau;              /* generation/recombination contact parameter [s] */
+  int acoustic;            /* acoustic phonon scattering on/off */
+  int optical;             /* optical phonon scattering on/off */
+  int scale;               /* kinetic-energy unknown scaling on/off */
+} shefill_config;
+
+const char *shefill_last_error(void);
+int shefill_create(shefill_handle *h, int device);
+int shefill_destroy(shefill_handle h);
+
+/* Builds structure and values.  Re-running with a new potential on the same
+ * mesh reuses buffers; the structure is rebuilt from the new energies. */
+int shefill_build(shefill_handle h, const shefill_device *dev, const shefill_config *cfg);
+
+/* sizes: [0] N_H, [1] n (reduced unknowns), [2] nnz, [3] n_odd (eliminated),
+ *        [4] nodes (cell, H) carrying unknowns, [5] cells, [6] interior facets */
+int shefill_sizes(shefill_handle h, int64_t *out7);
+
+/* Device pointers of the resident reduced system (owned by the handle, valid
+ * until the next build or destroy): row_ptr (int64, n+1), col (int32), val
+ * (FP64), rhs (FP64). */
+int shefill_device_system(shefill_handle h, const int64_t **d_row_ptr, const int32_t **d_col,
+                          const double **d_val, const double **d_rhs);
+/* Host copies of the reduced system, the unknown scaling exp(-E_kin/kT) and
+ * the column factors of the normalisation. */
+int shefill_get_system(shefill_handle h, int64_t *row_ptr, int32_t *col, double *val, double *rhs,
+                       double *unknown_scaling, double *col_factors);
+/* Block ranges (2*N_H values), as she::fill_block_indices writes them. */
+int shefill_block_ranges(shefill_handle h, int64_t *ranges);
+/* Even mapping: first unknown index and expansion order per (cell, H),
+ * cell-major [cells * N_H]; -1 / 0 where a node carries no unknown. */
+int shefill_mapping(shefill_handle h, int64_t *even_idx, int32_t *even_order);
+
+/* Device milliseconds of the last value fill (scalars + rows + normalisation). */
+int shefill_last_fill_ms(shefill_handle h, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

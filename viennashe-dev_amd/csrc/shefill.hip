@@ -1,0 +1,1102 @@
+// shefill.hip -- SHE coefficient fill on MI355X behind include/shefill.h.
+//
+// Host: structured mesh, energy grid, expansion orders, unknown mapping and the
+// sparsity of the reduced (odd-eliminated) system.  The pattern is value
+// independent on structured meshes: the drift coupling b_dir is nonzero only
+// where the diffusion coupling a_dir is (checked for L = 1..7 in both
+// directions), so every entry the reference creates is an a-coupling product.
+// Device: all FP64 values -- scalar coefficients per (cell, H) and (facet, H),
+// the eliminated rows -S^eo (S^oo)^-1 S^oe per row, unknown scaling and the
+// column/row normalisation -- written straight into the resident CSR.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "coupling_tables.inc"
+#include "she_physics.h"
+#include "shefill.h"
+
+using namespace shephys;
+
+namespace {
+
+thread_local std::string g_ferr;
+void ferr(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_ferr = buf;
+}
+#define FHIP(expr)                                                                    \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      ferr("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return -10;                                                                     \
+    }                                                                                 \
+  } while (0)
+
+constexpr int kB = 256;
+// facet slots of a cell (creation order of a quadrilateral's edges) and the
+// direction from the cell to the neighbour across each: +x 0, -x 1, +y 2, -y 3
+constexpr int kSlotDir[4] = {3, 1, 0, 2};  // bottom, left, right, top
+__host__ __device__ inline int rev_dir(int d) { return d ^ 1; }
+
+int even_count(int L) { int n = 0; for (int l = 0; l <= L; l += 2) n += 2 * l + 1; return n; }
+int odd_count(int L) { int n = 0; for (int l = 1; l <= L; l += 2) n += 2 * l + 1; return n; }
+std::vector<int> harmonics(int L, bool odd) {
+  std::vector<int> h;
+  for (int l = odd ? 1 : 0; l <= L; l += 2)
+    for (int m = -l; m <= l; ++m) h.push_back(l * l + l + m);
+  return h;
+}
+
+// --- coupling matrices with the reference's std::map semantics -----------------
+struct CMat {
+  int K = 0;
+  std::vector<char> p;
+  std::vector<double> v;
+  explicit CMat(int k = 0) : K(k), p((size_t)k * k, 0), v((size_t)k * k, 0.0) {}
+};
+CMat transpose(const CMat &m) {
+  CMat t(m.K);
+  for (int i = 0; i < m.K; ++i)
+    for (int j = 0; j < m.K; ++j)
+      if (m.p[i * m.K + j]) { t.p[j * m.K + i] = 1; t.v[j * m.K + i] = m.v[i * m.K + j]; }
+  return t;
+}
+// sparse_matrix::operator*(factor) (math/linalg_util.hpp:132-143) scales the
+// transposed position of every stored entry, inserting it when absent.
+CMat scaled(const CMat &m, double f) {
+  CMat r = m;
+  for (int i = 0; i < m.K; ++i)
+    for (int j = 0; j < m.K; ++j)
+      if (m.p[i * m.K + j]) { r.p[j * m.K + i] = 1; r.v[j * m.K + i] *= f; }
+  return r;
+}
+void add_into(CMat &d, const CMat &s) {
+  for (size_t i = 0; i < d.p.size(); ++i)
+    if (s.p[i]) { d.p[i] = 1; d.v[i] += s.v[i]; }
+}
+// coupling_matrix_in_direction (she/assemble_common.hpp:176-290) for the unit
+// direction of code d; 1D uses the x matrix only.
+CMat in_direction(const CMat *m, int d, int dim, int ct) {
+  const double px = d == 0 ? 1.0 : (d == 1 ? -1.0 : 0.0), py = d == 2 ? 1.0 : (d == 3 ? -1.0 : 0.0);
+  const double s = sqrt(m_dos(ct) / m_trans(ct));
+  CMat r = scaled(m[0], px * s);
+  if (dim > 1) add_into(r, scaled(m[1], py * s));
+  return r;
+}
+
+// --- device kernels -----------------------------------------------------------
+struct DevTables {
+  int K, NH, carrier, dim;
+  double T, kT, hw, rate_ac, rate_op, Nop, tau, spacing;
+  const double *Hval, *shc, *shf, *cvol, *fvol, *flen, *dn, *dp;
+  const int *contact, *fc1, *fc2;
+  const double *Ae, *Be, *Ao, *Bo;  // [4][K][K]
+  const int *ev_h, *od_h;           // harmonic index lists
+  const int *ev_off, *od_off;       // offsets per L in the lists (L+1 entries)
+};
+
+__device__ inline double kin_cell(const DevTables &t, int c, int H) {
+  return t.carrier == 0 ? t.Hval[H] - t.shc[c] : t.shc[c] - t.Hval[H];
+}
+__device__ inline double kin_facet(const DevTables &t, int f, int H) {
+  return t.carrier == 0 ? t.Hval[H] - t.shf[f] : t.shf[f] - t.Hval[H];
+}
+__device__ inline void kin3(const DevTables &t, bool facet, int e, int H, double &em, double &e0, double &ep,
+                            bool &hm, bool &hp) {
+  hm = H > 0;
+  hp = H < t.NH - 1;
+  e0 = facet ? kin_facet(t, e, H) : kin_cell(t, e, H);
+  em = hm ? (facet ? kin_facet(t, e, H - 1) : kin_cell(t, e, H - 1)) : 0.0;
+  ep = hp ? (facet ? kin_facet(t, e, H + 1) : kin_cell(t, e, H + 1)) : 0.0;
+}
+__device__ inline double avgZ(const DevTables &t, bool facet, int e, int H) {
+  double em, e0, ep;
+  bool hm, hp;
+  kin3(t, facet, e, H, em, e0, ep, hm, hp);
+  return avg_dos3(t.carrier, em, e0, ep, hm, hp);
+}
+__device__ inline double bheight(const DevTables &t, bool facet, int e, int H) {
+  double em, e0, ep;
+  bool hm, hp;
+  kin3(t, facet, e, H, em, e0, ep, hm, hp);
+  return box_height3(em, e0, ep, hm, hp);
+}
+
+// contact normalisation (she/boundary_conditions.hpp:74-101) and contact density
+__global__ void k_bnd_corr(DevTables t, int ncells, double *corr, double *ncont) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  if (!t.contact[c]) { corr[c] = 1.0; ncont[c] = 0.0; return; }
+  double s = 0;
+  for (int H = 1; H < t.NH - 1; ++H) {
+    double mid = kin_cell(t, c, H);
+    double lo = fmax((kin_cell(t, c, H - 1) + mid) / 2.0, 0.0);
+    double up = (kin_cell(t, c, H + 1) + mid) / 2.0;
+    if (lo < up && up >= 0) s += exp(-mid / t.kT) * avgZ(t, false, c, H) * bheight(t, false, c, H);
+  }
+  corr[c] = s * symmetry_factor(t.carrier);
+  ncont[c] = contact_density(t.T, t.dn[c], t.dp[c], t.carrier);
+}
+
+// Per facet node (f, H): diffusion coefficient, drift coefficient seen from each
+// cell, odd diagonal (out-scattering; no elastic round-off skip for odd rows).
+__global__ void k_fnode(DevTables t, int nfn, const int *fn_f, const int *fn_H, const int *fn_out,
+                        double *cdiff, double *cdrift, double *odiag) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nfn) return;
+  const int f = fn_f[k], H = fn_H[k], c1 = t.fc1[f], c2 = t.fc2[f];
+  double em, e0, ep;
+  bool hm, hp;
+  kin3(t, true, f, H, em, e0, ep, hm, hp);
+  const double area = t.fvol[f], len = t.flen[f];
+  cdiff[k] = integral_vZ3(t.carrier, em, e0, ep, hm, hp) * area;
+  // lower/upper kinetic energies of the facet box (assemble_common.hpp:323-388), interior H
+  const double lo = (kin_cell(t, c1, H - 1) + kin_cell(t, c2, H - 1) + kin_cell(t, c1, H) + kin_cell(t, c2, H)) / 4.0;
+  const double up = (kin_cell(t, c1, H + 1) + kin_cell(t, c2, H + 1) + kin_cell(t, c1, H) + kin_cell(t, c2, H)) / 4.0;
+  const double izhk = 0.5 * (dos(t.carrier, up) * velocity(t.carrier, up) - dos(t.carrier, lo) * velocity(t.carrier, lo));
+  // force_on_facet_accessor: (E_kin(cell) - E_kin(other)) / distance
+  const double f1 = (kin_cell(t, c1, H) - kin_cell(t, c2, H)) / len;
+  const double f2 = (kin_cell(t, c2, H) - kin_cell(t, c1, H)) / len;
+  cdrift[2 * k] = izhk * f1 * area * len / 2.0;
+  cdrift[2 * k + 1] = izhk * f2 * area * len / 2.0;
+  // out-scattering on the facet box, volume |f| * d / dim
+  const double V = area * (len / (double)t.dim);
+  const double Z0 = avg_dos3(t.carrier, em, e0, ep, hm, hp), h0 = box_height3(em, e0, ep, hm, hp);
+  double d = 0;
+  if (t.rate_ac > 0) d += 2.0 * kPi * t.rate_ac * Z0 * Z0 * V * h0;
+  for (int q = 0; q < 2; ++q) {
+    const int Hf = fn_out[2 * k + q];
+    if (Hf < 0) continue;
+    const double rate = q == 0 ? t.rate_op * (t.Nop + 1.0) : t.rate_op * t.Nop;
+    const double Zf = avgZ(t, true, f, Hf), hf = bheight(t, true, f, Hf);
+    d += 2.0 * kPi * rate * Z0 * Zf * V * fmin(h0, hf);
+  }
+  odiag[k] = d;
+}
+
+// Per cell node (c, H): diagonal scattering/boundary terms, rhs and the two
+// optical in-scattering coefficients (she/assemble_scattering.hpp:116-335,
+// she/assemble_boundary.hpp:69-183).
+__global__ void k_node(DevTables t, int nn, const int *nd_c, const int *nd_H, const int *nd_out, const int *nd_in,
+                       const double *corr, const double *ncont, double *diag0, double *diagr, double *rhs0,
+                       double *inco) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nn) return;
+  const int c = nd_c[k], H = nd_H[k];
+  double em, e0, ep;
+  bool hm, hp;
+  kin3(t, false, c, H, em, e0, ep, hm, hp);
+  const double V = t.cvol[c];
+  const double Z0 = avg_dos3(t.carrier, em, e0, ep, hm, hp), h0 = box_height3(em, e0, ep, hm, hp);
+  double d0 = 0, dr = 0, r0 = 0;
+  if (t.contact[c]) {
+    const double coeff = V * h0 / t.tau * Z0;
+    double bnd = 0.0;
+    if (H >= 1 && H < t.NH - 1) {
+      const double lo = fmax((em + e0) / 2.0, 0.0), up = (ep + e0) / 2.0;
+      if (lo < up && up >= 0) bnd = ncont[c] * exp(-e0 / t.kT) / corr[c];
+    }
+    d0 += coeff;
+    dr += coeff;
+    r0 += bnd * coeff;
+  }
+  // acoustic: elastic, the (0,0) out/in pair is skipped
+  if (t.rate_ac > 0) dr += 2.0 * kPi * t.rate_ac * Z0 * Z0 * V * h0;
+  // optical out (emission to H-k with N+1, absorption to H+k with N)
+  for (int q = 0; q < 2; ++q) {
+    const int Hf = nd_out[2 * k + q];
+    if (Hf < 0) continue;
+    const double rate = q == 0 ? t.rate_op * (t.Nop + 1.0) : t.rate_op * t.Nop;
+    const double Zf = avgZ(t, false, c, Hf), hf = bheight(t, false, c, Hf);
+    const double coeff = 2.0 * kPi * rate * Z0 * Zf * V * fmin(h0, hf);
+    d0 += coeff;
+    dr += coeff;
+  }
+  // optical in: from H-k (absorption, N) and from H+k (emission, N+1)
+  for (int q = 0; q < 2; ++q) {
+    const int Hi = nd_in[2 * k + q];
+    double v = 0;
+    if (Hi >= 0) {
+      const double rate = q == 0 ? t.rate_op * t.Nop : t.rate_op * (t.Nop + 1.0);
+      const double Zi = avgZ(t, false, c, Hi), hi = bheight(t, false, c, Hi);
+      v = -2.0 * kPi * rate * Zi * Z0 * V * fmin(hi, h0);
+    }
+    inco[2 * k + q] = v;
+  }
+  diag0[k] = d0;
+  diagr[k] = dr;
+  rhs0[k] = r0;
+}
+
+struct RowArgs {
+  int n;
+  const int *rp, *ci, *row_node;
+  const int *nd_c, *nd_H, *nd_base, *nd_L, *nd_fn, *nd_nb, *nd_inn;
+  const int *fn_f, *fn_L;
+  const double *cdiff, *cdrift, *odiag, *diag0, *diagr, *rhs0, *inco;
+  double *val, *rhs;
+};
+
+// One thread per row: value of every stored entry.
+__global__ __launch_bounds__(kB) void k_rows(DevTables t, RowArgs a) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= a.n) return;
+  const int nd = a.row_node[row];
+  const int c = a.nd_c[nd], base = a.nd_base[nd], Lc = a.nd_L[nd];
+  const int r = row - base;
+  const int K = t.K;
+  const int hr = t.ev_h[t.ev_off[Lc] + r];
+  // facets with odd unknowns: slot data
+  int fnk[4], nbk[4], dirk[4], side_c[4];
+  for (int s = 0; s < 4; ++s) {
+    fnk[s] = a.nd_fn[4 * nd + s];
+    nbk[s] = a.nd_nb[4 * nd + s];
+    dirk[s] = kSlotDir[s];
+    side_c[s] = 0;
+    if (fnk[s] >= 0) side_c[s] = (t.fc1[a.fn_f[fnk[s]]] == c) ? 0 : 1;
+  }
+  for (int j = a.rp[row]; j < a.rp[row + 1]; ++j) {
+    const int col = a.ci[j];
+    const int cn = a.row_node[col];
+    const int rq = col - a.nd_base[cn];
+    const int hq = t.ev_h[t.ev_off[a.nd_L[cn]] + rq];
+    double v = 0.0;
+    if (cn == nd) {
+      if (rq == r) v += (r == 0) ? a.diag0[nd] : a.diagr[nd];
+    } else if (cn == a.nd_inn[2 * nd]) {
+      v = (r == 0 && rq == 0) ? a.inco[2 * nd] : 0.0;
+    } else if (cn == a.nd_inn[2 * nd + 1]) {
+      v = (r == 0 && rq == 0) ? a.inco[2 * nd + 1] : 0.0;
+    }
+    // elimination through every facet whose far side is this column's node
+    for (int s = 0; s < 4; ++s) {
+      const int fk = fnk[s];
+      if (fk < 0) continue;
+      const bool self = (cn == nd);
+      if (!self && cn != nbk[s]) continue;
+      const int d = dirk[s];
+      const int dd = self ? rev_dir(d) : d;  // direction other -> column cell
+      const int Lf = a.fn_L[fk];
+      const double cd = a.cdiff[fk];
+      const double crow = a.cdrift[2 * fk + side_c[s]];
+      const double ccol = a.cdrift[2 * fk + (self ? side_c[s] : 1 - side_c[s])];
+      const double *Ae = t.Ae + (size_t)d * K * K, *Be = t.Be + (size_t)rev_dir(d) * K * K;
+      const double *Ao = t.Ao + (size_t)dd * K * K, *Bo = t.Bo + (size_t)dd * K * K;
+      double s_acc = 0.0;
+      const int ob = t.od_off[Lf], oe = t.od_off[Lf + 1];
+      for (int o = ob; o < oe; ++o) {
+        const int ho = t.od_h[o];
+        const double seo = cd * Ae[hr * K + ho] + crow * Be[hr * K + ho];
+        if (seo == 0.0) continue;
+        const double soe = cd * Ao[ho * K + hq] + ccol * Bo[ho * K + hq];
+        s_acc += seo * soe;
+      }
+      v -= s_acc / a.odiag[fk];
+    }
+    a.val[j] = v;
+  }
+  a.rhs[row] = (r == 0) ? a.rhs0[nd] : 0.0;
+}
+
+// unknown scaling of the columns + log-magnitude sums per column
+__global__ void k_scale_log(int n, const int *rp, const int *ci, const int *diag_pos, const unsigned char *diag_struct,
+                            const double *scal, double *val, double *logsum) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  for (int j = rp[row]; j < rp[row + 1]; ++j) {
+    const int col = ci[j];
+    double v = val[j] * scal[col];
+    val[j] = v;
+    if (j == diag_pos[row] && !diag_struct[row]) continue;
+    atomicAdd(&logsum[col], log(fabs(v)));
+  }
+}
+__global__ void k_colnorm(int n, const int *colcount, const int *diag_pos, const double *val, double *logsum,
+                          double *colfac) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double cn = exp(logsum[i] / (double)colcount[i]);
+  if (val[diag_pos[i]] < 0.0) cn *= -1.0;
+  logsum[i] = cn;  // reused as the column normaliser
+  colfac[i] = 1.0 / cn;
+}
+__global__ void k_rownorm(int n, const int *rp, const int *ci, const int *diag_pos, const double *cn, double *val,
+                          double *rhs) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  double s = 0.0;
+  for (int j = rp[row]; j < rp[row + 1]; ++j) {
+    double v = val[j] / cn[ci[j]];
+    val[j] = v;
+    s += v * v;
+  }
+  double rn = sqrt(s);
+  if (val[diag_pos[row]] < 0.0) rn *= -1.0;
+  for (int j = rp[row]; j < rp[row + 1]; ++j) val[j] /= rn;
+  rhs[row] /= rn;
+}
+__global__ void k_node_scaling(int n, const int *row_node, const int *nd_c, const int *nd_H, DevTables t, int on,
+                               double *scal) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const int nd = row_node[row];
+  scal[row] = on ? exp(-kin_cell(t, nd_c[nd], nd_H[nd]) / t.kT) : 1.0;
+}
+__global__ void k_rp64(int n, const int *rp, int64_t *rp64) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= n) rp64[i] = rp[i];
+}
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ~DBuf() { if (p) (void)hipFree(p); }
+  int alloc(size_t cnt) {
+    if (cnt <= n && p) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (hipMalloc(&p, sizeof(T) * std::max<size_t>(cnt, 1)) != hipSuccess) {
+      ferr("hipMalloc of %zu bytes failed", sizeof(T) * cnt);
+      return -11;
+    }
+    n = cnt;
+    return 0;
+  }
+};
+
+}  // namespace
+
+struct shefill_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double fill_ms = 0;
+  shefill_config cfg{};
+  int dim = 1, nx = 0, ny = 1, nc = 0, nf = 0, NH = 0, K = 0;
+  double T = 300;
+  std::vector<double> cx, cy, cvol, dn, dp, psi, fvol, flen, Hval, shc, shf;
+  std::vector<int> contact, fc1, fc2, cf, ordc, idxc, ordf, idxf, node_of, fnode_of;
+  // nodes
+  std::vector<int> nd_c, nd_H, nd_base, nd_L, nd_fn, nd_nb, nd_inn, nd_in, nd_out;
+  std::vector<int> fn_f, fn_H, fn_L, fn_out;
+  int64_t n = 0, n_odd = 0, nnz = 0;
+  std::vector<int> rp, ci, row_node, diag_pos, colcount;
+  std::vector<unsigned char> diag_struct;
+  std::vector<int64_t> blocks;
+  // device
+  DBuf<double> dHval, dshc, dshf, dcvol, dfvol, dflen, ddn, ddp, dAe, dBe, dAo, dBo;
+  DBuf<int> dcontact, dfc1, dfc2, dev_h, dod_h, dev_off, dod_off;
+  DBuf<int> dnd_c, dnd_H, dnd_base, dnd_L, dnd_fn, dnd_nb, dnd_inn, dnd_in, dnd_out, dfn_f, dfn_H, dfn_L, dfn_out;
+  DBuf<int> drp, dci, drow_node, ddiag_pos, dcolcount;
+  DBuf<unsigned char> ddiag_struct;
+  DBuf<double> dcorr, dncont, dcdiff, dcdrift, dodiag, ddiag0, ddiagr, drhs0, dinco;
+  DBuf<double> dval, drhs, dscal, dcolfac, dlog;
+  DBuf<int64_t> drp64;
+};
+
+namespace {
+
+template <typename T>
+int upload(shefill_ctx *c, DBuf<T> &b, const std::vector<T> &h) {
+  if (int e = b.alloc(h.size())) return e;
+  if (!h.empty()) FHIP(hipMemcpyAsync(b.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+double kin_c(const shefill_ctx *c, int cell, int H) {
+  return c->cfg.carrier == 0 ? c->Hval[H] - c->shc[cell] : c->shc[cell] - c->Hval[H];
+}
+double kin_f(const shefill_ctx *c, int f, int H) {
+  return c->cfg.carrier == 0 ? c->Hval[H] - c->shf[f] : c->shf[f] - c->Hval[H];
+}
+double avg_dos_host(const shefill_ctx *c, bool facet, int e, int H) {
+  auto k = [&](int h) { return facet ? kin_f(c, e, h) : kin_c(c, e, h); };
+  const bool hm = H > 0, hp = H < c->NH - 1;
+  return avg_dos3(c->cfg.carrier, hm ? k(H - 1) : 0.0, k(H), hp ? k(H + 1) : 0.0, hm, hp);
+}
+// energy_index_H: she/assemble_scattering.hpp:64-105
+long energy_index_host(const shefill_ctx *c, bool facet, int e, long hint, double E) {
+  auto k = [&](long h) { return facet ? kin_f(c, e, (int)h) : kin_c(c, e, (int)h); };
+  long r = hint;
+  if (E < k(r) || E > k(r)) {
+    long inc = 1;
+    double diff = E - k(r), diff0 = diff;
+    if (diff < 0) inc = -1;
+    while (r < (long)(c->NH - 1) && r > 0 && diff * diff0 > 0) {
+      r += inc;
+      diff = E - k(r);
+    }
+    if (diff * diff0 > 0) return -1;
+    double e1 = fabs(diff), e2 = e1;
+    if (r - inc >= 0 && r - inc < (long)c->NH) e2 = fabs(E - k(r - inc));
+    if (e2 < e1) return r - inc;
+  }
+  return r;
+}
+
+int build_geometry(shefill_ctx *c, const shefill_device *d) {
+  c->dim = d->dim;
+  c->nx = d->nx;
+  c->ny = d->dim > 1 ? d->ny : 1;
+  c->nc = c->nx * c->ny;
+  c->T = d->temperature;
+  const int nc = c->nc;
+  c->cx.assign(nc, 0); c->cy.assign(nc, 0); c->cvol.assign(nc, 0);
+  c->dn.assign(d->doping_n, d->doping_n + nc);
+  c->dp.assign(d->doping_p, d->doping_p + nc);
+  c->psi.assign(d->potential, d->potential + nc);
+  c->contact.assign(d->contact, d->contact + nc);
+  for (int j = 0; j < c->ny; ++j)
+    for (int i = 0; i < c->nx; ++i) {
+      const int k = j * c->nx + i;
+      if (c->dim == 1) {
+        c->cx[k] = (d->xv[i] + d->xv[i + 1]) / 2.0;
+        c->cvol[k] = fabs(d->xv[i + 1] - d->xv[i]);
+      } else {
+        c->cx[k] = (d->xv[i] + d->xv[i + 1] + d->xv[i] + d->xv[i + 1]) / 4.0;
+        c->cy[k] = (d->yv[j] + d->yv[j] + d->yv[j + 1] + d->yv[j + 1]) / 4.0;
+        c->cvol[k] = fabs(d->xv[i + 1] - d->xv[i]) * fabs(d->yv[j + 1] - d->yv[j]);
+      }
+    }
+  c->fc1.clear(); c->fc2.clear(); c->fvol.clear(); c->flen.clear();
+  c->cf.assign((size_t)nc * 4, -1);
+  for (int j = 0; j < c->ny; ++j)
+    for (int i = 0; i < c->nx; ++i) {
+      const int k = j * c->nx + i;
+      auto add = [&](int o, double vol, int slot_here, int slot_other) {
+        const int f = (int)c->fc1.size();
+        c->fc1.push_back(k);
+        c->fc2.push_back(o);
+        c->fvol.push_back(vol);
+        const double dx = c->cx[k] - c->cx[o], dy = c->dim > 1 ? c->cy[k] - c->cy[o] : 0.0;
+        c->flen.push_back(sqrt(dx * dx + dy * dy));
+        c->cf[(size_t)k * 4 + slot_here] = f;
+        c->cf[(size_t)o * 4 + slot_other] = f;
+      };
+      if (i + 1 < c->nx) add(k + 1, c->dim == 1 ? 1.0 : fabs(d->yv[j + 1] - d->yv[j]), 2, 1);
+      if (c->dim > 1 && j + 1 < c->ny) add(k + c->nx, fabs(d->xv[i + 1] - d->xv[i]), 3, 0);
+    }
+  c->nf = (int)c->fc1.size();
+  return 0;
+}
+
+int build_energies_orders_mapping(shefill_ctx *c) {
+  const shefill_config &cf = c->cfg;
+  // setup_energies (she/setup_energies.hpp:61-153)
+  double maxp = -1.7976931348623157e308, minp = 1.7976931348623157e308;
+  for (double p : c->psi) {
+    p = p + 0.0;
+    if (p < minp) minp = p;
+    if (p > maxp) maxp = p;
+  }
+  const double range = kQ * (maxp - minp) + 2.0 * cf.energy_spacing + cf.min_kin_range;
+  const double edge = cf.carrier == 0 ? -kQ * maxp - cf.energy_spacing + band_gap() / 2.0
+                                      : -kQ * minp + cf.energy_spacing - band_gap() / 2.0;
+  const double inc = (cf.carrier == 0 ? 1.0 : -1.0) * cf.energy_spacing;
+  const size_t NH = (size_t)(range / cf.energy_spacing) + 1;
+  c->NH = (int)NH;
+  c->Hval.resize(NH);
+  for (size_t h = 0; h < NH; ++h) c->Hval[h] = edge + h * inc;
+  c->shc.resize(c->nc);
+  for (int k = 0; k < c->nc; ++k)
+    c->shc[k] = cf.carrier == 0 ? -kQ * c->psi[k] + band_gap() / 2.0 : -kQ * c->psi[k] - band_gap() / 2.0;
+  c->shf.resize(c->nf);
+  for (int f = 0; f < c->nf; ++f) c->shf[f] = (c->shc[c->fc1[f]] + c->shc[c->fc2[f]]) / 2.0;
+  // expansion orders (she/expansion_order.hpp:262-363), non-adaptive
+  const int L = cf.L_max;
+  c->ordc.assign((size_t)c->nc * NH, L);
+  c->ordf.assign((size_t)c->nf * NH, L);
+  for (int k = 0; k < c->nc; ++k) {
+    for (int H = 1; H < (int)NH - 1; ++H) {
+      if (kin_c(c, k, H) <= 0) { c->ordc[(size_t)k * NH + H] = 1; continue; }
+      bool allpos = true;
+      for (int s = 0; s < 4; ++s) {
+        const int f = c->cf[(size_t)k * 4 + s];
+        if (f < 0) continue;
+        const int o = c->fc1[f] == k ? c->fc2[f] : c->fc1[f];
+        if (kin_c(c, o, H - 1) <= 0) allpos = false;
+      }
+      if (allpos) break;
+      c->ordc[(size_t)k * NH + H] = 1;
+      for (int s = 0; s < 4; ++s) {
+        const int f = c->cf[(size_t)k * 4 + s];
+        if (f < 0) continue;
+        const int o = c->fc1[f] == k ? c->fc2[f] : c->fc1[f];
+        c->ordc[(size_t)o * NH + H] = 1;
+      }
+    }
+  }
+  for (int it = 0; it < 10; ++it) {  // smooth_expansion_order: lowers the first cell of a facet
+    bool changed = false;
+    for (int f = 0; f < c->nf; ++f)
+      for (size_t H = 0; H < NH; ++H) {
+        int &o1 = c->ordc[(size_t)c->fc1[f] * NH + H];
+        const int o2 = c->ordc[(size_t)c->fc2[f] * NH + H];
+        if (o1 > o2 + 2) { changed = true; o1 = o2 + 2; }
+      }
+    if (!changed) break;
+  }
+  // mapping (mapping.hpp:97-329): energy-major, then odd unknowns on facets
+  c->idxc.assign((size_t)c->nc * NH, -1);
+  c->idxf.assign((size_t)c->nf * NH, -1);
+  int64_t u = 0;
+  for (size_t H = 0; H < NH; ++H)
+    for (int k = 0; k < c->nc; ++k) {
+      const size_t q = (size_t)k * NH + H;
+      if (H == 0 || H == NH - 1) { c->idxc[q] = -1; continue; }
+      if (avg_dos_host(c, false, k, (int)H) > 0 && kin_c(c, k, (int)H) < cf.max_kin_range) {
+        c->idxc[q] = (int)u;
+        u += even_count(c->ordc[q]);
+      } else {
+        c->idxc[q] = -1;
+        c->ordc[q] = 0;
+      }
+    }
+  if (u > 0x7fffffff) { ferr("more than 2^31 reduced unknowns"); return -3; }
+  c->n = u;
+  int64_t uo = 0;
+  for (size_t H = 0; H < NH; ++H)
+    for (int f = 0; f < c->nf; ++f) {
+      const size_t q = (size_t)f * NH + H;
+      if (c->idxc[(size_t)c->fc1[f] * NH + H] > -1 && c->idxc[(size_t)c->fc2[f] * NH + H] > -1 &&
+          avg_dos_host(c, true, f, (int)H) > 0) {
+        c->idxf[q] = 1;
+        uo += odd_count(c->ordf[q]);
+      } else {
+        c->idxf[q] = -1;
+        c->ordf[q] = 0;
+      }
+    }
+  c->n_odd = uo;
+  return 0;
+}
+
+// Pattern of one eliminated block: P[r] = sorted r' with some odd o linking them.
+using Pattern = std::vector<std::vector<int>>;
+
+struct Builder {
+  shefill_ctx *c;
+  std::vector<CMat> Ae, Be, Ao, Bo;  // [4]
+  std::vector<std::vector<int>> ev, od;
+  std::map<std::tuple<int, int, int, int, int>, Pattern> cache;
+
+  explicit Builder(shefill_ctx *cc) : c(cc) {}
+
+  int tables() {
+    const int L = c->cfg.L_max;
+    const int K = (L + 1) * (L + 1);
+    c->K = K;
+    CMat a[3] = {CMat(K), CMat(K), CMat(K)}, b[3] = {CMat(K), CMat(K), CMat(K)};
+    bool found = false;
+    for (const auto &t : she_tables::tables) {
+      if (t.L != L) continue;
+      found = true;
+      CMat &m = t.which < 3 ? a[t.which] : b[t.which - 3];
+      for (int q = 0; q < t.n; ++q) {
+        m.p[t.e[q].r * K + t.e[q].c] = 1;
+        m.v[t.e[q].r * K + t.e[q].c] = t.e[q].v;
+      }
+    }
+    if (!found) { ferr("expansion order %d has no precomputed coupling table", L); return 2; }
+    CMat aT[3] = {transpose(a[0]), transpose(a[1]), transpose(a[2])};
+    CMat bT[3] = {transpose(b[0]), transpose(b[1]), transpose(b[2])};
+    Ae.clear(); Be.clear(); Ao.clear(); Bo.clear();
+    for (int d = 0; d < 4; ++d) {
+      Ae.push_back(in_direction(a, d, c->dim, c->cfg.carrier));
+      Be.push_back(in_direction(b, d, c->dim, c->cfg.carrier));
+      Ao.push_back(in_direction(aT, d, c->dim, c->cfg.carrier));
+      Bo.push_back(in_direction(bT, d, c->dim, c->cfg.carrier));
+    }
+    ev.assign(L + 1, {});
+    od.assign(L + 1, {});
+    for (int l = 0; l <= L; ++l) { ev[l] = harmonics(l, false); od[l] = harmonics(l, true); }
+    return 0;
+  }
+
+  const Pattern &elim(int d, int La, int Lf, int Lb, int dd) {
+    auto key = std::make_tuple(d, La, Lf, Lb, dd);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    const int K = c->K;
+    Pattern P(ev[La].size());
+    for (size_t r = 0; r < ev[La].size(); ++r)
+      for (size_t q = 0; q < ev[Lb].size(); ++q) {
+        bool any = false;
+        for (int ho : od[Lf])
+          if (Ae[d].v[ev[La][r] * K + ho] != 0.0 && Ao[dd].v[ho * K + ev[Lb][q]] != 0.0) { any = true; break; }
+        if (any) P[r].push_back((int)q);
+      }
+    return cache.emplace(key, std::move(P)).first->second;
+  }
+};
+
+int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
+  const size_t NH = c->NH;
+  const double hw = optical_phonon_energy(c->cfg.energy_spacing);
+  if (c->cfg.optical && !(hw > 0)) { ferr("optical phonon energy snaps to zero on this grid"); return 2; }
+  // facet nodes
+  c->fnode_of.assign((size_t)c->nf * NH, -1);
+  c->fn_f.clear(); c->fn_H.clear(); c->fn_L.clear(); c->fn_out.clear();
+  for (size_t H = 0; H < NH; ++H)
+    for (int f = 0; f < c->nf; ++f) {
+      const size_t q = (size_t)f * NH + H;
+      if (c->idxf[q] < 0) continue;
+      c->fnode_of[q] = (int)c->fn_f.size();
+      c->fn_f.push_back(f);
+      c->fn_H.push_back((int)H);
+      c->fn_L.push_back(c->ordf[q]);
+      const double E = kin_f(c, f, (int)H);
+      for (int s = 0; s < 2; ++s) {
+        int Hf = -1;
+        if (c->cfg.optical) {
+          long h = energy_index_host(c, true, f, (long)H, s == 0 ? E - hw : E + hw);
+          if (h >= 0 && c->idxf[(size_t)f * NH + h] >= 0) Hf = (int)h;
+        }
+        c->fn_out.push_back(Hf);
+      }
+    }
+  // cell nodes in unknown order
+  c->node_of.assign((size_t)c->nc * NH, -1);
+  std::vector<int> order;
+  for (size_t H = 0; H < NH; ++H)
+    for (int k = 0; k < c->nc; ++k)
+      if (c->idxc[(size_t)k * NH + H] >= 0) {
+        c->node_of[(size_t)k * NH + H] = (int)order.size();
+        order.push_back(k * (int)NH + (int)H);
+      }
+  const int nn = (int)order.size();
+  c->nd_c.resize(nn); c->nd_H.resize(nn); c->nd_base.resize(nn); c->nd_L.resize(nn);
+  c->nd_fn.assign((size_t)nn * 4, -1); c->nd_nb.assign((size_t)nn * 4, -1);
+  c->nd_inn.assign((size_t)nn * 2, -1); c->nd_in.assign((size_t)nn * 2, -1); c->nd_out.assign((size_t)nn * 2, -1);
+  for (int k = 0; k < nn; ++k) {
+    const int cell = order[k] / (int)NH, H = order[k] % (int)NH;
+    const size_t q = (size_t)cell * NH + H;
+    c->nd_c[k] = cell;
+    c->nd_H[k] = H;
+    c->nd_base[k] = c->idxc[q];
+    c->nd_L[k] = c->ordc[q];
+    for (int s = 0; s < 4; ++s) {
+      const int f = c->cf[(size_t)cell * 4 + s];
+      if (f < 0) continue;
+      const int fn = c->fnode_of[(size_t)f * NH + H];
+      if (fn < 0) continue;
+      const int o = c->fc1[f] == cell ? c->fc2[f] : c->fc1[f];
+      c->nd_fn[(size_t)k * 4 + s] = fn;
+      c->nd_nb[(size_t)k * 4 + s] = c->node_of[(size_t)o * NH + H];
+    }
+    if (c->cfg.optical) {
+      const double E = kin_c(c, cell, H);
+      // out: to E - hw (slot 0) and E + hw (slot 1); in: from E - hw (slot 0) and E + hw (slot 1)
+      for (int s = 0; s < 2; ++s) {
+        long h = energy_index_host(c, false, cell, H, s == 0 ? E - hw : E + hw);
+        if (h >= 0 && c->idxc[(size_t)cell * NH + h] >= 0) {
+          c->nd_out[2 * k + s] = (int)h;
+          c->nd_in[2 * k + s] = (int)h;
+        }
+      }
+    }
+  }
+  // in-scattering partner nodes (need node ids of other energies: second pass)
+  for (int k = 0; k < nn; ++k)
+    for (int s = 0; s < 2; ++s)
+      if (c->nd_in[2 * k + s] >= 0) c->nd_inn[2 * k + s] = c->node_of[(size_t)c->nd_c[k] * NH + c->nd_in[2 * k + s]];
+  // rows: pattern per node
+  const int64_t n = c->n;
+  c->rp.assign(n + 1, 0);
+  c->row_node.assign(n, 0);
+  c->diag_pos.assign(n, 0);
+  c->diag_struct.assign(n, 0);
+  // row lists of one node, computed twice (count, fill) in parallel over nodes
+  auto node_rows = [&](int k, std::vector<std::vector<int>> &cols, std::vector<char> &dstruct) {
+    const int cell = c->nd_c[k], Lc = c->nd_L[k];
+    const int e = (int)B.ev[Lc].size();
+    cols.assign(e, {});
+    dstruct.assign(e, 0);
+    // blocks in ascending column order: lower partner, bottom, left, self, right, top, upper partner
+    struct Blk { int base; int kind; int slot; };
+    Blk blks[7];
+    int nb = 0;
+    if (c->nd_inn[2 * k] >= 0) blks[nb++] = {c->nd_base[c->nd_inn[2 * k]], 1, 0};
+    for (int s = 0; s < 2; ++s)
+      if (c->nd_fn[(size_t)k * 4 + s] >= 0) blks[nb++] = {c->nd_base[c->nd_nb[(size_t)k * 4 + s]], 2, s};
+    blks[nb++] = {c->nd_base[k], 0, -1};
+    for (int s = 2; s < 4; ++s)
+      if (c->nd_fn[(size_t)k * 4 + s] >= 0) blks[nb++] = {c->nd_base[c->nd_nb[(size_t)k * 4 + s]], 2, s};
+    if (c->nd_inn[2 * k + 1] >= 0) blks[nb++] = {c->nd_base[c->nd_inn[2 * k + 1]], 1, 1};
+    // self block pattern
+    std::vector<std::vector<char>> selfm(e, std::vector<char>(e, 0));
+    const bool optical_out = c->nd_out[2 * k] >= 0 || c->nd_out[2 * k + 1] >= 0;
+    for (int r = 0; r < e; ++r) {
+      if (c->contact[cell] || optical_out) selfm[r][r] = 1;
+      if (c->cfg.acoustic && r > 0) selfm[r][r] = 1;
+    }
+    for (int s = 0; s < 4; ++s) {
+      const int fn = c->nd_fn[(size_t)k * 4 + s];
+      if (fn < 0) continue;
+      const int d = kSlotDir[s];
+      const Pattern &P = B.elim(d, Lc, c->fn_L[fn], Lc, rev_dir(d));
+      for (int r = 0; r < e; ++r)
+        for (int q : P[r]) selfm[r][q] = 1;
+    }
+    for (int bi = 0; bi < nb; ++bi) {
+      const Blk &bk = blks[bi];
+      if (bk.kind == 1) {
+        cols[0].push_back(bk.base);
+      } else if (bk.kind == 0) {
+        for (int r = 0; r < e; ++r) {
+          dstruct[r] = selfm[r][r];
+          for (int q = 0; q < e; ++q)
+            if (selfm[r][q] || q == r) cols[r].push_back(bk.base + q);
+        }
+      } else {
+        const int s = bk.slot;
+        const int fn = c->nd_fn[(size_t)k * 4 + s], nbn = c->nd_nb[(size_t)k * 4 + s];
+        const int d = kSlotDir[s];
+        const Pattern &P = B.elim(d, Lc, c->fn_L[fn], c->nd_L[nbn], d);
+        for (int r = 0; r < e; ++r)
+          for (int q : P[r]) cols[r].push_back(bk.base + q);
+      }
+    }
+  };
+  // pattern cache is filled serially first (all key combinations of this device)
+  {
+    std::vector<std::vector<int>> cols;
+    std::vector<char> ds;
+    for (int k = 0; k < nn; ++k) {
+      bool needed = false;
+      for (int s = 0; s < 4 && !needed; ++s) {
+        const int fn = c->nd_fn[(size_t)k * 4 + s];
+        if (fn < 0) continue;
+        const int d = kSlotDir[s];
+        needed = !B.cache.count(std::make_tuple(d, c->nd_L[k], c->fn_L[fn], c->nd_L[k], rev_dir(d))) ||
+                 !B.cache.count(std::make_tuple(d, c->nd_L[k], c->fn_L[fn], c->nd_L[c->nd_nb[(size_t)k * 4 + s]], d));
+      }
+      if (needed) node_rows(k, cols, ds);
+    }
+  }
+  const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  auto parallel = [&](auto fn) {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t)
+      th.emplace_back([&, t] {
+        std::vector<std::vector<int>> cols;
+        std::vector<char> ds;
+        for (int k = (int)((int64_t)nn * t / nthreads); k < (int)((int64_t)nn * (t + 1) / nthreads); ++k)
+          fn(k, cols, ds);
+      });
+    for (auto &x : th) x.join();
+  };
+  parallel([&](int k, std::vector<std::vector<int>> &cols, std::vector<char> &ds) {
+    node_rows(k, cols, ds);
+    const int base = c->nd_base[k];
+    for (size_t r = 0; r < cols.size(); ++r) {
+      c->rp[base + r + 1] = (int)cols[r].size();
+      c->row_node[base + r] = k;
+      c->diag_struct[base + r] = ds[r];
+    }
+  });
+  for (int64_t i = 0; i < n; ++i) {
+    if ((int64_t)c->rp[i] + c->rp[i + 1] > 0x7fffffffLL) { ferr("more than 2^31 nonzeros"); return -3; }
+    c->rp[i + 1] += c->rp[i];
+  }
+  c->nnz = c->rp[n];
+  c->ci.assign(c->nnz, 0);
+  parallel([&](int k, std::vector<std::vector<int>> &cols, std::vector<char> &ds) {
+    node_rows(k, cols, ds);
+    const int base = c->nd_base[k];
+    for (size_t r = 0; r < cols.size(); ++r) {
+      int p = c->rp[base + r];
+      for (int col : cols[r]) {
+        if (col == base + (int)r) c->diag_pos[base + r] = p;
+        c->ci[p++] = col;
+      }
+    }
+  });
+  // structural entries per column (the diagonal inserted by the normalisation is not counted)
+  c->colcount.assign(n, 0);
+  for (int64_t i = 0; i < n; ++i)
+    for (int j = c->rp[i]; j < c->rp[i + 1]; ++j)
+      if (j != c->diag_pos[i] || c->diag_struct[i]) c->colcount[c->ci[j]]++;
+  // block ranges (she/linear_solver.hpp:42-83)
+  c->blocks.assign((size_t)2 * NH, 0);
+  int64_t last = 0;
+  for (size_t H = 0; H < NH; ++H) {
+    c->blocks[2 * H] = last;
+    for (int k = 0; k < c->nc; ++k) {
+      const int u = c->idxc[(size_t)k * NH + H];
+      if (u > -1) {
+        last = u;
+        c->blocks[2 * H] = last;
+        if (H > 0) c->blocks[2 * (H - 1) + 1] = last;
+        break;
+      }
+    }
+    if (c->blocks[2 * H + 1] < c->blocks[2 * H]) c->blocks[2 * H + 1] = c->blocks[2 * H];
+  }
+  c->blocks[2 * (NH - 1) + 1] = n;
+  return 0;
+}
+
+int grid(int64_t n) { return (int)std::max<int64_t>(1, (n + kB - 1) / kB); }
+
+int fill_values(shefill_ctx *c, Builder &B) {
+  const int K = c->K;
+  std::vector<double> Ae(4 * K * K), Be(4 * K * K), Ao(4 * K * K), Bo(4 * K * K);
+  for (int d = 0; d < 4; ++d)
+    for (int q = 0; q < K * K; ++q) {
+      Ae[d * K * K + q] = B.Ae[d].v[q];
+      Be[d * K * K + q] = B.Be[d].v[q];
+      Ao[d * K * K + q] = B.Ao[d].v[q];
+      Bo[d * K * K + q] = B.Bo[d].v[q];
+    }
+  std::vector<int> evh, odh, evo, odo;
+  for (int l = 0; l <= c->cfg.L_max; ++l) {
+    evo.push_back((int)evh.size());
+    odo.push_back((int)odh.size());
+    for (int h : B.ev[l]) evh.push_back(h);
+    for (int h : B.od[l]) odh.push_back(h);
+  }
+  evo.push_back((int)evh.size());
+  odo.push_back((int)odh.size());
+  int e = 0;
+  e = e ? e : upload(c, c->dHval, c->Hval);
+  e = e ? e : upload(c, c->dshc, c->shc);
+  e = e ? e : upload(c, c->dshf, c->shf);
+  e = e ? e : upload(c, c->dcvol, c->cvol);
+  e = e ? e : upload(c, c->dfvol, c->fvol);
+  e = e ? e : upload(c, c->dflen, c->flen);
+  e = e ? e : upload(c, c->ddn, c->dn);
+  e = e ? e : upload(c, c->ddp, c->dp);
+  e = e ? e : upload(c, c->dcontact, c->contact);
+  e = e ? e : upload(c, c->dfc1, c->fc1);
+  e = e ? e : upload(c, c->dfc2, c->fc2);
+  e = e ? e : upload(c, c->dAe, Ae);
+  e = e ? e : upload(c, c->dBe, Be);
+  e = e ? e : upload(c, c->dAo, Ao);
+  e = e ? e : upload(c, c->dBo, Bo);
+  e = e ? e : upload(c, c->dev_h, evh);
+  e = e ? e : upload(c, c->dod_h, odh);
+  e = e ? e : upload(c, c->dev_off, evo);
+  e = e ? e : upload(c, c->dod_off, odo);
+  e = e ? e : upload(c, c->dnd_c, c->nd_c);
+  e = e ? e : upload(c, c->dnd_H, c->nd_H);
+  e = e ? e : upload(c, c->dnd_base, c->nd_base);
+  e = e ? e : upload(c, c->dnd_L, c->nd_L);
+  e = e ? e : upload(c, c->dnd_fn, c->nd_fn);
+  e = e ? e : upload(c, c->dnd_nb, c->nd_nb);
+  e = e ? e : upload(c, c->dnd_inn, c->nd_inn);
+  e = e ? e : upload(c, c->dnd_in, c->nd_in);
+  e = e ? e : upload(c, c->dnd_out, c->nd_out);
+  e = e ? e : upload(c, c->dfn_f, c->fn_f);
+  e = e ? e : upload(c, c->dfn_H, c->fn_H);
+  e = e ? e : upload(c, c->dfn_L, c->fn_L);
+  e = e ? e : upload(c, c->dfn_out, c->fn_out);
+  e = e ? e : upload(c, c->drp, c->rp);
+  e = e ? e : upload(c, c->dci, c->ci);
+  e = e ? e : upload(c, c->drow_node, c->row_node);
+  e = e ? e : upload(c, c->ddiag_pos, c->diag_pos);
+  e = e ? e : upload(c, c->dcolcount, c->colcount);
+  e = e ? e : upload(c, c->ddiag_struct, c->diag_struct);
+  const int nn = (int)c->nd_c.size(), nfn = (int)c->fn_f.size();
+  const int64_t n = c->n;
+  e = e ? e : c->dcorr.alloc(c->nc);
+  e = e ? e : c->dncont.alloc(c->nc);
+  e = e ? e : c->dcdiff.alloc(nfn);
+  e = e ? e : c->dcdrift.alloc(2 * (size_t)nfn);
+  e = e ? e : c->dodiag.alloc(nfn);
+  e = e ? e : c->ddiag0.alloc(nn);
+  e = e ? e : c->ddiagr.alloc(nn);
+  e = e ? e : c->drhs0.alloc(nn);
+  e = e ? e : c->dinco.alloc(2 * (size_t)nn);
+  e = e ? e : c->dval.alloc(c->nnz);
+  e = e ? e : c->drhs.alloc(n);
+  e = e ? e : c->dscal.alloc(n);
+  e = e ? e : c->dcolfac.alloc(n);
+  e = e ? e : c->dlog.alloc(n);
+  e = e ? e : c->drp64.alloc(n + 1);
+  if (e) return e;
+
+  DevTables t{};
+  t.K = K;
+  t.NH = c->NH;
+  t.carrier = c->cfg.carrier;
+  t.dim = c->dim;
+  t.T = c->T;
+  t.kT = kKB * c->T;
+  t.hw = optical_phonon_energy(c->cfg.energy_spacing);
+  t.rate_ac = c->cfg.acoustic ? acoustic_rate(c->cfg.carrier, c->T) : 0.0;
+  t.rate_op = c->cfg.optical ? optical_rate(t.hw) : 0.0;
+  t.Nop = optical_phonon_number(t.hw, c->T);
+  t.tau = c->cfg.tau;
+  t.spacing = c->cfg.energy_spacing;
+  t.Hval = c->dHval.p; t.shc = c->dshc.p; t.shf = c->dshf.p; t.cvol = c->dcvol.p; t.fvol = c->dfvol.p;
+  t.flen = c->dflen.p; t.dn = c->ddn.p; t.dp = c->ddp.p; t.contact = c->dcontact.p; t.fc1 = c->dfc1.p;
+  t.fc2 = c->dfc2.p; t.Ae = c->dAe.p; t.Be = c->dBe.p; t.Ao = c->dAo.p; t.Bo = c->dBo.p;
+  t.ev_h = c->dev_h.p; t.od_h = c->dod_h.p; t.ev_off = c->dev_off.p; t.od_off = c->dod_off.p;
+
+  FHIP(hipEventRecord(c->e0, c->stream));
+  hipLaunchKernelGGL(k_bnd_corr, dim3(grid(c->nc)), dim3(kB), 0, c->stream, t, c->nc, c->dcorr.p, c->dncont.p);
+  if (nfn > 0)
+    hipLaunchKernelGGL(k_fnode, dim3(grid(nfn)), dim3(kB), 0, c->stream, t, nfn, c->dfn_f.p, c->dfn_H.p, c->dfn_out.p,
+                       c->dcdiff.p, c->dcdrift.p, c->dodiag.p);
+  if (nn > 0)
+    hipLaunchKernelGGL(k_node, dim3(grid(nn)), dim3(kB), 0, c->stream, t, nn, c->dnd_c.p, c->dnd_H.p, c->dnd_out.p,
+                       c->dnd_in.p, c->dcorr.p, c->dncont.p, c->ddiag0.p, c->ddiagr.p, c->drhs0.p, c->dinco.p);
+  RowArgs a{};
+  a.n = (int)n; a.rp = c->drp.p; a.ci = c->dci.p; a.row_node = c->drow_node.p; a.nd_c = c->dnd_c.p;
+  a.nd_H = c->dnd_H.p; a.nd_base = c->dnd_base.p; a.nd_L = c->dnd_L.p; a.nd_fn = c->dnd_fn.p; a.nd_nb = c->dnd_nb.p;
+  a.nd_inn = c->dnd_inn.p; a.fn_f = c->dfn_f.p; a.fn_L = c->dfn_L.p; a.cdiff = c->dcdiff.p; a.cdrift = c->dcdrift.p;
+  a.odiag = c->dodiag.p; a.diag0 = c->ddiag0.p; a.diagr = c->ddiagr.p; a.rhs0 = c->drhs0.p; a.inco = c->dinco.p;
+  a.val = c->dval.p; a.rhs = c->drhs.p;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_rows, dim3(grid(n)), dim3(kB), 0, c->stream, t, a);
+    hipLaunchKernelGGL(k_node_scaling, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->drow_node.p, c->dnd_c.p,
+                       c->dnd_H.p, t, c->cfg.scale, c->dscal.p);
+    FHIP(hipMemsetAsync(c->dlog.p, 0, sizeof(double) * n, c->stream));
+    hipLaunchKernelGGL(k_scale_log, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->drp.p, c->dci.p,
+                       c->ddiag_pos.p, c->ddiag_struct.p, c->dscal.p, c->dval.p, c->dlog.p);
+    hipLaunchKernelGGL(k_colnorm, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->dcolcount.p, c->ddiag_pos.p,
+                       c->dval.p, c->dlog.p, c->dcolfac.p);
+    hipLaunchKernelGGL(k_rownorm, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->drp.p, c->dci.p, c->ddiag_pos.p,
+                       c->dlog.p, c->dval.p, c->drhs.p);
+    hipLaunchKernelGGL(k_rp64, dim3(grid(n + 1)), dim3(kB), 0, c->stream, (int)n, c->drp.p, c->drp64.p);
+  }
+  FHIP(hipGetLastError());
+  FHIP(hipEventRecord(c->e1, c->stream));
+  FHIP(hipEventSynchronize(c->e1));
+  float ms = 0;
+  FHIP(hipEventElapsedTime(&ms, c->e0, c->e1));
+  c->fill_ms = ms;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int shefill_create(shefill_handle *h, int device) {
+  if (!h) { ferr("handle pointer is NULL"); return 1; }
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) { ferr("no HIP device available"); return -2; }
+  if (device < 0 || device >= cnt) { ferr("device %d out of range", device); return 2; }
+  FHIP(hipSetDevice(device));
+  auto *c = new shefill_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&c->e0) != hipSuccess ||
+      hipEventCreate(&c->e1) != hipSuccess) {
+    ferr("stream/event creation failed");
+    delete c;
+    return -10;
+  }
+  *h = c;
+  return 0;
+}
+
+int shefill_destroy(shefill_handle h) {
+  if (!h) return 1;
+  (void)hipSetDevice(h->device);
+  (void)hipStreamSynchronize(h->stream);
+  (void)hipEventDestroy(h->e0);
+  (void)hipEventDestroy(h->e1);
+  (void)hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+const char *shefill_last_error(void) { return g_ferr.c_str(); }
+
+int shefill_build(shefill_handle h, const shefill_device *d, const shefill_config *cfg) {
+  if (!h) return 1;
+  if (!d || !d->xv || !d->doping_n || !d->doping_p || !d->potential || !d->contact) { ferr("device arrays missing"); return 2; }
+  if (!cfg) return 3;
+  if (d->dim != 1 && d->dim != 2) { ferr("dim must be 1 or 2"); return 2; }
+  if (d->nx < 2 || (d->dim == 2 && (d->ny < 2 || !d->yv))) { ferr("mesh too small"); return 2; }
+  if (!(cfg->energy_spacing > 0) || !(cfg->tau > 0)) { ferr("energy spacing and tau must be positive"); return 3; }
+  if (cfg->carrier != 0 && cfg->carrier != 1) { ferr("carrier must be 0 or 1"); return 3; }
+  FHIP(hipSetDevice(h->device));
+  h->cfg = *cfg;
+  if (int e = build_geometry(h, d)) return e;
+  if (int e = build_energies_orders_mapping(h)) return e;
+  Builder B(h);
+  if (int e = B.tables()) return e == 2 ? 3 : e;
+  if (int e = build_nodes_and_pattern(h, B)) return e;
+  return fill_values(h, B);
+}
+
+int shefill_sizes(shefill_handle h, int64_t *o) {
+  if (!h) return 1;
+  if (!o) return 2;
+  o[0] = h->NH; o[1] = h->n; o[2] = h->nnz; o[3] = h->n_odd; o[4] = (int64_t)h->nd_c.size(); o[5] = h->nc; o[6] = h->nf;
+  return 0;
+}
+
+int shefill_device_system(shefill_handle h, const int64_t **rp, const int32_t **ci, const double **val,
+                          const double **rhs) {
+  if (!h) return 1;
+  if (!h->dval.p) { ferr("nothing built"); return -1; }
+  if (rp) *rp = h->drp64.p;
+  if (ci) *ci = h->dci.p;
+  if (val) *val = h->dval.p;
+  if (rhs) *rhs = h->drhs.p;
+  return 0;
+}
+
+int shefill_get_system(shefill_handle h, int64_t *rp, int32_t *ci, double *val, double *rhs, double *sc, double *cf) {
+  if (!h) return 1;
+  if (!h->dval.p) { ferr("nothing built"); return -1; }
+  FHIP(hipSetDevice(h->device));
+  if (rp) for (int64_t i = 0; i <= h->n; ++i) rp[i] = h->rp[i];
+  if (ci) std::memcpy(ci, h->ci.data(), sizeof(int32_t) * h->nnz);
+  if (val) FHIP(hipMemcpyAsync(val, h->dval.p, sizeof(double) * h->nnz, hipMemcpyDeviceToHost, h->stream));
+  if (rhs) FHIP(hipMemcpyAsync(rhs, h->drhs.p, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream));
+  if (sc) FHIP(hipMemcpyAsync(sc, h->dscal.p, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream));
+  if (cf) FHIP(hipMemcpyAsync(cf, h->dcolfac.p, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream));
+  FHIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int shefill_block_ranges(shefill_handle h, int64_t *r) {
+  if (!h) return 1;
+  if (!r) return 2;
+  std::memcpy(r, h->blocks.data(), sizeof(int64_t) * h->blocks.size());
+  return 0;
+}
+
+int shefill_mapping(shefill_handle h, int64_t *ei, int32_t *eo) {
+  if (!h) return 1;
+  if (!ei) return 2;
+  if (!eo) return 3;
+  for (size_t q = 0; q < h->idxc.size(); ++q) {
+    ei[q] = h->idxc[q];
+    eo[q] = h->ordc[q];
+  }
+  return 0;
+}
+
+int shefill_last_fill_ms(shefill_handle h, double *ms) {
+  if (!h) return 1;
+  if (!ms) return 2;
+  *ms = h->fill_ms;
+  return 0;
+}
+
+}  // extern "C"

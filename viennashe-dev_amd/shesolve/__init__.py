@@ -210,3 +210,125 @@ class Solver:
         a = C.c_double()
         _check(lib().shesolve_time_spmv(self._h, 1 if fp32 else 0, reps, C.byref(a)), "shesolve_time_spmv")
         return a.value
+
+
+# ---------------------------------------------------------------- coefficient fill (include/shefill.h)
+FILL_EXPORTS = ("shefill_last_error", "shefill_create", "shefill_destroy", "shefill_build", "shefill_sizes",
+                "shefill_device_system", "shefill_get_system", "shefill_block_ranges", "shefill_mapping",
+                "shefill_last_fill_ms")
+
+
+class FillDevice(C.Structure):
+    _fields_ = [("dim", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("xv", C.c_void_p), ("yv", C.c_void_p),
+                ("doping_n", C.c_void_p), ("doping_p", C.c_void_p), ("potential", C.c_void_p),
+                ("contact", C.c_void_p), ("temperature", C.c_double)]
+
+
+class FillConfig(C.Structure):
+    _fields_ = [("carrier", C.c_int), ("L_max", C.c_int), ("energy_spacing", C.c_double),
+                ("min_kin_range", C.c_double), ("max_kin_range", C.c_double), ("tau", C.c_double),
+                ("acoustic", C.c_int), ("optical", C.c_int), ("scale", C.c_int)]
+
+
+def _fill_lib():
+    L = lib()
+    if getattr(L, "_fill_ready", False):
+        return L
+    L.shefill_last_error.restype = C.c_char_p
+    L.shefill_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+    L.shefill_destroy.argtypes = [C.c_void_p]
+    L.shefill_build.argtypes = [C.c_void_p, C.POINTER(FillDevice), C.POINTER(FillConfig)]
+    L.shefill_sizes.argtypes = [C.c_void_p, _i64p]
+    L.shefill_device_system.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                        C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
+    L.shefill_get_system.argtypes = [C.c_void_p, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p]
+    L.shefill_block_ranges.argtypes = [C.c_void_p, _i64p]
+    L.shefill_mapping.argtypes = [C.c_void_p, _i64p, _i32p]
+    L.shefill_last_fill_ms.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    for name in FILL_EXPORTS:
+        if name != "shefill_last_error":
+            getattr(L, name).restype = C.c_int
+    L._fill_ready = True
+    return L
+
+
+def _fcheck(rc, what):
+    if rc != 0:
+        raise SheSolveError(f"{what} failed ({rc}): {_fill_lib().shefill_last_error().decode()}")
+
+
+class Fill:
+    """GPU coefficient fill of one structured device (resident reduced system)."""
+
+    def __init__(self, device=0):
+        L = _fill_lib()
+        h = C.c_void_p()
+        _fcheck(L.shefill_create(C.byref(h), device), "shefill_create")
+        self._h = h
+        self.sizes = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _fill_lib().shefill_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def build(self, dev, cfg):
+        keep = dict(xv=np.ascontiguousarray(dev.xv, np.float64), yv=np.ascontiguousarray(dev.yv, np.float64),
+                    dn=np.ascontiguousarray(dev.doping_n, np.float64),
+                    dp=np.ascontiguousarray(dev.doping_p, np.float64),
+                    psi=np.ascontiguousarray(dev.potential, np.float64),
+                    ct=np.ascontiguousarray(dev.contact, np.int32))
+        d = FillDevice(dev.dim, dev.nx, dev.ny, keep["xv"].ctypes.data, keep["yv"].ctypes.data,
+                       keep["dn"].ctypes.data, keep["dp"].ctypes.data, keep["psi"].ctypes.data,
+                       keep["ct"].ctypes.data, dev.temperature)
+        c = FillConfig(cfg.carrier, cfg.L_max, cfg.energy_spacing, cfg.min_kin_range, cfg.max_kin_range, cfg.tau,
+                       cfg.acoustic, cfg.optical, cfg.scale)
+        _fcheck(_fill_lib().shefill_build(self._h, C.byref(d), C.byref(c)), "shefill_build")
+        s = np.zeros(7, np.int64)
+        _fcheck(_fill_lib().shefill_sizes(self._h, s), "shefill_sizes")
+        self.sizes = dict(NH=int(s[0]), n=int(s[1]), nnz=int(s[2]), n_odd=int(s[3]), nodes=int(s[4]),
+                          cells=int(s[5]), facets=int(s[6]))
+        self.ncells = dev.ncells
+        return self.sizes
+
+    @property
+    def n(self):
+        return self.sizes["n"]
+
+    @property
+    def nnz(self):
+        return self.sizes["nnz"]
+
+    def device_system(self):
+        ptrs = [C.c_void_p() for _ in range(4)]
+        _fcheck(_fill_lib().shefill_device_system(self._h, *[C.byref(p) for p in ptrs]), "shefill_device_system")
+        return tuple(p.value for p in ptrs)
+
+    def system(self):
+        n, nnz = self.n, self.nnz
+        rp = np.zeros(n + 1, np.int64)
+        ci = np.zeros(max(nnz, 1), np.int32)
+        v = np.zeros(max(nnz, 1))
+        rhs, sc, cf = np.zeros(n), np.zeros(n), np.zeros(n)
+        _fcheck(_fill_lib().shefill_get_system(self._h, rp, ci, v, rhs, sc, cf), "shefill_get_system")
+        return rp, ci[:nnz], v[:nnz], rhs, sc, cf
+
+    def block_ranges(self):
+        r = np.zeros(2 * self.sizes["NH"], np.int64)
+        _fcheck(_fill_lib().shefill_block_ranges(self._h, r), "shefill_block_ranges")
+        return r.reshape(-1, 2)
+
+    def mapping(self):
+        m = self.ncells * self.sizes["NH"]
+        ei = np.zeros(m, np.int64)
+        eo = np.zeros(m, np.int32)
+        _fcheck(_fill_lib().shefill_mapping(self._h, ei, eo), "shefill_mapping")
+        return ei.reshape(self.ncells, -1), eo.reshape(self.ncells, -1)
+
+    def last_fill_ms(self):
+        v = C.c_double()
+        _fcheck(_fill_lib().shefill_last_fill_ms(self._h, C.byref(v)), "shefill_last_fill_ms")
+        return v.value
