@@ -64,6 +64,10 @@ int shefill_destroy(shefill_handle h);
  * mesh reuses buffers; the structure is rebuilt from the new energies. */
 int shefill_build(shefill_handle h, const shefill_device *dev, const shefill_config *cfg);
 
+/* Recomputes every value of the last build on the device (same structure and
+ * inputs): the per-solve coefficient fill of a Gummel step. */
+int shefill_refill(shefill_handle h);
+
 /* sizes: [0] N_H, [1] n (reduced unknowns), [2] nnz, [3] n_odd (eliminated),
  *        [4] nodes (cell, H) carrying unknowns, [5] cells, [6] interior facets */
 int shefill_sizes(shefill_handle h, int64_t *out7);

@@ -41,7 +41,10 @@ typedef struct shesolve_ctx *shesolve_handle;
 enum {
   SHESOLVE_PRECOND_NONE = 0,
   SHESOLVE_PRECOND_ILU0 = 1,        /* ILU(0), natural row order, level-scheduled */
-  SHESOLVE_PRECOND_BLOCK_ILU0 = 3   /* ILU(0) of the diagonal blocks (per-energy slabs) */
+  SHESOLVE_PRECOND_BLOCK_ILU0 = 3,  /* ILU(0) of the diagonal blocks (per-energy slabs) */
+  SHESOLVE_PRECOND_MC_ILU0 = 4      /* ILU(0) of the matrix in a greedy multicolour row
+                                       ordering: a few dependency levels instead of
+                                       hundreds; not in the reference, same stopping rule */
 };
 
 enum {
@@ -110,6 +113,8 @@ int shesolve_precond_apply(shesolve_handle h, const double *r, double *z);
  * by shesolve_factor_nnz. */
 int64_t shesolve_factor_nnz(shesolve_handle h);
 int shesolve_get_factors(shesolve_handle h, int64_t *row_ptr, int32_t *col, double *val);
+/* Row ordering of the factor (new -> old); the identity unless MC_ILU0. */
+int shesolve_get_ordering(shesolve_handle h, int32_t *perm);
 /* Level-schedule statistics: [0] forward levels, [1] backward levels. */
 int shesolve_levels(shesolve_handle h, int64_t *out2);
 

@@ -81,14 +81,51 @@ def test_ilu0_factors_match_oracle(name, kind):
     assert np.linalg.norm(z - z_ref) <= 1e-10 * np.linalg.norm(z_ref)
 
 
+def _pc(kind):
+    import shesolve
+    return {"ilu0": shesolve.PRECOND_ILU0, "block_ilu0": shesolve.PRECOND_BLOCK_ILU0,
+            "mc_ilu0": shesolve.PRECOND_MC_ILU0}[kind]
+
+
+@pytest.mark.parametrize("name", ["cfg1", "mos2d_L3"])
+def test_mc_ilu0_factors_match_oracle(name):
+    """Multicolour ILU0 = the oracle's ILU0 of the matrix permuted by the
+    solver's own ordering (rows and columns)."""
+    import scipy.sparse as sp
+    from oracle import oracle as O
+    s = oracle_system(name)
+    A, b, _, _ = s.reduced()
+    sv = _solver(precond=_pc("mc_ilu0"))
+    sv.set_matrix(A.row_ptr, A.col, A.val)
+    perm = sv.ordering().astype(np.int64)
+    assert np.array_equal(np.sort(perm), np.arange(A.n))
+    M = A.to_scipy()
+    Mp = M[perm][:, perm].tocsr()
+    Mp.sort_indices()
+    Ap = O.CSR(Mp.indptr, Mp.indices, Mp.data)
+    F = O.Precond(O.PC_ILU0, Ap).factors()
+    rp, ci, v = sv.factors()
+    assert np.array_equal(rp, F.row_ptr) and np.array_equal(ci, F.col)
+    assert np.max(np.abs(v - F.val) / np.maximum(np.abs(F.val), 1e-300)) <= 1e-10
+    lv = sv.levels()
+    assert lv[0] <= 16 and lv[1] <= 16, lv
+    # application in the natural ordering: z = P^T (LU)^-1 P r
+    r = np.random.default_rng(5).standard_normal(A.n)
+    z = sv.precond_apply(r)
+    zp = O.Precond(O.PC_ILU0, Ap).apply(r[perm])
+    z_ref = np.empty_like(zp)
+    z_ref[perm] = zp
+    assert np.linalg.norm(z - z_ref) <= 1e-10 * np.linalg.norm(z_ref)
+
+
 @pytest.mark.parametrize("name", SYSTEMS)
 @pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
-@pytest.mark.parametrize("kind", ["ilu0", "block_ilu0"])
+@pytest.mark.parametrize("kind", ["ilu0", "block_ilu0", "mc_ilu0"])
 def test_solve_parity(name, precision, kind):
     import shesolve
     s = oracle_system(name)
     A, b, _, _ = s.reduced()
-    pc = shesolve.PRECOND_ILU0 if kind == "ilu0" else shesolve.PRECOND_BLOCK_ILU0
+    pc = _pc(kind)
     prec = shesolve.FP64 if precision == "fp64" else shesolve.FP32_IR
     sv = _solver(precond=pc, precision=prec, tol=1e-10, max_iters=5000)
     sv.set_matrix(A.row_ptr, A.col, A.val, block_ranges=s.block_ranges() if kind != "ilu0" else None)

@@ -411,6 +411,8 @@ struct shefill_ctx {
   DBuf<double> dcorr, dncont, dcdiff, dcdrift, dodiag, ddiag0, ddiagr, drhs0, dinco;
   DBuf<double> dval, drhs, dscal, dcolfac, dlog;
   DBuf<int64_t> drp64;
+  DevTables t{};
+  bool built = false;
 };
 
 namespace {
@@ -857,6 +859,7 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
 }
 
 int grid(int64_t n) { return (int)std::max<int64_t>(1, (n + kB - 1) / kB); }
+int launch_fill(shefill_ctx *c);
 
 int fill_values(shefill_ctx *c, Builder &B) {
   const int K = c->K;
@@ -953,6 +956,15 @@ int fill_values(shefill_ctx *c, Builder &B) {
   t.fc2 = c->dfc2.p; t.Ae = c->dAe.p; t.Be = c->dBe.p; t.Ao = c->dAo.p; t.Bo = c->dBo.p;
   t.ev_h = c->dev_h.p; t.od_h = c->dod_h.p; t.ev_off = c->dev_off.p; t.od_off = c->dod_off.p;
 
+  c->t = t;
+  c->built = true;
+  return launch_fill(c);
+}
+
+int launch_fill(shefill_ctx *c) {
+  const DevTables &t = c->t;
+  const int nn = (int)c->nd_c.size(), nfn = (int)c->fn_f.size();
+  const int64_t n = c->n;
   FHIP(hipEventRecord(c->e0, c->stream));
   hipLaunchKernelGGL(k_bnd_corr, dim3(grid(c->nc)), dim3(kB), 0, c->stream, t, c->nc, c->dcorr.p, c->dncont.p);
   if (nfn > 0)
@@ -1040,6 +1052,13 @@ int shefill_build(shefill_handle h, const shefill_device *d, const shefill_confi
   if (int e = B.tables()) return e == 2 ? 3 : e;
   if (int e = build_nodes_and_pattern(h, B)) return e;
   return fill_values(h, B);
+}
+
+int shefill_refill(shefill_handle h) {
+  if (!h) return 1;
+  if (!h->built) { ferr("nothing built"); return -1; }
+  FHIP(hipSetDevice(h->device));
+  return launch_fill(h);
 }
 
 int shefill_sizes(shefill_handle h, int64_t *o) {

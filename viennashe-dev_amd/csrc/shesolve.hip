@@ -123,6 +123,148 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ 
   }
 }
 
+// ---------------------------------------------------------------- CSR-stream kernels
+// A row block is a run of consecutive rows holding at most kStreamNnz entries
+// (a single longer row forms a block of its own).  The block's entries are
+// streamed with fully coalesced loads into LDS as products v[k] * x[col[k]],
+// then each thread reduces whole rows from LDS.  Used for the SpMV and for the
+// sweeps of every contiguous level of the triangular factors.
+constexpr int kStreamNnz = 2048;
+constexpr int kStreamThreads = 256;
+constexpr int kStreamRows = 256;  // rows per block (rows without entries must not pile up)
+
+enum { MODE_SPMV = 0, MODE_LSWEEP = 1, MODE_USWEEP = 2 };
+
+template <typename T>
+__device__ inline T block_sum_t(T v) {
+  __shared__ T red[kStreamThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  T t = 0;
+  for (int w = 0; w < kStreamThreads / 64; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+// MODE_SPMV:   y[row] = (b ? b[row] - s : s), dots of y with d0, d1 (per persistent block)
+// MODE_LSWEEP: y[row] = b[perm ? perm[row] : row] - s           (y == x: the factor-space vector)
+// MODE_USWEEP: y[row] = dinv[row] * (y[row] - s), out2[perm[row]] = y[row] when perm
+template <typename TV, typename T, int MODE>
+__global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, const int *__restrict__ blocks,
+                                                           const int *__restrict__ rp, const int *__restrict__ ci,
+                                                           const TV *__restrict__ v, const T *x, T *y,
+                                                           const T *__restrict__ b, const int *__restrict__ perm,
+                                                           const T *__restrict__ dinv, T *__restrict__ out2,
+                                                           const T *__restrict__ d0, const T *__restrict__ d1,
+                                                           double *__restrict__ part) {
+  __shared__ T prod[kStreamNnz];
+  double a0 = 0.0, a1 = 0.0;
+  for (int bk = blockIdx.x; bk < nblk; bk += gridDim.x) {
+    const int rs = blocks[bk], re = blocks[bk + 1];
+    const int k0 = rp[rs], k1 = rp[re];
+    if (k1 - k0 <= kStreamNnz) {
+      for (int k = k0 + (int)threadIdx.x; k < k1; k += kStreamThreads) prod[k - k0] = (T)v[k] * x[ci[k]];
+      __syncthreads();
+      for (int row = rs + (int)threadIdx.x; row < re; row += kStreamThreads) {
+        T acc = 0;
+        const int e = rp[row + 1] - k0;
+        for (int k = rp[row] - k0; k < e; ++k) acc += prod[k];
+        if (MODE == MODE_SPMV) {
+          const T out = b ? (T)(b[row] - acc) : acc;
+          y[row] = out;
+          if (d0) a0 += (double)d0[row] * (double)out;
+          if (d1) a1 += (double)d1[row] * (double)out;
+        } else if (MODE == MODE_LSWEEP) {
+          y[row] = (perm ? b[perm[row]] : b[row]) - acc;
+        } else {
+          const T val = dinv[row] * (y[row] - acc);
+          y[row] = val;
+          if (perm) out2[perm[row]] = val;
+        }
+      }
+      __syncthreads();
+    } else {
+      // one long row
+      T acc = 0;
+      for (int k = k0 + (int)threadIdx.x; k < k1; k += kStreamThreads) acc += (T)v[k] * x[ci[k]];
+      acc = block_sum_t<T>(acc);
+      if (threadIdx.x == 0) {
+        const int row = rs;
+        if (MODE == MODE_SPMV) {
+          const T out = b ? (T)(b[row] - acc) : acc;
+          y[row] = out;
+          if (d0) a0 += (double)d0[row] * (double)out;
+          if (d1) a1 += (double)d1[row] * (double)out;
+        } else if (MODE == MODE_LSWEEP) {
+          y[row] = (perm ? b[perm[row]] : b[row]) - acc;
+        } else {
+          const T val = dinv[row] * (y[row] - acc);
+          y[row] = val;
+          if (perm) out2[perm[row]] = val;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (part) {
+    double vv[2] = {a0, a1};
+    block_sum<2>(vv);
+    if (threadIdx.x == 0) {
+      part[blockIdx.x] = vv[0];
+      part[kGrid + blockIdx.x] = vv[1];
+    }
+  }
+}
+
+// Row-list sweeps for levels that are not contiguous ranges (natural-order ILU0).
+template <int LPR, typename TV, typename T>
+__global__ __launch_bounds__(kBlock) void k_lsweep_rows(int cnt, const int *__restrict__ rows, const int *__restrict__ rp,
+                                                        const int *__restrict__ ci, const TV *__restrict__ v,
+                                                        const T *__restrict__ rhs, T *out, const int *__restrict__ perm) {
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = gtid / LPR, lane = gtid & (LPR - 1);
+  if (g >= cnt) return;
+  const int row = rows[g];
+  T acc = 0;
+  for (int k = rp[row] + lane; k < rp[row + 1]; k += LPR) acc += (T)v[k] * out[ci[k]];
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+  if (lane == 0) out[row] = (perm ? rhs[perm[row]] : rhs[row]) - acc;
+}
+template <int LPR, typename TV, typename T>
+__global__ __launch_bounds__(kBlock) void k_usweep_rows(int cnt, const int *__restrict__ rows, const int *__restrict__ rp,
+                                                        const int *__restrict__ ci, const TV *__restrict__ v,
+                                                        const T *__restrict__ dinv, T *out, const int *__restrict__ perm,
+                                                        T *__restrict__ out2) {
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = gtid / LPR, lane = gtid & (LPR - 1);
+  if (g >= cnt) return;
+  const int row = rows[g];
+  T acc = 0;
+  for (int k = rp[row] + lane; k < rp[row + 1]; k += LPR) acc += (T)v[k] * out[ci[k]];
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+  if (lane == 0) {
+    const T val = dinv[row] * (out[row] - acc);
+    out[row] = val;
+    if (perm) out2[perm[row]] = val;
+  }
+}
+
+// Split of the factor into strict lower / strict upper parts and 1/diagonal.
+__global__ __launch_bounds__(kBlock) void k_split(int n, const int *__restrict__ frp, const int *__restrict__ fdg,
+                                                  const double *__restrict__ f, const int *__restrict__ Lrp,
+                                                  const int *__restrict__ Urp, double *__restrict__ L,
+                                                  double *__restrict__ U, double *__restrict__ dinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = frp[i], d = fdg[i], e = frp[i + 1];
+  for (int k = b; k < d; ++k) L[Lrp[i] + (k - b)] = f[k];
+  for (int k = d + 1; k < e; ++k) U[Urp[i] + (k - d - 1)] = f[k];
+  dinv[i] = 1.0 / f[d];
+}
+
 // ---------------------------------------------------------------- vector kernels
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_copy_dot(int n, const T *__restrict__ a, T *__restrict__ out,
@@ -277,11 +419,13 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int op, const double *__res
 
 // ---------------------------------------------------------------- level-scheduled ILU(0)
 // Forward sweep (unit lower triangle): out[row] = rhs[row] - sum_{k<diag} L[k] out[col[k]]
+// With a permutation (multicolour ordering) the sweeps run in the permuted
+// index space: the right-hand side is gathered through perm (new -> old).
 template <int LPR, typename TV, typename T>
 __global__ __launch_bounds__(kBlock) void k_lsolve(int cnt, const int *__restrict__ rows, const int *__restrict__ rp,
                                                    const int *__restrict__ ci, const int *__restrict__ dg,
                                                    const TV *__restrict__ fv, const T *__restrict__ rhs,
-                                                   T *__restrict__ out) {
+                                                   T *__restrict__ out, const int *__restrict__ perm) {
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int g = gtid / LPR, lane = gtid & (LPR - 1);
   if (g >= cnt) return;
@@ -290,13 +434,16 @@ __global__ __launch_bounds__(kBlock) void k_lsolve(int cnt, const int *__restric
   for (int k = rp[row] + lane; k < dg[row]; k += LPR) s += (T)fv[k] * out[ci[k]];
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
-  if (lane == 0) out[row] = rhs[row] - s;
+  if (lane == 0) out[row] = (perm ? rhs[perm[row]] : rhs[row]) - s;
 }
 // Backward sweep (upper triangle with diagonal), in place on out.
+// Backward sweep; with a permutation the result is also scattered to the
+// natural order (out2[perm[row]]).
 template <int LPR, typename TV, typename T>
 __global__ __launch_bounds__(kBlock) void k_usolve(int cnt, const int *__restrict__ rows, const int *__restrict__ rp,
                                                    const int *__restrict__ ci, const int *__restrict__ dg,
-                                                   const TV *__restrict__ fv, T *__restrict__ out) {
+                                                   const TV *__restrict__ fv, T *__restrict__ out,
+                                                   const int *__restrict__ perm, T *__restrict__ out2) {
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int g = gtid / LPR, lane = gtid & (LPR - 1);
   if (g >= cnt) return;
@@ -306,7 +453,11 @@ __global__ __launch_bounds__(kBlock) void k_usolve(int cnt, const int *__restric
   for (int k = d + 1 + lane; k < rp[row + 1]; k += LPR) s += (T)fv[k] * out[ci[k]];
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
-  if (lane == 0) out[row] = (out[row] - s) / (T)fv[d];
+  if (lane == 0) {
+    const T v = (out[row] - s) / (T)fv[d];
+    out[row] = v;
+    if (perm) out2[perm[row]] = v;
+  }
 }
 
 // ILU(0) of the rows of one level; one 64-lane block per row, row kept in LDS.
@@ -376,6 +527,22 @@ struct shesolve_ctx {
   float *d_f32 = nullptr;
   std::vector<int> h_frp, h_fci, h_fdg;
   std::vector<int> blk_of_row;
+  int *d_perm = nullptr;            // multicolour ordering: new -> old row
+  double *z64 = nullptr;            // permuted-space work vectors of the sweeps
+  float *z32 = nullptr;
+  int ncolors = 0;
+  std::vector<int> h_perm;
+  // CSR-stream row blocks of A
+  int *d_ablk = nullptr;
+  int nablk = 0;
+  // split factor: strict lower / strict upper parts and 1/diagonal (factor space)
+  int *d_Lrp = nullptr, *d_Lci = nullptr, *d_Urp = nullptr, *d_Uci = nullptr, *d_Lsrc = nullptr;
+  double *d_L64 = nullptr, *d_U64 = nullptr, *d_dinv64 = nullptr;
+  float *d_L32 = nullptr, *d_U32 = nullptr, *d_dinv32 = nullptr;
+  int64_t nL = 0, nU = 0;
+  struct LevelRun { int r0, r1, blk_off, blk_cnt; };  // contiguous level: rows [r0,r1), row blocks
+  std::vector<LevelRun> runL, runU;
+  int *d_lblk = nullptr, *d_ublk = nullptr;
   Levels lvL, lvU;
   int lpr = 8, flpr = 8;
   // vectors (FP64 set and FP32 set)
@@ -430,56 +597,72 @@ struct KTimer {
 };
 
 template <typename TV, typename T, typename TA>
-void launch_spmv(shesolve_ctx *c, int lpr, const TV *v, const T *x, T *y, const T *b, const T *d0, const T *d1,
+void launch_spmv(shesolve_ctx *c, int /*lpr*/, const TV *v, const T *x, T *y, const T *b, const T *d0, const T *d1,
                  double *part) {
   KTimer kt(c, 0);
-  const int n = c->n;
-  int g = std::min<int64_t>(kGrid, grid_for((int64_t)n * lpr, kBlock));
-  if (part) g = kGrid;  // fixed partial count: every block writes its partial
-#define SPMV_CASE(L)                                                                                        \
-  case L:                                                                                                   \
-    hipLaunchKernelGGL((k_spmv<L, TV, T, TA>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rp, c->d_ci, v, \
-                       x, y, b, d0, d1, part);                                                              \
-    break;
-  switch (lpr) {
-    SPMV_CASE(2) SPMV_CASE(4) SPMV_CASE(8) SPMV_CASE(16) SPMV_CASE(32) SPMV_CASE(64)
-    default:
-      hipLaunchKernelGGL((k_spmv<8, TV, T, TA>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rp, c->d_ci, v, x, y,
-                         b, d0, d1, part);
-  }
-#undef SPMV_CASE
+  const int g = part ? kGrid : std::max(1, std::min(c->nablk, 4 * kGrid));
+  hipLaunchKernelGGL((k_stream<TV, T, MODE_SPMV>), dim3(g), dim3(kStreamThreads), 0, c->stream, c->nablk, c->d_ablk,
+                     c->d_rp, c->d_ci, v, x, y, b, (const int *)nullptr, (const T *)nullptr, (T *)nullptr, d0, d1,
+                     part);
 }
 
 template <typename TV, typename T>
-void launch_precond(shesolve_ctx *c, const TV *fv, const T *in, T *out) {
+void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out) {
   const int n = c->n;
   if (c->prm.precond == SHESOLVE_PRECOND_NONE || !c->d_frp) {
     hipMemcpyAsync(out, in, sizeof(T) * n, hipMemcpyDeviceToDevice, c->stream);
     return;
   }
+  const TV *Lv, *Uv, *dinv;
+  if constexpr (sizeof(TV) == 8) {
+    Lv = (const TV *)c->d_L64; Uv = (const TV *)c->d_U64; dinv = (const TV *)c->d_dinv64;
+  } else {
+    Lv = (const TV *)c->d_L32; Uv = (const TV *)c->d_U32; dinv = (const TV *)c->d_dinv32;
+  }
   const int L = c->flpr;
+  const int *perm = c->d_perm;
+  T *z = out;
+  if (perm) {
+    if constexpr (sizeof(T) == 8) z = (T *)c->z64; else z = (T *)c->z32;
+  }
   for (int l = 0; l < c->lvL.count(); ++l) {
     KTimer kt(c, 1);
+    const auto &run = c->runL[l];
+    if (run.blk_cnt > 0) {
+      const int g = std::max(1, std::min(run.blk_cnt, 4 * kGrid));
+      hipLaunchKernelGGL((k_stream<TV, T, MODE_LSWEEP>), dim3(g), dim3(kStreamThreads), 0, c->stream, run.blk_cnt,
+                         c->d_lblk + run.blk_off, c->d_Lrp, c->d_Lci, Lv, z, z, in, perm, (const T *)nullptr,
+                         (T *)nullptr, (const T *)nullptr, (const T *)nullptr, (double *)nullptr);
+      continue;
+    }
     int cnt = c->lvL.ptr[l + 1] - c->lvL.ptr[l];
     const int *rows = c->lvL.d_rows + c->lvL.ptr[l];
     int g = grid_for((int64_t)cnt * L, kBlock);
-#define LS_CASE(LL)                                                                                         \
-  case LL:                                                                                                  \
-    hipLaunchKernelGGL((k_lsolve<LL, TV, T>), dim3(g), dim3(kBlock), 0, c->stream, cnt, rows, c->d_frp,     \
-                       c->d_fci, c->d_fdg, fv, in, out);                                                    \
+#define LS_CASE(LL)                                                                                          \
+  case LL:                                                                                                   \
+    hipLaunchKernelGGL((k_lsweep_rows<LL, TV, T>), dim3(g), dim3(kBlock), 0, c->stream, cnt, rows, c->d_Lrp, \
+                       c->d_Lci, Lv, in, z, perm);                                                           \
     break;
     switch (L) { LS_CASE(2) LS_CASE(4) LS_CASE(8) LS_CASE(16) LS_CASE(32) default: LS_CASE(64) }
 #undef LS_CASE
   }
   for (int l = 0; l < c->lvU.count(); ++l) {
     KTimer kt(c, 2);
+    const auto &run = c->runU[l];
+    if (run.blk_cnt > 0) {
+      const int g = std::max(1, std::min(run.blk_cnt, 4 * kGrid));
+      hipLaunchKernelGGL((k_stream<TV, T, MODE_USWEEP>), dim3(g), dim3(kStreamThreads), 0, c->stream, run.blk_cnt,
+                         c->d_ublk + run.blk_off, c->d_Urp, c->d_Uci, Uv, z, z, (const T *)nullptr, perm,
+                         (const T *)dinv, out, (const T *)nullptr, (const T *)nullptr, (double *)nullptr);
+      continue;
+    }
     int cnt = c->lvU.ptr[l + 1] - c->lvU.ptr[l];
     const int *rows = c->lvU.d_rows + c->lvU.ptr[l];
     int g = grid_for((int64_t)cnt * L, kBlock);
-#define US_CASE(LL)                                                                                         \
-  case LL:                                                                                                  \
-    hipLaunchKernelGGL((k_usolve<LL, TV, T>), dim3(g), dim3(kBlock), 0, c->stream, cnt, rows, c->d_frp,     \
-                       c->d_fci, c->d_fdg, fv, out);                                                        \
+#define US_CASE(LL)                                                                                          \
+  case LL:                                                                                                   \
+    hipLaunchKernelGGL((k_usweep_rows<LL, TV, T>), dim3(g), dim3(kBlock), 0, c->stream, cnt, rows, c->d_Urp, \
+                       c->d_Uci, Uv, (const T *)dinv, z, perm, out);                                         \
     break;
     switch (L) { US_CASE(2) US_CASE(4) US_CASE(8) US_CASE(16) US_CASE(32) default: US_CASE(64) }
 #undef US_CASE
@@ -505,8 +688,28 @@ void free_matrix(shesolve_ctx *c) {
   if (!c->factor_is_A) { dfree(c->d_frp); dfree(c->d_fci); }
   c->d_frp = c->d_fci = nullptr;
   dfree(c->d_fdg); dfree(c->d_fpos); dfree(c->d_f64); dfree(c->d_f32);
+  dfree(c->d_perm); dfree(c->z64); dfree(c->z32);
+  dfree(c->d_ablk); c->nablk = 0;
+  dfree(c->d_Lrp); dfree(c->d_Lci); dfree(c->d_Urp); dfree(c->d_Uci); dfree(c->d_Lsrc);
+  dfree(c->d_L64); dfree(c->d_U64); dfree(c->d_dinv64); dfree(c->d_L32); dfree(c->d_U32); dfree(c->d_dinv32);
+  dfree(c->d_lblk); dfree(c->d_ublk);
+  c->runL.clear(); c->runU.clear();
   dfree(c->lvL.d_rows); dfree(c->lvU.d_rows);
   c->lvL.ptr.clear(); c->lvU.ptr.clear();
+}
+
+// Row blocks over rows [r0, r1) of a CSR (host row pointer): consecutive rows
+// holding at most kStreamNnz entries; a longer row forms a block of its own.
+void make_blocks(const int *rp, int r0, int r1, std::vector<int> &out) {
+  out.push_back(r0);
+  int start = r0;
+  for (int r = r0; r < r1; ++r) {
+    if (r > start && (rp[r + 1] - rp[start] > kStreamNnz || r - start >= kStreamRows)) {
+      out.push_back(r);
+      start = r;
+    }
+  }
+  if (out.back() != r1) out.push_back(r1);
 }
 
 // Level sets of the lower (forward) and upper (backward) triangles of the factor pattern.
@@ -543,23 +746,72 @@ void compute_levels(const std::vector<int> &rp, const std::vector<int> &ci, cons
 int analyse_and_factor(shesolve_ctx *c) {
   const int n = c->n;
   if (n <= 0) return 0;
-  // factor pattern
+  // factor pattern (in the factor's own index space) and the gather map from A
   if (c->prm.precond == SHESOLVE_PRECOND_NONE) return 0;
+  std::vector<int> pos, perm;
   if (c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILU0 && !c->blk_of_row.empty()) {
-    c->factor_is_A = false;
     c->h_frp.assign(n + 1, 0);
     c->h_fci.clear();
     c->h_fci.reserve(c->h_ci.size());
     for (int i = 0; i < n; ++i) {
       for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k)
-        if (c->blk_of_row[c->h_ci[k]] == c->blk_of_row[i]) c->h_fci.push_back(c->h_ci[k]);
+        if (c->blk_of_row[c->h_ci[k]] == c->blk_of_row[i]) {
+          c->h_fci.push_back(c->h_ci[k]);
+          pos.push_back(k);
+        }
       c->h_frp[i + 1] = (int)c->h_fci.size();
     }
+  } else if (c->prm.precond == SHESOLVE_PRECOND_MC_ILU0) {
+    // greedy colouring of the rows in natural order (colours of already
+    // coloured neighbours in the row are excluded); rows sorted by colour,
+    // stable, give the multicolour ordering new -> old
+    std::vector<int> colour(n, -1);
+    std::vector<int> mark(1024, -1);
+    int ncol = 0;
+    for (int i = 0; i < n; ++i) {
+      for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) {
+        const int cj = colour[c->h_ci[k]];
+        if (cj >= 0) mark[cj] = i;
+      }
+      int q = 0;
+      while (q < 1023 && mark[q] == i) ++q;
+      colour[i] = q;
+      ncol = std::max(ncol, q + 1);
+    }
+    c->ncolors = ncol;
+    std::vector<int> start(ncol + 1, 0);
+    for (int i = 0; i < n; ++i) start[colour[i] + 1]++;
+    for (int q = 0; q < ncol; ++q) start[q + 1] += start[q];
+    perm.assign(n, 0);
+    std::vector<int> iperm(n);
+    for (int i = 0; i < n; ++i) {
+      const int p = start[colour[i]]++;
+      perm[p] = i;
+      iperm[i] = p;
+    }
+    c->h_frp.assign(n + 1, 0);
+    c->h_fci.resize(c->h_ci.size());
+    pos.resize(c->h_ci.size());
+    std::vector<std::pair<int, int>> tmp;
+    int w = 0;
+    for (int p = 0; p < n; ++p) {
+      const int i = perm[p];
+      tmp.clear();
+      for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) tmp.emplace_back(iperm[c->h_ci[k]], k);
+      std::sort(tmp.begin(), tmp.end());
+      for (auto &e : tmp) {
+        c->h_fci[w] = e.first;
+        pos[w] = e.second;
+        ++w;
+      }
+      c->h_frp[p + 1] = w;
+    }
   } else {
-    c->factor_is_A = true;
     c->h_frp = c->h_rp;
     c->h_fci = c->h_ci;
   }
+  c->factor_is_A = pos.empty();
+  c->h_perm = perm;
   c->fnnz = c->h_frp[n];
   c->h_fdg.assign(n, -1);
   int maxlen = 0;
@@ -592,20 +844,80 @@ int analyse_and_factor(shesolve_ctx *c) {
     HIPCHK(hipMalloc(&c->d_fci, sizeof(int) * std::max<int64_t>(c->fnnz, 1)));
     HIPCHK(hipMemcpyAsync(c->d_frp, c->h_frp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_fci, c->h_fci.data(), sizeof(int) * c->fnnz, hipMemcpyHostToDevice, c->stream));
-    std::vector<int> pos;
-    pos.reserve(c->fnnz);
-    for (int i = 0; i < n; ++i)
-      for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k)
-        if (c->blk_of_row[c->h_ci[k]] == c->blk_of_row[i]) pos.push_back(k);
     HIPCHK(hipMalloc(&c->d_fpos, sizeof(int) * std::max<int64_t>(c->fnnz, 1)));
     HIPCHK(hipMemcpyAsync(c->d_fpos, pos.data(), sizeof(int) * c->fnnz, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
+  if (!perm.empty()) {
+    HIPCHK(hipMalloc(&c->d_perm, sizeof(int) * n));
+    HIPCHK(hipMemcpyAsync(c->d_perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMalloc(&c->z64, sizeof(double) * n));
+    HIPCHK(hipMalloc(&c->z32, sizeof(float) * n));
+  }
   HIPCHK(hipMalloc(&c->d_fdg, sizeof(int) * std::max(n, 1)));
   HIPCHK(hipMemcpyAsync(c->d_fdg, c->h_fdg.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMalloc(&c->d_f64, sizeof(double) * std::max<int64_t>(c->fnnz, 1)));
   HIPCHK(hipMalloc(&c->d_f32, sizeof(float) * std::max<int64_t>(c->fnnz, 1)));
   c->flpr = pick_lpr((double)c->fnnz / std::max(n, 1) / 2.0);
+  // split pattern of the factor
+  std::vector<int> Lrp(n + 1, 0), Urp(n + 1, 0), Lci, Uci;
+  Lci.reserve(c->fnnz / 2 + 1);
+  Uci.reserve(c->fnnz / 2 + 1);
+  for (int i = 0; i < n; ++i) {
+    for (int k = c->h_frp[i]; k < c->h_fdg[i]; ++k) Lci.push_back(c->h_fci[k]);
+    for (int k = c->h_fdg[i] + 1; k < c->h_frp[i + 1]; ++k) Uci.push_back(c->h_fci[k]);
+    Lrp[i + 1] = (int)Lci.size();
+    Uci.size();
+    Urp[i + 1] = (int)Uci.size();
+  }
+  c->nL = Lci.size();
+  c->nU = Uci.size();
+  // contiguous levels get CSR-stream row blocks
+  auto runs = [&](const Levels &lv, const std::vector<int> &rows, const std::vector<int> &rp,
+                  std::vector<shesolve_ctx::LevelRun> &out, std::vector<int> &blk) {
+    out.clear();
+    blk.clear();
+    for (int l = 0; l < lv.count(); ++l) {
+      const int b = lv.ptr[l], e = lv.ptr[l + 1];
+      bool contiguous = e > b;
+      for (int q = b + 1; q < e && contiguous; ++q) contiguous = rows[q] == rows[q - 1] + 1;
+      shesolve_ctx::LevelRun r{0, 0, 0, 0};
+      if (contiguous) {
+        r.r0 = rows[b];
+        r.r1 = rows[e - 1] + 1;
+        r.blk_off = (int)blk.size();
+        std::vector<int> bb;
+        make_blocks(rp.data(), r.r0, r.r1, bb);
+        // stored as boundaries: blk_cnt blocks need blk_cnt + 1 entries
+        for (int v : bb) blk.push_back(v);
+        r.blk_cnt = (int)bb.size() - 1;
+      }
+      out.push_back(r);
+    }
+  };
+  std::vector<int> lblk, ublk;
+  runs(c->lvL, Lrows, Lrp, c->runL, lblk);
+  runs(c->lvU, Urows, Urp, c->runU, ublk);
+  HIPCHK(hipMalloc(&c->d_Lrp, sizeof(int) * (n + 1)));
+  HIPCHK(hipMalloc(&c->d_Urp, sizeof(int) * (n + 1)));
+  HIPCHK(hipMalloc(&c->d_Lci, sizeof(int) * std::max<int64_t>(c->nL, 1)));
+  HIPCHK(hipMalloc(&c->d_Uci, sizeof(int) * std::max<int64_t>(c->nU, 1)));
+  HIPCHK(hipMalloc(&c->d_L64, sizeof(double) * std::max<int64_t>(c->nL, 1)));
+  HIPCHK(hipMalloc(&c->d_U64, sizeof(double) * std::max<int64_t>(c->nU, 1)));
+  HIPCHK(hipMalloc(&c->d_L32, sizeof(float) * std::max<int64_t>(c->nL, 1)));
+  HIPCHK(hipMalloc(&c->d_U32, sizeof(float) * std::max<int64_t>(c->nU, 1)));
+  HIPCHK(hipMalloc(&c->d_dinv64, sizeof(double) * n));
+  HIPCHK(hipMalloc(&c->d_dinv32, sizeof(float) * n));
+  HIPCHK(hipMalloc(&c->d_lblk, sizeof(int) * std::max<size_t>(lblk.size(), 1)));
+  HIPCHK(hipMalloc(&c->d_ublk, sizeof(int) * std::max<size_t>(ublk.size(), 1)));
+  HIPCHK(hipMemcpyAsync(c->d_Lrp, Lrp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_Urp, Urp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
+  if (c->nL) HIPCHK(hipMemcpyAsync(c->d_Lci, Lci.data(), sizeof(int) * c->nL, hipMemcpyHostToDevice, c->stream));
+  if (c->nU) HIPCHK(hipMemcpyAsync(c->d_Uci, Uci.data(), sizeof(int) * c->nU, hipMemcpyHostToDevice, c->stream));
+  if (!lblk.empty()) HIPCHK(hipMemcpyAsync(c->d_lblk, lblk.data(), sizeof(int) * lblk.size(), hipMemcpyHostToDevice, c->stream));
+  if (!ublk.empty()) HIPCHK(hipMemcpyAsync(c->d_ublk, ublk.data(), sizeof(int) * ublk.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -627,6 +939,17 @@ int refactor(shesolve_ctx *c) {
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->fnnz, kBlock * 4)), dim3(kBlock), 0, c->stream,
                      c->fnnz, c->d_f64, c->d_f32);
+  hipLaunchKernelGGL(k_split, dim3(grid_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->n, c->d_frp, c->d_fdg,
+                     c->d_f64, c->d_Lrp, c->d_Urp, c->d_L64, c->d_U64, c->d_dinv64);
+  if (c->nL)
+    hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->nL, kBlock * 4)), dim3(kBlock), 0, c->stream, c->nL,
+                       c->d_L64, c->d_L32);
+  if (c->nU)
+    hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->nU, kBlock * 4)), dim3(kBlock), 0, c->stream, c->nU,
+                       c->d_U64, c->d_U32);
+  hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->n, kBlock * 4)), dim3(kBlock), 0, c->stream,
+                     (int64_t)c->n, c->d_dinv64, c->d_dinv32);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -725,6 +1048,11 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
       *st = 0;
       break;
     }
+    if (!confirm_true && hs.res > 1e4) {
+      // FP32 inner solve diverging: give the refinement loop the chance to reject it
+      *st = 2;
+      break;
+    }
     if (hs.res <= tol) {
       // confirm with the explicit residual r = b - A x
       launch_spmv<TV, T, T>(c, c->lpr, Av, x, r, b, nullptr, nullptr, nullptr);
@@ -803,9 +1131,19 @@ int solve_device(shesolve_ctx *c, const double *b, double *x, int warm, shesolve
       r.status = 0;
     } else {
       r.status = 1;
+      double prev_rn = -1.0;
+      bool fallback = false;
+      double *xprev = c->v64[6];  // idle during the FP32 inner solves
       for (int k = 0; k <= c->prm.max_ir_steps; ++k) {
         double rn = 0;
         if (int e = residual64(c, b, x, r64, &rn)) return e;
+        if (prev_rn >= 0 && !(rn < prev_rn)) {
+          // the last correction did not reduce the FP64 residual: undo it and
+          // finish with FP64 BiCGStab from the best iterate
+          HIPCHK(hipMemcpyAsync(x, xprev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+          fallback = true;
+          break;
+        }
         r.rel_residual = rn / bn;
         if (r.rel_residual <= c->prm.tol) {
           r.status = 0;
@@ -813,6 +1151,8 @@ int solve_device(shesolve_ctx *c, const double *b, double *x, int warm, shesolve
         }
         if (k == c->prm.max_ir_steps || r.iterations >= c->prm.max_iters) break;
         r.ir_steps++;
+        prev_rn = rn;
+        HIPCHK(hipMemcpyAsync(xprev, x, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
         // r32 = r64 / ||r64|| ; d = 0 ; inner solve ; x += ||r|| * d
         hipLaunchKernelGGL(k_cast_scale, dim3(kGrid), dim3(kBlock), 0, c->stream, n, r64, r32, c->d_norm);
         // the correction d (n floats) lives in the FP64 slot 7, idle during the FP32 solve
@@ -820,17 +1160,22 @@ int solve_device(shesolve_ctx *c, const double *b, double *x, int warm, shesolve
         hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)n, dd, 0.0f);
         int st = 1;
         double ires = 0;
-        // keep ||r|| on the host: the inner solve reuses d_norm
         int it = bicgstab_dev<float, float>(c, c->d_v32, c->d_f32, r32, dd, c->prm.inner_tol,
                                             c->prm.max_iters - r.iterations, &st, &ires, false);
         if (it < 0) return it;
         r.iterations += it;
         HIPCHK(hipMemcpyAsync(c->d_norm, &rn, sizeof(double), hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_axpy_f32, dim3(kGrid), dim3(kBlock), 0, c->stream, n, x, dd, c->d_norm);
-        if (st == 2 && it <= 1) {
-          r.status = 2;
-          break;
-        }
+      }
+      if (fallback && r.iterations < c->prm.max_iters) {
+        int st = 1;
+        double res = 0;
+        int it = bicgstab_dev<double, double>(c, c->d_v64, c->d_f64, b, x, c->prm.tol,
+                                              c->prm.max_iters - r.iterations, &st, &res);
+        if (it < 0) return it;
+        r.iterations += it;
+        r.status = st;
+        r.rel_residual = res;
       }
     }
   }
@@ -897,6 +1242,13 @@ int set_matrix_common(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, 
   HIPCHK(hipMemcpyAsync(c->d_v64, val, sizeof(double) * nnz, device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         c->stream));
   c->lpr = c->prm.lanes_per_row > 0 ? c->prm.lanes_per_row : pick_lpr((double)nnz / n);
+  {
+    std::vector<int> ab;
+    make_blocks(c->h_rp.data(), 0, (int)n, ab);
+    c->nablk = (int)ab.size() - 1;
+    HIPCHK(hipMalloc(&c->d_ablk, sizeof(int) * ab.size()));
+    HIPCHK(hipMemcpyAsync(c->d_ablk, ab.data(), sizeof(int) * ab.size(), hipMemcpyHostToDevice, c->stream));
+  }
   if (int e = analyse_and_factor(c)) return e;
   if (int e = upload_values(c)) return e;
   if (int e = ensure_vectors(c)) return e;
@@ -926,7 +1278,10 @@ static int check_params(const shesolve_params *p) {
   if (!(p->tol > 0)) { set_err("tol must be positive"); return 1; }
   if (p->max_iters <= 0) { set_err("max_iters must be positive"); return 1; }
   if (p->precond != SHESOLVE_PRECOND_NONE && p->precond != SHESOLVE_PRECOND_ILU0 &&
-      p->precond != SHESOLVE_PRECOND_BLOCK_ILU0) { set_err("unknown preconditioner %d", p->precond); return 1; }
+      p->precond != SHESOLVE_PRECOND_BLOCK_ILU0 && p->precond != SHESOLVE_PRECOND_MC_ILU0) {
+    set_err("unknown preconditioner %d", p->precond);
+    return 1;
+  }
   if (p->precision != SHESOLVE_FP64 && p->precision != SHESOLVE_FP32_IR) { set_err("unknown precision"); return 1; }
   if (p->lanes_per_row != 0 && (p->lanes_per_row < 2 || p->lanes_per_row > 64 ||
                                 (p->lanes_per_row & (p->lanes_per_row - 1)))) {
@@ -1127,6 +1482,13 @@ int shesolve_get_factors(shesolve_handle h, int64_t *row_ptr, int32_t *col, doub
   std::memcpy(col, h->h_fci.data(), sizeof(int32_t) * h->fnnz);
   HIPCHK(hipMemcpyAsync(val, h->d_f64, sizeof(double) * h->fnnz, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int shesolve_get_ordering(shesolve_handle h, int32_t *perm) {
+  if (!h) return 1;
+  if (!perm) return 2;
+  for (int i = 0; i < h->n; ++i) perm[i] = h->h_perm.empty() ? i : h->h_perm[i];
   return 0;
 }
 

@@ -14,7 +14,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBPATH = os.path.join(PKG, "libshesolve.so")
 
-PRECOND_NONE, PRECOND_ILU0, PRECOND_BLOCK_ILU0 = 0, 1, 3
+PRECOND_NONE, PRECOND_ILU0, PRECOND_BLOCK_ILU0, PRECOND_MC_ILU0 = 0, 1, 3, 4
 FP64, FP32_IR = 0, 1
 
 # symbols declared in include/shesolve.h
@@ -24,6 +24,7 @@ EXPORTS = (
     "shesolve_update_values_device", "shesolve_solve", "shesolve_solve_device", "shesolve_spmv",
     "shesolve_spmv_f32", "shesolve_precond_apply", "shesolve_factor_nnz", "shesolve_get_factors",
     "shesolve_levels", "shesolve_set_profiling", "shesolve_kernel_stats", "shesolve_time_spmv",
+    "shesolve_get_ordering",
 )
 
 
@@ -78,6 +79,7 @@ def lib():
     L.shesolve_factor_nnz.argtypes = [C.c_void_p]
     L.shesolve_get_factors.argtypes = [C.c_void_p, _i64p, _i32p, _f64p]
     L.shesolve_levels.argtypes = [C.c_void_p, _i64p]
+    L.shesolve_get_ordering.argtypes = [C.c_void_p, _i32p]
     L.shesolve_set_profiling.argtypes = [C.c_void_p, C.c_int]
     L.shesolve_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.shesolve_time_spmv.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
@@ -192,6 +194,11 @@ class Solver:
         _check(lib().shesolve_get_factors(self._h, rp, ci, v), "shesolve_get_factors")
         return rp, ci[:nnz], v[:nnz]
 
+    def ordering(self):
+        p = np.zeros(self.n, np.int32)
+        _check(lib().shesolve_get_ordering(self._h, p), "shesolve_get_ordering")
+        return p
+
     def levels(self):
         o = np.zeros(2, np.int64)
         _check(lib().shesolve_levels(self._h, o), "shesolve_levels")
@@ -215,7 +222,7 @@ class Solver:
 # ---------------------------------------------------------------- coefficient fill (include/shefill.h)
 FILL_EXPORTS = ("shefill_last_error", "shefill_create", "shefill_destroy", "shefill_build", "shefill_sizes",
                 "shefill_device_system", "shefill_get_system", "shefill_block_ranges", "shefill_mapping",
-                "shefill_last_fill_ms")
+                "shefill_last_fill_ms", "shefill_refill")
 
 
 class FillDevice(C.Structure):
@@ -239,6 +246,7 @@ def _fill_lib():
     L.shefill_destroy.argtypes = [C.c_void_p]
     L.shefill_build.argtypes = [C.c_void_p, C.POINTER(FillDevice), C.POINTER(FillConfig)]
     L.shefill_sizes.argtypes = [C.c_void_p, _i64p]
+    L.shefill_refill.argtypes = [C.c_void_p]
     L.shefill_device_system.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
     L.shefill_get_system.argtypes = [C.c_void_p, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p]
@@ -315,6 +323,9 @@ class Fill:
         rhs, sc, cf = np.zeros(n), np.zeros(n), np.zeros(n)
         _fcheck(_fill_lib().shefill_get_system(self._h, rp, ci, v, rhs, sc, cf), "shefill_get_system")
         return rp, ci[:nnz], v[:nnz], rhs, sc, cf
+
+    def refill(self):
+        _fcheck(_fill_lib().shefill_refill(self._h), "shefill_refill")
 
     def block_ranges(self):
         r = np.zeros(2 * self.sizes["NH"], np.int64)

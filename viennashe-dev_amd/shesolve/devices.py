@@ -82,15 +82,79 @@ def built_in_potential(T, dn, dp):
     return np.where((dn <= 0) & (dp <= 0), 0.0, out)
 
 
+EPS_SI = 11.9 * 1.0e7 / (4.0 * 3.1415926535897932384626433832795 * 299792458.0 * 299792458.0)
+
+
+def poisson_potential(dev, quasi_fermi, max_newton=100, tol=1e-10):
+    """Electrostatic potential of a structured device from the nonlinear
+    Poisson equation with Boltzmann carriers at given quasi-Fermi levels:
+        div(eps grad psi) = -q (p - n + N_D - N_A),
+        n = n_aux exp((psi - phi)/V_T),  p = p_aux exp((phi - psi)/V_T),
+    cell-centred finite volumes on the device's own mesh, Dirichlet
+    psi = built-in potential + applied voltage on contact cells, zero flux
+    elsewhere; damped Newton from the built-in guess.  This is the smooth
+    drift-diffusion-like initial guess the SHE solve starts from."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    T = dev.temperature
+    Vt = KB * T / Q
+    naux, paux = _aux(T, 0), _aux(T, 1)
+    nx, ny = dev.nx, dev.ny
+    nc = nx * ny
+    dx = np.diff(dev.xv)
+    dy = np.diff(dev.yv) if dev.dim > 1 else np.array([1.0])
+    xc = 0.5 * (dev.xv[:-1] + dev.xv[1:])
+    yc = 0.5 * (dev.yv[:-1] + dev.yv[1:]) if dev.dim > 1 else np.array([0.0])
+    vol = (dx[None, :] * dy[:, None]).ravel()
+    rows, cols, vals = [], [], []
+    # face couplings eps * area / distance
+    i = np.arange(nc).reshape(ny, nx)
+    if nx > 1:
+        a = i[:, :-1].ravel(); b = i[:, 1:].ravel()
+        area = np.repeat(dy, nx - 1) if dev.dim > 1 else np.ones(nx - 1)
+        g = EPS_SI * area / np.tile(np.diff(xc), ny)
+        rows += [a, b]; cols += [b, a]; vals += [g, g]
+    if dev.dim > 1 and ny > 1:
+        a = i[:-1, :].ravel(); b = i[1:, :].ravel()
+        g = EPS_SI * np.tile(dx, ny - 1) / np.repeat(np.diff(yc), nx)
+        rows += [a, b]; cols += [b, a]; vals += [g, g]
+    rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    G = sp.csr_matrix((vals, (rows, cols)), shape=(nc, nc))
+    Lap = G - sp.diags(np.asarray(G.sum(axis=1)).ravel())
+    C = dev.doping_n - dev.doping_p
+    phi = np.asarray(quasi_fermi, dtype=np.float64)
+    contact = np.asarray(dev.contact, bool)
+    psi = built_in_potential(T, dev.doping_n, dev.doping_p) + phi
+    fixed = psi.copy()
+    free = ~contact
+    for _ in range(max_newton):
+        n = naux * np.exp((psi - phi) / Vt)
+        p = paux * np.exp((phi - psi) / Vt)
+        F = Lap @ psi + Q * vol * (p - n + C)
+        J = Lap + sp.diags(-Q * vol * (p + n) / Vt)
+        Jf = J[free][:, free].tocsc()
+        d = np.zeros(nc)
+        d[free] = spl.spsolve(Jf, -F[free])
+        step = np.max(np.abs(d))
+        if step > 10 * Vt:
+            d *= 10 * Vt / step
+        psi = psi + d
+        psi[contact] = fixed[contact]
+        if step < tol:
+            break
+    return psi
+
+
 def _vertices(start, length, points):
     j = np.arange(points, dtype=np.float64)
     return start + length * j / (float(points) - 1.0)
 
 
 def diode_1d(length_nm=250.0, vertices=251, nplus_frac=0.2, n_plus=1e24, n_low=1e21,
-             bias=0.5, T=300.0):
+             bias=0.5, T=300.0, poisson=False):
     """n+ - n - n+ diode on [0, length]; n+ regions are nplus_frac of the length
-    at each end, the first and last cells are the contacts (0 V and ``bias``)."""
+    at each end, the first and last cells are the contacts (0 V and ``bias``).
+    poisson=True replaces the built-in guess by the nonlinear Poisson potential."""
     L = length_nm * 1e-9
     xv = _vertices(0.0, L, vertices)
     nx = vertices - 1
@@ -100,12 +164,14 @@ def diode_1d(length_nm=250.0, vertices=251, nplus_frac=0.2, n_plus=1e24, n_low=1
     psi = built_in_potential(T, dn, dp) + bias * (xc - xc[0]) / (xc[-1] - xc[0])
     contact = np.zeros(nx, dtype=np.int32)
     contact[0] = contact[-1] = 1
-    return Device(1, nx, 1, xv, np.array([0.0, 1.0]), dn, dp, psi, contact, T,
-                  name=f"diode1d_{vertices}v")
+    dev = Device(1, nx, 1, xv, np.array([0.0, 1.0]), dn, dp, psi, contact, T, name=f"diode1d_{vertices}v")
+    if poisson:
+        dev.potential = poisson_potential(dev, bias * (xc - xc[0]) / (xc[-1] - xc[0]))
+    return dev
 
 
 def mosfet_2d(width_nm=200.0, height_nm=100.0, vx=161, vy=81, pocket_w_nm=50.0,
-              pocket_h_nm=20.0, n_plus=1e26, p_body=1e23, v_drain=0.5, T=300.0):
+              pocket_h_nm=20.0, n_plus=1e26, p_body=1e23, v_drain=0.5, T=300.0, poisson=True):
     """MOSFET-like rectangle: n+ source/drain pockets at the top corners of a
     p-body.  Contacts: top-row cells inside the pockets (source 0 V, drain
     ``v_drain``) and the bottom row (body, 0 V)."""
@@ -124,8 +190,10 @@ def mosfet_2d(width_nm=200.0, height_nm=100.0, vx=161, vy=81, pocket_w_nm=50.0,
     contact = np.zeros((ny, nx), dtype=np.int32)
     contact[ny - 1, :] = np.where(pocket[ny - 1, :], 1, 0)
     contact[0, :] = 1
-    return Device(2, nx, ny, xv, yv, dn, dp, psi, contact.ravel().astype(np.int32), T,
-                  name=f"mosfet2d_{vx}x{vy}")
+    dev = Device(2, nx, ny, xv, yv, dn, dp, psi, contact.ravel().astype(np.int32), T, name=f"mosfet2d_{vx}x{vy}")
+    if poisson:
+        dev.potential = poisson_potential(dev, v_drain * ((X - xc[0]) / (xc[-1] - xc[0])).ravel())
+    return dev
 
 
 def nin_2d(width_nm=100.0, height_nm=40.0, vx=41, vy=17, nplus_frac=0.2, n_plus=1e24, n_low=1e21,
