@@ -57,7 +57,7 @@ typedef struct {
   int max_iters;         /* BiCGStab iteration cap per (inner) solve, default 1000 */
   int precond;           /* SHESOLVE_PRECOND_* */
   int precision;         /* SHESOLVE_FP64 or SHESOLVE_FP32_IR */
-  double inner_tol;      /* relative reduction of each FP32 inner solve, default 1e-5 */
+  double inner_tol;      /* relative reduction of each FP32 inner solve, default 1e-2 */
   int max_ir_steps;      /* cap on refinement steps, default 30 */
   int lanes_per_row;     /* SpMV lanes per row (0 = pick from nnz/row) */
 } shesolve_params;

@@ -164,7 +164,25 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, const int *
     const int rs = blocks[bk], re = blocks[bk + 1];
     const int k0 = rp[rs], k1 = rp[re];
     if (k1 - k0 <= kStreamNnz) {
-      for (int k = k0 + (int)threadIdx.x; k < k1; k += kStreamThreads) prod[k - k0] = (T)v[k] * x[ci[k]];
+      // all column indices and values first, then the gathers: 8 independent
+      // loads in flight per thread
+      constexpr int kItems = kStreamNnz / kStreamThreads;
+      const int cnt = k1 - k0;
+      int cc[kItems];
+      TV vv[kItems];
+#pragma unroll
+      for (int u = 0; u < kItems; ++u) {
+        const int k = (int)threadIdx.x + u * kStreamThreads;
+        if (k < cnt) {
+          cc[u] = __builtin_nontemporal_load(ci + k0 + k);
+          vv[u] = __builtin_nontemporal_load(v + k0 + k);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kItems; ++u) {
+        const int k = (int)threadIdx.x + u * kStreamThreads;
+        if (k < cnt) prod[k] = (T)vv[u] * x[cc[u]];
+      }
       __syncthreads();
       for (int row = rs + (int)threadIdx.x; row < re; row += kStreamThreads) {
         T acc = 0;
@@ -531,7 +549,7 @@ struct shesolve_ctx {
   double *z64 = nullptr;            // permuted-space work vectors of the sweeps
   float *z32 = nullptr;
   int ncolors = 0;
-  std::vector<int> h_perm;
+  std::vector<int> h_perm, h_colour;
   // CSR-stream row blocks of A
   int *d_ablk = nullptr;
   int nablk = 0;
@@ -762,9 +780,19 @@ int analyse_and_factor(shesolve_ctx *c) {
       c->h_frp[i + 1] = (int)c->h_fci.size();
     }
   } else if (c->prm.precond == SHESOLVE_PRECOND_MC_ILU0) {
-    // greedy colouring of the rows in natural order (colours of already
-    // coloured neighbours in the row are excluded); rows sorted by colour,
-    // stable, give the multicolour ordering new -> old
+    // greedy colouring of the rows in natural order on the symmetrised
+    // pattern: rows of one colour are mutually independent in both
+    // triangles, so each colour is one level of both sweeps; rows sorted by
+    // colour (stable) give the multicolour ordering new -> old
+    std::vector<int> trp(n + 1, 0), tci(c->h_ci.size());
+    for (int i = 0; i < n; ++i)
+      for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) trp[c->h_ci[k] + 1]++;
+    for (int i = 0; i < n; ++i) trp[i + 1] += trp[i];
+    {
+      std::vector<int> fillp(trp.begin(), trp.end() - 1);
+      for (int i = 0; i < n; ++i)
+        for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) tci[fillp[c->h_ci[k]]++] = i;
+    }
     std::vector<int> colour(n, -1);
     std::vector<int> mark(1024, -1);
     int ncol = 0;
@@ -773,12 +801,17 @@ int analyse_and_factor(shesolve_ctx *c) {
         const int cj = colour[c->h_ci[k]];
         if (cj >= 0) mark[cj] = i;
       }
+      for (int k = trp[i]; k < trp[i + 1]; ++k) {
+        const int cj = colour[tci[k]];
+        if (cj >= 0) mark[cj] = i;
+      }
       int q = 0;
       while (q < 1023 && mark[q] == i) ++q;
       colour[i] = q;
       ncol = std::max(ncol, q + 1);
     }
     c->ncolors = ncol;
+    c->h_colour = colour;
     std::vector<int> start(ncol + 1, 0);
     for (int i = 0; i < n; ++i) start[colour[i] + 1]++;
     for (int q = 0; q < ncol; ++q) start[q + 1] += start[q];
@@ -830,6 +863,34 @@ int analyse_and_factor(shesolve_ctx *c) {
   }
   std::vector<int> Lptr, Lrows, Uptr, Urows;
   compute_levels(c->h_frp, c->h_fci, c->h_fdg, n, Lptr, Lrows, Uptr, Urows);
+  if (!perm.empty()) {
+    // multicolour ordering: colour q occupies the contiguous range
+    // [cstart[q], cstart[q+1]) of the factor; use whole colours as levels
+    // (forward: ascending colours, backward: descending) when every factor
+    // entry links different colours
+    std::vector<int> cstart(1, 0), colour_of(n);
+    for (int p = 0; p < n; ++p) {
+      if (p > 0 && c->h_colour[perm[p]] != c->h_colour[perm[p - 1]]) cstart.push_back(p);
+      colour_of[p] = (int)cstart.size() - 1;
+    }
+    cstart.push_back(n);
+    bool valid = true;
+    for (int p = 0; p < n && valid; ++p)
+      for (int k = c->h_frp[p]; k < c->h_frp[p + 1]; ++k)
+        if (c->h_fci[k] != p && colour_of[c->h_fci[k]] == colour_of[p]) { valid = false; break; }
+    if (valid) {
+      const int nq = (int)cstart.size() - 1;
+      Lptr.assign(cstart.begin(), cstart.end());
+      Lrows.resize(n);
+      for (int p = 0; p < n; ++p) Lrows[p] = p;
+      Uptr.assign(1, 0);
+      Urows.clear();
+      for (int q = nq - 1; q >= 0; --q) {
+        for (int p = cstart[q]; p < cstart[q + 1]; ++p) Urows.push_back(p);
+        Uptr.push_back((int)Urows.size());
+      }
+    }
+  }
   c->lvL.ptr = Lptr;
   c->lvU.ptr = Uptr;
   HIPCHK(hipMalloc(&c->lvL.d_rows, sizeof(int) * std::max(n, 1)));
@@ -1267,7 +1328,7 @@ void shesolve_default_params(shesolve_params *p) {
   p->max_iters = 1000;
   p->precond = SHESOLVE_PRECOND_ILU0;
   p->precision = SHESOLVE_FP64;
-  p->inner_tol = 1e-5;
+  p->inner_tol = 1e-2;
   p->max_ir_steps = 30;
   p->lanes_per_row = 0;
 }
