@@ -113,6 +113,11 @@ int shesolve_precond_apply(shesolve_handle h, const double *r, double *z);
  * by shesolve_factor_nnz. */
 int64_t shesolve_factor_nnz(shesolve_handle h);
 int shesolve_get_factors(shesolve_handle h, int64_t *row_ptr, int32_t *col, double *val);
+/* Launch shape the setup autotuner picked for the CSR-stream kernels: entries
+ * per thread (row block = 256 * items entries), XCD-contiguous block mapping
+ * (0/1), and the six timings it compared (FP32 SpMV, microseconds; order
+ * items 4/8/16 x mapping strided/contiguous).  Any pointer may be NULL. */
+int shesolve_stream_config(shesolve_handle h, int32_t *items, int32_t *xcd, double *tune_us6);
 /* Row ordering of the factor (new -> old); the identity unless MC_ILU0. */
 int shesolve_get_ordering(shesolve_handle h, int32_t *perm);
 /* Level-schedule statistics: [0] forward levels, [1] backward levels. */

@@ -80,6 +80,7 @@ def lib():
     L.oracle_precond_factors.argtypes = [C.c_void_p, _i64p, _i32p, _f64p]
     L.oracle_precond_free.argtypes = [C.c_void_p]
     L.oracle_spmv.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, _f64p]
+    L.oracle_dispersion.argtypes = [C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.oracle_bicgstab.restype = C.c_int
     L.oracle_bicgstab.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, _f64p, C.c_void_p, C.c_double,
                                   C.c_int, C.c_double, C.POINTER(C.c_double)]
@@ -99,6 +100,12 @@ def coupling_tables():
 
 
 NAMES = ("a_x", "a_y", "a_z", "b_x", "b_y", "b_z")
+
+
+def dispersion(carrier, eps):
+    z, v = C.c_double(), C.c_double()
+    lib().oracle_dispersion(carrier, eps, C.byref(z), C.byref(v))
+    return z.value, v.value
 
 
 class CSR:

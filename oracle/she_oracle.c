@@ -1104,6 +1104,11 @@ struct oracle_precond {
   int64_t *diag;   /* position of the diagonal in each row */
 };
 
+void oracle_dispersion(int carrier, double eps, double *Z, double *v) {
+  *Z = dos(carrier, eps);
+  *v = velocity(carrier, eps);
+}
+
 void oracle_spmv(int64_t n, const int64_t *rp, const int32_t *ci, const double *v, const double *x, double *y) {
   for (int64_t i = 0; i < n; ++i) {
     double s = 0.0;
@@ -1151,7 +1156,7 @@ static int ilut_factor(oracle_precond *p, int64_t n, const int64_t *rp, const in
   double *w = (double *)calloc((size_t)n, sizeof(double));
   char *nz = (char *)calloc((size_t)n, 1);
   int32_t *lst = (int32_t *)malloc((size_t)n * sizeof(int32_t));
-  jv *buf = (jv *)malloc((size_t)n * sizeof(jv));
+  jv *buf = (jv *)malloc((size_t)(2 * n + 2) * sizeof(jv));
   int64_t cap = rp[n] * 2 + n + 16, used = 0;
   p->rp = (int64_t *)malloc((size_t)(n + 1) * sizeof(int64_t));
   p->ci = (int32_t *)malloc((size_t)cap * sizeof(int32_t));

@@ -117,6 +117,9 @@ double oracle_finish(const oracle_system *s, const double *x_reduced, double *x_
 void oracle_postproc(const oracle_system *s, const double *x_full, double *density,
                      double *facet_current);
 
+/* Modena density of states [1/(J m^3)] and velocity [m/s] at kinetic energy eps [J]. */
+void oracle_dispersion(int carrier, double eps, double *Z, double *v);
+
 /* ---------------- solvers on CSR (FP64) ---------------- */
 enum { ORACLE_PC_NONE = 0, ORACLE_PC_ILU0 = 1, ORACLE_PC_ILUT = 2, ORACLE_PC_BLOCK_ILU0 = 3 };
 

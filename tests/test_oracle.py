@@ -1,0 +1,334 @@
+"""Pins of the CPU oracle (no GPU).
+
+The oracle restates the reference's SHE assembly, elimination, normalisation
+and ILU/BiCGStab solve.  Each check below is independent of the oracle's own
+code: golden data held by the reference (coupling tables, recorded dispersion
+samples), closed forms, textbook identities (ILU0 reproduces A on its
+pattern, exact LU on a tridiagonal matrix), direct solves, and the physical
+invariants the reference's own tests check (Maxwellian equilibrium,
+tests/src/equilibrium_resistor.cpp:121-197; current conservation,
+tests/src/ushape_2d.cpp:196-204).
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+from scipy.special import lpmv
+
+from conftest import oracle_system
+
+Q = 1.602176565e-19
+KB = 1.3806488e-23
+
+
+# ------------------------------------------------------------------ coupling tables
+def _Y(n, m, th, ph):
+    """Real spherical harmonic as math/spherical_harmonics.hpp:98-120 defines it."""
+    am = abs(m)
+    norm = math.sqrt((2 * n + 1) * math.factorial(n - am) / (2 * math.pi * math.factorial(n + am)))
+    P = lpmv(am, n, np.cos(th))
+    if m > 0:
+        return norm * P * np.cos(m * ph)
+    if m == 0:
+        return norm * P / math.sqrt(2.0)
+    return norm * P * np.sin(m * ph)
+
+
+def _quadrature_tables(L):
+    """a and b coupling integrals (harmonics_coupling.hpp:35-205) by Gauss-Legendre
+    in theta and the trapezoid rule in phi (exact for the trigonometric part)."""
+    xg, wg = np.polynomial.legendre.leggauss(64)
+    th = 0.5 * np.pi * (xg + 1.0)
+    wt = 0.5 * np.pi * wg
+    nph = 64
+    ph = 2 * np.pi * np.arange(nph) / nph
+    wp = 2 * np.pi / nph
+    TH, PH = np.meshgrid(th, ph, indexing="ij")
+    W = wt[:, None] * wp
+    K = (L + 1) ** 2
+    idx = [(l, m) for l in range(L + 1) for m in range(-l, l + 1)]
+    Y = {(l, m): _Y(l, m, TH, PH) for (l, m) in idx}
+    h = 1e-6
+    dth = {(l, m): (_Y(l, m, TH + h, PH) - _Y(l, m, TH - h, PH)) / (2 * h) for (l, m) in idx}
+    dph = {(l, m): (m * _Y(l, -m, TH, PH) if m != 0 else 0.0 * TH) for (l, m) in idx}
+    S, C_, St, Ct = np.sin(TH), np.cos(TH), np.sin(PH), np.cos(PH)
+    out = {k: np.zeros((K, K)) for k in ("a_x", "a_y", "a_z", "b_x", "b_y", "b_z")}
+    for (l, m) in idx:
+        i = l * l + l + m
+        for (l1, m1) in idx:
+            if abs(l - l1) != 1:
+                continue
+            j = l1 * l1 + l1 + m1
+            Y1 = Y[(l1, m1)]
+            out["a_x"][i, j] = np.sum(W * Y[(l, m)] * S * Ct * Y1 * S)
+            out["a_y"][i, j] = np.sum(W * Y[(l, m)] * S * St * Y1 * S)
+            out["a_z"][i, j] = np.sum(W * Y[(l, m)] * C_ * Y1 * S)
+            out["b_x"][i, j] = np.sum(W * (dth[(l, m)] * C_ * Ct * S - dph[(l, m)] * St) * Y1)
+            out["b_y"][i, j] = np.sum(W * (dth[(l, m)] * C_ * St * S + dph[(l, m)] * Ct) * Y1)
+            out["b_z"][i, j] = np.sum(W * (dth[(l, m)] * (-1.0) * S) * Y1 * S)
+    return out
+
+
+@pytest.mark.parametrize("L", [1, 3])
+def test_coupling_fixture_is_the_coupling_integral(L):
+    from oracle import oracle as O
+    tab = O.coupling_tables()[str(L)]
+    ref = _quadrature_tables(L)
+    K = (L + 1) ** 2
+    for name, entries in tab.items():
+        M = np.zeros((K, K))
+        for i, j, v in entries:
+            assert M[i, j] == 0.0, f"{name}({i},{j}) assigned twice"
+            M[i, j] = v
+        Q_ = ref[name]
+        # every stored constant is the integral (the tables carry ~1e-9 quadrature error)
+        assert np.max(np.abs(M - np.where(M != 0, Q_, 0.0))) <= 2e-7, name
+        # and every integral above 1e-8 is stored, except the l=0 row of b whose
+        # derivative vanishes (b(0, l=1) = 0) -- nothing is missing or transposed
+        missing = (np.abs(Q_) > 1e-6) & (M == 0.0)
+        assert not missing.any(), (name, np.argwhere(missing)[:5])
+
+
+def test_dispersion_known_values():
+    """Modena band at 0.1 eV: density of states and velocity recorded from the
+    compiled reference headers (1.3717e45, 2.783e5 m/s), and the closed form."""
+    from oracle import oracle as O
+    Z, v = O.dispersion(0, 0.1 * Q)
+    assert abs(Z / 1.3717e45 - 1) < 1e-4
+    assert abs(v / 2.783e5 - 1) < 2e-4
+    hbar = 6.62606896e-34 / (2 * math.pi)
+    m0 = 9.10938291e-31
+    md, ml, mt = 0.32 * m0, 0.98 * m0, 0.19 * m0
+    mc = 3.0 / (1 / ml + 2 / mt)
+    a, e = 0.5 / Q, 0.1 * Q
+    Zc = md * math.sqrt(2 * md) / (2 * math.pi ** 2 * hbar ** 3) * (1 + 2 * a * e) * math.sqrt(e * (1 + a * e))
+    vc = math.sqrt(mc / md) * math.sqrt(2 * e * (1 + a * e) / md) / (1 + 2 * a * e)
+    assert abs(Z / Zc - 1) < 1e-13 and abs(v / vc - 1) < 1e-13
+    assert O.dispersion(0, -1e-21) == (0.0, 0.0)
+
+
+# ------------------------------------------------------------------ solvers
+def _tridiag(n, seed=0):
+    rng = np.random.default_rng(seed)
+    d = 4.0 + rng.random(n)
+    lo, up = -1.0 - rng.random(n - 1), -1.0 - rng.random(n - 1)
+    return sp.diags([lo, d, up], [-1, 0, 1]).tocsr()
+
+
+def _csr(M):
+    from oracle import oracle as O
+    M = sp.csr_matrix(M)
+    M.sort_indices()
+    return O.CSR(M.indptr, M.indices, M.data)
+
+
+def test_ilu0_is_exact_lu_on_tridiagonal():
+    from oracle import oracle as O
+    M = _tridiag(200)
+    A = _csr(M)
+    b = np.arange(200.0) + 1
+    P = O.Precond(O.PC_ILU0, A)
+    x, it, res = O.bicgstab(A, b, precond=P, tol=1e-13)
+    assert it == 1 and res <= 1e-13
+    assert np.allclose(x, spl.spsolve(M.tocsc(), b), rtol=1e-12)
+
+
+def test_block_ilu0_exact_on_block_diagonal_and_ilut_full_fill_is_lu():
+    from oracle import oracle as O
+    M = sp.block_diag([_tridiag(50, 1), _tridiag(70, 2)]).tocsr()
+    A = _csr(M)
+    b = np.ones(120)
+    P = O.Precond(O.PC_BLOCK_ILU0, A, ranges=[[0, 50], [50, 120]])
+    x, it, res = O.bicgstab(A, b, precond=P, tol=1e-13)
+    assert it == 1
+    # a dense-ish random matrix: ILUT without dropping equals LU
+    rng = np.random.default_rng(3)
+    D = sp.random(60, 60, density=0.2, random_state=3) + sp.eye(60) * 8
+    A2 = _csr(D)
+    P2 = O.Precond(O.PC_ILUT, A2, drop=0.0, fill=60)
+    x, it, res = O.bicgstab(A2, rng.random(60), precond=P2, tol=1e-12)
+    assert it == 1 and res <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["cfg1", "diode_L3", "mos2d_L3"])
+def test_ilu0_reproduces_A_on_its_pattern(name):
+    """ILU(0) defining property: (L U)_ij = a_ij for every (i, j) in the pattern of A."""
+    from oracle import oracle as O
+    s = oracle_system(name)
+    A, b, _, _ = s.reduced()
+    F = O.Precond(O.PC_ILU0, A).factors().to_scipy().tocsr()
+    assert np.array_equal(F.indptr, A.row_ptr) and np.array_equal(F.indices, A.col)
+    L = sp.tril(F, -1) + sp.eye(A.n)
+    U = sp.triu(F)
+    LU = (L @ U).tocsr()
+    M = A.to_scipy().tocsr()
+    vals = np.asarray(LU[M.nonzero()]).ravel()
+    ref = M.data
+    scale = np.abs(L).max() * np.abs(U).max()
+    assert np.max(np.abs(vals - ref)) <= 1e-12 * max(scale, 1.0)
+
+
+@pytest.mark.parametrize("name", ["cfg1", "diode_L3", "mos2d_L3"])
+def test_bicgstab_matches_direct_solve(name):
+    from oracle import oracle as O
+    s = oracle_system(name)
+    A, b, _, _ = s.reduced()
+    x, it, res = O.bicgstab(A, b, precond=O.Precond(O.PC_ILU0, A), tol=1e-13, max_iters=5000)
+    xd = spl.spsolve(A.to_scipy().tocsc(), b)
+    assert res <= 1e-13
+    assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
+
+
+# ------------------------------------------------------------------ front end
+@pytest.mark.parametrize("name", ["cfg1", "diode_L3", "mos2d_L3"])
+def test_elimination_and_recovery_solve_the_full_system(name):
+    """recover(eliminate(A)) gives the solution of the full even+odd system, and
+    the odd-odd block is diagonal (she/eliminate.hpp:80-354)."""
+    s = oracle_system(name)
+    A, b, _, _ = s.reduced()
+    xd = spl.spsolve(A.to_scipy().tocsc(), b)
+    xf, res = s.finish(xd)
+    assert res <= 1e-10
+    Fm, fb = s.full()
+    M = Fm.to_scipy().tocsr()
+    ne = s.n_even
+    oo = M[ne:, ne:].tocoo()
+    assert np.all(oo.row == oo.col) and np.all(oo.data > 0)
+    assert np.linalg.norm(M @ xf - fb) <= 1e-10 * np.linalg.norm(fb)
+
+
+@pytest.mark.parametrize("name", ["cfg1", "mos2d_L3"])
+def test_normalisation_identities(name):
+    """row_normalize_system (math/linalg_util.hpp:200-282): A = R * A0 * S * C with
+    S the unknown scaling exp(-E_kin/kT), C the column factors (geometric mean of
+    each column set to one), R the inverse row 2-norms; rows end with unit norm
+    and positive diagonal."""
+    s = oracle_system(name)
+    A, b, sc, cf = s.reduced()
+    A0, b0 = s.unscaled()
+    M, M0 = A.to_scipy().tocsr(), A0.to_scipy().tocsr()
+    rows = np.sqrt(np.asarray(M.multiply(M).sum(axis=1)).ravel())
+    assert np.allclose(rows, 1.0, rtol=1e-12)
+    assert np.all(M.diagonal() > 0)
+    Cs = M0 @ sp.diags(sc * cf)
+    rn = np.sqrt(np.asarray(Cs.multiply(Cs).sum(axis=1)).ravel())
+    R = sp.diags(1.0 / rn)
+    assert abs(R @ Cs - M).max() <= 1e-12
+    assert np.allclose(b, b0 / rn, rtol=1e-12, atol=0)
+    # geometric mean of |column| after unknown scaling and column factor = 1
+    Sc = (M0 @ sp.diags(sc * cf)).tocsc()
+    for j in np.random.default_rng(0).choice(A.n, 200, replace=False):
+        col = Sc.data[Sc.indptr[j]:Sc.indptr[j + 1]]
+        assert abs(np.mean(np.log(np.abs(col)))) <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["cfg1", "diode_L3", "nin2d_L3", "mos2d_L3"])
+def test_mapping_rules(name):
+    """Energy-major numbering (mapping.hpp:204-271): H outer, cells inner, no
+    unknowns at H = 0 and N_H - 1, none where the averaged density of states is
+    zero; even_unknowns_on_node(L) per node; orders L_max or 1 (band edge)."""
+    s = oracle_system(name)
+    ei, eo, oi, oo = s.mapping()
+    H, sh = s.energies()
+    kin = H[None, :] - sh[:, None]
+    assert np.all(ei[:, 0] < 0) and np.all(ei[:, -1] < 0)
+    nodes = [(h, c) for h in range(s.NH) for c in range(s.ncells) if ei[c, h] >= 0]
+    expect = 0
+    for h, c in nodes:
+        assert ei[c, h] == expect
+        L = eo[c, h]
+        assert L in (1, s.cfg.L_max) or (s.cfg.L_max >= 5 and L in (3,))
+        expect += sum(2 * l + 1 for l in range(0, L + 1, 2))
+        # some point of the energy box is above the band edge
+        assert max(kin[c, h], (kin[c, h] + kin[c, h + 1]) / 2) > 0
+    assert expect == s.n_even
+    # cells strictly inside the band (all three box points below zero) carry nothing
+    inside_gap = (kin[:, 1:-1] <= 0) & (kin[:, 2:] + kin[:, 1:-1] <= 0) & (kin[:, :-2] + kin[:, 1:-1] <= 0)
+    assert np.all(ei[:, 1:-1][inside_gap] < 0)
+
+
+@pytest.mark.parametrize("name", ["cfg1", "mos2d_L3"])
+def test_pattern_couples_only_facet_neighbours_and_phonon_partners(name):
+    """Reduced rows couple a node to itself, its facet neighbours at the same H,
+    and (harmonic 0 only) to the same cell at H +- hbar*omega/dE."""
+    s = oracle_system(name)
+    A, b, _, _ = s.reduced()
+    ei, eo, _, _ = s.mapping()
+    n = A.n
+    node_c = np.zeros(n, np.int64)
+    node_h = np.zeros(n, np.int64)
+    harm = np.zeros(n, np.int64)
+    for c in range(s.ncells):
+        for h in range(s.NH):
+            u = ei[c, h]
+            if u >= 0:
+                e = sum(2 * l + 1 for l in range(0, eo[c, h] + 1, 2))
+                node_c[u:u + e] = c
+                node_h[u:u + e] = h
+                harm[u:u + e] = np.arange(e)
+    k = round(0.062 / (s.cfg.energy_spacing / Q))
+    M = A.to_scipy().tocoo()
+    r, c = M.row, M.col
+    same_h = node_h[r] == node_h[c]
+    nx = s.dev.nx
+    dc = np.abs(node_c[r] - node_c[c])
+    neighbour = (dc == 0) | (dc == 1) | (dc == nx)
+    assert np.all(neighbour[same_h])
+    other = ~same_h
+    assert np.all(node_c[r][other] == node_c[c][other])
+    assert np.all(np.abs(node_h[r] - node_h[c])[other] == k)
+    assert np.all((harm[r] == 0)[other] & (harm[c] == 0)[other])
+
+
+# ------------------------------------------------------------------ physics invariants
+def _resistor(dim, T, dE):
+    from shesolve import devices as D
+    if dim == 1:
+        dev = D.diode_1d(length_nm=60.0, vertices=31, n_plus=1e23, n_low=1e23, bias=0.0, T=T)
+    else:
+        dev = D.nin_2d(width_nm=100.0, height_nm=20.0, vx=11, vy=3, n_plus=1e23, n_low=1e23, bias=0.0, T=T)
+    cfg = D.SheConfig(L_max=1, energy_spacing=dE * Q, min_kin_range=0.5 * Q)
+    return dev, cfg
+
+
+@pytest.mark.parametrize("dim,T", [(1, 300.0), (2, 300.0), (2, 400.0)])
+def test_equilibrium_is_maxwellian(dim, T):
+    """Zero bias, uniform doping: f_00(H) proportional to exp(-E_kin/kT) to 1e-3 and
+    the odd (current-carrying) coefficients vanish (equilibrium_resistor.cpp:121-197)."""
+    import scipy.sparse.linalg as spl2
+    from oracle import oracle as O
+    dev, cfg = _resistor(dim, T, 0.0062)
+    s = O.OracleSystem(dev, cfg)
+    A, b, _, _ = s.reduced()
+    x = spl2.spsolve(A.to_scipy().tocsc(), b)
+    xf, res = s.finish(x)
+    ei, eo, oi, oo = s.mapping()
+    H, sh = s.energies()
+    ratios = []
+    for c in range(s.ncells):
+        for h in range(s.NH):
+            if ei[c, h] >= 0:
+                ratios.append(xf[ei[c, h]] / math.exp(-(H[h] - sh[c]) / (KB * T)))
+    ratios = np.array(ratios)
+    assert np.max(np.abs(ratios / np.median(ratios) - 1)) <= 1e-3
+    odd = xf[s.n_even:]
+    assert np.max(np.abs(odd)) <= 1e-12 * np.max(np.abs(xf[:s.n_even]))
+
+
+@pytest.mark.parametrize("name", ["cfg1", "diode_L3", "nin2d_L3"])
+def test_current_conservation(name):
+    """Terminal currents of the two contacts cancel to 1e-5 (ushape_2d.cpp:196-204)
+    and the contact density follows the Maxwellian boundary normalisation."""
+    from test_gpu_solver import terminal_currents
+    s = oracle_system(name)
+    A, b, _, _ = s.reduced()
+    x = spl.spsolve(A.to_scipy().tocsc(), b)
+    xf, _ = s.finish(x)
+    n, j = s.postproc(xf)
+    I = terminal_currents(s, j)
+    assert abs(I[0] + I[1]) <= 1e-5 * np.max(np.abs(I))
+    # majority density in the contact cells is the doping (G/R contacts, tau = 1e-16 s)
+    ct = np.nonzero(s.dev.contact)[0]
+    assert np.allclose(n[ct], s.dev.doping_n[ct], rtol=2e-2)

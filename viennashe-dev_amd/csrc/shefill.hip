@@ -315,18 +315,20 @@ __global__ __launch_bounds__(kB) void k_rows(DevTables t, RowArgs a) {
   a.rhs[row] = (r == 0) ? a.rhs0[nd] : 0.0;
 }
 
-// unknown scaling of the columns + log-magnitude sums per column
-__global__ void k_scale_log(int n, const int *rp, const int *ci, const int *diag_pos, const unsigned char *diag_struct,
-                            const double *scal, double *val, double *logsum) {
+// unknown scaling of the columns (she/rescaling.hpp:89-107)
+__global__ void k_scale(int n, const int *rp, const int *ci, const double *scal, double *val) {
   int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n) return;
-  for (int j = rp[row]; j < rp[row + 1]; ++j) {
-    const int col = ci[j];
-    double v = val[j] * scal[col];
-    val[j] = v;
-    if (j == diag_pos[row] && !diag_struct[row]) continue;
-    atomicAdd(&logsum[col], log(fabs(v)));
-  }
+  for (int j = rp[row]; j < rp[row + 1]; ++j) val[j] *= scal[ci[j]];
+}
+// log-magnitude sum of the structural entries of each column, in ascending
+// row order (the order of the transposed std::map rows)
+__global__ void k_collog(int n, const int *cptr, const int *cpos, const double *val, double *logsum) {
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n) return;
+  double s = 0.0;
+  for (int k = cptr[col]; k < cptr[col + 1]; ++k) s += log(fabs(val[cpos[k]]));
+  logsum[col] = s;
 }
 __global__ void k_colnorm(int n, const int *colcount, const int *diag_pos, const double *val, double *logsum,
                           double *colfac) {
@@ -399,14 +401,14 @@ struct shefill_ctx {
   std::vector<int> nd_c, nd_H, nd_base, nd_L, nd_fn, nd_nb, nd_inn, nd_in, nd_out;
   std::vector<int> fn_f, fn_H, fn_L, fn_out;
   int64_t n = 0, n_odd = 0, nnz = 0;
-  std::vector<int> rp, ci, row_node, diag_pos, colcount;
+  std::vector<int> rp, ci, row_node, diag_pos, colcount, cptr, cpos;
   std::vector<unsigned char> diag_struct;
   std::vector<int64_t> blocks;
   // device
   DBuf<double> dHval, dshc, dshf, dcvol, dfvol, dflen, ddn, ddp, dAe, dBe, dAo, dBo;
   DBuf<int> dcontact, dfc1, dfc2, dev_h, dod_h, dev_off, dod_off;
   DBuf<int> dnd_c, dnd_H, dnd_base, dnd_L, dnd_fn, dnd_nb, dnd_inn, dnd_in, dnd_out, dfn_f, dfn_H, dfn_L, dfn_out;
-  DBuf<int> drp, dci, drow_node, ddiag_pos, dcolcount;
+  DBuf<int> drp, dci, drow_node, ddiag_pos, dcolcount, dcptr, dcpos;
   DBuf<unsigned char> ddiag_struct;
   DBuf<double> dcorr, dncont, dcdiff, dcdrift, dodiag, ddiag0, ddiagr, drhs0, dinco;
   DBuf<double> dval, drhs, dscal, dcolfac, dlog;
@@ -838,6 +840,16 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
   for (int64_t i = 0; i < n; ++i)
     for (int j = c->rp[i]; j < c->rp[i + 1]; ++j)
       if (j != c->diag_pos[i] || c->diag_struct[i]) c->colcount[c->ci[j]]++;
+  // column-ordered positions of the structural entries (rows ascending)
+  c->cptr.assign(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) c->cptr[i + 1] = c->cptr[i] + c->colcount[i];
+  c->cpos.assign(c->cptr[n], 0);
+  {
+    std::vector<int> fillp(c->cptr.begin(), c->cptr.end() - 1);
+    for (int64_t i = 0; i < n; ++i)
+      for (int j = c->rp[i]; j < c->rp[i + 1]; ++j)
+        if (j != c->diag_pos[i] || c->diag_struct[i]) c->cpos[fillp[c->ci[j]]++] = j;
+  }
   // block ranges (she/linear_solver.hpp:42-83)
   c->blocks.assign((size_t)2 * NH, 0);
   int64_t last = 0;
@@ -918,6 +930,8 @@ int fill_values(shefill_ctx *c, Builder &B) {
   e = e ? e : upload(c, c->drow_node, c->row_node);
   e = e ? e : upload(c, c->ddiag_pos, c->diag_pos);
   e = e ? e : upload(c, c->dcolcount, c->colcount);
+  e = e ? e : upload(c, c->dcptr, c->cptr);
+  e = e ? e : upload(c, c->dcpos, c->cpos);
   e = e ? e : upload(c, c->ddiag_struct, c->diag_struct);
   const int nn = (int)c->nd_c.size(), nfn = (int)c->fn_f.size();
   const int64_t n = c->n;
@@ -983,9 +997,10 @@ int launch_fill(shefill_ctx *c) {
     hipLaunchKernelGGL(k_rows, dim3(grid(n)), dim3(kB), 0, c->stream, t, a);
     hipLaunchKernelGGL(k_node_scaling, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->drow_node.p, c->dnd_c.p,
                        c->dnd_H.p, t, c->cfg.scale, c->dscal.p);
-    FHIP(hipMemsetAsync(c->dlog.p, 0, sizeof(double) * n, c->stream));
-    hipLaunchKernelGGL(k_scale_log, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->drp.p, c->dci.p,
-                       c->ddiag_pos.p, c->ddiag_struct.p, c->dscal.p, c->dval.p, c->dlog.p);
+    hipLaunchKernelGGL(k_scale, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->drp.p, c->dci.p, c->dscal.p,
+                       c->dval.p);
+    hipLaunchKernelGGL(k_collog, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->dcptr.p, c->dcpos.p, c->dval.p,
+                       c->dlog.p);
     hipLaunchKernelGGL(k_colnorm, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->dcolcount.p, c->ddiag_pos.p,
                        c->dval.p, c->dlog.p, c->dcolfac.p);
     hipLaunchKernelGGL(k_rownorm, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->drp.p, c->dci.p, c->ddiag_pos.p,

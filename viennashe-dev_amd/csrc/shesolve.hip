@@ -129,9 +129,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ 
 // streamed with fully coalesced loads into LDS as products v[k] * x[col[k]],
 // then each thread reduces whole rows from LDS.  Used for the SpMV and for the
 // sweeps of every contiguous level of the triangular factors.
-constexpr int kStreamNnz = 2048;
 constexpr int kStreamThreads = 256;
 constexpr int kStreamRows = 256;  // rows per block (rows without entries must not pile up)
+constexpr int kStreamMaxNnz = 16 * kStreamThreads;
+constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 
 enum { MODE_SPMV = 0, MODE_LSWEEP = 1, MODE_USWEEP = 2 };
 
@@ -150,23 +151,34 @@ __device__ inline T block_sum_t(T v) {
 // MODE_SPMV:   y[row] = (b ? b[row] - s : s), dots of y with d0, d1 (per persistent block)
 // MODE_LSWEEP: y[row] = b[perm ? perm[row] : row] - s           (y == x: the factor-space vector)
 // MODE_USWEEP: y[row] = dinv[row] * (y[row] - s), out2[perm[row]] = y[row] when perm
-template <typename TV, typename T, int MODE>
-__global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, const int *__restrict__ blocks,
+// xcd: 0 = row blocks strided over the whole grid; 1 = each XCD (blockIdx % 8)
+// works through its own contiguous eighth of the row blocks, so the gathered
+// vector entries of neighbouring rows are reused in that XCD's L2.
+template <typename TV, typename T, int MODE, int ITEMS>
+__global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, const int *__restrict__ blocks,
                                                            const int *__restrict__ rp, const int *__restrict__ ci,
                                                            const TV *__restrict__ v, const T *x, T *y,
                                                            const T *__restrict__ b, const int *__restrict__ perm,
                                                            const T *__restrict__ dinv, T *__restrict__ out2,
                                                            const T *__restrict__ d0, const T *__restrict__ d1,
                                                            double *__restrict__ part) {
+  constexpr int kStreamNnz = ITEMS * kStreamThreads;
   __shared__ T prod[kStreamNnz];
   double a0 = 0.0, a1 = 0.0;
-  for (int bk = blockIdx.x; bk < nblk; bk += gridDim.x) {
+  int bk0 = blockIdx.x, bk1 = nblk, bstep = gridDim.x;
+  if (xcd) {
+    const int x = blockIdx.x % kXcds, per = gridDim.x / kXcds;
+    bk0 = (int)((int64_t)nblk * x / kXcds) + (int)blockIdx.x / kXcds;
+    bk1 = (int)((int64_t)nblk * (x + 1) / kXcds);
+    bstep = per;
+  }
+  for (int bk = bk0; bk < bk1; bk += bstep) {
     const int rs = blocks[bk], re = blocks[bk + 1];
     const int k0 = rp[rs], k1 = rp[re];
     if (k1 - k0 <= kStreamNnz) {
       // all column indices and values first, then the gathers: 8 independent
       // loads in flight per thread
-      constexpr int kItems = kStreamNnz / kStreamThreads;
+      constexpr int kItems = ITEMS;
       const int cnt = k1 - k0;
       int cc[kItems];
       TV vv[kItems];
@@ -550,9 +562,12 @@ struct shesolve_ctx {
   float *z32 = nullptr;
   int ncolors = 0;
   std::vector<int> h_perm, h_colour;
-  // CSR-stream row blocks of A
+  // CSR-stream row blocks of A and the tuned launch shape
   int *d_ablk = nullptr;
   int nablk = 0;
+  int items = 8;   // entries per thread of a row block (row block = 256 * items entries)
+  int xcd = 1;     // XCD-contiguous block mapping
+  double tune_us[6] = {0, 0, 0, 0, 0, 0};
   // split factor: strict lower / strict upper parts and 1/diagonal (factor space)
   int *d_Lrp = nullptr, *d_Lci = nullptr, *d_Urp = nullptr, *d_Uci = nullptr, *d_Lsrc = nullptr;
   double *d_L64 = nullptr, *d_U64 = nullptr, *d_dinv64 = nullptr;
@@ -614,14 +629,33 @@ struct KTimer {
   }
 };
 
+// grid of a stream launch: a multiple of the XCD count, at most 4 kGrid
+inline int stream_grid(int nblk, bool fixed) {
+  if (fixed) return kGrid;
+  int g = std::min(nblk, 4 * kGrid);
+  g = (g + kXcds - 1) / kXcds * kXcds;
+  return std::max(g, kXcds);
+}
+
+template <typename TV, typename T, int MODE>
+void launch_stream(shesolve_ctx *c, int g, int nblk, const int *blocks, const int *rp, const int *ci, const TV *v,
+                   const T *x, T *y, const T *b, const int *perm, const T *dinv, T *out2, const T *d0, const T *d1,
+                   double *part) {
+#define ST_CASE(IT)                                                                                                \
+  case IT:                                                                                                         \
+    hipLaunchKernelGGL((k_stream<TV, T, MODE, IT>), dim3(g), dim3(kStreamThreads), 0, c->stream, nblk, c->xcd,     \
+                       blocks, rp, ci, v, x, y, b, perm, dinv, out2, d0, d1, part);                                \
+    break;
+  switch (c->items) { ST_CASE(4) ST_CASE(16) default: ST_CASE(8) }
+#undef ST_CASE
+}
+
 template <typename TV, typename T, typename TA>
 void launch_spmv(shesolve_ctx *c, int /*lpr*/, const TV *v, const T *x, T *y, const T *b, const T *d0, const T *d1,
                  double *part) {
   KTimer kt(c, 0);
-  const int g = part ? kGrid : std::max(1, std::min(c->nablk, 4 * kGrid));
-  hipLaunchKernelGGL((k_stream<TV, T, MODE_SPMV>), dim3(g), dim3(kStreamThreads), 0, c->stream, c->nablk, c->d_ablk,
-                     c->d_rp, c->d_ci, v, x, y, b, (const int *)nullptr, (const T *)nullptr, (T *)nullptr, d0, d1,
-                     part);
+  launch_stream<TV, T, MODE_SPMV>(c, stream_grid(c->nablk, part != nullptr), c->nablk, c->d_ablk, c->d_rp, c->d_ci,
+                                  v, x, y, b, (const int *)nullptr, (const T *)nullptr, (T *)nullptr, d0, d1, part);
 }
 
 template <typename TV, typename T>
@@ -647,10 +681,9 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     KTimer kt(c, 1);
     const auto &run = c->runL[l];
     if (run.blk_cnt > 0) {
-      const int g = std::max(1, std::min(run.blk_cnt, 4 * kGrid));
-      hipLaunchKernelGGL((k_stream<TV, T, MODE_LSWEEP>), dim3(g), dim3(kStreamThreads), 0, c->stream, run.blk_cnt,
-                         c->d_lblk + run.blk_off, c->d_Lrp, c->d_Lci, Lv, z, z, in, perm, (const T *)nullptr,
-                         (T *)nullptr, (const T *)nullptr, (const T *)nullptr, (double *)nullptr);
+      launch_stream<TV, T, MODE_LSWEEP>(c, stream_grid(run.blk_cnt, false), run.blk_cnt, c->d_lblk + run.blk_off,
+                                        c->d_Lrp, c->d_Lci, Lv, z, z, in, perm, (const T *)nullptr, (T *)nullptr,
+                                        (const T *)nullptr, (const T *)nullptr, (double *)nullptr);
       continue;
     }
     int cnt = c->lvL.ptr[l + 1] - c->lvL.ptr[l];
@@ -668,10 +701,9 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     KTimer kt(c, 2);
     const auto &run = c->runU[l];
     if (run.blk_cnt > 0) {
-      const int g = std::max(1, std::min(run.blk_cnt, 4 * kGrid));
-      hipLaunchKernelGGL((k_stream<TV, T, MODE_USWEEP>), dim3(g), dim3(kStreamThreads), 0, c->stream, run.blk_cnt,
-                         c->d_ublk + run.blk_off, c->d_Urp, c->d_Uci, Uv, z, z, (const T *)nullptr, perm,
-                         (const T *)dinv, out, (const T *)nullptr, (const T *)nullptr, (double *)nullptr);
+      launch_stream<TV, T, MODE_USWEEP>(c, stream_grid(run.blk_cnt, false), run.blk_cnt, c->d_ublk + run.blk_off,
+                                        c->d_Urp, c->d_Uci, Uv, z, z, (const T *)nullptr, perm, (const T *)dinv, out,
+                                        (const T *)nullptr, (const T *)nullptr, (double *)nullptr);
       continue;
     }
     int cnt = c->lvU.ptr[l + 1] - c->lvU.ptr[l];
@@ -717,12 +749,12 @@ void free_matrix(shesolve_ctx *c) {
 }
 
 // Row blocks over rows [r0, r1) of a CSR (host row pointer): consecutive rows
-// holding at most kStreamNnz entries; a longer row forms a block of its own.
-void make_blocks(const int *rp, int r0, int r1, std::vector<int> &out) {
+// holding at most `cap` entries; a longer row forms a block of its own.
+void make_blocks(const int *rp, int r0, int r1, int cap, std::vector<int> &out) {
   out.push_back(r0);
   int start = r0;
   for (int r = r0; r < r1; ++r) {
-    if (r > start && (rp[r + 1] - rp[start] > kStreamNnz || r - start >= kStreamRows)) {
+    if (r > start && (rp[r + 1] - rp[start] > cap || r - start >= kStreamRows)) {
       out.push_back(r);
       start = r;
     }
@@ -949,7 +981,7 @@ int analyse_and_factor(shesolve_ctx *c) {
         r.r1 = rows[e - 1] + 1;
         r.blk_off = (int)blk.size();
         std::vector<int> bb;
-        make_blocks(rp.data(), r.r0, r.r1, bb);
+        make_blocks(rp.data(), r.r0, r.r1, c->items * kStreamThreads, bb);
         // stored as boundaries: blk_cnt blocks need blk_cnt + 1 entries
         for (int v : bb) blk.push_back(v);
         r.blk_cnt = (int)bb.size() - 1;
@@ -1277,6 +1309,63 @@ int set_pattern_host(shesolve_ctx *c, int64_t n, int64_t nnz, const int64_t *rp,
   return 0;
 }
 
+int ensure_vectors(shesolve_ctx *c);
+
+// Autotuner of the CSR-stream launch shape: entries per thread (4, 8, 16) x
+// block-to-XCD mapping (strided, contiguous), each timed on three FP32 SpMV
+// launches of this matrix (values cast below); the fastest is kept for the
+// SpMV and for the sweeps of the factors.
+int tune_stream(shesolve_ctx *c) {
+  const int n = c->n;
+  const int cand_items[3] = {4, 8, 16};
+  std::vector<int> best_blocks;
+  double best = 1e300;
+  int best_items = 8, best_xcd = 1;
+  HIPCHK(hipMalloc(&c->d_v32, sizeof(float) * std::max<int64_t>(c->nnz, 1)));
+  hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->nnz, kBlock * 4)), dim3(kBlock), 0, c->stream, c->nnz,
+                     c->d_v64, c->d_v32);
+  if (int e = ensure_vectors(c)) return e;
+  hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)n, c->v32[0], 1.0f);
+  for (int ii = 0; ii < 3; ++ii) {
+    std::vector<int> ab;
+    make_blocks(c->h_rp.data(), 0, n, cand_items[ii] * kStreamThreads, ab);
+    int *d_ab = nullptr;
+    HIPCHK(hipMalloc(&d_ab, sizeof(int) * ab.size()));
+    HIPCHK(hipMemcpyAsync(d_ab, ab.data(), sizeof(int) * ab.size(), hipMemcpyHostToDevice, c->stream));
+    for (int xm = 0; xm < 2; ++xm) {
+      c->items = cand_items[ii];
+      c->xcd = xm;
+      c->d_ablk = d_ab;
+      c->nablk = (int)ab.size() - 1;
+      launch_spmv<float, float, float>(c, 0, c->d_v32, c->v32[0], c->v32[1], nullptr, nullptr, nullptr, nullptr);
+      HIPCHK(hipEventRecord(c->ev0, c->stream));
+      for (int r = 0; r < 3; ++r)
+        launch_spmv<float, float, float>(c, 0, c->d_v32, c->v32[0], c->v32[1], nullptr, nullptr, nullptr, nullptr);
+      HIPCHK(hipEventRecord(c->ev1, c->stream));
+      HIPCHK(hipEventSynchronize(c->ev1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      c->tune_us[ii * 2 + xm] = ms * 1e3 / 3;
+      if (ms < best) {
+        best = ms;
+        best_items = cand_items[ii];
+        best_xcd = xm;
+        best_blocks = ab;
+      }
+    }
+    c->d_ablk = nullptr;
+    HIPCHK(hipFree(d_ab));
+  }
+  c->items = best_items;
+  c->xcd = best_xcd;
+  c->nablk = (int)best_blocks.size() - 1;
+  HIPCHK(hipMalloc(&c->d_ablk, sizeof(int) * best_blocks.size()));
+  HIPCHK(hipMemcpyAsync(c->d_ablk, best_blocks.data(), sizeof(int) * best_blocks.size(), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int set_matrix_common(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, const int64_t *rp,
                       const int32_t *ci, const double *val, int nblocks, const int64_t *ranges) {
   auto t0 = std::chrono::steady_clock::now();
@@ -1297,19 +1386,12 @@ int set_matrix_common(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, 
   HIPCHK(hipMalloc(&c->d_rp, sizeof(int) * (n + 1)));
   HIPCHK(hipMalloc(&c->d_ci, sizeof(int) * std::max<int64_t>(nnz, 1)));
   HIPCHK(hipMalloc(&c->d_v64, sizeof(double) * std::max<int64_t>(nnz, 1)));
-  HIPCHK(hipMalloc(&c->d_v32, sizeof(float) * std::max<int64_t>(nnz, 1)));
   HIPCHK(hipMemcpyAsync(c->d_rp, c->h_rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_ci, c->h_ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_v64, val, sizeof(double) * nnz, device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         c->stream));
   c->lpr = c->prm.lanes_per_row > 0 ? c->prm.lanes_per_row : pick_lpr((double)nnz / n);
-  {
-    std::vector<int> ab;
-    make_blocks(c->h_rp.data(), 0, (int)n, ab);
-    c->nablk = (int)ab.size() - 1;
-    HIPCHK(hipMalloc(&c->d_ablk, sizeof(int) * ab.size()));
-    HIPCHK(hipMemcpyAsync(c->d_ablk, ab.data(), sizeof(int) * ab.size(), hipMemcpyHostToDevice, c->stream));
-  }
+  if (int e = tune_stream(c)) return e;
   if (int e = analyse_and_factor(c)) return e;
   if (int e = upload_values(c)) return e;
   if (int e = ensure_vectors(c)) return e;
@@ -1543,6 +1625,15 @@ int shesolve_get_factors(shesolve_handle h, int64_t *row_ptr, int32_t *col, doub
   std::memcpy(col, h->h_fci.data(), sizeof(int32_t) * h->fnnz);
   HIPCHK(hipMemcpyAsync(val, h->d_f64, sizeof(double) * h->fnnz, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int shesolve_stream_config(shesolve_handle h, int32_t *items, int32_t *xcd, double *tune_us6) {
+  if (!h) return 1;
+  if (items) *items = h->items;
+  if (xcd) *xcd = h->xcd;
+  if (tune_us6)
+    for (int k = 0; k < 6; ++k) tune_us6[k] = h->tune_us[k];
   return 0;
 }
 

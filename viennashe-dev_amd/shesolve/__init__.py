@@ -24,7 +24,7 @@ EXPORTS = (
     "shesolve_update_values_device", "shesolve_solve", "shesolve_solve_device", "shesolve_spmv",
     "shesolve_spmv_f32", "shesolve_precond_apply", "shesolve_factor_nnz", "shesolve_get_factors",
     "shesolve_levels", "shesolve_set_profiling", "shesolve_kernel_stats", "shesolve_time_spmv",
-    "shesolve_get_ordering",
+    "shesolve_get_ordering", "shesolve_stream_config",
 )
 
 
@@ -80,6 +80,7 @@ def lib():
     L.shesolve_get_factors.argtypes = [C.c_void_p, _i64p, _i32p, _f64p]
     L.shesolve_levels.argtypes = [C.c_void_p, _i64p]
     L.shesolve_get_ordering.argtypes = [C.c_void_p, _i32p]
+    L.shesolve_stream_config.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _f64p]
     L.shesolve_set_profiling.argtypes = [C.c_void_p, C.c_int]
     L.shesolve_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.shesolve_time_spmv.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
@@ -193,6 +194,12 @@ class Solver:
         v = np.zeros(max(nnz, 1))
         _check(lib().shesolve_get_factors(self._h, rp, ci, v), "shesolve_get_factors")
         return rp, ci[:nnz], v[:nnz]
+
+    def stream_config(self):
+        it, xc = C.c_int32(), C.c_int32()
+        t = np.zeros(6)
+        _check(lib().shesolve_stream_config(self._h, C.byref(it), C.byref(xc), t), "shesolve_stream_config")
+        return {"items": it.value, "xcd": xc.value, "tune_us": [round(v, 1) for v in t]}
 
     def ordering(self):
         p = np.zeros(self.n, np.int32)
