@@ -118,13 +118,17 @@ def main():
     d_x = ctypes.c_void_p()
     assert hip.hipMalloc(ctypes.byref(d_x), ctypes.c_size_t(8 * n)) == 0
 
-    def run(precision, steps, warmup):
+    def run(precision, steps, warmup, live=False):
         sv = shesolve.Solver(0, precond=pc, precision=precision, tol=1e-10, max_iters=5000)
         t = time.perf_counter()
         sv.set_matrix_device(n, nnz, d_rp, d_ci, d_val, block_ranges=blk)
         log(f"[bench] precision={precision} setup {time.perf_counter() - t:.1f} s, levels {sv.levels()}")
         lat, reps = [], []
         for k in range(warmup + steps):
+            if live and k == warmup:
+                # SpMV launches of the timed steps bracketed by HIP events on the
+                # solver stream, recorded without waiting (read after the steps)
+                sv.set_profiling(2)
             t = time.perf_counter()
             F.refill()
             sv.update_values_device(d_val)
@@ -139,8 +143,12 @@ def main():
                 reps.append(rep)
         return sv, lat, reps
 
-    sv32, lat32, reps32 = run(shesolve.FP32_IR, args.steps, args.warmup)
-    spmv32_us = sv32.time_spmv(fp32=True, reps=20)
+    sv32, lat32, reps32 = run(shesolve.FP32_IR, args.steps, args.warmup, live=True)
+    spmv32_us, spmv32_launches = sv32.kernel_stats(0)
+    spmv64_ir_us, spmv64_ir_launches = sv32.kernel_stats(4)
+    sv32.set_profiling(0)
+    log(f"[bench] live FP32 SpMV: {spmv32_us:.1f} us avg over {spmv32_launches} launches in the timed steps")
+    plain32_us = sv32.time_spmv(fp32=True, reps=20)
     spmv64_us = sv32.time_spmv(fp32=False, reps=20)
     prof = {}
     if args.profile_kernels:
@@ -174,6 +182,12 @@ def main():
     total = sum(lat32)
     b32 = spmv_bytes(n, nnz, 4, 4)
     achieved = b32 / (spmv32_us * 1e-6) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_stream_spmv_cfg2_traffic.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        if tr.get("n") == n and tr.get("nnz") == nnz:
+            traffic = tr["traffic_bytes"]
     result = {
         "metric": "SHE solves/s at rel. residual 1e-10 (FP32+IR), p50/p99 latency",
         "value": len(lat32) / total,
@@ -197,10 +211,16 @@ def main():
         "ir_steps": reps32[-1]["ir_steps"],
         "fp64": fp64,
         "speedup_vs_fp64": (fp64["ms_per_solve"] / (1e3 * total / len(lat32))) if fp64 else None,
-        "roofline": {"bound": "hbm", "kernel": "csr_spmv_fp32", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "avg_us": spmv32_us, "algorithmic_bytes": b32,
-                     "fp64_spmv_gbs": spmv_bytes(n, nnz, 8, 8) / (spmv64_us * 1e-6) / 1e9},
+        "roofline": {"bound": "hbm", "kernel": "k_stream<float,float,0,8> (CSR-stream SpMV, FP32)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "avg_us": spmv32_us, "launches": spmv32_launches,
+                     "algorithmic_bytes": b32,
+                     "traffic_source": "profiles/k_stream_spmv_cfg2_fp32ir_pmc.md" if traffic else None,
+                     "plain_launch_us": plain32_us,
+                     "plain_launch_gbs": b32 / (plain32_us * 1e-6) / 1e9,
+                     "fp64_spmv_us": spmv64_us,
+                     "fp64_spmv_gbs": spmv_bytes(n, nnz, 8, 8) / (spmv64_us * 1e-6) / 1e9,
+                     "fp64_ir_residual_spmv_us": spmv64_ir_us},
         "kernel_stats_us": prof or None,
     }
     if not args.no_cpu:

@@ -90,6 +90,22 @@ int shefill_mapping(shefill_handle h, int64_t *even_idx, int32_t *even_order);
 /* Device milliseconds of the last value fill (scalars + rows + normalisation). */
 int shefill_last_fill_ms(shefill_handle h, double *ms);
 
+/* Full solution from a reduced one: the even unknowns scaled back,
+ * x_full[i] = x[i] * scaling[i] * col_factor[i] (she/linear_solver.hpp:172-181),
+ * then the odd unknowns recovered from their rows of the full system,
+ * f_o = (b_o - S^oe f_e) / S^oo_oo (she/eliminate.hpp:305-354).  x has n
+ * entries, x_full n + n_odd (even unknowns first, then the odd ones in the odd
+ * mapping's order).  _device: device buffers; otherwise host buffers. */
+int shefill_recover(shefill_handle h, const double *x, double *x_full);
+int shefill_recover_device(shefill_handle h, const double *d_x, double *d_x_full);
+/* Carrier density per cell [m^-3] (she/postproc/carrier_density.hpp:68-131) and
+ * current per interior facet (she/postproc/current_density.hpp:48-131, facet
+ * order of the build: per cell the right then the top neighbour) from a full
+ * solution.  density or current may be NULL.  _device: device buffers. */
+int shefill_postproc(shefill_handle h, const double *x_full, double *density, double *facet_current);
+int shefill_postproc_device(shefill_handle h, const double *d_x_full, double *d_density,
+                            double *d_facet_current);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,9 +42,14 @@ enum {
   SHESOLVE_PRECOND_NONE = 0,
   SHESOLVE_PRECOND_ILU0 = 1,        /* ILU(0), natural row order, level-scheduled */
   SHESOLVE_PRECOND_BLOCK_ILU0 = 3,  /* ILU(0) of the diagonal blocks (per-energy slabs) */
-  SHESOLVE_PRECOND_MC_ILU0 = 4      /* ILU(0) of the matrix in a greedy multicolour row
+  SHESOLVE_PRECOND_MC_ILU0 = 4,     /* ILU(0) of the matrix in a greedy multicolour row
                                        ordering: a few dependency levels instead of
                                        hundreds; not in the reference, same stopping rule */
+  SHESOLVE_PRECOND_ILUK = 5         /* ILU(k) on the level-of-fill pattern, k = fill_level,
+                                       natural order, level-scheduled; drop_tol > 0 adds
+                                       ILUT's threshold rule inside that pattern.  The
+                                       GPU form of the ILUT(drop, fill) slot of the
+                                       reference config (solvers/config.hpp:166-186) */
 };
 
 enum {
@@ -60,6 +65,8 @@ typedef struct {
   double inner_tol;      /* relative reduction of each FP32 inner solve, default 1e-2 */
   int max_ir_steps;      /* cap on refinement steps, default 30 */
   int lanes_per_row;     /* SpMV lanes per row (0 = pick from nnz/row) */
+  int fill_level;        /* ILUK: level of fill k, default 1 */
+  double drop_tol;       /* ILUK: relative drop tolerance (0 = plain ILU(k)), default 0 */
 } shesolve_params;
 
 typedef struct {
@@ -123,16 +130,69 @@ int shesolve_get_ordering(shesolve_handle h, int32_t *perm);
 /* Level-schedule statistics: [0] forward levels, [1] backward levels. */
 int shesolve_levels(shesolve_handle h, int64_t *out2);
 
-/* Timing of one kernel class over the last solve: average device time per
- * launch in microseconds and the launch count, for SpMV (kind 0), forward
- * sweep level (1), backward sweep level (2), vector kernels (3).  Needs
- * profiling enabled with shesolve_set_profiling(h, 1) before the solve. */
+/* Timing of one kernel class over the solves since profiling was switched on:
+ * average device time per launch in microseconds and the launch count, for
+ * the FP32 SpMV (kind 0), forward sweep level (1), backward sweep level (2),
+ * vector kernels (3), FP64 SpMV (4: FP64 solves and refinement residuals).
+ * shesolve_set_profiling(h, mode) before the solves: mode 1 brackets every
+ * launch of the classes with events and waits for each (the solve slows
+ * down); mode 2 times only the SpMV launches (kinds 0 and 4), with events
+ * recorded on the solver stream without waiting, resolved when
+ * shesolve_kernel_stats is called; 0 switches profiling off.  Switching
+ * resets the statistics. */
 int shesolve_set_profiling(shesolve_handle h, int on);
 int shesolve_kernel_stats(shesolve_handle h, int kind, double *avg_us, int64_t *launches);
 /* Times `reps` back-to-back SpMV launches (FP64 if fp32 == 0, else FP32) with
  * HIP events on the solver stream; returns the average per launch in
  * microseconds. */
 int shesolve_time_spmv(shesolve_handle h, int fp32, int reps, double *avg_us);
+
+/* ---------------- batched solves ----------------
+ * Many independent systems of uneven size (bias points, devices, expansion
+ * orders) solved together.  The reference solves them one solvers::solve call
+ * at a time (src/solvers/solve.hpp:47-87); here they are stored block-
+ * diagonally on one device and every BiCGStab phase is one launch over all
+ * systems, with per-system scalars, stopping tests and convergence masks kept
+ * on the device; one iteration is captured into a HIP graph and replayed.
+ * Each system keeps its own result: the same preconditioner (factored per
+ * system: the factor of a block-diagonal matrix is block-diagonal), the same
+ * stopping rule on its own ||b_s - A_s x_s|| / ||b_s|| as a single solve.
+ * The parameters (precond, precision, tolerances) apply to every system.
+ * ILU sweeps run one workgroup per system over its own level schedule. */
+typedef struct shesolve_batch_ctx *shesolve_batch;
+
+int shesolve_batch_create(shesolve_batch *h, int device, const shesolve_params *p);
+int shesolve_batch_destroy(shesolve_batch h);
+int shesolve_batch_set_params(shesolve_batch h, const shesolve_params *p);
+/* count systems; system s has n[s] rows, CSR row_ptr[s] (int64, n[s]+1),
+ * col[s] (int32, ascending per row), val[s] (FP64); block ranges per system
+ * for BLOCK_ILU0 (nblocks and block_ranges may be NULL).  Host buffers, or
+ * device buffers for the _device variant.  Total rows and entries must fit
+ * 32-bit indices. */
+int shesolve_batch_set_systems(shesolve_batch h, int count, const int64_t *n, const int64_t *const *row_ptr,
+                               const int32_t *const *col, const double *const *val, const int *nblocks,
+                               const int64_t *const *block_ranges);
+int shesolve_batch_set_systems_device(shesolve_batch h, int count, const int64_t *n,
+                                      const int64_t *const *d_row_ptr, const int32_t *const *d_col,
+                                      const double *const *d_val, const int *nblocks,
+                                      const int64_t *const *block_ranges);
+/* New values for every system on the stored patterns; refactorises. */
+int shesolve_batch_update_values(shesolve_batch h, const double *const *val);
+int shesolve_batch_update_values_device(shesolve_batch h, const double *const *d_val);
+/* Solves every system; reports[s] (may be NULL) receives system s's result. */
+int shesolve_batch_solve(shesolve_batch h, const double *const *b, double *const *x, int warm_start,
+                         shesolve_report *reports);
+int shesolve_batch_solve_device(shesolve_batch h, const double *const *d_b, double *const *d_x, int warm_start,
+                                shesolve_report *reports);
+/* use_graph: replay one captured iteration (default 1); poll: iterations
+ * between host checks of the convergence masks (default 8). */
+int shesolve_batch_options(shesolve_batch h, int use_graph, int poll);
+/* [0] systems, [1] total rows, [2] total entries, [3] deepest per-system level
+ * schedule, [4] levels of the concatenated schedule, [5] factor entries. */
+int shesolve_batch_info(shesolve_batch h, int64_t *out6);
+/* The underlying single-system handle of the concatenated matrix (SpMV,
+ * factors, timing helpers); owned by the batch. */
+shesolve_handle shesolve_batch_solver(shesolve_batch h);
 
 #ifdef __cplusplus
 }

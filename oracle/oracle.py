@@ -242,7 +242,7 @@ class OracleSystem:
         return dens, cur[: self.nfacets]
 
 
-PC_NONE, PC_ILU0, PC_ILUT, PC_BLOCK_ILU0 = 0, 1, 2, 3
+PC_NONE, PC_ILU0, PC_ILUT, PC_BLOCK_ILU0, PC_ILUK = 0, 1, 2, 3, 5
 
 
 class Precond:

@@ -1117,14 +1117,26 @@ void oracle_spmv(int64_t n, const int64_t *rp, const int32_t *ci, const double *
   }
 }
 
-/* ILU(0), row-wise IKJ variant: Saad 2nd ed. Alg. 10.4 */
-static int ilu0_factor(oracle_precond *p) {
+/* ILU(0), row-wise IKJ variant: Saad 2nd ed. Alg. 10.4, on the pattern held in
+ * p (A's pattern for ILU0, the level-of-fill pattern for ILU(k)).  drop > 0
+ * applies ILUT's dual threshold rule (Saad Alg. 10.6, lines 5 and 10) inside
+ * the fixed pattern: a multiplier |l_ik| < drop * ||a_i||_2 is set to zero and
+ * not used, and so is an upper entry below the same bound after the row is
+ * done; dropped entries stay in the pattern as zeros. */
+static int ilu0_factor_drop(oracle_precond *p, double drop);
+static int ilu0_factor(oracle_precond *p) { return ilu0_factor_drop(p, 0.0); }
+static int ilu0_factor_drop(oracle_precond *p, double drop) {
   int64_t n = p->n;
   int64_t *pos = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
   for (int64_t i = 0; i < n; ++i) pos[i] = -1;
   p->diag = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
   for (int64_t i = 0; i < n; ++i) {
-    for (int64_t k = p->rp[i]; k < p->rp[i + 1]; ++k) pos[p->ci[k]] = k;
+    double rn = 0.0;
+    for (int64_t k = p->rp[i]; k < p->rp[i + 1]; ++k) {
+      pos[p->ci[k]] = k;
+      rn += p->v[k] * p->v[k];
+    }
+    const double tau = drop * sqrt(rn);
     p->diag[i] = pos[i];
     if (pos[i] < 0) { free(pos); set_err("ILU0: missing diagonal in row %lld", (long long)i); return -1; }
     for (int64_t k = p->rp[i]; k < p->rp[i + 1]; ++k) {
@@ -1132,15 +1144,90 @@ static int ilu0_factor(oracle_precond *p) {
       if (kc >= i) break;
       double piv = p->v[p->diag[kc]];
       p->v[k] /= piv;
+      if (drop > 0.0 && fabs(p->v[k]) < tau) { p->v[k] = 0.0; continue; }
       double lik = p->v[k];
       for (int64_t q = p->diag[kc] + 1; q < p->rp[kc + 1]; ++q) {
         int64_t j = pos[p->ci[q]];
         if (j >= 0) p->v[j] -= lik * p->v[q];
       }
     }
+    if (drop > 0.0)
+      for (int64_t k = p->diag[i] + 1; k < p->rp[i + 1]; ++k)
+        if (fabs(p->v[k]) < tau) p->v[k] = 0.0;
     for (int64_t k = p->rp[i]; k < p->rp[i + 1]; ++k) pos[p->ci[k]] = -1;
   }
   free(pos);
+  return 0;
+}
+
+/* Symbolic ILU(k): level-of-fill pattern (Saad 2nd ed. Alg. 10.5).  A's
+ * entries have level 0; eliminating with row j gives column c the level
+ * lev(i,j) + lev(j,c) + 1, the minimum over all paths; columns with level <= k
+ * are kept.  Rows are processed in order, the lower part of each row in
+ * ascending column order.  Output: row pointer and columns (ascending). */
+static int iluk_symbolic(int64_t n, const int64_t *rp, const int32_t *ci, int k, int64_t **out_rp,
+                         int32_t **out_ci) {
+  int *lev = (int *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  int32_t *lst = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+  for (int64_t i = 0; i < n; ++i) lev[i] = -1;
+  int64_t cap = rp[n] * 2 + 16, used = 0;
+  int64_t *orp = (int64_t *)malloc((size_t)(n + 1) * sizeof(int64_t));
+  int32_t *oci = (int32_t *)malloc((size_t)cap * sizeof(int32_t));
+  unsigned char *olev = (unsigned char *)malloc((size_t)cap);
+  int64_t *odg = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+  orp[0] = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    int32_t nl = 0;
+    for (int64_t q = rp[i]; q < rp[i + 1]; ++q) {
+      lev[ci[q]] = 0;
+      lst[nl++] = ci[q];
+    }
+    /* ascending processing of the lower part: selection of the next smallest */
+    int32_t done = -1;
+    for (;;) {
+      int32_t j = (int32_t)i;
+      for (int32_t t = 0; t < nl; ++t)
+        if (lst[t] > done && lst[t] < j) j = lst[t];
+      if (j >= i) break;
+      done = j;
+      const int lj = lev[j];
+      for (int64_t q = odg[j] + 1; q < orp[j + 1]; ++q) {
+        const int32_t c = oci[q];
+        const int nlv = lj + olev[q] + 1;
+        if (nlv > k) continue;
+        if (lev[c] < 0) { lev[c] = nlv; lst[nl++] = c; }
+        else if (nlv < lev[c]) lev[c] = nlv;
+      }
+    }
+    /* sort the row's columns */
+    for (int32_t a = 1; a < nl; ++a) {
+      int32_t v = lst[a], b = a - 1;
+      while (b >= 0 && lst[b] > v) { lst[b + 1] = lst[b]; --b; }
+      lst[b + 1] = v;
+    }
+    if (used + nl > cap) {
+      cap = (used + nl) * 2;
+      oci = (int32_t *)realloc(oci, (size_t)cap * sizeof(int32_t));
+      olev = (unsigned char *)realloc(olev, (size_t)cap);
+    }
+    odg[i] = -1;
+    for (int32_t t = 0; t < nl; ++t) {
+      if (lst[t] == i) odg[i] = used;
+      oci[used] = lst[t];
+      olev[used] = (unsigned char)lev[lst[t]];
+      ++used;
+      lev[lst[t]] = -1;
+    }
+    if (odg[i] < 0) {
+      free(lev); free(lst); free(orp); free(oci); free(olev); free(odg);
+      set_err("ILU(k): missing diagonal in row %lld", (long long)i);
+      return -1;
+    }
+    orp[i + 1] = used;
+  }
+  free(lev); free(lst); free(olev); free(odg);
+  *out_rp = orp;
+  *out_ci = oci;
   return 0;
 }
 
@@ -1234,6 +1321,20 @@ oracle_precond *oracle_precond_create(int kind, int64_t n, const int64_t *rp, co
   if (kind == ORACLE_PC_NONE) return p;
   if (kind == ORACLE_PC_ILUT) {
     if (ilut_factor(p, n, rp, ci, val, drop, fill)) { oracle_precond_free(p); return NULL; }
+    return p;
+  }
+  if (kind == ORACLE_PC_ILUK) {
+    /* fill = level k; A's values scattered into the level-k pattern */
+    if (iluk_symbolic(n, rp, ci, fill, &p->rp, &p->ci)) { oracle_precond_free(p); return NULL; }
+    p->v = (double *)calloc((size_t)(p->rp[n] > 0 ? p->rp[n] : 1), sizeof(double));
+    for (int64_t i = 0; i < n; ++i) {
+      int64_t q = p->rp[i];
+      for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
+        while (p->ci[q] < ci[k]) ++q;
+        p->v[q] = val[k];
+      }
+    }
+    if (ilu0_factor_drop(p, drop)) { oracle_precond_free(p); return NULL; }
     return p;
   }
   /* ILU0 on A, or on the block-diagonal part of A for block ILU0 */

@@ -121,7 +121,10 @@ void oracle_postproc(const oracle_system *s, const double *x_full, double *densi
 void oracle_dispersion(int carrier, double eps, double *Z, double *v);
 
 /* ---------------- solvers on CSR (FP64) ---------------- */
-enum { ORACLE_PC_NONE = 0, ORACLE_PC_ILU0 = 1, ORACLE_PC_ILUT = 2, ORACLE_PC_BLOCK_ILU0 = 3 };
+enum { ORACLE_PC_NONE = 0, ORACLE_PC_ILU0 = 1, ORACLE_PC_ILUT = 2, ORACLE_PC_BLOCK_ILU0 = 3, ORACLE_PC_ILUK = 5 };
+/* ORACLE_PC_ILUK: ILU(k) on the level-of-fill pattern (Saad Alg. 10.5 symbolic,
+ * Alg. 10.4 numeric), k = ilut_fill; ilut_drop > 0 adds ILUT's threshold rule
+ * inside that pattern (dropped entries kept as zeros). */
 
 typedef struct oracle_precond oracle_precond;
 oracle_precond *oracle_precond_create(int kind, int64_t n, const int64_t *row_ptr,

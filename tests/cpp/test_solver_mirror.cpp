@@ -62,6 +62,19 @@ static int cpu_checks() {
   threw = false;
   try { solve(A, b, conf); } catch (invalid_linear_solver_exception const &) { threw = true; }
   CHECK(threw);
+  // an empty row is rejected before any solve (src/solvers/solve.hpp:53-58)
+  sparse_matrix<double> E(2, 2);
+  E(0, 0) = 1.0;
+  std::vector<double> b2(2, 1.0);
+  threw = false;
+  try { solve(E, b2, conf); } catch (linear_solver_exception const &) { threw = true; }
+  CHECK(threw);
+  // the ILU(k) knobs
+  CHECK(conf.gpu_fill_level() == 1 && conf.gpu_drop_tolerance() == 0.0);
+  conf.gpu_fill_level(2);
+  conf.gpu_drop_tolerance(1e-3);
+  conf.gpu_drop_tolerance(2.0);
+  CHECK(conf.gpu_fill_level() == 2 && conf.gpu_drop_tolerance() == 1e-3);
   std::printf("cpu checks passed\n");
   return 0;
 }

@@ -43,3 +43,36 @@ def test_fill_matches_oracle(name):
     big = np.abs(A.val) > 1e-3
     assert rel(v[big], A.val[big]) <= 1e-10, rel(v[big], A.val[big])
     assert np.linalg.norm(rhs - b) <= 1e-11 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("name", ["cfg1", "diode_L3", "diode_L5", "nin2d_L3", "mos2d_L1", "mos2d_L3"])
+def test_recovery_and_postproc_match_oracle(name):
+    """Scale back + odd recovery and the post-processing (carrier density per
+    cell, current per interior facet) on the GPU against the oracle, applied to
+    the same vectors: a random reduced vector (the recovery is linear, so every
+    odd row is exercised) and the oracle's converged solution."""
+    import shesolve
+    from oracle import oracle as O
+    s = oracle_system(name)
+    F = shesolve.Fill(0)
+    F.build(s.dev, s.cfg)
+    A, b, _, _ = s.reduced()
+    rng = np.random.default_rng(11)
+    xr = rng.standard_normal(A.n)
+    xf = F.recover(xr)
+    xf_ref, _ = s.finish(xr)
+    assert xf.shape == xf_ref.shape
+    assert np.linalg.norm(xf - xf_ref) <= 1e-12 * np.linalg.norm(xf_ref)
+    assert np.max(np.abs(xf - xf_ref)) <= 1e-10 * np.max(np.abs(xf_ref))
+    P = O.Precond(O.PC_ILUK, A, drop=0.0, fill=1)
+    x, it, res = O.bicgstab(A, b, precond=P, tol=1e-12, max_iters=3000)
+    assert res <= 1e-12
+    xf = F.recover(x)
+    xf_ref, full_res = s.finish(x)
+    assert np.linalg.norm(xf - xf_ref) <= 1e-12 * np.linalg.norm(xf_ref)
+    dens, cur = F.postproc(xf_ref)
+    dens_ref, cur_ref = s.postproc(xf_ref)
+    assert np.max(np.abs(dens - dens_ref)) <= 1e-12 * np.max(np.abs(dens_ref))
+    if len(cur_ref):
+        assert np.max(np.abs(cur - cur_ref)) <= 1e-11 * np.max(np.abs(cur_ref))
+    F.close()

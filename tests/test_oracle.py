@@ -152,6 +152,76 @@ def test_block_ilu0_exact_on_block_diagonal_and_ilut_full_fill_is_lu():
     assert it == 1 and res <= 1e-12
 
 
+def _laplace5(nx, ny):
+    T = sp.diags([-1.0, 4.0, -1.0], [-1, 0, 1], shape=(nx, nx))
+    S = sp.diags([-1.0, -1.0], [-1, 1], shape=(ny, ny))
+    return (sp.kron(sp.eye(ny), T) + sp.kron(S, sp.eye(nx))).tocsr()
+
+
+def test_iluk_level_of_fill_on_the_five_point_laplacian():
+    """Level-of-fill pattern (Saad, Iterative Methods, 2nd ed., sec. 10.3.3):
+    ILU(0) keeps the five diagonals; ILU(1) adds exactly the two fill diagonals
+    at offsets -(nx-1) and +(nx-1) (eliminating the south neighbour brings in its
+    east neighbour, eliminating the west neighbour brings in its north
+    neighbour); ILU(k) with k >= nx reproduces the exact LU of this banded
+    matrix, so M = A."""
+    from oracle import oracle as O
+    nx, ny = 7, 6
+    M = _laplace5(nx, ny)
+    A = _csr(M)
+    n = nx * ny
+    F0 = O.Precond(O.PC_ILUK, A, drop=0.0, fill=0).factors()
+    assert np.array_equal(F0.row_ptr, A.row_ptr) and np.array_equal(F0.col, A.col)
+    F1 = O.Precond(O.PC_ILUK, A, drop=0.0, fill=1).factors()
+    for i in range(n):
+        gx, gy = i % nx, i // nx
+        want = {i}
+        if gx > 0: want.add(i - 1)
+        if gx < nx - 1: want.add(i + 1)
+        if gy > 0: want.add(i - nx)
+        if gy < ny - 1: want.add(i + nx)
+        if gy > 0 and gx < nx - 1: want.add(i - nx + 1)
+        if gy < ny - 1 and gx > 0: want.add(i + nx - 1)
+        got = set(F1.col[F1.row_ptr[i]:F1.row_ptr[i + 1]].tolist())
+        assert got == want, (i, sorted(got ^ want))
+    P = O.Precond(O.PC_ILUK, A, drop=0.0, fill=2 * nx)
+    x = np.random.default_rng(0).random(n)
+    assert np.allclose(P.apply(M @ x), x, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["cfg1", "diode_L5"])
+def test_iluk_zero_level_is_ilu0_and_dropping_rules(name):
+    """ILU(k) at k = 0 is ILU(0) entry for entry; ILU(k) keeps (L U)_ij = a_ij on
+    A's pattern (its defining property on the extended pattern); with a drop
+    tolerance every stored multiplier and upper entry is either zero or at
+    least the row's threshold, and a drop tolerance of 1e-300 drops nothing."""
+    from oracle import oracle as O
+    A, b, _, _ = oracle_system(name).reduced()
+    F0 = O.Precond(O.PC_ILU0, A).factors()
+    Fk0 = O.Precond(O.PC_ILUK, A, drop=0.0, fill=0).factors()
+    assert np.array_equal(F0.row_ptr, Fk0.row_ptr) and np.array_equal(F0.col, Fk0.col)
+    assert np.array_equal(F0.val, Fk0.val)
+    P1 = O.Precond(O.PC_ILUK, A, drop=0.0, fill=1)
+    F1 = P1.factors()
+    Fs = F1.to_scipy().tocsr()
+    L = sp.tril(Fs, -1) + sp.eye(A.n)
+    U = sp.triu(Fs)
+    LU = (L @ U).tocsr()
+    Mx = A.to_scipy().tocsr()
+    vals = np.asarray(LU[Mx.nonzero()]).ravel()
+    assert np.max(np.abs(vals - Mx.data)) <= 1e-11 * max(np.abs(L).max() * np.abs(U).max(), 1.0)
+    Ft = O.Precond(O.PC_ILUK, A, drop=1e-300, fill=1).factors()
+    assert np.array_equal(Ft.val, F1.val)
+    drop = 1e-3
+    Fd = O.Precond(O.PC_ILUK, A, drop=drop, fill=2).factors()
+    rn = np.sqrt(np.add.reduceat(A.val ** 2, A.row_ptr[:-1]))
+    rows = np.repeat(np.arange(A.n), np.diff(Fd.row_ptr))
+    off = Fd.col != rows
+    v = np.abs(Fd.val[off])
+    assert np.all((v == 0) | (v >= drop * rn[rows[off]]))
+    assert np.count_nonzero(Fd.val[off] == 0) > 0
+
+
 @pytest.mark.parametrize("name", ["cfg1", "diode_L3", "mos2d_L3"])
 def test_ilu0_reproduces_A_on_its_pattern(name):
     """ILU(0) defining property: (L U)_ij = a_ij for every (i, j) in the pattern of A."""

@@ -245,6 +245,105 @@ __global__ void k_node(DevTables t, int nn, const int *nd_c, const int *nd_H, co
   rhs0[k] = r0;
 }
 
+// ---- solution back end: scale back, odd recovery, carrier density, current
+// x_full[i] = x[i] * scaling[i] * col_factor[i] for the even unknowns
+// (she/linear_solver.hpp:172-181)
+__global__ void k_scale_back(int n, const double *x, const double *scal, const double *colfac, int on, double *xf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  xf[i] = on ? x[i] * (scal[i] * colfac[i]) : x[i];
+}
+
+struct RecArgs {
+  int n, NH, nfn;
+  const int *fn_f, *fn_H, *fn_L, *fn_base, *fdir, *node_of, *nd_base, *nd_L;
+  const double *cdiff, *cdrift, *odiag;
+};
+
+// Odd unknowns of one facet node from its row of the full system:
+// f_o = (b_o - sum_e S^oe f_e) / S^oo_oo with b_o = 0 (she/eliminate.hpp:305-354).
+// S^oe couples to the even unknowns of both cells X of the facet:
+// diffusion * a^T + drift seen from X * b^T, in the direction from the other
+// cell to X -- the same entries the elimination used.
+__global__ void k_recover_odd(DevTables t, RecArgs a, double *xf) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.nfn) return;
+  const int f = a.fn_f[k], H = a.fn_H[k], Lf = a.fn_L[k], K = t.K;
+  const int d12 = a.fdir[f];
+  const int ob = t.od_off[Lf], oe = t.od_off[Lf + 1];
+  double *out = xf + a.n + a.fn_base[k];
+  for (int o = ob; o < oe; ++o) {
+    const int ho = t.od_h[o];
+    double acc = 0.0;
+    for (int side = 0; side < 2; ++side) {
+      const int X = side ? t.fc2[f] : t.fc1[f];
+      const int dir = side ? d12 : rev_dir(d12);
+      const int nX = a.node_of[(size_t)X * a.NH + H];
+      const int base = a.nd_base[nX], LX = a.nd_L[nX];
+      const double *Ao = t.Ao + (size_t)dir * K * K, *Bo = t.Bo + (size_t)dir * K * K;
+      const double cd = a.cdiff[k], cr = a.cdrift[2 * k + side];
+      for (int r = t.ev_off[LX]; r < t.ev_off[LX + 1]; ++r) {
+        const int hq = t.ev_h[r];
+        const double soe = cd * Ao[ho * K + hq] + cr * Bo[ho * K + hq];
+        acc += soe * xf[base + (r - t.ev_off[LX])];
+      }
+    }
+    out[o - ob] = -acc / a.odiag[k];
+  }
+}
+
+// carrier density per cell (she/postproc/carrier_density.hpp:68-131, window [0, 1.0])
+__global__ void k_density(DevTables t, int nc, const int *idxc, const double *xf, double *dens) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  double n = 0.0;
+  for (int H = 1; H < t.NH - 1; ++H) {
+    const int u = idxc[(size_t)c * t.NH + H];
+    if (u < 0) continue;
+    double lo = (kin_cell(t, c, H - 1) + kin_cell(t, c, H)) / 2.0;
+    if (lo < 0.0) lo = 0.0;
+    double up = (kin_cell(t, c, H + 1) + kin_cell(t, c, H)) / 2.0;
+    if (up > 1.0) up = 1.0;
+    if (up >= 0 && lo < up) n += xf[u] * avgZ(t, false, c, H) * bheight(t, false, c, H);
+  }
+  dens[c] = n * symmetry_factor(t.carrier);
+}
+
+struct CurArgs {
+  int nf, n;
+  const int *fnode_of, *fn_base;
+  const double *cx, *cy;
+  double ax[3], ay[3];
+};
+
+// current through each interior facet (she/postproc/current_density.hpp:48-131):
+// projection of the l = 1 components on the facet normal
+__global__ void k_current(DevTables t, CurArgs a, const double *xf, double *cur) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= a.nf) return;
+  const int c1 = t.fc1[f], c2 = t.fc2[f];
+  double nx = a.cx[c2] - a.cx[c1], ny = t.dim > 1 ? a.cy[c2] - a.cy[c1] : 0.0;
+  const double nn = sqrt(nx * nx + ny * ny);
+  nx /= nn;
+  ny /= nn;
+  const double s = sqrt(m_dos(t.carrier) / m_trans(t.carrier));
+  double vn = 0.0;
+  for (int H = 1; H < t.NH - 1; ++H) {
+    const int k = a.fnode_of[(size_t)f * t.NH + H];
+    if (k < 0) continue;
+    const double *x = xf + a.n + a.fn_base[k];
+    double em, e0, ep;
+    bool hm, hp;
+    kin3(t, true, f, H, em, e0, ep, hm, hp);
+    const double factor = integral_vZ3(t.carrier, em, e0, ep, hm, hp) * symmetry_factor(t.carrier);
+    vn += (a.ax[0] * x[0] + a.ax[1] * x[1] + a.ax[2] * x[2]) * factor * s * nx;
+    if (t.dim > 1) vn += (a.ay[0] * x[0] + a.ay[1] * x[1] + a.ay[2] * x[2]) * factor * s * ny;
+  }
+  const double pol = t.carrier == 0 ? -1.0 : 1.0;
+  const double Y00 = sqrt(1.0 / (2.0 * kPi)) / sqrt(2.0);
+  cur[f] = -pol * kQ * vn / Y00;
+}
+
 struct RowArgs {
   int n;
   const int *rp, *ci, *row_node;
@@ -400,6 +499,9 @@ struct shefill_ctx {
   // nodes
   std::vector<int> nd_c, nd_H, nd_base, nd_L, nd_fn, nd_nb, nd_inn, nd_in, nd_out;
   std::vector<int> fn_f, fn_H, fn_L, fn_out;
+  std::vector<int> fdir;     // direction code of c1 -> c2 per facet (0 = +x, 2 = +y)
+  std::vector<int> fn_base;  // first odd unknown of each facet node, relative to n
+  double a_l01[2][3] = {{0, 0, 0}, {0, 0, 0}};  // raw a_x, a_y entries (0, 1..3)
   int64_t n = 0, n_odd = 0, nnz = 0;
   std::vector<int> rp, ci, row_node, diag_pos, colcount, cptr, cpos;
   std::vector<unsigned char> diag_struct;
@@ -413,6 +515,8 @@ struct shefill_ctx {
   DBuf<double> dcorr, dncont, dcdiff, dcdrift, dodiag, ddiag0, ddiagr, drhs0, dinco;
   DBuf<double> dval, drhs, dscal, dcolfac, dlog;
   DBuf<int64_t> drp64;
+  DBuf<int> dfdir, dfn_base, didxc, dfnode_of, dnode_of;
+  DBuf<double> dcx, dcy;
   DevTables t{};
   bool built = false;
 };
@@ -481,7 +585,7 @@ int build_geometry(shefill_ctx *c, const shefill_device *d) {
         c->cvol[k] = fabs(d->xv[i + 1] - d->xv[i]) * fabs(d->yv[j + 1] - d->yv[j]);
       }
     }
-  c->fc1.clear(); c->fc2.clear(); c->fvol.clear(); c->flen.clear();
+  c->fc1.clear(); c->fc2.clear(); c->fvol.clear(); c->flen.clear(); c->fdir.clear();
   c->cf.assign((size_t)nc * 4, -1);
   for (int j = 0; j < c->ny; ++j)
     for (int i = 0; i < c->nx; ++i) {
@@ -495,6 +599,7 @@ int build_geometry(shefill_ctx *c, const shefill_device *d) {
         c->flen.push_back(sqrt(dx * dx + dy * dy));
         c->cf[(size_t)k * 4 + slot_here] = f;
         c->cf[(size_t)o * 4 + slot_other] = f;
+        c->fdir.push_back(kSlotDir[slot_here]);
       };
       if (i + 1 < c->nx) add(k + 1, c->dim == 1 ? 1.0 : fabs(d->yv[j + 1] - d->yv[j]), 2, 1);
       if (c->dim > 1 && j + 1 < c->ny) add(k + c->nx, fabs(d->xv[i + 1] - d->xv[i]), 3, 0);
@@ -621,6 +726,8 @@ struct Builder {
       }
     }
     if (!found) { ferr("expansion order %d has no precomputed coupling table", L); return 2; }
+    for (int w = 0; w < 2; ++w)
+      for (int q = 1; q <= 3 && q < K; ++q) c->a_l01[w][q - 1] = a[w].v[q];
     CMat aT[3] = {transpose(a[0]), transpose(a[1]), transpose(a[2])};
     CMat bT[3] = {transpose(b[0]), transpose(b[1]), transpose(b[2])};
     Ae.clear(); Be.clear(); Ao.clear(); Bo.clear();
@@ -678,6 +785,14 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
         c->fn_out.push_back(Hf);
       }
     }
+  c->fn_base.assign(c->fn_f.size(), 0);
+  {
+    int acc = 0;
+    for (size_t k = 0; k < c->fn_f.size(); ++k) {
+      c->fn_base[k] = acc;
+      acc += odd_count(c->fn_L[k]);
+    }
+  }
   // cell nodes in unknown order
   c->node_of.assign((size_t)c->nc * NH, -1);
   std::vector<int> order;
@@ -925,6 +1040,13 @@ int fill_values(shefill_ctx *c, Builder &B) {
   e = e ? e : upload(c, c->dfn_H, c->fn_H);
   e = e ? e : upload(c, c->dfn_L, c->fn_L);
   e = e ? e : upload(c, c->dfn_out, c->fn_out);
+  e = e ? e : upload(c, c->dfdir, c->fdir);
+  e = e ? e : upload(c, c->dfn_base, c->fn_base);
+  e = e ? e : upload(c, c->didxc, c->idxc);
+  e = e ? e : upload(c, c->dfnode_of, c->fnode_of);
+  e = e ? e : upload(c, c->dnode_of, c->node_of);
+  e = e ? e : upload(c, c->dcx, c->cx);
+  e = e ? e : upload(c, c->dcy, c->cy);
   e = e ? e : upload(c, c->drp, c->rp);
   e = e ? e : upload(c, c->dci, c->ci);
   e = e ? e : upload(c, c->drow_node, c->row_node);
@@ -1131,6 +1253,95 @@ int shefill_last_fill_ms(shefill_handle h, double *ms) {
   if (!ms) return 2;
   *ms = h->fill_ms;
   return 0;
+}
+
+int shefill_recover_device(shefill_handle h, const double *d_x, double *d_x_full) {
+  if (!h) { ferr("handle is NULL"); return 1; }
+  if (!h->built) { ferr("nothing built"); return -1; }
+  if (!d_x) { ferr("x is NULL"); return 2; }
+  if (!d_x_full) { ferr("x_full is NULL"); return 3; }
+  FHIP(hipSetDevice(h->device));
+  const int n = (int)h->n, nfn = (int)h->fn_f.size();
+  if (n > 0)
+    hipLaunchKernelGGL(k_scale_back, dim3(grid(n)), dim3(kB), 0, h->stream, n, d_x, h->dscal.p, h->dcolfac.p,
+                       h->cfg.scale, d_x_full);
+  RecArgs a{};
+  a.n = n; a.NH = h->NH; a.nfn = nfn;
+  a.fn_f = h->dfn_f.p; a.fn_H = h->dfn_H.p; a.fn_L = h->dfn_L.p; a.fn_base = h->dfn_base.p; a.fdir = h->dfdir.p;
+  a.node_of = h->dnode_of.p; a.nd_base = h->dnd_base.p; a.nd_L = h->dnd_L.p;
+  a.cdiff = h->dcdiff.p; a.cdrift = h->dcdrift.p; a.odiag = h->dodiag.p;
+  if (nfn > 0) hipLaunchKernelGGL(k_recover_odd, dim3(grid(nfn)), dim3(kB), 0, h->stream, h->t, a, d_x_full);
+  FHIP(hipGetLastError());
+  FHIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int shefill_postproc_device(shefill_handle h, const double *d_x_full, double *d_density, double *d_current) {
+  if (!h) { ferr("handle is NULL"); return 1; }
+  if (!h->built) { ferr("nothing built"); return -1; }
+  if (!d_x_full) { ferr("x_full is NULL"); return 2; }
+  FHIP(hipSetDevice(h->device));
+  if (d_density && h->nc > 0)
+    hipLaunchKernelGGL(k_density, dim3(grid(h->nc)), dim3(kB), 0, h->stream, h->t, h->nc, h->didxc.p, d_x_full,
+                       d_density);
+  if (d_current && h->nf > 0) {
+    CurArgs a{};
+    a.nf = h->nf; a.n = (int)h->n; a.fnode_of = h->dfnode_of.p; a.fn_base = h->dfn_base.p;
+    a.cx = h->dcx.p; a.cy = h->dcy.p;
+    for (int q = 0; q < 3; ++q) { a.ax[q] = h->a_l01[0][q]; a.ay[q] = h->a_l01[1][q]; }
+    hipLaunchKernelGGL(k_current, dim3(grid(h->nf)), dim3(kB), 0, h->stream, h->t, a, d_x_full, d_current);
+  }
+  FHIP(hipGetLastError());
+  FHIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int shefill_recover(shefill_handle h, const double *x, double *x_full) {
+  if (!h) { ferr("handle is NULL"); return 1; }
+  if (!h->built) { ferr("nothing built"); return -1; }
+  if (!x) { ferr("x is NULL"); return 2; }
+  if (!x_full) { ferr("x_full is NULL"); return 3; }
+  FHIP(hipSetDevice(h->device));
+  const size_t n = (size_t)h->n, N = n + (size_t)h->n_odd;
+  double *dx = nullptr, *dxf = nullptr;
+  FHIP(hipMalloc(&dx, sizeof(double) * std::max<size_t>(n, 1)));
+  if (hipMalloc(&dxf, sizeof(double) * std::max<size_t>(N, 1)) != hipSuccess) {
+    hipFree(dx);
+    ferr("out of device memory");
+    return -10;
+  }
+  int e = 0;
+  if (hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) { ferr("copy failed"); e = -10; }
+  if (!e) e = shefill_recover_device(h, dx, dxf);
+  if (!e && hipMemcpy(x_full, dxf, sizeof(double) * N, hipMemcpyDeviceToHost) != hipSuccess) {
+    ferr("copy failed");
+    e = -10;
+  }
+  hipFree(dx);
+  hipFree(dxf);
+  return e;
+}
+
+int shefill_postproc(shefill_handle h, const double *x_full, double *density, double *current) {
+  if (!h) { ferr("handle is NULL"); return 1; }
+  if (!h->built) { ferr("nothing built"); return -1; }
+  if (!x_full) { ferr("x_full is NULL"); return 2; }
+  FHIP(hipSetDevice(h->device));
+  const size_t N = (size_t)h->n + (size_t)h->n_odd;
+  double *dxf = nullptr, *dd = nullptr, *dc = nullptr;
+  FHIP(hipMalloc(&dxf, sizeof(double) * std::max<size_t>(N, 1)));
+  FHIP(hipMalloc(&dd, sizeof(double) * std::max(h->nc, 1)));
+  FHIP(hipMalloc(&dc, sizeof(double) * std::max(h->nf, 1)));
+  int e = 0;
+  if (hipMemcpy(dxf, x_full, sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess) { ferr("copy failed"); e = -10; }
+  if (!e) e = shefill_postproc_device(h, dxf, density ? dd : nullptr, current ? dc : nullptr);
+  if (!e && density && hipMemcpy(density, dd, sizeof(double) * h->nc, hipMemcpyDeviceToHost) != hipSuccess) e = -10;
+  if (!e && current && h->nf > 0 && hipMemcpy(current, dc, sizeof(double) * h->nf, hipMemcpyDeviceToHost) != hipSuccess)
+    e = -10;
+  hipFree(dxf);
+  hipFree(dd);
+  hipFree(dc);
+  return e;
 }
 
 }  // extern "C"

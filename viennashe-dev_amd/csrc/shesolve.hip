@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <queue>
 #include <vector>
 
 #include "shesolve.h"
@@ -353,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_update_xr(int n, T *__restrict__ x, 
 __global__ __launch_bounds__(kBlock) void k_gather(int64_t n, const int *__restrict__ pos, const double *__restrict__ src,
                                                    double *__restrict__ dst) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = src[pos[i]];
+    dst[i] = pos[i] >= 0 ? src[pos[i]] : 0.0;  // fill positions of ILU(k) start at zero
 }
 
 // FP64 iterative-refinement helpers
@@ -492,22 +493,33 @@ __global__ __launch_bounds__(kBlock) void k_usolve(int cnt, const int *__restric
 
 // ILU(0) of the rows of one level; one 64-lane block per row, row kept in LDS.
 // Row-wise IKJ order (Saad, Iterative Methods, 2nd ed., Alg. 10.4).
+// drop > 0: threshold rule of ILUT inside the fixed pattern -- multipliers with
+// |l| < drop * ||a_row||_2 are set to zero and not used, upper entries below the
+// same bound are zeroed once the row is done.
 __global__ __launch_bounds__(64) void k_ilu0_level(int cnt, const int *__restrict__ rows, const int *__restrict__ rp,
                                                    const int *__restrict__ ci, const int *__restrict__ dg,
-                                                   double *__restrict__ fv) {
+                                                   double *__restrict__ fv, double drop) {
   __shared__ int cols[kMaxRowLDS];
   __shared__ double vals[kMaxRowLDS];
   if ((int)blockIdx.x >= cnt) return;
   const int row = rows[blockIdx.x];
   const int beg = rp[row], len = rp[row + 1] - beg, nl = dg[row] - beg;
+  double sq = 0.0;
   for (int t = threadIdx.x; t < len; t += 64) {
     cols[t] = ci[beg + t];
     vals[t] = fv[beg + t];
+    sq += vals[t] * vals[t];
   }
+  const double tau = drop > 0.0 ? drop * sqrt(wave_sum(sq)) : 0.0;
   __syncthreads();
   for (int t = 0; t < nl; ++t) {
     const int kc = cols[t];
-    const double l = vals[t] / fv[dg[kc]];
+    double l = vals[t] / fv[dg[kc]];
+    if (drop > 0.0 && fabs(l) < tau) {
+      if (threadIdx.x == 0) vals[t] = 0.0;
+      __syncthreads();
+      continue;
+    }
     const int ub = dg[kc] + 1, ue = rp[kc + 1];
     for (int q = ub + threadIdx.x; q < ue; q += 64) {
       const int j = ci[q];
@@ -521,7 +533,206 @@ __global__ __launch_bounds__(64) void k_ilu0_level(int cnt, const int *__restric
     if (threadIdx.x == 0) vals[t] = l;
     __syncthreads();
   }
-  for (int t = threadIdx.x; t < len; t += 64) fv[beg + t] = vals[t];
+  for (int t = threadIdx.x; t < len; t += 64) {
+    double v = vals[t];
+    if (drop > 0.0 && t > nl && fabs(v) < tau) v = 0.0;
+    fv[beg + t] = v;
+  }
+}
+
+// ---------------------------------------------------------------- batched BiCGStab
+// Systems are concatenated block-diagonally; system s owns rows
+// [sys_ptr[s], sys_ptr[s+1]).  Every system runs its own recurrence: scalars,
+// dot products, stopping test and a convergence mask per system, all on the
+// device, so that a sequence of iterations needs no host round trip.
+constexpr int kBatchParts = 32;  // partial sums per system and dot product
+
+struct BSys {
+  double rho, alpha, omega, beta;
+  double bnorm, res, tol, div_limit;
+  int active;     // 1 while the system iterates
+  int status;     // 0 converged, 1 iteration cap, 2 breakdown / divergence
+  int iters;      // iterations of the current solve
+  int max_iters;
+  int bd, pad[3];
+};
+
+// NV dot products per system into part[(q * S + s) * kBatchParts + j]
+// (systems with mask[s] == 0, or with bs[s].active == 0 when mask is NULL, are skipped)
+template <typename T, int NV>
+__global__ __launch_bounds__(kBlock) void k_bdot(int S, const int *__restrict__ sys_ptr, const BSys *__restrict__ bs,
+                                                 const int *__restrict__ mask, const T *a0, const T *b0, const T *a1,
+                                                 const T *b1, double *__restrict__ part) {
+  const int s = blockIdx.y, j = blockIdx.x;
+  if (mask ? !mask[s] : !bs[s].active) return;
+  double vv[2] = {0.0, 0.0};
+  const int r0 = sys_ptr[s], len = sys_ptr[s + 1] - r0;
+  const int lo = r0 + (int)((int64_t)len * j / kBatchParts), hi = r0 + (int)((int64_t)len * (j + 1) / kBatchParts);
+  for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock) {
+    vv[0] += (double)a0[i] * (double)b0[i];
+    if (NV > 1) vv[1] += (double)a1[i] * (double)b1[i];
+  }
+  block_sum<2>(vv);
+  if (threadIdx.x == 0) {
+    part[(0 * S + s) * kBatchParts + j] = vv[0];
+    if (NV > 1) part[(1 * S + s) * kBatchParts + j] = vv[1];
+  }
+}
+
+__device__ inline double bsum(const double *part, int S, int q, int s) {
+  double t = 0.0;
+  for (int j = 0; j < kBatchParts; ++j) t += part[(q * S + s) * kBatchParts + j];
+  return t;
+}
+
+// Per-system finalise, one thread per system:
+//   0 init from (r,r), (b,b)      1 alpha = rho / (r0,v)      2 omega = (t,s)/(t,t)
+//   3 stopping test from (r,r), (r0,r); beta, rho for the next iteration
+//   5 nrm[s] = sqrt(dot 0) for every system (no mask)
+__global__ void k_bfinalize(int op, int S, const double *__restrict__ part, BSys *bs, double *__restrict__ nrm) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  if (op == 5) {
+    nrm[s] = sqrt(bsum(part, S, 0, s));
+    return;
+  }
+  BSys &c = bs[s];
+  if (!c.active) return;
+  if (op == 0) {
+    const double rr = bsum(part, S, 0, s), bb = bsum(part, S, 1, s);
+    c.bnorm = sqrt(bb);
+    c.res = c.bnorm > 0 ? sqrt(rr) / c.bnorm : 0.0;
+    c.rho = rr;
+    c.alpha = c.omega = 1.0;
+    c.beta = 0.0;
+    c.bd = 0;
+    c.iters = 0;
+    c.status = 1;
+    if (c.bnorm == 0.0 || c.res <= c.tol || rr == 0.0) {
+      c.status = 0;
+      c.active = 0;
+    }
+  } else if (op == 1) {
+    const double r0v = bsum(part, S, 0, s);
+    if (r0v == 0.0 || !isfinite(r0v)) { c.bd = 1; c.alpha = 0.0; }
+    else c.alpha = c.rho / r0v;
+  } else if (op == 2) {
+    const double ts = bsum(part, S, 0, s), tt = bsum(part, S, 1, s);
+    c.omega = tt > 0.0 ? ts / tt : 0.0;
+  } else if (op == 3) {
+    const double rr = bsum(part, S, 0, s), r0r = bsum(part, S, 1, s);
+    c.res = c.bnorm > 0 ? sqrt(rr) / c.bnorm : 0.0;
+    c.iters += 1;
+    if (!isfinite(c.res) || c.res > c.div_limit) {
+      c.status = 2;
+      c.active = 0;
+    } else if (c.res <= c.tol) {
+      c.status = 0;
+      c.active = 0;
+    } else if (c.bd || c.omega == 0.0 || r0r == 0.0) {
+      c.status = 2;
+      c.active = 0;
+    } else if (c.iters >= c.max_iters) {
+      c.status = 1;
+      c.active = 0;
+    } else {
+      c.beta = (r0r / c.rho) * (c.alpha / c.omega);
+      c.rho = r0r;
+    }
+  }
+}
+
+// Masked vector updates (rows of inactive systems are left alone):
+//   0 p = r + beta (p - omega v)          a = p, c1 = r, c2 = v
+//   1 s = r - alpha v                     a = s, c1 = r, c2 = v
+//   2 x += alpha ph + omega sh; r = s - omega t    a = x, bb = r, c1 = ph, c2 = sh, c3 = s, c4 = t
+//   3 restart: r0 = r, p = 0, v = 0       a = r0, bb = p, cw = v, c1 = r
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bupdate(int op, int n, const int *__restrict__ row_sys,
+                                                    const BSys *__restrict__ bs, T *a, T *bb, T *cw, const T *c1,
+                                                    const T *c2, const T *c3, const T *c4) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const BSys &c = bs[row_sys[i]];
+    if (!c.active) continue;
+    if (op == 0) {
+      a[i] = c1[i] + (T)c.beta * (a[i] - (T)c.omega * c2[i]);
+    } else if (op == 1) {
+      a[i] = c1[i] - (T)c.alpha * c2[i];
+    } else if (op == 2) {
+      a[i] += (T)c.alpha * c1[i] + (T)c.omega * c2[i];
+      bb[i] = c3[i] - (T)c.omega * c4[i];
+    } else {
+      a[i] = c1[i];
+      bb[i] = 0;
+      cw[i] = 0;
+    }
+  }
+}
+
+// Masked FP64 refinement helpers (mask = int per system):
+//   0 r32 = r64 / nrm[s]      1 x += nrm[s] * d      2 dst = src (FP64 copy)     3 x = 0     4 f = 0
+__global__ __launch_bounds__(kBlock) void k_bref(int op, int n, const int *__restrict__ row_sys,
+                                                 const int *__restrict__ mask, double *__restrict__ x,
+                                                 const double *__restrict__ src, float *__restrict__ f,
+                                                 const double *__restrict__ nrm) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int s = row_sys[i];
+    if (!mask[s]) continue;
+    if (op == 0) f[i] = (float)(src[i] * (nrm[s] > 0 ? 1.0 / nrm[s] : 1.0));
+    else if (op == 1) x[i] += (double)f[i] * nrm[s];
+    else if (op == 2) x[i] = src[i];
+    else if (op == 3) x[i] = 0.0;
+    else f[i] = 0.0f;
+  }
+}
+
+// Sets the per-system state before a batched solve: active from mask, tolerance,
+// iteration cap and divergence limit.
+__global__ void k_binit(int S, BSys *bs, const int *__restrict__ mask, double tol, int max_iters, double div_limit) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  BSys &c = bs[s];
+  c.active = mask[s] ? 1 : 0;
+  c.status = mask[s] ? 1 : 0;
+  c.tol = tol;
+  c.max_iters = max_iters;
+  c.div_limit = div_limit;
+  c.iters = 0;
+  c.bd = 0;
+}
+
+// Triangular sweep of segment-local level schedules, one workgroup per
+// segment (one system of a batch): the workgroup walks its segment's levels in
+// order, rows of a level spread over its threads (LPR lanes per row), a
+// barrier between levels.  Communication stays inside one CU.
+//   lower: z[row] = in[row] - sum L[k] z[col[k]]
+//   upper: z[row] = dinv[row] (z[row] - sum U[k] z[col[k]])
+constexpr int kSweepThreads = 1024;
+template <int LPR, typename TV, typename T, bool UPPER>
+__global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict__ seg_lvl, const int *__restrict__ lvl_ptr,
+                                                           const int *__restrict__ rows, const int *__restrict__ rp,
+                                                           const int *__restrict__ ci, const TV *__restrict__ v,
+                                                           const T *__restrict__ in, T *z, const TV *__restrict__ dinv,
+                                                           const BSys *__restrict__ bs) {
+  const int s = blockIdx.x;
+  if (bs && !bs[s].active) return;
+  const int l0 = seg_lvl[s], l1 = seg_lvl[s + 1];
+  const int g = threadIdx.x / LPR, lane = threadIdx.x & (LPR - 1);
+  for (int l = l0; l < l1; ++l) {
+    const int q0 = lvl_ptr[l], q1 = lvl_ptr[l + 1];
+    for (int q = q0 + g; q < q1; q += kSweepThreads / LPR) {
+      const int row = rows[q];
+      T acc = 0;
+      for (int k = rp[row] + lane; k < rp[row + 1]; k += LPR) acc += (T)v[k] * z[ci[k]];
+#pragma unroll
+      for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+      if (lane == 0) {
+        if (UPPER) z[row] = (T)dinv[row] * (z[row] - acc);
+        else z[row] = in[row] - acc;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------- host side
@@ -578,6 +789,15 @@ struct shesolve_ctx {
   int *d_lblk = nullptr, *d_ublk = nullptr;
   Levels lvL, lvU;
   int lpr = 8, flpr = 8;
+  // segment-local level schedules (systems of a batch): segment boundaries in
+  // rows, and per segment its levels (seg -> level range, level -> row range
+  // of the level-ordered row list) for the one-workgroup-per-segment sweeps
+  std::vector<int> seg_ptr;
+  int nseg = 0;
+  int *d_segL = nullptr, *d_lvlL = nullptr, *d_rowsL = nullptr;
+  int *d_segU = nullptr, *d_lvlU = nullptr, *d_rowsU = nullptr;
+  const BSys *seg_mask = nullptr;  // device per-segment state; inactive segments are skipped
+  int seg_levels_max = 0;
   // vectors (FP64 set and FP32 set)
   double *v64[9] = {};
   float *v32[9] = {};
@@ -587,10 +807,15 @@ struct shesolve_ctx {
   int vec_n = 0;
   double last_setup_ms = 0;
   // profiling
-  bool prof = false;
-  double ktime[4] = {0, 0, 0, 0};
-  int64_t kcount[4] = {0, 0, 0, 0};
+  int prof = 0;  // 0 off, 1 every timed launch waited for, 2 SpMV launches timed live
+  double ktime[5] = {0, 0, 0, 0, 0};
+  int64_t kcount[5] = {0, 0, 0, 0, 0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // live SpMV timing (profiling mode 2): event pairs recorded around SpMV
+  // launches without waiting, resolved when the statistics are read
+  std::vector<hipEvent_t> evpool;
+  std::vector<int> evkind;  // kind per recorded pair
+  int evused = 0;
 };
 
 namespace {
@@ -611,14 +836,37 @@ int pick_lpr(double avg) {
 }
 
 // timing helper: brackets a launch with events when profiling is on
+void resolve_events(shesolve_ctx *c) {
+  if (!c->evused) return;
+  hipEventSynchronize(c->evpool[2 * c->evused - 1]);
+  for (int i = 0; i < c->evused; ++i) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->evpool[2 * i], c->evpool[2 * i + 1]);
+    c->ktime[c->evkind[i]] += ms * 1e3;
+    c->kcount[c->evkind[i]] += 1;
+  }
+  c->evused = 0;
+}
+
 struct KTimer {
   shesolve_ctx *c;
   int kind;
+  int slot = -1;
   KTimer(shesolve_ctx *c_, int k) : c(c_), kind(k) {
-    if (c->prof) hipEventRecord(c->ev0, c->stream);
+    if (c->prof == 2) {
+      if (kind != 0 && kind != 4) return;
+      if (2 * (c->evused + 1) > (int)c->evpool.size()) resolve_events(c);
+      slot = c->evused++;
+      c->evkind[slot] = kind;
+      hipEventRecord(c->evpool[2 * slot], c->stream);
+    } else if (c->prof) {
+      hipEventRecord(c->ev0, c->stream);
+    }
   }
   ~KTimer() {
-    if (c->prof) {
+    if (c->prof == 2) {
+      if (slot >= 0) hipEventRecord(c->evpool[2 * slot + 1], c->stream);
+    } else if (c->prof) {
       hipEventRecord(c->ev1, c->stream);
       hipEventSynchronize(c->ev1);
       float ms = 0;
@@ -653,7 +901,7 @@ void launch_stream(shesolve_ctx *c, int g, int nblk, const int *blocks, const in
 template <typename TV, typename T, typename TA>
 void launch_spmv(shesolve_ctx *c, int /*lpr*/, const TV *v, const T *x, T *y, const T *b, const T *d0, const T *d1,
                  double *part) {
-  KTimer kt(c, 0);
+  KTimer kt(c, sizeof(TV) == 8 ? 4 : 0);
   launch_stream<TV, T, MODE_SPMV>(c, stream_grid(c->nablk, part != nullptr), c->nablk, c->d_ablk, c->d_rp, c->d_ci,
                                   v, x, y, b, (const int *)nullptr, (const T *)nullptr, (T *)nullptr, d0, d1, part);
 }
@@ -673,6 +921,21 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
   }
   const int L = c->flpr;
   const int *perm = c->d_perm;
+  if (c->nseg > 0 && !perm) {
+    KTimer kt(c, 1);
+#define WG_CASE(LL)                                                                                            \
+  case LL:                                                                                                     \
+    hipLaunchKernelGGL((k_wgsweep<LL, TV, T, false>), dim3(c->nseg), dim3(kSweepThreads), 0, c->stream,        \
+                       c->d_segL, c->d_lvlL, c->d_rowsL, c->d_Lrp, c->d_Lci, Lv, in, out, (const TV *)nullptr, \
+                       c->seg_mask);                                                                           \
+    hipLaunchKernelGGL((k_wgsweep<LL, TV, T, true>), dim3(c->nseg), dim3(kSweepThreads), 0, c->stream,         \
+                       c->d_segU, c->d_lvlU, c->d_rowsU, c->d_Urp, c->d_Uci, Uv, (const T *)nullptr, out, dinv, \
+                       c->seg_mask);                                                                           \
+    break;
+    switch (L) { WG_CASE(2) WG_CASE(4) WG_CASE(8) WG_CASE(16) WG_CASE(32) default: WG_CASE(64) }
+#undef WG_CASE
+    return;
+  }
   T *z = out;
   if (perm) {
     if constexpr (sizeof(T) == 8) z = (T *)c->z64; else z = (T *)c->z32;
@@ -746,6 +1009,9 @@ void free_matrix(shesolve_ctx *c) {
   c->runL.clear(); c->runU.clear();
   dfree(c->lvL.d_rows); dfree(c->lvU.d_rows);
   c->lvL.ptr.clear(); c->lvU.ptr.clear();
+  dfree(c->d_segL); dfree(c->d_lvlL); dfree(c->d_rowsL);
+  dfree(c->d_segU); dfree(c->d_lvlU); dfree(c->d_rowsU);
+  c->nseg = 0;
 }
 
 // Row blocks over rows [r0, r1) of a CSR (host row pointer): consecutive rows
@@ -791,6 +1057,133 @@ void compute_levels(const std::vector<int> &rp, const std::vector<int> &ci, cons
     maxl = std::max(maxl, l);
   }
   bucket(maxl, Uptr, Urows);
+}
+
+// Segment-local level schedules of both triangles of the factor: for each
+// segment (a contiguous row range that no factor entry leaves), its rows
+// ordered by level, ascending row within a level.
+int segment_levels(shesolve_ctx *c) {
+  const int n = c->n;
+  const int S = (int)c->seg_ptr.size() - 1;
+  const auto &rp = c->h_frp, &ci = c->h_fci, &dg = c->h_fdg;
+  std::vector<int> lev(n);
+  for (int s = 0; s < S; ++s)
+    for (int i = c->seg_ptr[s]; i < c->seg_ptr[s + 1]; ++i)
+      for (int k = rp[i]; k < rp[i + 1]; ++k)
+        if (ci[k] < c->seg_ptr[s] || ci[k] >= c->seg_ptr[s + 1]) {
+          set_err("factor entry (%d, %d) couples two systems", i, ci[k]);
+          return 2;
+        }
+  c->seg_levels_max = 0;
+  for (int upper = 0; upper < 2; ++upper) {
+    std::vector<int> seg(S + 1, 0), lvl(1, 0), rows;
+    rows.reserve(n);
+    for (int s = 0; s < S; ++s) {
+      const int r0 = c->seg_ptr[s], r1 = c->seg_ptr[s + 1];
+      int maxl = -1;
+      if (!upper) {
+        for (int i = r0; i < r1; ++i) {
+          int l = 0;
+          for (int k = rp[i]; k < dg[i]; ++k) l = std::max(l, lev[ci[k]] + 1);
+          lev[i] = l;
+          maxl = std::max(maxl, l);
+        }
+      } else {
+        for (int i = r1 - 1; i >= r0; --i) {
+          int l = 0;
+          for (int k = dg[i] + 1; k < rp[i + 1]; ++k) l = std::max(l, lev[ci[k]] + 1);
+          lev[i] = l;
+          maxl = std::max(maxl, l);
+        }
+      }
+      std::vector<int> cnt(maxl + 2, 0);
+      for (int i = r0; i < r1; ++i) cnt[lev[i] + 1]++;
+      for (int l = 0; l <= maxl; ++l) cnt[l + 1] += cnt[l];
+      const int base = (int)rows.size();
+      rows.resize(base + (r1 - r0));
+      std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+      for (int i = r0; i < r1; ++i) rows[base + fillp[lev[i]]++] = i;
+      for (int l = 0; l <= maxl; ++l) lvl.push_back(base + cnt[l + 1]);
+      seg[s + 1] = (int)lvl.size() - 1;
+      c->seg_levels_max = std::max(c->seg_levels_max, maxl + 1);
+    }
+    int *&dseg = upper ? c->d_segU : c->d_segL;
+    int *&dlvl = upper ? c->d_lvlU : c->d_lvlL;
+    int *&drows = upper ? c->d_rowsU : c->d_rowsL;
+    HIPCHK(hipMalloc(&dseg, sizeof(int) * seg.size()));
+    HIPCHK(hipMalloc(&dlvl, sizeof(int) * lvl.size()));
+    HIPCHK(hipMalloc(&drows, sizeof(int) * std::max<size_t>(rows.size(), 1)));
+    HIPCHK(hipMemcpy(dseg, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dlvl, lvl.data(), sizeof(int) * lvl.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(drows, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice));
+  }
+  c->nseg = S;
+  return 0;
+}
+
+// Level-of-fill pattern of ILU(k), k = prm.fill_level: entries of A have level
+// 0; eliminating row j from row i gives column c > j the level
+// lev(i,j) + lev(j,c) + 1, kept (minimum over j) when <= k.  Rows in order, the
+// lower part of a row in ascending column order (a min-heap of pending
+// columns).  pos maps factor entries to A's entries, -1 for fill.
+int iluk_pattern(shesolve_ctx *c, std::vector<int> &pos) {
+  const int n = c->n, k = c->prm.fill_level;
+  const auto &rp = c->h_rp, &ci = c->h_ci;
+  std::vector<int> frp(n + 1, 0), fci, udg(n);
+  std::vector<unsigned char> flev;
+  fci.reserve(c->h_ci.size() * 3 / 2);
+  flev.reserve(c->h_ci.size() * 3 / 2);
+  pos.clear();
+  pos.reserve(c->h_ci.size() * 3 / 2);
+  std::vector<int> lev(n, -1), src(n, -1), touched;
+  std::priority_queue<int, std::vector<int>, std::greater<int>> heap;
+  for (int i = 0; i < n; ++i) {
+    touched.clear();
+    for (int q = rp[i]; q < rp[i + 1]; ++q) {
+      lev[ci[q]] = 0;
+      src[ci[q]] = q;
+      touched.push_back(ci[q]);
+      if (ci[q] < i) heap.push(ci[q]);
+    }
+    while (!heap.empty()) {
+      const int j = heap.top();
+      heap.pop();
+      const int lj = lev[j];
+      for (int q = udg[j] + 1; q < frp[j + 1]; ++q) {
+        const int col = fci[q], nl = lj + flev[q] + 1;
+        if (nl > k) continue;
+        if (lev[col] < 0) {
+          lev[col] = nl;
+          touched.push_back(col);
+          if (col < i) heap.push(col);
+        } else if (nl < lev[col]) {
+          lev[col] = nl;
+        }
+      }
+    }
+    std::sort(touched.begin(), touched.end());
+    udg[i] = -1;
+    for (int col : touched) {
+      if (col == i) udg[i] = (int)fci.size();
+      fci.push_back(col);
+      flev.push_back((unsigned char)lev[col]);
+      pos.push_back(src[col]);
+      lev[col] = -1;
+      src[col] = -1;
+    }
+    if (udg[i] < 0) {
+      set_err("ILU(k): row %d has no diagonal entry", i);
+      return 2;
+    }
+    if ((int64_t)fci.size() > 0x7fffffff) {
+      set_err("ILU(k) factor too large for 32-bit indices");
+      return 2;
+    }
+    frp[i + 1] = (int)fci.size();
+  }
+  c->h_frp.swap(frp);
+  c->h_fci.swap(fci);
+  return 0;
 }
 
 int analyse_and_factor(shesolve_ctx *c) {
@@ -871,6 +1264,8 @@ int analyse_and_factor(shesolve_ctx *c) {
       }
       c->h_frp[p + 1] = w;
     }
+  } else if (c->prm.precond == SHESOLVE_PRECOND_ILUK && c->prm.fill_level > 0) {
+    if (int e = iluk_pattern(c, pos)) return e;
   } else {
     c->h_frp = c->h_rp;
     c->h_fci = c->h_ci;
@@ -895,6 +1290,9 @@ int analyse_and_factor(shesolve_ctx *c) {
   }
   std::vector<int> Lptr, Lrows, Uptr, Urows;
   compute_levels(c->h_frp, c->h_fci, c->h_fdg, n, Lptr, Lrows, Uptr, Urows);
+  if (!c->seg_ptr.empty() && perm.empty()) {
+    if (int e = segment_levels(c)) return e;
+  }
   if (!perm.empty()) {
     // multicolour ordering: colour q occupies the contiguous range
     // [cstart[q], cstart[q+1]) of the factor; use whole colours as levels
@@ -1027,7 +1425,8 @@ int refactor(shesolve_ctx *c) {
   for (int l = 0; l < c->lvL.count(); ++l) {
     int cnt = c->lvL.ptr[l + 1] - c->lvL.ptr[l];
     hipLaunchKernelGGL(k_ilu0_level, dim3(cnt), dim3(64), 0, c->stream, cnt, c->lvL.d_rows + c->lvL.ptr[l],
-                       c->d_frp, c->d_fci, c->d_fdg, c->d_f64);
+                       c->d_frp, c->d_fci, c->d_fdg, c->d_f64,
+                       c->prm.precond == SHESOLVE_PRECOND_ILUK ? c->prm.drop_tol : 0.0);
   }
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->fnnz, kBlock * 4)), dim3(kBlock), 0, c->stream,
@@ -1367,7 +1766,9 @@ int tune_stream(shesolve_ctx *c) {
 }
 
 int set_matrix_common(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, const int64_t *rp,
-                      const int32_t *ci, const double *val, int nblocks, const int64_t *ranges) {
+                      const int32_t *ci, const double *val, int nblocks, const int64_t *ranges,
+                      int val_device = -1) {
+  if (val_device < 0) val_device = device_src ? 1 : 0;
   auto t0 = std::chrono::steady_clock::now();
   HIPCHK(hipSetDevice(c->device));
   free_matrix(c);
@@ -1388,7 +1789,7 @@ int set_matrix_common(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, 
   HIPCHK(hipMalloc(&c->d_v64, sizeof(double) * std::max<int64_t>(nnz, 1)));
   HIPCHK(hipMemcpyAsync(c->d_rp, c->h_rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_ci, c->h_ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_v64, val, sizeof(double) * nnz, device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+  HIPCHK(hipMemcpyAsync(c->d_v64, val, sizeof(double) * nnz, val_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         c->stream));
   c->lpr = c->prm.lanes_per_row > 0 ? c->prm.lanes_per_row : pick_lpr((double)nnz / n);
   if (int e = tune_stream(c)) return e;
@@ -1396,6 +1797,437 @@ int set_matrix_common(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, 
   if (int e = upload_values(c)) return e;
   if (int e = ensure_vectors(c)) return e;
   c->last_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+}  // namespace
+
+
+// ================================================================ batched solves
+struct shesolve_batch_ctx {
+  shesolve_ctx *c = nullptr;       // the concatenated block-diagonal system
+  int S = 0;
+  std::vector<int> sys_ptr;        // row offsets of the systems
+  std::vector<int64_t> val_ptr;    // value offsets of the systems
+  int *d_sys_ptr = nullptr, *d_row_sys = nullptr, *d_mask = nullptr;
+  BSys *d_bs = nullptr;
+  double *d_bpart = nullptr, *d_nrm = nullptr;
+  double *x64 = nullptr, *b64 = nullptr, *xprev = nullptr;
+  float *dd = nullptr;
+  std::vector<BSys> h_bs;
+  std::vector<shesolve_report> rep;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // one iteration: [0] FP64, [1] FP32
+  bool use_graph = true;
+  int poll = 8;                    // iterations between host checks of the masks
+  double last_setup_ms = 0;
+};
+
+namespace {
+
+void batch_free(shesolve_batch_ctx *b) {
+  if (b->c) b->c->seg_mask = nullptr;
+  for (auto &g : b->graph)
+    if (g) { hipGraphExecDestroy(g); g = nullptr; }
+  dfree(b->d_sys_ptr); dfree(b->d_row_sys); dfree(b->d_mask); dfree(b->d_bs);
+  dfree(b->d_bpart); dfree(b->d_nrm); dfree(b->x64); dfree(b->b64); dfree(b->xprev); dfree(b->dd);
+}
+
+template <typename T>
+void bdot(shesolve_batch_ctx *b, const int *mask, const T *a0, const T *b0, const T *a1, const T *b1) {
+  shesolve_ctx *c = b->c;
+  KTimer kt(c, 3);
+  if (a1)
+    hipLaunchKernelGGL((k_bdot<T, 2>), dim3(kBatchParts, b->S), dim3(kBlock), 0, c->stream, b->S, b->d_sys_ptr,
+                       b->d_bs, mask, a0, b0, a1, b1, b->d_bpart);
+  else
+    hipLaunchKernelGGL((k_bdot<T, 1>), dim3(kBatchParts, b->S), dim3(kBlock), 0, c->stream, b->S, b->d_sys_ptr,
+                       b->d_bs, mask, a0, b0, a1, b1, b->d_bpart);
+}
+void bfin(shesolve_batch_ctx *b, int op) {
+  hipLaunchKernelGGL(k_bfinalize, dim3((b->S + 63) / 64), dim3(64), 0, b->c->stream, op, b->S, b->d_bpart, b->d_bs,
+                     b->d_nrm);
+}
+template <typename T>
+void bupd(shesolve_batch_ctx *b, int op, T *a, T *bb, T *cw, const T *c1, const T *c2, const T *c3, const T *c4) {
+  shesolve_ctx *c = b->c;
+  KTimer kt(c, 3);
+  hipLaunchKernelGGL((k_bupdate<T>), dim3(kGrid), dim3(kBlock), 0, c->stream, op, c->n, b->d_row_sys, b->d_bs, a, bb,
+                     cw, c1, c2, c3, c4);
+}
+void bref(shesolve_batch_ctx *b, int op, double *x, const double *src, float *f) {
+  hipLaunchKernelGGL(k_bref, dim3(kGrid), dim3(kBlock), 0, b->c->stream, op, b->c->n, b->d_row_sys, b->d_mask, x, src,
+                     f, b->d_nrm);
+}
+int upload_mask(shesolve_batch_ctx *b, const std::vector<int> &m) {
+  HIPCHK(hipMemcpyAsync(b->d_mask, m.data(), sizeof(int) * b->S, hipMemcpyHostToDevice, b->c->stream));
+  return 0;
+}
+// Per-system norms sqrt((u,u)) of the masked systems into d_nrm and `out`.
+template <typename T>
+int bnorms(shesolve_batch_ctx *b, const T *u, std::vector<double> &out) {
+  bdot<T>(b, b->d_mask, u, u, (const T *)nullptr, (const T *)nullptr);
+  bfin(b, 5);
+  out.resize(b->S);
+  HIPCHK(hipMemcpyAsync(out.data(), b->d_nrm, sizeof(double) * b->S, hipMemcpyDeviceToHost, b->c->stream));
+  HIPCHK(hipStreamSynchronize(b->c->stream));
+  return 0;
+}
+
+// One iteration of the batched right-preconditioned BiCGStab (kernels only,
+// so that it can be captured into a graph).
+template <typename TV, typename T>
+void bbicg_iter(shesolve_batch_ctx *b, T *x) {
+  shesolve_ctx *c = b->c;
+  T **V;
+  const TV *Av, *Mv;
+  if constexpr (sizeof(T) == 8) { V = (T **)c->v64; Av = (const TV *)c->d_v64; Mv = (const TV *)c->d_f64; }
+  else { V = (T **)c->v32; Av = (const TV *)c->d_v32; Mv = (const TV *)c->d_f32; }
+  T *r = V[0], *r0 = V[1], *p = V[2], *v = V[3], *ph = V[4], *s = V[5], *sh = V[6], *t = V[7];
+  bupd<T>(b, 0, p, nullptr, nullptr, r, v, nullptr, nullptr);
+  launch_precond<TV, T>(c, Mv, p, ph);
+  launch_spmv<TV, T, T>(c, c->lpr, Av, ph, v, nullptr, nullptr, nullptr, nullptr);
+  bdot<T>(b, nullptr, r0, v, (const T *)nullptr, (const T *)nullptr);
+  bfin(b, 1);
+  bupd<T>(b, 1, s, nullptr, nullptr, r, v, nullptr, nullptr);
+  launch_precond<TV, T>(c, Mv, s, sh);
+  launch_spmv<TV, T, T>(c, c->lpr, Av, sh, t, nullptr, nullptr, nullptr, nullptr);
+  bdot<T>(b, nullptr, t, s, t, t);
+  bfin(b, 2);
+  bupd<T>(b, 2, x, r, nullptr, ph, sh, s, t);
+  bdot<T>(b, nullptr, r, r, r0, r);
+  bfin(b, 3);
+}
+
+// Batched BiCGStab on the systems selected by mask (host vector), from the
+// initial guess in x.  Per-system results in b->h_bs.
+template <typename TV, typename T>
+int bbicg_run(shesolve_batch_ctx *b, const T *rhs, T *x, double tol, int max_iters, double div_limit,
+              const std::vector<int> &mask) {
+  shesolve_ctx *c = b->c;
+  T **V;
+  const TV *Av;
+  if constexpr (sizeof(T) == 8) { V = (T **)c->v64; Av = (const TV *)c->d_v64; }
+  else { V = (T **)c->v32; Av = (const TV *)c->d_v32; }
+  T *r = V[0], *r0 = V[1], *p = V[2], *v = V[3];
+  if (int e = upload_mask(b, mask)) return e;
+  hipLaunchKernelGGL(k_binit, dim3((b->S + 63) / 64), dim3(64), 0, c->stream, b->S, b->d_bs, b->d_mask, tol, max_iters,
+                     div_limit);
+  launch_spmv<TV, T, T>(c, c->lpr, Av, x, r, rhs, nullptr, nullptr, nullptr);
+  bupd<T>(b, 3, r0, p, v, r, nullptr, nullptr, nullptr);
+  bdot<T>(b, nullptr, r, r, rhs, rhs);
+  bfin(b, 0);
+  HIPCHK(hipGetLastError());
+  const int gi = sizeof(T) == 8 ? 0 : 1;
+  const bool graph = b->use_graph && !c->prof;
+  if (graph && !b->graph[gi]) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    bbicg_iter<TV, T>(b, x);
+    HIPCHK(hipStreamEndCapture(c->stream, &g));
+    HIPCHK(hipGraphInstantiate(&b->graph[gi], g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+  }
+  b->h_bs.resize(b->S);
+  for (int done = 0; done < max_iters;) {
+    HIPCHK(hipMemcpyAsync(b->h_bs.data(), b->d_bs, sizeof(BSys) * b->S, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    bool any = false;
+    for (auto &q : b->h_bs) any |= q.active != 0;
+    if (!any) break;
+    const int k = std::min(b->poll, max_iters - done);
+    for (int i = 0; i < k; ++i) {
+      if (graph) HIPCHK(hipGraphLaunch(b->graph[gi], c->stream));
+      else bbicg_iter<TV, T>(b, x);
+    }
+    done += k;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(b->h_bs.data(), b->d_bs, sizeof(BSys) * b->S, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// True FP64 residuals ||b_s - A_s x_s|| of the masked systems (r64 = b - A x).
+int bresidual(shesolve_batch_ctx *b, std::vector<double> &rn) {
+  shesolve_ctx *c = b->c;
+  double *r64 = c->v64[8];
+  launch_spmv<double, double, double>(c, c->lpr, c->d_v64, b->x64, r64, b->b64, nullptr, nullptr, nullptr);
+  return bnorms<double>(b, r64, rn);
+}
+
+int batch_solve_internal(shesolve_batch_ctx *b, int warm) {
+  shesolve_ctx *c = b->c;
+  const int S = b->S;
+  const auto &prm = c->prm;
+  b->rep.assign(S, shesolve_report{});
+  std::vector<int> all(S, 1), act(S, 0), fb(S, 0);
+  std::vector<double> bn, rn, prev(S, -1.0);
+  if (int e = upload_mask(b, all)) return e;
+  if (!warm) bref(b, 3, b->x64, nullptr, nullptr);
+  if (int e = bnorms<double>(b, b->b64, bn)) return e;
+  std::vector<int> zero_rhs(S, 0);
+  bool anyz = false;
+  for (int s = 0; s < S; ++s)
+    if (bn[s] == 0.0) { zero_rhs[s] = 1; anyz = true; }
+  if (anyz) {
+    if (int e = upload_mask(b, zero_rhs)) return e;
+    bref(b, 3, b->x64, nullptr, nullptr);
+  }
+  for (int s = 0; s < S; ++s) {
+    b->rep[s].status = zero_rhs[s] ? 0 : 1;
+    act[s] = !zero_rhs[s];
+  }
+  if (prm.precision == SHESOLVE_FP64) {
+    for (int attempt = 0; attempt < 4; ++attempt) {
+      bool any = false;
+      for (int s = 0; s < S; ++s) any |= act[s] != 0;
+      if (!any) break;
+      int cap = 0;
+      for (int s = 0; s < S; ++s)
+        if (act[s]) cap = std::max(cap, prm.max_iters - b->rep[s].iterations);
+      if (cap <= 0) break;
+      if (int e = bbicg_run<double, double>(b, b->b64, b->x64, prm.tol, cap, 1e300, act)) return e;
+      for (int s = 0; s < S; ++s)
+        if (act[s]) {
+          b->rep[s].iterations += b->h_bs[s].iters;
+          b->rep[s].status = b->h_bs[s].status;
+        }
+      // confirm converged systems with the explicit residual; restart the others
+      if (int e = upload_mask(b, act)) return e;
+      if (int e = bresidual(b, rn)) return e;
+      for (int s = 0; s < S; ++s) {
+        if (!act[s]) continue;
+        b->rep[s].rel_residual = rn[s] / bn[s];
+        if (b->rep[s].rel_residual <= prm.tol) { b->rep[s].status = 0; act[s] = 0; }
+        else if (b->rep[s].status != 0 || b->rep[s].iterations >= prm.max_iters) act[s] = 0;
+        else b->rep[s].status = 1;  // recurrence converged but the true residual did not: restart
+      }
+    }
+  } else {
+    float *r32 = c->v32[8];
+    // an FP32 inner solve that stalls (iteration cap) or breaks down hands its
+    // system to the FP64 solve after its correction has been checked
+    std::vector<int> inner_failed(S, 0);
+    const int inner_cap = std::min(prm.max_iters, 1000);
+    for (int k = 0; k <= prm.max_ir_steps; ++k) {
+      if (int e = upload_mask(b, act)) return e;
+      if (int e = bresidual(b, rn)) return e;
+      std::vector<int> revert(S, 0);
+      bool anyrev = false, any = false;
+      for (int s = 0; s < S; ++s) {
+        if (!act[s]) continue;
+        if (prev[s] >= 0 && !(rn[s] < prev[s])) {
+          // the last correction did not reduce the FP64 residual: undo it and
+          // finish this system with FP64 BiCGStab
+          revert[s] = 1;
+          anyrev = true;
+          fb[s] = 1;
+          act[s] = 0;
+          continue;
+        }
+        b->rep[s].rel_residual = rn[s] / bn[s];
+        if (b->rep[s].rel_residual <= prm.tol) { b->rep[s].status = 0; act[s] = 0; }
+        else if (inner_failed[s]) { fb[s] = 1; act[s] = 0; }
+        else if (k == prm.max_ir_steps || b->rep[s].iterations >= prm.max_iters) act[s] = 0;
+        any |= act[s] != 0;
+      }
+      if (anyrev) {
+        if (int e = upload_mask(b, revert)) return e;
+        bref(b, 2, b->x64, b->xprev, nullptr);
+      }
+      if (!any) break;
+      if (int e = upload_mask(b, act)) return e;
+      // d_nrm holds ||r_s|| of the masked systems from bresidual
+      bref(b, 2, b->xprev, b->x64, nullptr);
+      bref(b, 0, nullptr, c->v64[8], r32);
+      bref(b, 4, nullptr, nullptr, b->dd);
+      for (int s = 0; s < S; ++s)
+        if (act[s]) { prev[s] = rn[s]; b->rep[s].ir_steps++; }
+      if (int e = bbicg_run<float, float>(b, r32, b->dd, prm.inner_tol, inner_cap, 1e4, act)) return e;
+      for (int s = 0; s < S; ++s)
+        if (act[s]) {
+          b->rep[s].iterations += b->h_bs[s].iters;
+          if (b->h_bs[s].status != 0) inner_failed[s] = 1;
+        }
+      // x += ||r_s|| d  (bbicg_run left d_mask = act and d_nrm untouched)
+      bref(b, 1, b->x64, nullptr, b->dd);
+    }
+    bool anyfb = false;
+    for (int s = 0; s < S; ++s) anyfb |= fb[s] != 0;
+    if (anyfb) {
+      if (int e = bbicg_run<double, double>(b, b->b64, b->x64, prm.tol, prm.max_iters, 1e300, fb)) return e;
+      for (int s = 0; s < S; ++s)
+        if (fb[s]) {
+          b->rep[s].iterations += b->h_bs[s].iters;
+          b->rep[s].status = b->h_bs[s].status;
+        }
+    }
+  }
+  // final explicit residuals of every system with a nonzero right-hand side
+  for (int s = 0; s < S; ++s) act[s] = !zero_rhs[s];
+  if (int e = upload_mask(b, act)) return e;
+  if (int e = bresidual(b, rn)) return e;
+  for (int s = 0; s < S; ++s) {
+    if (zero_rhs[s]) { b->rep[s].rel_residual = 0; b->rep[s].status = 0; continue; }
+    b->rep[s].rel_residual = rn[s] / bn[s];
+    if (b->rep[s].rel_residual <= prm.tol) b->rep[s].status = 0;
+    else if (b->rep[s].status == 0) b->rep[s].status = 1;
+  }
+  return 0;
+}
+
+int batch_set_systems(shesolve_batch_ctx *b, bool device_src, int count, const int64_t *n, const int64_t *const *rp,
+                      const int32_t *const *ci, const double *const *val, const int *nblocks,
+                      const int64_t *const *ranges) {
+  auto t0 = std::chrono::steady_clock::now();
+  shesolve_ctx *c = b->c;
+  HIPCHK(hipSetDevice(c->device));
+  if (count <= 0) { set_err("count must be positive"); return 2; }
+  if (!n) { set_err("n is NULL"); return 3; }
+  if (!rp) { set_err("row_ptr is NULL"); return 4; }
+  if (!ci) { set_err("col is NULL"); return 5; }
+  if (!val) { set_err("val is NULL"); return 6; }
+  batch_free(b);
+  int64_t N = 0, NNZ = 0;
+  std::vector<int64_t> nnz(count);
+  std::vector<std::vector<int64_t>> hrp(count);
+  for (int s = 0; s < count; ++s) {
+    if (n[s] <= 0 || !rp[s] || !ci[s] || !val[s]) {
+      set_err("system %d: empty or missing arrays", s);
+      return 3;
+    }
+    hrp[s].resize(n[s] + 1);
+    if (device_src)
+      HIPCHK(hipMemcpy(hrp[s].data(), rp[s], sizeof(int64_t) * (n[s] + 1), hipMemcpyDeviceToHost));
+    else
+      std::copy(rp[s], rp[s] + n[s] + 1, hrp[s].begin());
+    nnz[s] = hrp[s][n[s]] - hrp[s][0];
+    if (hrp[s][0] != 0 || nnz[s] < 0) { set_err("system %d: row_ptr must start at 0", s); return 4; }
+    N += n[s];
+    NNZ += nnz[s];
+  }
+  if (N > 0x7fffffff || NNZ > 0x7fffffff) {
+    set_err("batch too large for 32-bit indices: %lld rows, %lld entries", (long long)N, (long long)NNZ);
+    return 2;
+  }
+  std::vector<int64_t> crp(N + 1);
+  std::vector<int32_t> cci(NNZ);
+  std::vector<int64_t> cranges;
+  b->S = count;
+  b->sys_ptr.assign(count + 1, 0);
+  b->val_ptr.assign(count + 1, 0);
+  std::vector<int32_t> tmpc;
+  for (int s = 0; s < count; ++s) {
+    const int64_t r0 = b->sys_ptr[s], v0 = b->val_ptr[s];
+    for (int64_t i = 0; i <= n[s]; ++i) crp[r0 + i] = v0 + hrp[s][i];
+    const int32_t *src = ci[s];
+    if (device_src) {
+      tmpc.resize(nnz[s]);
+      HIPCHK(hipMemcpy(tmpc.data(), ci[s], sizeof(int32_t) * nnz[s], hipMemcpyDeviceToHost));
+      src = tmpc.data();
+    }
+    for (int64_t k = 0; k < nnz[s]; ++k) {
+      if (src[k] < 0 || src[k] >= n[s]) { set_err("system %d: column index out of range", s); return 5; }
+      cci[v0 + k] = (int32_t)(src[k] + r0);
+    }
+    if (nblocks && ranges && nblocks[s] > 0 && ranges[s]) {
+      for (int q = 0; q < nblocks[s]; ++q) {
+        cranges.push_back(r0 + std::max<int64_t>(0, std::min<int64_t>(n[s], ranges[s][2 * q])));
+        cranges.push_back(r0 + std::max<int64_t>(0, std::min<int64_t>(n[s], ranges[s][2 * q + 1])));
+      }
+    } else {
+      cranges.push_back(r0);
+      cranges.push_back(r0 + n[s]);
+    }
+    b->sys_ptr[s + 1] = (int)(r0 + n[s]);
+    b->val_ptr[s + 1] = v0 + nnz[s];
+  }
+  // values: one concatenated buffer (device staging for device sources)
+  double *dval = nullptr;
+  std::vector<double> hval;
+  if (device_src) {
+    HIPCHK(hipMalloc(&dval, sizeof(double) * std::max<int64_t>(NNZ, 1)));
+    for (int s = 0; s < count; ++s)
+      HIPCHK(hipMemcpyAsync(dval + b->val_ptr[s], val[s], sizeof(double) * nnz[s], hipMemcpyDeviceToDevice,
+                            c->stream));
+  } else {
+    hval.resize(NNZ);
+    for (int s = 0; s < count; ++s) std::copy(val[s], val[s] + nnz[s], hval.begin() + b->val_ptr[s]);
+  }
+  c->seg_ptr = b->sys_ptr;
+  int e = set_matrix_common(c, false, N, NNZ, crp.data(), cci.data(), device_src ? dval : hval.data(),
+                            (int)(cranges.size() / 2), cranges.data(), device_src ? 1 : 0);
+  if (dval) hipFree(dval);
+  if (e) return e;
+  std::vector<int> row_sys(N);
+  for (int s = 0; s < count; ++s)
+    for (int i = b->sys_ptr[s]; i < b->sys_ptr[s + 1]; ++i) row_sys[i] = s;
+  HIPCHK(hipMalloc(&b->d_sys_ptr, sizeof(int) * (count + 1)));
+  HIPCHK(hipMalloc(&b->d_row_sys, sizeof(int) * N));
+  HIPCHK(hipMalloc(&b->d_mask, sizeof(int) * count));
+  HIPCHK(hipMalloc(&b->d_bs, sizeof(BSys) * count));
+  HIPCHK(hipMalloc(&b->d_bpart, sizeof(double) * 2 * count * kBatchParts));
+  HIPCHK(hipMalloc(&b->d_nrm, sizeof(double) * count));
+  HIPCHK(hipMalloc(&b->x64, sizeof(double) * N));
+  HIPCHK(hipMalloc(&b->b64, sizeof(double) * N));
+  HIPCHK(hipMalloc(&b->xprev, sizeof(double) * N));
+  HIPCHK(hipMalloc(&b->dd, sizeof(float) * N));
+  HIPCHK(hipMemcpy(b->d_sys_ptr, b->sys_ptr.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_row_sys, row_sys.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(b->d_bs, 0, sizeof(BSys) * count));
+  c->seg_mask = b->d_bs;
+  b->last_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+int batch_update_values(shesolve_batch_ctx *b, bool device_src, const double *const *val) {
+  shesolve_ctx *c = b->c;
+  if (!c->d_rp) { set_err("no systems set"); return -1; }
+  if (!val) { set_err("val is NULL"); return 2; }
+  HIPCHK(hipSetDevice(c->device));
+  for (int s = 0; s < b->S; ++s) {
+    if (!val[s]) { set_err("system %d: values missing", s); return 2; }
+    HIPCHK(hipMemcpyAsync(c->d_v64 + b->val_ptr[s], val[s], sizeof(double) * (b->val_ptr[s + 1] - b->val_ptr[s]),
+                          device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+  }
+  return upload_values(c);
+}
+
+int batch_solve(shesolve_batch_ctx *b, bool device_src, const double *const *rhs, double *const *x, int warm,
+                shesolve_report *reports) {
+  auto t0 = std::chrono::steady_clock::now();
+  shesolve_ctx *c = b->c;
+  if (!c->d_rp) { set_err("no systems set"); return -1; }
+  if (!rhs) { set_err("b is NULL"); return 2; }
+  if (!x) { set_err("x is NULL"); return 3; }
+  HIPCHK(hipSetDevice(c->device));
+  const hipMemcpyKind in = device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  const hipMemcpyKind out = device_src ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  for (int s = 0; s < b->S; ++s) {
+    if (!rhs[s] || !x[s]) { set_err("system %d: b or x missing", s); return 2; }
+    const size_t bytes = sizeof(double) * (b->sys_ptr[s + 1] - b->sys_ptr[s]);
+    HIPCHK(hipMemcpyAsync(b->b64 + b->sys_ptr[s], rhs[s], bytes, in, c->stream));
+    if (warm) HIPCHK(hipMemcpyAsync(b->x64 + b->sys_ptr[s], x[s], bytes, in, c->stream));
+  }
+  if (!device_src) {
+    // reject non-finite right-hand sides (src/solvers/solve.hpp:55-59)
+    for (int s = 0; s < b->S; ++s)
+      for (int i = 0; i < b->sys_ptr[s + 1] - b->sys_ptr[s]; ++i)
+        if (!std::isfinite(rhs[s][i])) { set_err("system %d: right-hand side is not finite", s); return 2; }
+  }
+  if (int e = batch_solve_internal(b, warm)) return e;
+  for (int s = 0; s < b->S; ++s) {
+    const size_t bytes = sizeof(double) * (b->sys_ptr[s + 1] - b->sys_ptr[s]);
+    HIPCHK(hipMemcpyAsync(x[s], b->x64 + b->sys_ptr[s], bytes, out, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (reports)
+    for (int s = 0; s < b->S; ++s) {
+      reports[s] = b->rep[s];
+      reports[s].setup_ms = b->last_setup_ms;
+      reports[s].solve_ms = ms;
+    }
   return 0;
 }
 
@@ -1413,6 +2245,8 @@ void shesolve_default_params(shesolve_params *p) {
   p->inner_tol = 1e-2;
   p->max_ir_steps = 30;
   p->lanes_per_row = 0;
+  p->fill_level = 1;
+  p->drop_tol = 0.0;
 }
 
 const char *shesolve_last_error(void) { return g_err.c_str(); }
@@ -1421,11 +2255,14 @@ static int check_params(const shesolve_params *p) {
   if (!(p->tol > 0)) { set_err("tol must be positive"); return 1; }
   if (p->max_iters <= 0) { set_err("max_iters must be positive"); return 1; }
   if (p->precond != SHESOLVE_PRECOND_NONE && p->precond != SHESOLVE_PRECOND_ILU0 &&
-      p->precond != SHESOLVE_PRECOND_BLOCK_ILU0 && p->precond != SHESOLVE_PRECOND_MC_ILU0) {
+      p->precond != SHESOLVE_PRECOND_BLOCK_ILU0 && p->precond != SHESOLVE_PRECOND_MC_ILU0 &&
+      p->precond != SHESOLVE_PRECOND_ILUK) {
     set_err("unknown preconditioner %d", p->precond);
     return 1;
   }
   if (p->precision != SHESOLVE_FP64 && p->precision != SHESOLVE_FP32_IR) { set_err("unknown precision"); return 1; }
+  if (p->fill_level < 0 || p->fill_level > 8) { set_err("fill_level must be in [0, 8]"); return 1; }
+  if (!(p->drop_tol >= 0) || p->drop_tol >= 1) { set_err("drop_tol must be in [0, 1)"); return 1; }
   if (p->lanes_per_row != 0 && (p->lanes_per_row < 2 || p->lanes_per_row > 64 ||
                                 (p->lanes_per_row & (p->lanes_per_row - 1)))) {
     set_err("lanes_per_row must be 0 or a power of two in [2, 64]");
@@ -1471,6 +2308,7 @@ int shesolve_destroy(shesolve_handle h) {
   dfree(h->d_norm);
   hipEventDestroy(h->ev0);
   hipEventDestroy(h->ev1);
+  for (auto &e : h->evpool) hipEventDestroy(e);
   hipStreamDestroy(h->stream);
   delete h;
   return 0;
@@ -1654,14 +2492,23 @@ int shesolve_levels(shesolve_handle h, int64_t *out2) {
 
 int shesolve_set_profiling(shesolve_handle h, int on) {
   if (!h) return 1;
-  h->prof = on != 0;
-  for (int k = 0; k < 4; ++k) { h->ktime[k] = 0; h->kcount[k] = 0; }
+  if (on < 0 || on > 2) { set_err("profiling mode must be 0, 1 or 2"); return 2; }
+  resolve_events(h);
+  h->prof = on;
+  for (int k = 0; k < 5; ++k) { h->ktime[k] = 0; h->kcount[k] = 0; }
+  if (on == 2 && h->evpool.empty()) {
+    h->evpool.resize(2 * 4096);
+    h->evkind.resize(4096);
+    for (auto &e : h->evpool)
+      if (hipEventCreate(&e) != hipSuccess) { set_err("event creation failed"); return -10; }
+  }
   return 0;
 }
 
 int shesolve_kernel_stats(shesolve_handle h, int kind, double *avg_us, int64_t *launches) {
   if (!h) return 1;
-  if (kind < 0 || kind > 3) return 2;
+  if (kind < 0 || kind > 4) return 2;
+  resolve_events(h);
   if (avg_us) *avg_us = h->kcount[kind] ? h->ktime[kind] / h->kcount[kind] : 0.0;
   if (launches) *launches = h->kcount[kind];
   return 0;
@@ -1674,8 +2521,9 @@ int shesolve_time_spmv(shesolve_handle h, int fp32, int reps, double *avg_us) {
   if (!h->d_rp) { set_err("no matrix set"); return -1; }
   HIPCHK(hipSetDevice(h->device));
   if (int e = ensure_vectors(h)) return e;
-  bool was = h->prof;
-  h->prof = false;
+  resolve_events(h);
+  const int was = h->prof;
+  h->prof = 0;
   if (fp32) {
     hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, h->stream, (int64_t)h->n, h->v32[0], 1.0f);
     launch_spmv<float, float, float>(h, h->lpr, h->d_v32, h->v32[0], h->v32[1], nullptr, nullptr, nullptr, nullptr);
@@ -1701,4 +2549,84 @@ int shesolve_time_spmv(shesolve_handle h, int fp32, int reps, double *avg_us) {
   return 0;
 }
 
+// ---------------------------------------------------------------- batched solves
+int shesolve_batch_create(shesolve_batch *h, int device, const shesolve_params *p) {
+  if (!h) { set_err("handle pointer is NULL"); return 1; }
+  shesolve_handle c = nullptr;
+  if (int e = shesolve_create(&c, device, p)) return e;
+  shesolve_batch_ctx *b = new shesolve_batch_ctx();
+  b->c = c;
+  *h = b;
+  return 0;
+}
+int shesolve_batch_destroy(shesolve_batch h) {
+  if (!h) return 1;
+  hipSetDevice(h->c->device);
+  hipStreamSynchronize(h->c->stream);
+  batch_free(h);
+  shesolve_destroy(h->c);
+  delete h;
+  return 0;
+}
+int shesolve_batch_set_params(shesolve_batch h, const shesolve_params *p) {
+  if (!h) { set_err("handle is NULL"); return 1; }
+  if (p && p->precond != h->c->prm.precond && h->c->d_rp) {
+    set_err("the preconditioner kind is fixed once systems are set");
+    return 2;
+  }
+  for (auto &g : h->graph)
+    if (g) { hipGraphExecDestroy(g); g = nullptr; }
+  return shesolve_set_params(h->c, p);
+}
+int shesolve_batch_set_systems(shesolve_batch h, int count, const int64_t *n, const int64_t *const *row_ptr,
+                               const int32_t *const *col, const double *const *val, const int *nblocks,
+                               const int64_t *const *block_ranges) {
+  if (!h) { set_err("handle is NULL"); return 1; }
+  return batch_set_systems(h, false, count, n, row_ptr, col, val, nblocks, block_ranges);
+}
+int shesolve_batch_set_systems_device(shesolve_batch h, int count, const int64_t *n, const int64_t *const *d_row_ptr,
+                                      const int32_t *const *d_col, const double *const *d_val, const int *nblocks,
+                                      const int64_t *const *block_ranges) {
+  if (!h) { set_err("handle is NULL"); return 1; }
+  return batch_set_systems(h, true, count, n, d_row_ptr, d_col, d_val, nblocks, block_ranges);
+}
+int shesolve_batch_update_values(shesolve_batch h, const double *const *val) {
+  if (!h) { set_err("handle is NULL"); return 1; }
+  return batch_update_values(h, false, val);
+}
+int shesolve_batch_update_values_device(shesolve_batch h, const double *const *d_val) {
+  if (!h) { set_err("handle is NULL"); return 1; }
+  return batch_update_values(h, true, d_val);
+}
+int shesolve_batch_solve(shesolve_batch h, const double *const *b, double *const *x, int warm_start,
+                         shesolve_report *reports) {
+  if (!h) { set_err("handle is NULL"); return 1; }
+  return batch_solve(h, false, b, x, warm_start, reports);
+}
+int shesolve_batch_solve_device(shesolve_batch h, const double *const *d_b, double *const *d_x, int warm_start,
+                                shesolve_report *reports) {
+  if (!h) { set_err("handle is NULL"); return 1; }
+  return batch_solve(h, true, d_b, d_x, warm_start, reports);
+}
+int shesolve_batch_options(shesolve_batch h, int use_graph, int poll) {
+  if (!h) { set_err("handle is NULL"); return 1; }
+  if (poll < 1) { set_err("poll must be >= 1"); return 3; }
+  h->use_graph = use_graph != 0;
+  h->poll = poll;
+  return 0;
+}
+int shesolve_batch_info(shesolve_batch h, int64_t *out6) {
+  if (!h) { set_err("handle is NULL"); return 1; }
+  if (!out6) { set_err("out is NULL"); return 2; }
+  out6[0] = h->S;
+  out6[1] = h->c->n;
+  out6[2] = h->c->nnz;
+  out6[3] = h->c->seg_levels_max;
+  out6[4] = h->c->lvL.count();
+  out6[5] = h->c->fnnz;
+  return 0;
+}
+shesolve_handle shesolve_batch_solver(shesolve_batch h) { return h ? h->c : nullptr; }
+
 }  // extern "C"
+

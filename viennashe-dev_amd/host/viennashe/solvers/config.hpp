@@ -49,11 +49,12 @@ class linear_solver_config : public linear_solver_ids {
  public:
   typedef std::vector<std::pair<std::size_t, std::size_t> > block_preconditioner_boundaries_container;
   enum gpu_precision_type { gpu_fp64 = 0, gpu_fp32_refined = 1 };
-  enum gpu_preconditioner_type { gpu_ilu0 = 1, gpu_block_ilu0 = 3 };
+  enum gpu_preconditioner_type { gpu_ilu0 = 1, gpu_block_ilu0 = 3, gpu_mc_ilu0 = 4, gpu_iluk = 5 };
 
   linear_solver_config()
       : id_(serial_linear_solver), tol_(1e-13), max_iters_(1000), ilut_entries_(60), ilut_drop_tol_(1e-4),
-        do_scale_(true), gpu_device_(0), gpu_precision_(gpu_fp32_refined), gpu_precond_(gpu_ilu0) {}
+        do_scale_(true), gpu_device_(0), gpu_precision_(gpu_fp32_refined), gpu_precond_(gpu_ilu0),
+        gpu_fill_level_(1), gpu_drop_tol_(0.0) {}
 
   void scale(bool value) { do_scale_ = value; }
   bool scale() const { return do_scale_; }
@@ -105,6 +106,11 @@ class linear_solver_config : public linear_solver_ids {
   void gpu_precision(gpu_precision_type p) { gpu_precision_ = p; }
   gpu_preconditioner_type gpu_preconditioner() const { return gpu_precond_; }
   void gpu_preconditioner(gpu_preconditioner_type p) { gpu_precond_ = p; }
+  /** gpu_iluk: level of fill k and relative drop tolerance (0 = plain ILU(k)) */
+  int gpu_fill_level() const { return gpu_fill_level_; }
+  void gpu_fill_level(int k) { if (k >= 0) gpu_fill_level_ = k; }
+  double gpu_drop_tolerance() const { return gpu_drop_tol_; }
+  void gpu_drop_tolerance(double tol) { if (tol >= 0 && tol < 1) gpu_drop_tol_ = tol; }
 
  private:
   long id_;
@@ -117,6 +123,8 @@ class linear_solver_config : public linear_solver_ids {
   int gpu_device_;
   gpu_precision_type gpu_precision_;
   gpu_preconditioner_type gpu_precond_;
+  int gpu_fill_level_;
+  double gpu_drop_tol_;
 };
 
 }  // namespace solvers

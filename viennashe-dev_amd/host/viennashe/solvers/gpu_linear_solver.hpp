@@ -86,6 +86,12 @@ std::vector<double> solve(viennashe::math::sparse_matrix<NumericT> const &A, std
       ranges.push_back(static_cast<int64_t>(blk[k].first));
       ranges.push_back(static_cast<int64_t>(blk[k].second));
     }
+  } else if (config.gpu_preconditioner() == linear_solver_config::gpu_iluk) {
+    p.precond = SHESOLVE_PRECOND_ILUK;
+    p.fill_level = config.gpu_fill_level();
+    p.drop_tol = config.gpu_drop_tolerance();
+  } else if (config.gpu_preconditioner() == linear_solver_config::gpu_mc_ilu0) {
+    p.precond = SHESOLVE_PRECOND_MC_ILU0;
   } else {
     p.precond = SHESOLVE_PRECOND_ILU0;
   }
@@ -110,6 +116,9 @@ template <typename NumericT>
 std::vector<double> solve(viennashe::math::sparse_matrix<NumericT> &A, std::vector<double> const &b,
                           linear_solver_config const &config) {
   if (A.size1() != b.size()) throw linear_solver_exception("solve(): size mismatch of matrix and right-hand side");
+  // matrix_consistency_check: an empty row is an error (src/solvers/solve.hpp:53-58)
+  for (std::size_t i = 0; i < A.size1(); ++i)
+    if (A.row(i).empty()) throw linear_solver_exception("solve(): Found empty row in system_matrix.");
   double nrm = 0.0;
   for (std::size_t i = 0; i < b.size(); ++i) {
     if (!std::isfinite(b[i])) {

@@ -14,7 +14,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBPATH = os.path.join(PKG, "libshesolve.so")
 
-PRECOND_NONE, PRECOND_ILU0, PRECOND_BLOCK_ILU0, PRECOND_MC_ILU0 = 0, 1, 3, 4
+PRECOND_NONE, PRECOND_ILU0, PRECOND_BLOCK_ILU0, PRECOND_MC_ILU0, PRECOND_ILUK = 0, 1, 3, 4, 5
 FP64, FP32_IR = 0, 1
 
 # symbols declared in include/shesolve.h
@@ -25,12 +25,17 @@ EXPORTS = (
     "shesolve_spmv_f32", "shesolve_precond_apply", "shesolve_factor_nnz", "shesolve_get_factors",
     "shesolve_levels", "shesolve_set_profiling", "shesolve_kernel_stats", "shesolve_time_spmv",
     "shesolve_get_ordering", "shesolve_stream_config",
+    "shesolve_batch_create", "shesolve_batch_destroy", "shesolve_batch_set_params", "shesolve_batch_set_systems",
+    "shesolve_batch_set_systems_device", "shesolve_batch_update_values", "shesolve_batch_update_values_device",
+    "shesolve_batch_solve", "shesolve_batch_solve_device", "shesolve_batch_options", "shesolve_batch_info",
+    "shesolve_batch_solver",
 )
 
 
 class Params(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iters", C.c_int), ("precond", C.c_int), ("precision", C.c_int),
-                ("inner_tol", C.c_double), ("max_ir_steps", C.c_int), ("lanes_per_row", C.c_int)]
+                ("inner_tol", C.c_double), ("max_ir_steps", C.c_int), ("lanes_per_row", C.c_int),
+                ("fill_level", C.c_int), ("drop_tol", C.c_double)]
 
 
 class Report(C.Structure):
@@ -84,8 +89,23 @@ def lib():
     L.shesolve_set_profiling.argtypes = [C.c_void_p, C.c_int]
     L.shesolve_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.shesolve_time_spmv.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    VP = C.POINTER(C.c_void_p)
+    L.shesolve_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(Params)]
+    L.shesolve_batch_destroy.argtypes = [C.c_void_p]
+    L.shesolve_batch_set_params.argtypes = [C.c_void_p, C.POINTER(Params)]
+    L.shesolve_batch_set_systems.argtypes = [C.c_void_p, C.c_int, _i64p, VP, VP, VP, C.c_void_p, VP]
+    L.shesolve_batch_set_systems_device.argtypes = [C.c_void_p, C.c_int, _i64p, VP, VP, VP, C.c_void_p, VP]
+    L.shesolve_batch_update_values.argtypes = [C.c_void_p, VP]
+    L.shesolve_batch_update_values_device.argtypes = [C.c_void_p, VP]
+    L.shesolve_batch_solve.argtypes = [C.c_void_p, VP, VP, C.c_int, C.POINTER(Report)]
+    L.shesolve_batch_solve_device.argtypes = [C.c_void_p, VP, VP, C.c_int, C.POINTER(Report)]
+    L.shesolve_batch_options.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.shesolve_batch_info.argtypes = [C.c_void_p, _i64p]
+    L.shesolve_batch_solver.argtypes = [C.c_void_p]
+    L.shesolve_batch_solver.restype = C.c_void_p
     for name in EXPORTS:
-        if name not in ("shesolve_last_error", "shesolve_factor_nnz", "shesolve_default_params"):
+        if name not in ("shesolve_last_error", "shesolve_factor_nnz", "shesolve_default_params",
+                        "shesolve_batch_solver"):
             getattr(L, name).restype = C.c_int
     L.shesolve_default_params.restype = None
     _lib = L
@@ -212,7 +232,8 @@ class Solver:
         return int(o[0]), int(o[1])
 
     def set_profiling(self, on=True):
-        _check(lib().shesolve_set_profiling(self._h, 1 if on else 0), "shesolve_set_profiling")
+        """on: False/0 off, True/1 every timed launch waited for, 2 SpMV launches timed live."""
+        _check(lib().shesolve_set_profiling(self._h, int(on)), "shesolve_set_profiling")
 
     def kernel_stats(self, kind):
         a = C.c_double()
@@ -226,10 +247,120 @@ class Solver:
         return a.value
 
 
+def _ptrs(items):
+    """ctypes array of void pointers from numpy arrays or integer addresses."""
+    arr = (C.c_void_p * len(items))()
+    for i, a in enumerate(items):
+        arr[i] = a.ctypes.data if isinstance(a, np.ndarray) else a
+    return arr
+
+
+class BatchSolver:
+    """Many independent systems solved together on one device
+    (shesolve_batch_* in include/shesolve.h)."""
+
+    def __init__(self, device=0, **params):
+        self.params = default_params(**params)
+        h = C.c_void_p()
+        _check(lib().shesolve_batch_create(C.byref(h), device, C.byref(self.params)), "shesolve_batch_create")
+        self._h = h
+        self.sizes = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().shesolve_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        _check(lib().shesolve_batch_set_params(self._h, C.byref(self.params)), "shesolve_batch_set_params")
+
+    def options(self, use_graph=True, poll=8):
+        _check(lib().shesolve_batch_options(self._h, 1 if use_graph else 0, poll), "shesolve_batch_options")
+
+    @staticmethod
+    def _ranges(block_ranges, count):
+        if block_ranges is None:
+            return None, None, []
+        keep = [np.ascontiguousarray(np.asarray(r, np.int64).reshape(-1)) for r in block_ranges]
+        nb = (C.c_int * count)(*[len(r) // 2 for r in keep])
+        return nb, _ptrs(keep), keep
+
+    def set_systems(self, systems, block_ranges=None):
+        """systems: list of (row_ptr, col, val) host arrays."""
+        rps = [np.ascontiguousarray(r, np.int64) for r, _, _ in systems]
+        cis = [np.ascontiguousarray(c, np.int32) for _, c, _ in systems]
+        vs = [np.ascontiguousarray(v, np.float64) for _, _, v in systems]
+        n = np.array([len(r) - 1 for r in rps], np.int64)
+        nb, rr, keep = self._ranges(block_ranges, len(systems))
+        _check(lib().shesolve_batch_set_systems(self._h, len(systems), n, _ptrs(rps), _ptrs(cis), _ptrs(vs),
+                                                C.cast(nb, C.c_void_p) if nb is not None else None, rr),
+               "shesolve_batch_set_systems")
+        self.sizes = [int(v) for v in n]
+
+    def set_systems_device(self, n, d_row_ptr, d_col, d_val, block_ranges=None):
+        """Device CSRs: lists of sizes and device addresses (int)."""
+        nn = np.ascontiguousarray(n, np.int64)
+        nb, rr, keep = self._ranges(block_ranges, len(nn))
+        _check(lib().shesolve_batch_set_systems_device(self._h, len(nn), nn, _ptrs(d_row_ptr), _ptrs(d_col),
+                                                       _ptrs(d_val), C.cast(nb, C.c_void_p) if nb is not None else None,
+                                                       rr), "shesolve_batch_set_systems_device")
+        self.sizes = [int(v) for v in nn]
+
+    def update_values(self, vals):
+        vs = [np.ascontiguousarray(v, np.float64) for v in vals]
+        _check(lib().shesolve_batch_update_values(self._h, _ptrs(vs)), "shesolve_batch_update_values")
+
+    def update_values_device(self, d_vals):
+        _check(lib().shesolve_batch_update_values_device(self._h, _ptrs(d_vals)), "shesolve_batch_update_values_device")
+
+    def solve(self, bs, x0s=None):
+        bb = [np.ascontiguousarray(b, np.float64) for b in bs]
+        xs = [np.zeros(n) for n in self.sizes] if x0s is None else [np.array(x, np.float64, copy=True) for x in x0s]
+        reps = (Report * len(bb))()
+        _check(lib().shesolve_batch_solve(self._h, _ptrs(bb), _ptrs(xs), 0 if x0s is None else 1, reps),
+               "shesolve_batch_solve")
+        return xs, [r.as_dict() for r in reps]
+
+    def solve_device(self, d_bs, d_xs, warm_start=False):
+        reps = (Report * len(d_bs))()
+        _check(lib().shesolve_batch_solve_device(self._h, _ptrs(d_bs), _ptrs(d_xs), 1 if warm_start else 0, reps),
+               "shesolve_batch_solve_device")
+        return [r.as_dict() for r in reps]
+
+    def info(self):
+        o = np.zeros(6, np.int64)
+        _check(lib().shesolve_batch_info(self._h, o), "shesolve_batch_info")
+        return dict(zip(("systems", "rows", "nnz", "seg_levels_max", "levels", "factor_nnz"), (int(v) for v in o)))
+
+    def kernel_stats(self, kind):
+        """Kernel timing of the underlying concatenated system (see Solver.kernel_stats)."""
+        h = lib().shesolve_batch_solver(self._h)
+        a = C.c_double()
+        n = C.c_int64()
+        _check(lib().shesolve_kernel_stats(h, kind, C.byref(a), C.byref(n)), "shesolve_kernel_stats")
+        return a.value, n.value
+
+    def set_profiling(self, on=True):
+        h = lib().shesolve_batch_solver(self._h)
+        _check(lib().shesolve_set_profiling(h, int(on)), "shesolve_set_profiling")
+
+    def time_spmv(self, fp32=False, reps=20):
+        h = lib().shesolve_batch_solver(self._h)
+        a = C.c_double()
+        _check(lib().shesolve_time_spmv(h, 1 if fp32 else 0, reps, C.byref(a)), "shesolve_time_spmv")
+        return a.value
+
+
 # ---------------------------------------------------------------- coefficient fill (include/shefill.h)
 FILL_EXPORTS = ("shefill_last_error", "shefill_create", "shefill_destroy", "shefill_build", "shefill_sizes",
                 "shefill_device_system", "shefill_get_system", "shefill_block_ranges", "shefill_mapping",
-                "shefill_last_fill_ms", "shefill_refill")
+                "shefill_last_fill_ms", "shefill_refill", "shefill_recover", "shefill_recover_device",
+                "shefill_postproc", "shefill_postproc_device")
 
 
 class FillDevice(C.Structure):
@@ -260,6 +391,10 @@ def _fill_lib():
     L.shefill_block_ranges.argtypes = [C.c_void_p, _i64p]
     L.shefill_mapping.argtypes = [C.c_void_p, _i64p, _i32p]
     L.shefill_last_fill_ms.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    L.shefill_recover.argtypes = [C.c_void_p, _f64p, _f64p]
+    L.shefill_recover_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.shefill_postproc.argtypes = [C.c_void_p, _f64p, C.c_void_p, C.c_void_p]
+    L.shefill_postproc_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     for name in FILL_EXPORTS:
         if name != "shefill_last_error":
             getattr(L, name).restype = C.c_int
@@ -350,3 +485,24 @@ class Fill:
         v = C.c_double()
         _fcheck(_fill_lib().shefill_last_fill_ms(self._h, C.byref(v)), "shefill_last_fill_ms")
         return v.value
+
+    def recover(self, x):
+        """Full solution (even unknowns scaled back, odd unknowns recovered) on the GPU."""
+        xf = np.zeros(self.sizes["n"] + self.sizes["n_odd"])
+        _fcheck(_fill_lib().shefill_recover(self._h, np.ascontiguousarray(x, np.float64), xf), "shefill_recover")
+        return xf
+
+    def recover_device(self, d_x, d_x_full):
+        _fcheck(_fill_lib().shefill_recover_device(self._h, d_x, d_x_full), "shefill_recover_device")
+
+    def postproc(self, x_full):
+        """Carrier density per cell and current per interior facet on the GPU."""
+        dens = np.zeros(self.sizes["cells"])
+        cur = np.zeros(max(self.sizes["facets"], 1))
+        _fcheck(_fill_lib().shefill_postproc(self._h, np.ascontiguousarray(x_full, np.float64), dens.ctypes.data,
+                                             cur.ctypes.data), "shefill_postproc")
+        return dens, cur[:self.sizes["facets"]]
+
+    def postproc_device(self, d_x_full, d_density, d_current):
+        _fcheck(_fill_lib().shefill_postproc_device(self._h, d_x_full, d_density, d_current),
+                "shefill_postproc_device")
