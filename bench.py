@@ -77,6 +77,110 @@ def cpu_baseline(rp, ci, val, b, fp64_iters, budget_s):
     }
 
 
+def bench_batch(steps, count=96, seed=1234, fp64=True):
+    """The batched workload: `count` 1D diodes drawn from a seed (length
+    U[100,1000] nm, n+ 20 % each side, L in {1,3,5}, dE U[2,10] meV to 1.0 eV,
+    bias U[0,1.5] V), solved together by the batched FP32+IR BiCGStab (one
+    launch per phase over all systems, per-system masks, graph-replayed
+    iterations).  Preconditioner: ILU(0) per system, ILU(1) for L = 5 (ILU(0)
+    diverges there).  A step = GPU refill of every system + refactorisation +
+    batched solve to 1e-10."""
+    import ctypes
+    import shesolve
+    from shesolve import devices as D
+    hip = ctypes.CDLL("libamdhip64.so")
+    t0 = time.perf_counter()
+    fills, Ls = [], []
+    for dev, cfg in D.batch_diodes(count, seed):
+        F = shesolve.Fill(0)
+        F.build(dev, cfg)
+        fills.append(F)
+        Ls.append(cfg.L_max)
+    t_build = time.perf_counter() - t0
+    ns = [F.n for F in fills]
+    ptr = [F.device_system() for F in fills]
+    levels = [1 if L >= 5 else 0 for L in Ls]
+
+    def make(precision):
+        bs = shesolve.BatchSolver(0, precond=shesolve.PRECOND_ILUK, precision=precision, tol=1e-10, max_iters=3000)
+        bs.set_fill_levels(levels)
+        t = time.perf_counter()
+        bs.set_systems_device(ns, [p[0] for p in ptr], [p[1] for p in ptr], [p[2] for p in ptr])
+        return bs, time.perf_counter() - t
+
+    dx = []
+    for n in ns:
+        p = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(8 * n)) == 0
+        dx.append(p.value)
+
+    def run(bs, k):
+        lat, reps = [], None
+        for step in range(k):
+            t = time.perf_counter()
+            for F in fills:
+                F.refill()
+            bs.update_values_device([p[2] for p in ptr])
+            reps = bs.solve_device([p[3] for p in ptr], dx)
+            lat.append(time.perf_counter() - t)
+            bad = [r for r in reps if r["status"] != 0 or not r["rel_residual"] <= 1e-10]
+            log(f"[bench] batch step {step}: {lat[-1] * 1e3:.0f} ms, iterations max "
+                f"{max(r['iterations'] for r in reps)}, {len(bad)} not converged")
+            if bad:
+                raise SystemExit(f"batched solve did not reach 1e-10: {bad[:3]}")
+        return lat, reps
+
+    bs, t_setup = make(shesolve.FP32_IR)
+    info = bs.info()
+    log(f"[bench] batch of {count}: {info}, build {t_build:.1f} s, setup {t_setup:.1f} s")
+    run(bs, 1)
+    lat, reps = run(bs, steps)
+    bs.close()
+    out = {
+        "workload": f"{count} 1D diodes (seed {seed}): L in {{1,3,5}}, 100-1000 nm, dE 2-10 meV to 1.0 eV",
+        "systems": count, "rows": info["rows"], "nnz": info["nnz"], "factor_nnz": info["factor_nnz"],
+        "deepest_level_schedule": info["seg_levels_max"],
+        "solves_per_s": count * len(lat) / sum(lat), "ms_per_batch": 1e3 * sum(lat) / len(lat),
+        "p50_ms": 1e3 * percentile(lat, 50), "p99_ms": 1e3 * percentile(lat, 99),
+        "iterations_max": max(r["iterations"] for r in reps),
+        "iterations_mean": float(np.mean([r["iterations"] for r in reps])),
+        "max_rel_residual": max(r["rel_residual"] for r in reps),
+        "setup_s": t_setup, "build_s": t_build, "precond": "ILU(0); ILU(1) for L=5",
+    }
+    if fp64:
+        bs, _ = make(shesolve.FP64)
+        lat64, reps64 = run(bs, 1)
+        bs.close()
+        out["fp64_ms_per_batch"] = 1e3 * lat64[0]
+        out["fp64_iterations_max"] = max(r["iterations"] for r in reps64)
+        out["speedup_vs_fp64"] = lat64[0] / (sum(lat) / len(lat))
+    for p in dx:
+        hip.hipFree(ctypes.c_void_p(p))
+    for F in fills:
+        F.close()
+    return out
+
+
+def bench_serving(count, rate=200.0, slots=8, seed=2024):
+    """The serving workload: Poisson arrivals at `rate` req/s, 80 % 1D diodes
+    (batched-workload distribution), 20 % 2D MOSFET-like meshes 41x21..161x81
+    with L in {1,3}; each request = 5 warm-started solves with the potential
+    perturbed by N(0, 5 mV) per cell between solves; continuous admission under
+    a 200 GB HBM budget."""
+    from shesolve import serving
+    reqs = serving.make_requests(count, rate=rate, seed=seed)
+    srv = serving.Server(slots=slots, hbm_budget=200e9)
+    res = srv.run(reqs, realtime=True)
+    out = serving.summarize(res, srv.stats["makespan"])
+    out.update({"workload": f"{count} requests at {rate:g}/s, 80% diodes / 20% 2D MOSFET-like, 5 solves each",
+                "slots": slots, "hbm_budget_gb": 200, "peak_admitted_gb": srv.stats["peak_footprint"] / 1e9,
+                "max_inflight": srv.stats["max_inflight"],
+                "mix": {k: sum(1 for r in reqs if r.kind == k) for k in ("diode", "mosfet")}})
+    if not out["all_converged"]:
+        raise SystemExit(f"serving: a solve did not converge: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +194,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--profile-kernels", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the SHE solve of one device is not sharded "
@@ -223,8 +331,17 @@ def main():
                      "fp64_ir_residual_spmv_us": spmv64_ir_us},
         "kernel_stats_us": prof or None,
     }
-    if not args.no_cpu:
-        rp, ci, val, rhs, _, _ = F.system()
+    host_system = F.system() if not args.no_cpu else None
+    F.close()
+    hip.hipFree(d_x)
+    if not args.no_batch:
+        result["batch"] = bench_batch(args.batch_steps)
+        log(f"[bench] batch: {result['batch']}")
+    if not args.no_serve:
+        result["serving"] = bench_serving(args.serve_requests)
+        log(f"[bench] serving: {result['serving']}")
+    if host_system is not None:
+        rp, ci, val, rhs, _, _ = host_system
         log(f"[bench] CPU baseline (budget {args.cpu_budget:.0f} s)")
         result["cpu_baseline"] = cpu_baseline(rp, ci, val, rhs, ref_iters or reps32[-1]["iterations"],
                                               args.cpu_budget)

@@ -87,6 +87,10 @@ int shefill_block_ranges(shefill_handle h, int64_t *ranges);
  * cell-major [cells * N_H]; -1 / 0 where a node carries no unknown. */
 int shefill_mapping(shefill_handle h, int64_t *even_idx, int32_t *even_order);
 
+/* 64-bit hash of the reduced pattern (n, row pointer, columns) of the last
+ * build: equal hashes across builds mean the solver can keep its analysis and
+ * take new values with shesolve_update_values_device. */
+int shefill_pattern_hash(shefill_handle h, uint64_t *out);
 /* Device milliseconds of the last value fill (scalars + rows + normalisation). */
 int shefill_last_fill_ms(shefill_handle h, double *ms);
 

@@ -67,6 +67,9 @@ typedef struct {
   int lanes_per_row;     /* SpMV lanes per row (0 = pick from nnz/row) */
   int fill_level;        /* ILUK: level of fill k, default 1 */
   double drop_tol;       /* ILUK: relative drop tolerance (0 = plain ILU(k)), default 0 */
+  int autotune;          /* 1: time the CSR-stream launch shapes at set-up and keep the fastest
+                            (default); 0: fixed shape (2048-entry row blocks, strided), so that
+                            results do not depend on timings */
 } shesolve_params;
 
 typedef struct {
@@ -184,6 +187,10 @@ int shesolve_batch_solve(shesolve_batch h, const double *const *b, double *const
                          shesolve_report *reports);
 int shesolve_batch_solve_device(shesolve_batch h, const double *const *d_b, double *const *d_x, int warm_start,
                                 shesolve_report *reports);
+/* ILU(k) level per system for SHESOLVE_PRECOND_ILUK (0 = ILU(0) for that
+ * system), used by the next shesolve_batch_set_systems*; count must then equal
+ * the number of systems.  count = 0 goes back to params.fill_level for all. */
+int shesolve_batch_set_fill_levels(shesolve_batch h, int count, const int *levels);
 /* use_graph: replay one captured iteration (default 1); poll: iterations
  * between host checks of the convergence masks (default 8). */
 int shesolve_batch_options(shesolve_batch h, int use_graph, int poll);

@@ -1205,6 +1205,25 @@ int shefill_sizes(shefill_handle h, int64_t *o) {
   return 0;
 }
 
+int shefill_pattern_hash(shefill_handle h, uint64_t *out) {
+  if (!h) { ferr("handle is NULL"); return 1; }
+  if (!out) { ferr("out is NULL"); return 2; }
+  if (!h->built) { ferr("nothing built"); return -1; }
+  // FNV-1a over n, the row pointer and the column indices
+  uint64_t x = 1469598103934665603ULL;
+  auto mix = [&](uint64_t v) {
+    for (int b = 0; b < 8; ++b) {
+      x ^= (v >> (8 * b)) & 0xff;
+      x *= 1099511628211ULL;
+    }
+  };
+  mix((uint64_t)h->n);
+  for (int v : h->rp) mix((uint64_t)(uint32_t)v);
+  for (int v : h->ci) mix((uint64_t)(uint32_t)v);
+  *out = x;
+  return 0;
+}
+
 int shefill_device_system(shefill_handle h, const int64_t **rp, const int32_t **ci, const double **val,
                           const double **rhs) {
   if (!h) return 1;

@@ -20,9 +20,12 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <queue>
+#include <thread>
+#include <atomic>
 #include <vector>
 
 #include "shesolve.h"
@@ -357,6 +360,16 @@ __global__ __launch_bounds__(kBlock) void k_gather(int64_t n, const int *__restr
     dst[i] = pos[i] >= 0 ? src[pos[i]] : 0.0;  // fill positions of ILU(k) start at zero
 }
 
+// dir 0: out[i] = in[perm[i]] (to the permuted ordering); 1: out[perm[i]] = in[i] (back)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_permute(int n, const int *__restrict__ perm, const T *__restrict__ in,
+                                                    T *__restrict__ out, int dir) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (dir == 0) out[i] = in[perm[i]];
+  else out[perm[i]] = in[i];
+}
+
 // FP64 iterative-refinement helpers
 __global__ __launch_bounds__(kBlock) void k_cast_scale(int n, const double *__restrict__ a, float *__restrict__ out,
                                                        const double *__restrict__ nrm) {
@@ -546,6 +559,7 @@ __global__ __launch_bounds__(64) void k_ilu0_level(int cnt, const int *__restric
 // dot products, stopping test and a convergence mask per system, all on the
 // device, so that a sequence of iterations needs no host round trip.
 constexpr int kBatchParts = 32;  // partial sums per system and dot product
+constexpr int kStallWindow = 60;  // FP32 inner solves: iterations without a 1 % gain before giving up
 
 struct BSys {
   double rho, alpha, omega, beta;
@@ -554,7 +568,10 @@ struct BSys {
   int status;     // 0 converged, 1 iteration cap, 2 breakdown / divergence
   int iters;      // iterations of the current solve
   int max_iters;
-  int bd, pad[3];
+  int bd, best_it;
+  double best;    // best residual so far (stall test of FP32 inner solves)
+  int stall;      // iterations without a 1 % gain before giving up (0 = off)
+  int pad;
 };
 
 // NV dot products per system into part[(q * S + s) * kBatchParts + j]
@@ -607,6 +624,8 @@ __global__ void k_bfinalize(int op, int S, const double *__restrict__ part, BSys
     c.beta = 0.0;
     c.bd = 0;
     c.iters = 0;
+    c.best = c.res;
+    c.best_it = 0;
     c.status = 1;
     if (c.bnorm == 0.0 || c.res <= c.tol || rr == 0.0) {
       c.status = 0;
@@ -623,7 +642,11 @@ __global__ void k_bfinalize(int op, int S, const double *__restrict__ part, BSys
     const double rr = bsum(part, S, 0, s), r0r = bsum(part, S, 1, s);
     c.res = c.bnorm > 0 ? sqrt(rr) / c.bnorm : 0.0;
     c.iters += 1;
-    if (!isfinite(c.res) || c.res > c.div_limit) {
+    if (c.res < 0.99 * c.best) {
+      c.best = c.res;
+      c.best_it = c.iters;
+    }
+    if (!isfinite(c.res) || c.res > c.div_limit || (c.stall > 0 && c.iters - c.best_it >= c.stall)) {
       c.status = 2;
       c.active = 0;
     } else if (c.res <= c.tol) {
@@ -688,50 +711,150 @@ __global__ __launch_bounds__(kBlock) void k_bref(int op, int n, const int *__res
 
 // Sets the per-system state before a batched solve: active from mask, tolerance,
 // iteration cap and divergence limit.
-__global__ void k_binit(int S, BSys *bs, const int *__restrict__ mask, double tol, int max_iters, double div_limit) {
+__global__ void k_binit(int S, BSys *bs, const int *__restrict__ mask, double tol, int max_iters, double div_limit,
+                        int stall, const int *__restrict__ caps) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
   BSys &c = bs[s];
   c.active = mask[s] ? 1 : 0;
   c.status = mask[s] ? 1 : 0;
   c.tol = tol;
-  c.max_iters = max_iters;
+  c.max_iters = caps ? min(caps[s], max_iters) : max_iters;
   c.div_limit = div_limit;
+  c.stall = stall;
   c.iters = 0;
   c.bd = 0;
 }
 
 // Triangular sweep of segment-local level schedules, one workgroup per
 // segment (one system of a batch): the workgroup walks its segment's levels in
-// order, rows of a level spread over its threads (LPR lanes per row), a
-// barrier between levels.  Communication stays inside one CU.
-//   lower: z[row] = in[row] - sum L[k] z[col[k]]
-//   upper: z[row] = dinv[row] (z[row] - sum U[k] z[col[k]])
+// order, a barrier between levels, so all communication stays inside one CU.
+//
+// Storage is in level order: position q of the segment's schedule is row
+// rows[q], its entries lrp[q]..lrp[q+1], and each entry names its column by
+// the column's *position* cq (the schedule orders dependencies before their
+// users).  The outputs of the most recent positions are kept in an LDS window
+// of W slots (slot = position mod W): a dependency at position qc is read from
+// LDS when qc >= q1 - W (q1 = end of the current level, so no write of this
+// level can hit its slot) and from z[rows[qc]] otherwise.  A level's outputs
+// go to z and, for positions >= q1 - W, to the window.  The row data of the
+// next level (rows, entries, right-hand side, 1/diagonal) is loaded before
+// the current level is computed, so the only loads left on the critical path
+// are the window reads.
+//   lower: z[row] = in[row] - sum L[k] x(cq[k])
+//   upper: z[row] = dinv[row] (z[row] - sum U[k] x(cq[k]))
 constexpr int kSweepThreads = 1024;
-template <int LPR, typename TV, typename T, bool UPPER>
+template <int LPR, int R, int J, typename TV, typename T, bool UPPER>
 __global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict__ seg_lvl, const int *__restrict__ lvl_ptr,
-                                                           const int *__restrict__ rows, const int *__restrict__ rp,
-                                                           const int *__restrict__ ci, const TV *__restrict__ v,
+                                                           const int *__restrict__ rows, const int *__restrict__ lrp,
+                                                           const int *__restrict__ cq, const TV *__restrict__ v,
                                                            const T *__restrict__ in, T *z, const TV *__restrict__ dinv,
-                                                           const BSys *__restrict__ bs) {
+                                                           const BSys *__restrict__ bs, int W) {
+  extern __shared__ unsigned char lds_raw[];
+  T *win = reinterpret_cast<T *>(lds_raw);
   const int s = blockIdx.x;
   if (bs && !bs[s].active) return;
+  constexpr int G = kSweepThreads / LPR;
+  const int Wm = W - 1;
   const int l0 = seg_lvl[s], l1 = seg_lvl[s + 1];
   const int g = threadIdx.x / LPR, lane = threadIdx.x & (LPR - 1);
-  for (int l = l0; l < l1; ++l) {
-    const int q0 = lvl_ptr[l], q1 = lvl_ptr[l + 1];
-    for (int q = q0 + g; q < q1; q += kSweepThreads / LPR) {
-      const int row = rows[q];
-      T acc = 0;
-      for (int k = rp[row] + lane; k < rp[row + 1]; k += LPR) acc += (T)v[k] * z[ci[k]];
+  // prefetched first chunk of a level: R rows per lane group, J entries per lane
+  int prow[R], pkb[R], pke[R], pc[R][J];
+  T prhs[R], pd[R];
+  TV pv[R][J];
+  auto fetch = [&](int q0, int q1) {
 #pragma unroll
-      for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
-      if (lane == 0) {
-        if (UPPER) z[row] = (T)dinv[row] * (z[row] - acc);
-        else z[row] = in[row] - acc;
+    for (int r = 0; r < R; ++r) {
+      const int q = q0 + g + r * G;
+      const bool ok = q < q1;
+      prow[r] = ok ? rows[q] : 0;
+      pkb[r] = ok ? lrp[q] : 0;
+      pke[r] = ok ? lrp[q + 1] : 0;
+      prhs[r] = ok ? (UPPER ? z[prow[r]] : in[prow[r]]) : (T)0;
+      pd[r] = (UPPER && ok) ? (T)dinv[prow[r]] : (T)1;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int k = pkb[r] + lane + j * LPR;
+        const bool kk = k < pke[r];
+        pc[r][j] = kk ? cq[k] : 0;
+        pv[r][j] = kk ? v[k] : (TV)0;
+      }
+    }
+  };
+  if (l0 >= l1) return;
+  int q0 = lvl_ptr[l0], q1 = lvl_ptr[l0 + 1];
+  fetch(q0, q1);
+  for (int l = l0; l < l1; ++l) {
+    // this level's first chunk comes from the prefetch registers
+    int row[R], kb[R], ke[R], c[R][J];
+    T rhs[R], dd[R];
+    TV vv[R][J];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      row[r] = prow[r]; kb[r] = pkb[r]; ke[r] = pke[r]; rhs[r] = prhs[r]; dd[r] = pd[r];
+#pragma unroll
+      for (int j = 0; j < J; ++j) { c[r][j] = pc[r][j]; vv[r][j] = pv[r][j]; }
+    }
+    const int nq1 = l + 1 < l1 ? lvl_ptr[l + 2] : q1;
+    if (l + 1 < l1) fetch(q1, nq1);
+    const int lo = q1 - W;  // oldest position the window may serve during this level
+    for (int base = q0 + g; base < q1; base += G * R) {
+      if (base != q0 + g) {
+        // later chunks of a large level: load them now
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int q = base + r * G;
+          const bool ok = q < q1;
+          row[r] = ok ? rows[q] : 0;
+          kb[r] = ok ? lrp[q] : 0;
+          ke[r] = ok ? lrp[q + 1] : 0;
+          rhs[r] = ok ? (UPPER ? z[row[r]] : in[row[r]]) : (T)0;
+          dd[r] = (UPPER && ok) ? (T)dinv[row[r]] : (T)1;
+#pragma unroll
+          for (int j = 0; j < J; ++j) {
+            const int k = kb[r] + lane + j * LPR;
+            const bool kk = k < ke[r];
+            c[r][j] = kk ? cq[k] : 0;
+            vv[r][j] = kk ? v[k] : (TV)0;
+          }
+        }
+      }
+      T acc[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        acc[r] = 0;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          const int k = kb[r] + lane + j * LPR;
+          if (k < ke[r]) {
+            const int qc = c[r][j];
+            const T xv = qc >= lo ? win[qc & Wm] : z[rows[qc]];
+            acc[r] += (T)vv[r][j] * xv;
+          }
+        }
+        // entries beyond the prefetched J per lane
+        for (int k = kb[r] + lane + J * LPR; k < ke[r]; k += LPR) {
+          const int qc = cq[k];
+          const T xv = qc >= lo ? win[qc & Wm] : z[rows[qc]];
+          acc[r] += (T)v[k] * xv;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        T a = acc[r];
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, LPR);
+        const int q = base + r * G;
+        if (lane == 0 && q < q1) {
+          const T out = UPPER ? dd[r] * (rhs[r] - a) : rhs[r] - a;
+          z[row[r]] = out;
+          if (q >= lo) win[q & Wm] = out;
+        }
       }
     }
     __syncthreads();
+    q0 = q1;
+    q1 = nq1;
   }
 }
 
@@ -769,6 +892,17 @@ struct shesolve_ctx {
   std::vector<int> h_frp, h_fci, h_fdg;
   std::vector<int> blk_of_row;
   int *d_perm = nullptr;            // multicolour ordering: new -> old row
+  // solve space: the matrix the Krylov iteration multiplies by -- the stored
+  // matrix, or for MC_ILU0 that matrix permuted into the colour ordering (rows
+  // and columns), so that neither the SpMV nor the sweeps go through perm;
+  // b and x are permuted once per solve
+  bool solve_perm = false;
+  const int *s_rp = nullptr, *s_ci = nullptr, *s_blk = nullptr;
+  int s_nblk = 0;
+  double *d_sv64 = nullptr;
+  float *d_sv32 = nullptr;
+  int *d_sblk = nullptr;
+  double *pb64 = nullptr, *px64 = nullptr;
   double *z64 = nullptr;            // permuted-space work vectors of the sweeps
   float *z32 = nullptr;
   int ncolors = 0;
@@ -793,9 +927,12 @@ struct shesolve_ctx {
   // rows, and per segment its levels (seg -> level range, level -> row range
   // of the level-ordered row list) for the one-workgroup-per-segment sweeps
   std::vector<int> seg_ptr;
+  std::vector<int> seg_fill;  // ILU(k) level per segment (batches; empty = prm.fill_level)
   int nseg = 0;
   int *d_segL = nullptr, *d_lvlL = nullptr, *d_rowsL = nullptr;
   int *d_segU = nullptr, *d_lvlU = nullptr, *d_rowsU = nullptr;
+  int *d_loLrp = nullptr, *d_loUrp = nullptr;  // level-ordered row pointers of the split factor
+  std::vector<int> h_segrowsL, h_segrowsU;     // level-ordered rows (host copy during analysis)
   const BSys *seg_mask = nullptr;  // device per-segment state; inactive segments are skipped
   int seg_levels_max = 0;
   // vectors (FP64 set and FP32 set)
@@ -825,6 +962,10 @@ void dfree(T *&p) {
   if (p) hipFree(p);
   p = nullptr;
 }
+
+// solve-space values (FP64 / FP32)
+inline const double *sv64(const shesolve_ctx *c) { return c->solve_perm ? c->d_sv64 : c->d_v64; }
+inline const float *sv32(const shesolve_ctx *c) { return c->solve_perm ? c->d_sv32 : c->d_v32; }
 
 int pick_lpr(double avg) {
   if (avg <= 3) return 2;
@@ -902,8 +1043,18 @@ template <typename TV, typename T, typename TA>
 void launch_spmv(shesolve_ctx *c, int /*lpr*/, const TV *v, const T *x, T *y, const T *b, const T *d0, const T *d1,
                  double *part) {
   KTimer kt(c, sizeof(TV) == 8 ? 4 : 0);
-  launch_stream<TV, T, MODE_SPMV>(c, stream_grid(c->nablk, part != nullptr), c->nablk, c->d_ablk, c->d_rp, c->d_ci,
-                                  v, x, y, b, (const int *)nullptr, (const T *)nullptr, (T *)nullptr, d0, d1, part);
+  const int *rp = c->s_rp ? c->s_rp : c->d_rp, *ci = c->s_rp ? c->s_ci : c->d_ci;
+  const int *blk = c->s_rp ? c->s_blk : c->d_ablk;
+  const int nblk = c->s_rp ? c->s_nblk : c->nablk;
+  launch_stream<TV, T, MODE_SPMV>(c, stream_grid(nblk, part != nullptr), nblk, blk, rp, ci, v, x, y, b,
+                                  (const int *)nullptr, (const T *)nullptr, (T *)nullptr, d0, d1, part);
+}
+
+// LDS window of the segment sweeps: 64 KiB (the per-workgroup limit without
+// an opt-in), as a power of two number of entries
+template <typename T>
+int sweep_window() {
+  return (int)(65536 / sizeof(T));
 }
 
 template <typename TV, typename T>
@@ -920,19 +1071,21 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     Lv = (const TV *)c->d_L32; Uv = (const TV *)c->d_U32; dinv = (const TV *)c->d_dinv32;
   }
   const int L = c->flpr;
-  const int *perm = c->d_perm;
+  const int *perm = c->solve_perm ? nullptr : c->d_perm;
   if (c->nseg > 0 && !perm) {
     KTimer kt(c, 1);
-#define WG_CASE(LL)                                                                                            \
-  case LL:                                                                                                     \
-    hipLaunchKernelGGL((k_wgsweep<LL, TV, T, false>), dim3(c->nseg), dim3(kSweepThreads), 0, c->stream,        \
-                       c->d_segL, c->d_lvlL, c->d_rowsL, c->d_Lrp, c->d_Lci, Lv, in, out, (const TV *)nullptr, \
-                       c->seg_mask);                                                                           \
-    hipLaunchKernelGGL((k_wgsweep<LL, TV, T, true>), dim3(c->nseg), dim3(kSweepThreads), 0, c->stream,         \
-                       c->d_segU, c->d_lvlU, c->d_rowsU, c->d_Urp, c->d_Uci, Uv, (const T *)nullptr, out, dinv, \
-                       c->seg_mask);                                                                           \
+    const int W = sweep_window<T>();
+    const size_t lds = sizeof(T) * (size_t)W;
+#define WG_CASE(LL)                                                                                               \
+  case LL:                                                                                                        \
+    hipLaunchKernelGGL((k_wgsweep<LL, 4, 2, TV, T, false>), dim3(c->nseg), dim3(kSweepThreads), lds, c->stream,   \
+                       c->d_segL, c->d_lvlL, c->d_rowsL, c->d_loLrp, c->d_Lci, Lv, in, out, (const TV *)nullptr,  \
+                       c->seg_mask, W);                                                                           \
+    hipLaunchKernelGGL((k_wgsweep<LL, 4, 2, TV, T, true>), dim3(c->nseg), dim3(kSweepThreads), lds, c->stream,    \
+                       c->d_segU, c->d_lvlU, c->d_rowsU, c->d_loUrp, c->d_Uci, Uv, (const T *)nullptr, out, dinv, \
+                       c->seg_mask, W);                                                                           \
     break;
-    switch (L) { WG_CASE(2) WG_CASE(4) WG_CASE(8) WG_CASE(16) WG_CASE(32) default: WG_CASE(64) }
+    switch (L) { WG_CASE(2) WG_CASE(4) default: WG_CASE(8) }
 #undef WG_CASE
     return;
   }
@@ -1002,6 +1155,10 @@ void free_matrix(shesolve_ctx *c) {
   c->d_frp = c->d_fci = nullptr;
   dfree(c->d_fdg); dfree(c->d_fpos); dfree(c->d_f64); dfree(c->d_f32);
   dfree(c->d_perm); dfree(c->z64); dfree(c->z32);
+  dfree(c->d_sv64); dfree(c->d_sv32); dfree(c->d_sblk); dfree(c->pb64); dfree(c->px64);
+  c->solve_perm = false;
+  c->s_rp = c->s_ci = c->s_blk = nullptr;
+  c->s_nblk = 0;
   dfree(c->d_ablk); c->nablk = 0;
   dfree(c->d_Lrp); dfree(c->d_Lci); dfree(c->d_Urp); dfree(c->d_Uci); dfree(c->d_Lsrc);
   dfree(c->d_L64); dfree(c->d_U64); dfree(c->d_dinv64); dfree(c->d_L32); dfree(c->d_U32); dfree(c->d_dinv32);
@@ -1011,6 +1168,7 @@ void free_matrix(shesolve_ctx *c) {
   c->lvL.ptr.clear(); c->lvU.ptr.clear();
   dfree(c->d_segL); dfree(c->d_lvlL); dfree(c->d_rowsL);
   dfree(c->d_segU); dfree(c->d_lvlU); dfree(c->d_rowsU);
+  dfree(c->d_loLrp); dfree(c->d_loUrp);
   c->nseg = 0;
 }
 
@@ -1116,41 +1274,44 @@ int segment_levels(shesolve_ctx *c) {
     HIPCHK(hipMemcpy(dseg, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dlvl, lvl.data(), sizeof(int) * lvl.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(drows, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice));
+    (upper ? c->h_segrowsU : c->h_segrowsL).swap(rows);
   }
   c->nseg = S;
   return 0;
 }
 
-// Level-of-fill pattern of ILU(k), k = prm.fill_level: entries of A have level
+// Level-of-fill pattern of ILU(k) over rows [r0, r1): entries of A have level
 // 0; eliminating row j from row i gives column c > j the level
 // lev(i,j) + lev(j,c) + 1, kept (minimum over j) when <= k.  Rows in order, the
 // lower part of a row in ascending column order (a min-heap of pending
-// columns).  pos maps factor entries to A's entries, -1 for fill.
-int iluk_pattern(shesolve_ctx *c, std::vector<int> &pos) {
-  const int n = c->n, k = c->prm.fill_level;
-  const auto &rp = c->h_rp, &ci = c->h_ci;
-  std::vector<int> frp(n + 1, 0), fci, udg(n);
+// columns).  The rows may only reach columns in [r0, r1) (a diagonal block).
+// Output row lengths in len[i - r0], columns and the A-entry of each factor
+// entry (-1 for fill) appended to fci / pos.
+int iluk_rows(const std::vector<int> &rp, const std::vector<int> &ci, int r0, int r1, int k, std::vector<int> &len,
+              std::vector<int> &fci, std::vector<int> &pos) {
+  const int m = r1 - r0;
+  std::vector<int> frp(m + 1, 0), udg(m);
   std::vector<unsigned char> flev;
-  fci.reserve(c->h_ci.size() * 3 / 2);
-  flev.reserve(c->h_ci.size() * 3 / 2);
-  pos.clear();
-  pos.reserve(c->h_ci.size() * 3 / 2);
-  std::vector<int> lev(n, -1), src(n, -1), touched;
+  std::vector<int> lev(m, -1), src(m, -1), touched;
   std::priority_queue<int, std::vector<int>, std::greater<int>> heap;
-  for (int i = 0; i < n; ++i) {
+  len.assign(m, 0);
+  for (int i = 0; i < m; ++i) {
     touched.clear();
-    for (int q = rp[i]; q < rp[i + 1]; ++q) {
-      lev[ci[q]] = 0;
-      src[ci[q]] = q;
-      touched.push_back(ci[q]);
-      if (ci[q] < i) heap.push(ci[q]);
+    const int gi = r0 + i;
+    for (int q = rp[gi]; q < rp[gi + 1]; ++q) {
+      const int cl = ci[q] - r0;
+      if (cl < 0 || cl >= m) return -1;
+      lev[cl] = 0;
+      src[cl] = q;
+      touched.push_back(cl);
+      if (cl < i) heap.push(cl);
     }
     while (!heap.empty()) {
       const int j = heap.top();
       heap.pop();
       const int lj = lev[j];
       for (int q = udg[j] + 1; q < frp[j + 1]; ++q) {
-        const int col = fci[q], nl = lj + flev[q] + 1;
+        const int col = fci[q] - r0, nl = lj + flev[q] + 1;
         if (nl > k) continue;
         if (lev[col] < 0) {
           lev[col] = nl;
@@ -1165,24 +1326,62 @@ int iluk_pattern(shesolve_ctx *c, std::vector<int> &pos) {
     udg[i] = -1;
     for (int col : touched) {
       if (col == i) udg[i] = (int)fci.size();
-      fci.push_back(col);
+      fci.push_back(col + r0);
       flev.push_back((unsigned char)lev[col]);
       pos.push_back(src[col]);
       lev[col] = -1;
       src[col] = -1;
     }
-    if (udg[i] < 0) {
-      set_err("ILU(k): row %d has no diagonal entry", i);
-      return 2;
-    }
-    if ((int64_t)fci.size() > 0x7fffffff) {
-      set_err("ILU(k) factor too large for 32-bit indices");
-      return 2;
-    }
+    if (udg[i] < 0) return -2 - gi;
     frp[i + 1] = (int)fci.size();
+    len[i] = frp[i + 1] - frp[i];
   }
-  c->h_frp.swap(frp);
-  c->h_fci.swap(fci);
+  return 0;
+}
+
+// ILU(k) pattern of the whole matrix: one diagonal block per segment (systems
+// of a batch, built in parallel, each with its own level) or the whole matrix.
+// pos maps factor entries to A's entries, -1 for fill.
+int iluk_pattern(shesolve_ctx *c, std::vector<int> &pos) {
+  const int n = c->n;
+  std::vector<int> seg = c->seg_ptr;
+  if (seg.empty()) seg = {0, n};
+  const int S = (int)seg.size() - 1;
+  std::vector<int> kseg(S, c->prm.fill_level);
+  if ((int)c->seg_fill.size() == S) kseg = c->seg_fill;
+  std::vector<std::vector<int>> lens(S), fcis(S), poss(S);
+  std::vector<int> err(S, 0);
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (;;) {
+      const int s = next.fetch_add(1);
+      if (s >= S) break;
+      err[s] = iluk_rows(c->h_rp, c->h_ci, seg[s], seg[s + 1], kseg[s], lens[s], fcis[s], poss[s]);
+    }
+  };
+  const int nt = std::max(1, std::min<int>(S, (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(work);
+  work();
+  for (auto &t : th) t.join();
+  int64_t total = 0;
+  for (int s = 0; s < S; ++s) {
+    if (err[s] == -1) { set_err("ILU(k): an entry of segment %d leaves its diagonal block", s); return 2; }
+    if (err[s] < -1) { set_err("ILU(k): row %d has no diagonal entry", -(err[s] + 2)); return 2; }
+    total += (int64_t)fcis[s].size();
+  }
+  if (total > 0x7fffffff) { set_err("ILU(k) factor too large for 32-bit indices"); return 2; }
+  c->h_frp.assign(n + 1, 0);
+  c->h_fci.resize(total);
+  pos.resize(total);
+  int64_t w = 0;
+  for (int s = 0; s < S; ++s) {
+    for (int i = 0; i < seg[s + 1] - seg[s]; ++i) c->h_frp[seg[s] + i + 1] = (int)(w += lens[s][i]) ;
+    std::copy(fcis[s].begin(), fcis[s].end(), c->h_fci.begin() + (w - (int64_t)fcis[s].size()));
+    std::copy(poss[s].begin(), poss[s].end(), pos.begin() + (w - (int64_t)poss[s].size()));
+    std::vector<int>().swap(fcis[s]);
+    std::vector<int>().swap(poss[s]);
+  }
   return 0;
 }
 
@@ -1264,7 +1463,7 @@ int analyse_and_factor(shesolve_ctx *c) {
       }
       c->h_frp[p + 1] = w;
     }
-  } else if (c->prm.precond == SHESOLVE_PRECOND_ILUK && c->prm.fill_level > 0) {
+  } else if (c->prm.precond == SHESOLVE_PRECOND_ILUK && (c->prm.fill_level > 0 || !c->seg_fill.empty())) {
     if (int e = iluk_pattern(c, pos)) return e;
   } else {
     c->h_frp = c->h_rp;
@@ -1364,6 +1563,38 @@ int analyse_and_factor(shesolve_ctx *c) {
   }
   c->nL = Lci.size();
   c->nU = Uci.size();
+  std::vector<int> loLrp, loUrp;
+  if (c->nseg > 0) {
+    // segment sweeps: store each triangle in level order; the natural-indexed
+    // "row pointer" becomes the start of each row in that storage (k_split
+    // writes through it), lrp[q] the start of the q-th level-ordered row
+    // columns become positions in the same schedule (the sweep's window index)
+    auto reorder = [&](std::vector<int> &rp, std::vector<int> &ci, const std::vector<int> &rows,
+                       std::vector<int> &lrp) {
+      std::vector<int> start(n + 1, 0), lci(ci.size()), qpos(n);
+      for (int q = 0; q < n; ++q) qpos[rows[q]] = q;
+      lrp.assign(n + 1, 0);
+      int w = 0;
+      for (int q = 0; q < n; ++q) {
+        const int row = rows[q];
+        start[row] = w;
+        lrp[q] = w;
+        for (int k = rp[row]; k < rp[row + 1]; ++k) lci[w++] = qpos[ci[k]];
+      }
+      lrp[n] = w;
+      start[n] = w;
+      rp.swap(start);
+      ci.swap(lci);
+    };
+    reorder(Lrp, Lci, c->h_segrowsL, loLrp);
+    reorder(Urp, Uci, c->h_segrowsU, loUrp);
+    std::vector<int>().swap(c->h_segrowsL);
+    std::vector<int>().swap(c->h_segrowsU);
+    HIPCHK(hipMalloc(&c->d_loLrp, sizeof(int) * (n + 1)));
+    HIPCHK(hipMalloc(&c->d_loUrp, sizeof(int) * (n + 1)));
+    HIPCHK(hipMemcpy(c->d_loLrp, loLrp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_loUrp, loUrp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  }
   // contiguous levels get CSR-stream row blocks
   auto runs = [&](const Levels &lv, const std::vector<int> &rows, const std::vector<int> &rp,
                   std::vector<shesolve_ctx::LevelRun> &out, std::vector<int> &blk) {
@@ -1388,8 +1619,10 @@ int analyse_and_factor(shesolve_ctx *c) {
     }
   };
   std::vector<int> lblk, ublk;
-  runs(c->lvL, Lrows, Lrp, c->runL, lblk);
-  runs(c->lvU, Urows, Urp, c->runU, ublk);
+  if (c->nseg == 0) {
+    runs(c->lvL, Lrows, Lrp, c->runL, lblk);
+    runs(c->lvU, Urows, Urp, c->runU, ublk);
+  }
   HIPCHK(hipMalloc(&c->d_Lrp, sizeof(int) * (n + 1)));
   HIPCHK(hipMalloc(&c->d_Urp, sizeof(int) * (n + 1)));
   HIPCHK(hipMalloc(&c->d_Lci, sizeof(int) * std::max<int64_t>(c->nL, 1)));
@@ -1416,7 +1649,9 @@ int analyse_and_factor(shesolve_ctx *c) {
 // Copies A's values into the factor pattern (gather for block ILU) and factorises.
 int refactor(shesolve_ctx *c) {
   if (c->prm.precond == SHESOLVE_PRECOND_NONE || !c->d_frp) return 0;
-  if (c->factor_is_A) {
+  if (c->solve_perm) {
+    HIPCHK(hipMemcpyAsync(c->d_f64, c->d_sv64, sizeof(double) * c->fnnz, hipMemcpyDeviceToDevice, c->stream));
+  } else if (c->factor_is_A) {
     HIPCHK(hipMemcpyAsync(c->d_f64, c->d_v64, sizeof(double) * c->nnz, hipMemcpyDeviceToDevice, c->stream));
   } else {
     hipLaunchKernelGGL(k_gather, dim3(grid_for(c->fnnz, kBlock * 4)), dim3(kBlock), 0, c->stream, c->fnnz, c->d_fpos,
@@ -1447,10 +1682,44 @@ int refactor(shesolve_ctx *c) {
 }
 
 int upload_values(shesolve_ctx *c) {
-  hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->nnz, kBlock * 4)), dim3(kBlock), 0, c->stream, c->nnz,
-                     c->d_v64, c->d_v32);
+  if (c->solve_perm) {
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(c->fnnz, kBlock * 4)), dim3(kBlock), 0, c->stream, c->fnnz, c->d_fpos,
+                       c->d_v64, c->d_sv64);
+    hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->fnnz, kBlock * 4)), dim3(kBlock), 0, c->stream,
+                       c->fnnz, c->d_sv64, c->d_sv32);
+  } else {
+    hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->nnz, kBlock * 4)), dim3(kBlock), 0, c->stream,
+                       c->nnz, c->d_v64, c->d_v32);
+  }
   HIPCHK(hipGetLastError());
   return refactor(c);
+}
+
+// Solve space of a multicolour factor: the permuted matrix (the factor's
+// pattern), its CSR-stream row blocks, and the permuted b / x of a solve.
+int setup_solve_space(shesolve_ctx *c) {
+  c->solve_perm = c->d_perm != nullptr && c->seg_ptr.empty();
+  if (!c->solve_perm) {
+    c->s_rp = c->d_rp;
+    c->s_ci = c->d_ci;
+    c->s_blk = c->d_ablk;
+    c->s_nblk = c->nablk;
+    return 0;
+  }
+  dfree(c->d_v32);  // the natural-order FP32 copy served the launch autotuner only
+  HIPCHK(hipMalloc(&c->d_sv64, sizeof(double) * std::max<int64_t>(c->fnnz, 1)));
+  HIPCHK(hipMalloc(&c->d_sv32, sizeof(float) * std::max<int64_t>(c->fnnz, 1)));
+  HIPCHK(hipMalloc(&c->pb64, sizeof(double) * std::max(c->n, 1)));
+  HIPCHK(hipMalloc(&c->px64, sizeof(double) * std::max(c->n, 1)));
+  std::vector<int> blk;
+  make_blocks(c->h_frp.data(), 0, c->n, c->items * kStreamThreads, blk);
+  HIPCHK(hipMalloc(&c->d_sblk, sizeof(int) * blk.size()));
+  HIPCHK(hipMemcpy(c->d_sblk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice));
+  c->s_rp = c->d_frp;
+  c->s_ci = c->d_fci;
+  c->s_blk = c->d_sblk;
+  c->s_nblk = (int)blk.size() - 1;
+  return 0;
 }
 
 // One BiCGStab solve (right preconditioned) on the device, precision T.
@@ -1507,7 +1776,8 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
     *res_out = hs.res;
     return 0;
   }
-  int it = 0;
+  int it = 0, best_it = 0;
+  double best_res = hs.res;
   while (it < max_iters) {
     ++it;
     {
@@ -1545,6 +1815,16 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
       *st = 2;
       break;
     }
+    if (!confirm_true) {
+      // FP32 inner solve stalled: no 1 % gain over kStallWindow iterations
+      if (hs.res < 0.99 * best_res) {
+        best_res = hs.res;
+        best_it = it;
+      } else if (it - best_it >= kStallWindow) {
+        *st = 2;
+        break;
+      }
+    }
     if (hs.res <= tol) {
       // confirm with the explicit residual r = b - A x
       launch_spmv<TV, T, T>(c, c->lpr, Av, x, r, b, nullptr, nullptr, nullptr);
@@ -1581,7 +1861,7 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
 
 // FP64 residual norm ||b - A x|| into d_norm[slot] and r64 = b - A x
 int residual64(shesolve_ctx *c, const double *b, const double *x, double *r, double *out) {
-  launch_spmv<double, double, double>(c, c->lpr, c->d_v64, x, r, b, nullptr, nullptr, nullptr);
+  launch_spmv<double, double, double>(c, c->lpr, sv64(c), x, r, b, nullptr, nullptr, nullptr);
   hipLaunchKernelGGL((k_copy_dot<double>), dim3(kGrid), dim3(kBlock), 0, c->stream, c->n, r, r, c->d_part);
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, c->d_part, c->d_sc, c->d_norm);
   HIPCHK(hipMemcpyAsync(out, c->d_norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1589,7 +1869,24 @@ int residual64(shesolve_ctx *c, const double *b, const double *x, double *r, dou
   return 0;
 }
 
+int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_report *rep);
+
+// Solve in the solve space: b and x go through the multicolour permutation
+// once per solve when the matrix is stored permuted.
 int solve_device(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_report *rep) {
+  if (!c->solve_perm) return solve_core(c, b, x, warm, rep);
+  const int g = grid_for(c->n, kBlock);
+  hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(kBlock), 0, c->stream, c->n, c->d_perm, b, c->pb64, 0);
+  if (warm)
+    hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(kBlock), 0, c->stream, c->n, c->d_perm, x, c->px64, 0);
+  if (int e = solve_core(c, c->pb64, c->px64, warm, rep)) return e;
+  hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(kBlock), 0, c->stream, c->n, c->d_perm, c->px64, x, 1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_report *rep) {
   auto t0 = std::chrono::steady_clock::now();
   shesolve_report r{};
   if (!c->d_rp) {
@@ -1602,7 +1899,7 @@ int solve_device(shesolve_ctx *c, const double *b, double *x, int warm, shesolve
   if (c->prm.precision == SHESOLVE_FP64) {
     int st = 1;
     double res = 0;
-    int it = bicgstab_dev<double, double>(c, c->d_v64, c->d_f64, b, x, c->prm.tol, c->prm.max_iters, &st, &res);
+    int it = bicgstab_dev<double, double>(c, sv64(c), c->d_f64, b, x, c->prm.tol, c->prm.max_iters, &st, &res);
     if (it < 0) return it;
     r.iterations = it;
     r.status = st;
@@ -1624,7 +1921,9 @@ int solve_device(shesolve_ctx *c, const double *b, double *x, int warm, shesolve
     } else {
       r.status = 1;
       double prev_rn = -1.0;
-      bool fallback = false;
+      bool fallback = false, inner_failed = false;
+      const int inner_cap = std::min(c->prm.max_iters, 1000);
+      int first_it = -1;
       double *xprev = c->v64[6];  // idle during the FP32 inner solves
       for (int k = 0; k <= c->prm.max_ir_steps; ++k) {
         double rn = 0;
@@ -1641,6 +1940,12 @@ int solve_device(shesolve_ctx *c, const double *b, double *x, int warm, shesolve
           r.status = 0;
           break;
         }
+        if (inner_failed) {
+          // the last FP32 solve stalled or broke down: keep its (useful)
+          // correction and finish with FP64 BiCGStab
+          fallback = true;
+          break;
+        }
         if (k == c->prm.max_ir_steps || r.iterations >= c->prm.max_iters) break;
         r.ir_steps++;
         prev_rn = rn;
@@ -1652,18 +1957,21 @@ int solve_device(shesolve_ctx *c, const double *b, double *x, int warm, shesolve
         hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)n, dd, 0.0f);
         int st = 1;
         double ires = 0;
-        int it = bicgstab_dev<float, float>(c, c->d_v32, c->d_f32, r32, dd, c->prm.inner_tol,
-                                            c->prm.max_iters - r.iterations, &st, &ires, false);
+        const int cap = first_it < 0 ? inner_cap : std::max(40, 3 * first_it);
+        int it = bicgstab_dev<float, float>(c, sv32(c), c->d_f32, r32, dd, c->prm.inner_tol,
+                                            std::min(cap, c->prm.max_iters - r.iterations), &st, &ires, false);
         if (it < 0) return it;
+        if (first_it < 0) first_it = it;
         r.iterations += it;
+        if (st != 0) inner_failed = true;
         HIPCHK(hipMemcpyAsync(c->d_norm, &rn, sizeof(double), hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_axpy_f32, dim3(kGrid), dim3(kBlock), 0, c->stream, n, x, dd, c->d_norm);
       }
-      if (fallback && r.iterations < c->prm.max_iters) {
+      if (fallback) {
+        // the FP64 solve is the safety net: it gets the full iteration budget
         int st = 1;
         double res = 0;
-        int it = bicgstab_dev<double, double>(c, c->d_v64, c->d_f64, b, x, c->prm.tol,
-                                              c->prm.max_iters - r.iterations, &st, &res);
+        int it = bicgstab_dev<double, double>(c, sv64(c), c->d_f64, b, x, c->prm.tol, c->prm.max_iters, &st, &res);
         if (it < 0) return it;
         r.iterations += it;
         r.status = st;
@@ -1726,12 +2034,14 @@ int tune_stream(shesolve_ctx *c) {
   if (int e = ensure_vectors(c)) return e;
   hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)n, c->v32[0], 1.0f);
   for (int ii = 0; ii < 3; ++ii) {
+    if (!c->prm.autotune && cand_items[ii] != 8) continue;
     std::vector<int> ab;
     make_blocks(c->h_rp.data(), 0, n, cand_items[ii] * kStreamThreads, ab);
     int *d_ab = nullptr;
     HIPCHK(hipMalloc(&d_ab, sizeof(int) * ab.size()));
     HIPCHK(hipMemcpyAsync(d_ab, ab.data(), sizeof(int) * ab.size(), hipMemcpyHostToDevice, c->stream));
     for (int xm = 0; xm < 2; ++xm) {
+      if (!c->prm.autotune && xm != 0) continue;
       c->items = cand_items[ii];
       c->xcd = xm;
       c->d_ablk = d_ab;
@@ -1794,6 +2104,7 @@ int set_matrix_common(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, 
   c->lpr = c->prm.lanes_per_row > 0 ? c->prm.lanes_per_row : pick_lpr((double)nnz / n);
   if (int e = tune_stream(c)) return e;
   if (int e = analyse_and_factor(c)) return e;
+  if (int e = setup_solve_space(c)) return e;
   if (int e = upload_values(c)) return e;
   if (int e = ensure_vectors(c)) return e;
   c->last_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1809,7 +2120,7 @@ struct shesolve_batch_ctx {
   int S = 0;
   std::vector<int> sys_ptr;        // row offsets of the systems
   std::vector<int64_t> val_ptr;    // value offsets of the systems
-  int *d_sys_ptr = nullptr, *d_row_sys = nullptr, *d_mask = nullptr;
+  int *d_sys_ptr = nullptr, *d_row_sys = nullptr, *d_mask = nullptr, *d_caps = nullptr;
   BSys *d_bs = nullptr;
   double *d_bpart = nullptr, *d_nrm = nullptr;
   double *x64 = nullptr, *b64 = nullptr, *xprev = nullptr;
@@ -1819,6 +2130,7 @@ struct shesolve_batch_ctx {
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // one iteration: [0] FP64, [1] FP32
   bool use_graph = true;
   int poll = 8;                    // iterations between host checks of the masks
+  bool verbose = getenv("SHESOLVE_VERBOSE") != nullptr;  // progress lines on stderr
   double last_setup_ms = 0;
 };
 
@@ -1828,7 +2140,7 @@ void batch_free(shesolve_batch_ctx *b) {
   if (b->c) b->c->seg_mask = nullptr;
   for (auto &g : b->graph)
     if (g) { hipGraphExecDestroy(g); g = nullptr; }
-  dfree(b->d_sys_ptr); dfree(b->d_row_sys); dfree(b->d_mask); dfree(b->d_bs);
+  dfree(b->d_sys_ptr); dfree(b->d_row_sys); dfree(b->d_mask); dfree(b->d_caps); dfree(b->d_bs);
   dfree(b->d_bpart); dfree(b->d_nrm); dfree(b->x64); dfree(b->b64); dfree(b->xprev); dfree(b->dd);
 }
 
@@ -1902,7 +2214,7 @@ void bbicg_iter(shesolve_batch_ctx *b, T *x) {
 // initial guess in x.  Per-system results in b->h_bs.
 template <typename TV, typename T>
 int bbicg_run(shesolve_batch_ctx *b, const T *rhs, T *x, double tol, int max_iters, double div_limit,
-              const std::vector<int> &mask) {
+              const std::vector<int> &mask, const std::vector<int> *caps = nullptr) {
   shesolve_ctx *c = b->c;
   T **V;
   const TV *Av;
@@ -1910,8 +2222,10 @@ int bbicg_run(shesolve_batch_ctx *b, const T *rhs, T *x, double tol, int max_ite
   else { V = (T **)c->v32; Av = (const TV *)c->d_v32; }
   T *r = V[0], *r0 = V[1], *p = V[2], *v = V[3];
   if (int e = upload_mask(b, mask)) return e;
+  if (caps) HIPCHK(hipMemcpyAsync(b->d_caps, caps->data(), sizeof(int) * b->S, hipMemcpyHostToDevice, c->stream));
+  // FP32 inner solves (finite divergence limit) also stop when they stall
   hipLaunchKernelGGL(k_binit, dim3((b->S + 63) / 64), dim3(64), 0, c->stream, b->S, b->d_bs, b->d_mask, tol, max_iters,
-                     div_limit);
+                     div_limit, div_limit < 1e300 ? kStallWindow : 0, caps ? (const int *)b->d_caps : nullptr);
   launch_spmv<TV, T, T>(c, c->lpr, Av, x, r, rhs, nullptr, nullptr, nullptr);
   bupd<T>(b, 3, r0, p, v, r, nullptr, nullptr, nullptr);
   bdot<T>(b, nullptr, r, r, rhs, rhs);
@@ -1931,9 +2245,12 @@ int bbicg_run(shesolve_batch_ctx *b, const T *rhs, T *x, double tol, int max_ite
   for (int done = 0; done < max_iters;) {
     HIPCHK(hipMemcpyAsync(b->h_bs.data(), b->d_bs, sizeof(BSys) * b->S, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    bool any = false;
-    for (auto &q : b->h_bs) any |= q.active != 0;
-    if (!any) break;
+    int nact = 0;
+    for (auto &q : b->h_bs) nact += q.active != 0;
+    if (!nact) break;
+    if (b->verbose && done % (8 * b->poll) == 0 && done > 0)
+      fprintf(stderr, "[shesolve batch]   %s iteration %d: %d systems active\n", sizeof(T) == 8 ? "FP64" : "FP32",
+              done, nact);
     const int k = std::min(b->poll, max_iters - done);
     for (int i = 0; i < k; ++i) {
       if (graph) HIPCHK(hipGraphLaunch(b->graph[gi], c->stream));
@@ -2007,7 +2324,7 @@ int batch_solve_internal(shesolve_batch_ctx *b, int warm) {
     float *r32 = c->v32[8];
     // an FP32 inner solve that stalls (iteration cap) or breaks down hands its
     // system to the FP64 solve after its correction has been checked
-    std::vector<int> inner_failed(S, 0);
+    std::vector<int> inner_failed(S, 0), first_it(S, -1), caps(S, 0);
     const int inner_cap = std::min(prm.max_iters, 1000);
     for (int k = 0; k <= prm.max_ir_steps; ++k) {
       if (int e = upload_mask(b, act)) return e;
@@ -2043,17 +2360,35 @@ int batch_solve_internal(shesolve_batch_ctx *b, int warm) {
       bref(b, 4, nullptr, nullptr, b->dd);
       for (int s = 0; s < S; ++s)
         if (act[s]) { prev[s] = rn[s]; b->rep[s].ir_steps++; }
-      if (int e = bbicg_run<float, float>(b, r32, b->dd, prm.inner_tol, inner_cap, 1e4, act)) return e;
+      // later corrections get at most 3x the iterations the system's first
+      // FP32 solve needed (at least 40): FP32 solves of a correction that
+      // stop making progress are handed to FP64 early
+      for (int s = 0; s < S; ++s) caps[s] = first_it[s] < 0 ? inner_cap : std::max(40, 3 * first_it[s]);
+      if (int e = bbicg_run<float, float>(b, r32, b->dd, prm.inner_tol, inner_cap, 1e4, act, &caps)) return e;
+      for (int s = 0; s < S; ++s)
+        if (act[s] && first_it[s] < 0) first_it[s] = b->h_bs[s].iters;
+      int nact = 0, maxit = 0, nfail = 0;
       for (int s = 0; s < S; ++s)
         if (act[s]) {
           b->rep[s].iterations += b->h_bs[s].iters;
           if (b->h_bs[s].status != 0) inner_failed[s] = 1;
+          ++nact;
+          maxit = std::max(maxit, b->h_bs[s].iters);
+          nfail += b->h_bs[s].status != 0;
         }
+      if (b->verbose)
+        fprintf(stderr, "[shesolve batch] refinement step %d: %d systems, inner iterations max %d, %d stalled\n", k,
+                nact, maxit, nfail);
       // x += ||r_s|| d  (bbicg_run left d_mask = act and d_nrm untouched)
       bref(b, 1, b->x64, nullptr, b->dd);
     }
     bool anyfb = false;
     for (int s = 0; s < S; ++s) anyfb |= fb[s] != 0;
+    if (anyfb && b->verbose) {
+      int nfb = 0;
+      for (int s = 0; s < S; ++s) nfb += fb[s];
+      fprintf(stderr, "[shesolve batch] FP64 solve for %d systems\n", nfb);
+    }
     if (anyfb) {
       if (int e = bbicg_run<double, double>(b, b->b64, b->x64, prm.tol, prm.max_iters, 1e300, fb)) return e;
       for (int s = 0; s < S; ++s)
@@ -2084,6 +2419,10 @@ int batch_set_systems(shesolve_batch_ctx *b, bool device_src, int count, const i
   HIPCHK(hipSetDevice(c->device));
   if (count <= 0) { set_err("count must be positive"); return 2; }
   if (!n) { set_err("n is NULL"); return 3; }
+  if (c->prm.precond == SHESOLVE_PRECOND_MC_ILU0) {
+    set_err("the multicolour ordering is not available for batches (use ILU0, BLOCK_ILU0 or ILUK)");
+    return -3;
+  }
   if (!rp) { set_err("row_ptr is NULL"); return 4; }
   if (!ci) { set_err("col is NULL"); return 5; }
   if (!val) { set_err("val is NULL"); return 6; }
@@ -2155,6 +2494,10 @@ int batch_set_systems(shesolve_batch_ctx *b, bool device_src, int count, const i
     for (int s = 0; s < count; ++s) std::copy(val[s], val[s] + nnz[s], hval.begin() + b->val_ptr[s]);
   }
   c->seg_ptr = b->sys_ptr;
+  if (!c->seg_fill.empty() && (int)c->seg_fill.size() != count) {
+    set_err("%d fill levels set for %d systems", (int)c->seg_fill.size(), count);
+    return 2;
+  }
   int e = set_matrix_common(c, false, N, NNZ, crp.data(), cci.data(), device_src ? dval : hval.data(),
                             (int)(cranges.size() / 2), cranges.data(), device_src ? 1 : 0);
   if (dval) hipFree(dval);
@@ -2165,6 +2508,7 @@ int batch_set_systems(shesolve_batch_ctx *b, bool device_src, int count, const i
   HIPCHK(hipMalloc(&b->d_sys_ptr, sizeof(int) * (count + 1)));
   HIPCHK(hipMalloc(&b->d_row_sys, sizeof(int) * N));
   HIPCHK(hipMalloc(&b->d_mask, sizeof(int) * count));
+  HIPCHK(hipMalloc(&b->d_caps, sizeof(int) * count));
   HIPCHK(hipMalloc(&b->d_bs, sizeof(BSys) * count));
   HIPCHK(hipMalloc(&b->d_bpart, sizeof(double) * 2 * count * kBatchParts));
   HIPCHK(hipMalloc(&b->d_nrm, sizeof(double) * count));
@@ -2247,6 +2591,7 @@ void shesolve_default_params(shesolve_params *p) {
   p->lanes_per_row = 0;
   p->fill_level = 1;
   p->drop_tol = 0.0;
+  p->autotune = 1;
 }
 
 const char *shesolve_last_error(void) { return g_err.c_str(); }
@@ -2413,7 +2758,14 @@ int shesolve_spmv(shesolve_handle h, const double *x, double *y) {
   HIPCHK(hipSetDevice(h->device));
   if (int e = ensure_vectors(h)) return e;
   HIPCHK(hipMemcpyAsync(h->v64[0], x, sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
-  launch_spmv<double, double, double>(h, h->lpr, h->d_v64, h->v64[0], h->v64[1], nullptr, nullptr, nullptr, nullptr);
+  if (h->solve_perm) {
+    const int g = grid_for(h->n, kBlock);
+    hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(kBlock), 0, h->stream, h->n, h->d_perm, h->v64[0], h->v64[2], 0);
+    launch_spmv<double, double, double>(h, h->lpr, sv64(h), h->v64[2], h->v64[3], nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(kBlock), 0, h->stream, h->n, h->d_perm, h->v64[3], h->v64[1], 1);
+  } else {
+    launch_spmv<double, double, double>(h, h->lpr, sv64(h), h->v64[0], h->v64[1], nullptr, nullptr, nullptr, nullptr);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(y, h->v64[1], sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -2428,7 +2780,14 @@ int shesolve_spmv_f32(shesolve_handle h, const float *x, float *y) {
   HIPCHK(hipSetDevice(h->device));
   if (int e = ensure_vectors(h)) return e;
   HIPCHK(hipMemcpyAsync(h->v32[0], x, sizeof(float) * h->n, hipMemcpyHostToDevice, h->stream));
-  launch_spmv<float, float, float>(h, h->lpr, h->d_v32, h->v32[0], h->v32[1], nullptr, nullptr, nullptr, nullptr);
+  if (h->solve_perm) {
+    const int g = grid_for(h->n, kBlock);
+    hipLaunchKernelGGL((k_permute<float>), dim3(g), dim3(kBlock), 0, h->stream, h->n, h->d_perm, h->v32[0], h->v32[2], 0);
+    launch_spmv<float, float, float>(h, h->lpr, sv32(h), h->v32[2], h->v32[3], nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_permute<float>), dim3(g), dim3(kBlock), 0, h->stream, h->n, h->d_perm, h->v32[3], h->v32[1], 1);
+  } else {
+    launch_spmv<float, float, float>(h, h->lpr, sv32(h), h->v32[0], h->v32[1], nullptr, nullptr, nullptr, nullptr);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(y, h->v32[1], sizeof(float) * h->n, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -2443,7 +2802,18 @@ int shesolve_precond_apply(shesolve_handle h, const double *rr, double *z) {
   HIPCHK(hipSetDevice(h->device));
   if (int e = ensure_vectors(h)) return e;
   HIPCHK(hipMemcpyAsync(h->v64[0], rr, sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
-  launch_precond<double, double>(h, h->d_f64, h->v64[0], h->v64[1]);
+  // every segment of a batch takes part, whatever its convergence mask says
+  const BSys *mask = h->seg_mask;
+  h->seg_mask = nullptr;
+  if (h->solve_perm) {
+    const int g = grid_for(h->n, kBlock);
+    hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(kBlock), 0, h->stream, h->n, h->d_perm, h->v64[0], h->v64[2], 0);
+    launch_precond<double, double>(h, h->d_f64, h->v64[2], h->v64[3]);
+    hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(kBlock), 0, h->stream, h->n, h->d_perm, h->v64[3], h->v64[1], 1);
+  } else {
+    launch_precond<double, double>(h, h->d_f64, h->v64[0], h->v64[1]);
+  }
+  h->seg_mask = mask;
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(z, h->v64[1], sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -2526,18 +2896,18 @@ int shesolve_time_spmv(shesolve_handle h, int fp32, int reps, double *avg_us) {
   h->prof = 0;
   if (fp32) {
     hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, h->stream, (int64_t)h->n, h->v32[0], 1.0f);
-    launch_spmv<float, float, float>(h, h->lpr, h->d_v32, h->v32[0], h->v32[1], nullptr, nullptr, nullptr, nullptr);
+    launch_spmv<float, float, float>(h, h->lpr, sv32(h), h->v32[0], h->v32[1], nullptr, nullptr, nullptr, nullptr);
   } else {
     hipLaunchKernelGGL((k_fill<double>), dim3(kGrid), dim3(kBlock), 0, h->stream, (int64_t)h->n, h->v64[0], 1.0);
-    launch_spmv<double, double, double>(h, h->lpr, h->d_v64, h->v64[0], h->v64[1], nullptr, nullptr, nullptr,
+    launch_spmv<double, double, double>(h, h->lpr, sv64(h), h->v64[0], h->v64[1], nullptr, nullptr, nullptr,
                                         nullptr);
   }
   HIPCHK(hipEventRecord(h->ev0, h->stream));
   for (int r = 0; r < reps; ++r) {
     if (fp32)
-      launch_spmv<float, float, float>(h, h->lpr, h->d_v32, h->v32[0], h->v32[1], nullptr, nullptr, nullptr, nullptr);
+      launch_spmv<float, float, float>(h, h->lpr, sv32(h), h->v32[0], h->v32[1], nullptr, nullptr, nullptr, nullptr);
     else
-      launch_spmv<double, double, double>(h, h->lpr, h->d_v64, h->v64[0], h->v64[1], nullptr, nullptr, nullptr,
+      launch_spmv<double, double, double>(h, h->lpr, sv64(h), h->v64[0], h->v64[1], nullptr, nullptr, nullptr,
                                           nullptr);
   }
   HIPCHK(hipEventRecord(h->ev1, h->stream));
@@ -2607,6 +2977,15 @@ int shesolve_batch_solve_device(shesolve_batch h, const double *const *d_b, doub
                                 shesolve_report *reports) {
   if (!h) { set_err("handle is NULL"); return 1; }
   return batch_solve(h, true, d_b, d_x, warm_start, reports);
+}
+int shesolve_batch_set_fill_levels(shesolve_batch h, int count, const int *levels) {
+  if (!h) { set_err("handle is NULL"); return 1; }
+  if (count < 0) { set_err("count must be >= 0"); return 2; }
+  if (count > 0 && !levels) { set_err("levels is NULL"); return 3; }
+  for (int s = 0; s < count; ++s)
+    if (levels[s] < 0 || levels[s] > 8) { set_err("fill level of system %d must be in [0, 8]", s); return 3; }
+  h->c->seg_fill.assign(levels, levels + count);
+  return 0;
 }
 int shesolve_batch_options(shesolve_batch h, int use_graph, int poll) {
   if (!h) { set_err("handle is NULL"); return 1; }

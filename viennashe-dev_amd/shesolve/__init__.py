@@ -28,14 +28,14 @@ EXPORTS = (
     "shesolve_batch_create", "shesolve_batch_destroy", "shesolve_batch_set_params", "shesolve_batch_set_systems",
     "shesolve_batch_set_systems_device", "shesolve_batch_update_values", "shesolve_batch_update_values_device",
     "shesolve_batch_solve", "shesolve_batch_solve_device", "shesolve_batch_options", "shesolve_batch_info",
-    "shesolve_batch_solver",
+    "shesolve_batch_solver", "shesolve_batch_set_fill_levels",
 )
 
 
 class Params(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iters", C.c_int), ("precond", C.c_int), ("precision", C.c_int),
                 ("inner_tol", C.c_double), ("max_ir_steps", C.c_int), ("lanes_per_row", C.c_int),
-                ("fill_level", C.c_int), ("drop_tol", C.c_double)]
+                ("fill_level", C.c_int), ("drop_tol", C.c_double), ("autotune", C.c_int)]
 
 
 class Report(C.Structure):
@@ -100,6 +100,7 @@ def lib():
     L.shesolve_batch_solve.argtypes = [C.c_void_p, VP, VP, C.c_int, C.POINTER(Report)]
     L.shesolve_batch_solve_device.argtypes = [C.c_void_p, VP, VP, C.c_int, C.POINTER(Report)]
     L.shesolve_batch_options.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.shesolve_batch_set_fill_levels.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
     L.shesolve_batch_info.argtypes = [C.c_void_p, _i64p]
     L.shesolve_batch_solver.argtypes = [C.c_void_p]
     L.shesolve_batch_solver.restype = C.c_void_p
@@ -279,6 +280,12 @@ class BatchSolver:
             setattr(self.params, k, v)
         _check(lib().shesolve_batch_set_params(self._h, C.byref(self.params)), "shesolve_batch_set_params")
 
+    def set_fill_levels(self, levels):
+        """ILU(k) level per system (0 = ILU(0)) for PRECOND_ILUK; applies at the next set_systems."""
+        lv = np.ascontiguousarray(levels, np.int32)
+        _check(lib().shesolve_batch_set_fill_levels(self._h, len(lv), lv.ctypes.data if len(lv) else None),
+               "shesolve_batch_set_fill_levels")
+
     def options(self, use_graph=True, poll=8):
         _check(lib().shesolve_batch_options(self._h, 1 if use_graph else 0, poll), "shesolve_batch_options")
 
@@ -349,6 +356,13 @@ class BatchSolver:
         h = lib().shesolve_batch_solver(self._h)
         _check(lib().shesolve_set_profiling(h, int(on)), "shesolve_set_profiling")
 
+    def precond_apply(self, r):
+        """z = M^-1 r with the FP64 factor of the whole block-diagonal system (concatenated vectors)."""
+        h = lib().shesolve_batch_solver(self._h)
+        z = np.zeros(sum(self.sizes))
+        _check(lib().shesolve_precond_apply(h, np.ascontiguousarray(r, np.float64), z), "shesolve_precond_apply")
+        return z
+
     def time_spmv(self, fp32=False, reps=20):
         h = lib().shesolve_batch_solver(self._h)
         a = C.c_double()
@@ -360,7 +374,7 @@ class BatchSolver:
 FILL_EXPORTS = ("shefill_last_error", "shefill_create", "shefill_destroy", "shefill_build", "shefill_sizes",
                 "shefill_device_system", "shefill_get_system", "shefill_block_ranges", "shefill_mapping",
                 "shefill_last_fill_ms", "shefill_refill", "shefill_recover", "shefill_recover_device",
-                "shefill_postproc", "shefill_postproc_device")
+                "shefill_postproc", "shefill_postproc_device", "shefill_pattern_hash")
 
 
 class FillDevice(C.Structure):
@@ -392,6 +406,7 @@ def _fill_lib():
     L.shefill_mapping.argtypes = [C.c_void_p, _i64p, _i32p]
     L.shefill_last_fill_ms.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.shefill_recover.argtypes = [C.c_void_p, _f64p, _f64p]
+    L.shefill_pattern_hash.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
     L.shefill_recover_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.shefill_postproc.argtypes = [C.c_void_p, _f64p, C.c_void_p, C.c_void_p]
     L.shefill_postproc_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -484,6 +499,11 @@ class Fill:
     def last_fill_ms(self):
         v = C.c_double()
         _fcheck(_fill_lib().shefill_last_fill_ms(self._h, C.byref(v)), "shefill_last_fill_ms")
+        return v.value
+
+    def pattern_hash(self):
+        v = C.c_uint64()
+        _fcheck(_fill_lib().shefill_pattern_hash(self._h, C.byref(v)), "shefill_pattern_hash")
         return v.value
 
     def recover(self, x):

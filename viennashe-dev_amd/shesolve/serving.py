@@ -1,0 +1,349 @@
+"""Request loop for SHE solve sequences on one GPU, with continuous admission.
+
+A request is one Gummel-like sequence of SHE solves on one device. It mirrors what
+simulator::run does per nonlinear iteration (simulator.hpp:895-1096):
+1. set up energies, expansion orders and mapping again (the pattern follows the potential);
+2. assemble;
+3. solve, warm-started from the previous iterate transferred to the new energy grid
+   (she::transfer_to_new_h_space plays this part in the reference);
+4. perturb the potential before the next step (here: N(0, sigma) per cell).
+
+Serving is continuous:
+- A request is admitted as soon as it has arrived, a slot is free and its estimated
+  device footprint fits the HBM budget. Nothing waits for a batch to fill or drain.
+- Each slot is a host thread that owns its own fill and solver handles, so its own HIP
+  stream. The kernels of admitted requests therefore overlap on the device.
+- The library calls release the GIL, so the slots' host work (pattern analysis, level
+  schedules) also runs in parallel.
+- The matrix, preconditioner, Krylov vectors and previous iterate of a request stay
+  resident for its whole sequence.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import math
+import threading
+import time
+
+import numpy as np
+
+from . import devices as D
+
+_hip = None
+
+
+def _hiplib():
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        _hip.hipFree.argtypes = [C.c_void_p]
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    return _hip
+
+
+class DeviceBuffer:
+    """A plain device allocation of n doubles (freed on close)."""
+
+    def __init__(self, n):
+        self.n = n
+        p = C.c_void_p()
+        rc = _hiplib().hipMalloc(C.byref(p), C.c_size_t(8 * max(n, 1)))
+        if rc != 0:
+            raise MemoryError(f"hipMalloc of {8 * n} bytes failed ({rc})")
+        self.ptr = p.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        assert a.size == self.n
+        assert _hiplib().hipMemcpy(self.ptr, a.ctypes.data, 8 * self.n, 1) == 0
+
+    def download(self):
+        a = np.empty(self.n)
+        assert _hiplib().hipMemcpy(a.ctypes.data, self.ptr, 8 * self.n, 2) == 0
+        return a
+
+    def close(self):
+        if self.ptr:
+            _hiplib().hipFree(self.ptr)
+            self.ptr = None
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float            # seconds after the start of the run
+    kind: str                 # "diode" or "mosfet"
+    dev: D.Device
+    cfg: D.SheConfig
+    seed: int
+    steps: int = 5
+    sigma: float = 5e-3       # potential perturbation per cell between steps [V]
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    kind: str
+    arrival: float
+    start: float
+    finish: float
+    reports: list
+    n: list
+    x: np.ndarray             # last reduced solution
+    density: np.ndarray       # carrier density of the last step
+    footprint: float
+    desc: str = ""
+
+    @property
+    def latency(self):
+        return self.finish - self.arrival
+
+
+def _even_count(L):
+    """Even harmonics up to order L (odd L): 1, 6, 15, 28 for L = 1, 3, 5, 7."""
+    return L * (L + 1) // 2
+
+
+def footprint_estimate(dev: D.Device, cfg: D.SheConfig) -> float:
+    """Upper estimate of the device bytes one request holds.
+
+    The count covers:
+    - the fill's reduced system (values, columns, row pointer);
+    - the solver's FP64 and FP32 copies;
+    - the level-of-fill factor;
+    - 18 Krylov vectors;
+    - per-row fill data.
+
+    The energy count follows setup_energies (she/setup_energies.hpp:108-120)."""
+    span = float(np.max(dev.potential) - np.min(dev.potential))
+    nh = int((D.Q * span + 2 * cfg.energy_spacing + cfg.min_kin_range) / cfg.energy_spacing) + 1
+    rows = dev.ncells * max(nh - 2, 1) * _even_count(cfg.L_max)
+    per_row = {1: {1: 5.0, 3: 7.0, 5: 11.0, 7: 16.0}, 2: {1: 6.0, 3: 11.0, 5: 20.0, 7: 30.0}}[dev.dim][cfg.L_max]
+    return rows * (per_row * 52.0 + 220.0)
+
+
+def _preconditioner(dev, cfg):
+    import shesolve
+    if dev.dim > 1:
+        return dict(precond=shesolve.PRECOND_MC_ILU0)
+    # natural order on 1D chains; ILU(1) where ILU(0) diverges (L >= 5)
+    return dict(precond=shesolve.PRECOND_ILUK, fill_level=1 if cfg.L_max >= 5 else 0)
+
+
+def _transfer(x_old, map_old, shift, map_new, n_new):
+    """Previous iterate onto the new unknowns: the same cell, the same energy
+    (new level h holds the energy of old level h + shift), the harmonics both
+    orders share."""
+    ei_o, eo_o = map_old
+    ei_n, eo_n = map_new
+    x0 = np.zeros(n_new)
+    nh_o, nh_n = ei_o.shape[1], ei_n.shape[1]
+    h_n = np.arange(nh_n)
+    h_o = h_n + shift
+    ok = (h_o >= 0) & (h_o < nh_o)
+    h_n, h_o = h_n[ok], h_o[ok]
+    a_n = ei_n[:, h_n]
+    a_o = ei_o[:, h_o]
+    k_n = np.where(a_n >= 0, eo_n[:, h_n] * (eo_n[:, h_n] + 1) // 2, 0)
+    k_o = np.where(a_o >= 0, eo_o[:, h_o] * (eo_o[:, h_o] + 1) // 2, 0)
+    k = np.minimum(k_n, k_o)
+    sel = (a_n >= 0) & (a_o >= 0) & (k > 0)
+    dst, src, cnt = a_n[sel], a_o[sel], k[sel]
+    for j in range(int(cnt.max()) if cnt.size else 0):
+        m = cnt > j
+        x0[dst[m] + j] = x_old[src[m] + j]
+    return x0
+
+
+def run_request(req: Request, tol=1e-10, max_iters=3000, device=0, clock=None, autotune=True):
+    """One request, start to finish, on the calling thread.  autotune=False
+    pins the launch shape, so that the result cannot depend on timings."""
+    import shesolve
+    start = clock() if clock else time.perf_counter()
+    F = shesolve.Fill(device)
+    pc = _preconditioner(req.dev, req.cfg)
+    sv = shesolve.Solver(device, precision=shesolve.FP32_IR, tol=tol, max_iters=max_iters,
+                         autotune=1 if autotune else 0, **pc)
+    rng = np.random.default_rng(req.seed)
+    dev = dataclasses.replace(req.dev, potential=np.array(req.dev.potential, np.float64))
+    reps, ns = [], []
+    prev = None
+    pattern = None
+    dx = None
+    try:
+        for step in range(req.steps):
+            if step > 0:
+                dev = dataclasses.replace(dev, potential=dev.potential + rng.normal(0.0, req.sigma, dev.ncells))
+            sz = F.build(dev, req.cfg)
+            n, nnz = sz["n"], sz["nnz"]
+            d_rp, d_ci, d_val, d_rhs = F.device_system()
+            h = F.pattern_hash()
+            if h == pattern and dx is not None and dx.n == n:
+                sv.update_values_device(d_val)
+            else:
+                sv.set_matrix_device(n, nnz, d_rp, d_ci, d_val)
+                pattern = h
+            mapping = F.mapping()
+            if dx is None or dx.n != n:
+                if dx is not None:
+                    dx.close()
+                dx = DeviceBuffer(n)
+            warm = prev is not None
+            if warm:
+                x_old, map_old, psimax_old = prev
+                # new level h has the energy of old level h + shift (edge = -q max(psi) - dE + Eg/2)
+                shift = int(round(-(float(np.max(dev.potential)) - psimax_old) * D.Q / req.cfg.energy_spacing))
+                dx.upload(_transfer(x_old, map_old, shift, mapping, n))
+            rep = sv.solve_device(d_rhs, dx.ptr, warm_start=warm)
+            reps.append(rep)
+            ns.append(n)
+            x = dx.download()
+            prev = (x, mapping, float(np.max(dev.potential)))
+        xf = F.recover(prev[0])
+        dens, _ = F.postproc(xf)
+    finally:
+        if dx is not None:
+            dx.close()
+        sv.close()
+        F.close()
+    finish = clock() if clock else time.perf_counter()
+    desc = (f"{req.kind} {req.dev.nx}x{req.dev.ny} L={req.cfg.L_max} "
+            f"dE={req.cfg.energy_spacing / D.EV * 1e3:.2f} meV")
+    return Result(req.rid, req.kind, req.arrival, start, finish, reps, ns, prev[0], dens,
+                  footprint_estimate(req.dev, req.cfg), desc)
+
+
+_mos_cache = {}
+
+
+def _mosfet(vx, vy):
+    key = (vx, vy)
+    if key not in _mos_cache:
+        _mos_cache[key] = D.mosfet_2d(vx=vx, vy=vy)
+    return _mos_cache[key]
+
+
+def make_requests(count, rate=200.0, seed=2024, mos_frac=0.2, mesh=(41, 161), steps=5, sigma=5e-3,
+                  diode_length=(100, 1000)):
+    """Poisson arrivals at `rate` per second.
+
+    - A fraction `mos_frac` are 2D MOSFET-like devices: cfg2 geometry, vx uniform in
+      `mesh`, vy = (vx - 1) / 2 + 1, L in {1, 3}, 10 meV to 1.5 eV.
+    - The rest are 1D diodes drawn like the batched workload: length uniform in
+      `diode_length` nm with 1 nm cells, n+ 20 % of the length each side, L in
+      {1, 3, 5}, energy step U[2, 10] meV to 1.0 eV, bias U[0, 1.5] V."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        if rng.random() < mos_frac:
+            vx = int(rng.integers(mesh[0], mesh[1] + 1))
+            vx += 1 - vx % 2
+            vy = (vx - 1) // 2 + 1
+            L = int(rng.choice([1, 3]))
+            dev = _mosfet(vx, vy)
+            cfg = D.SheConfig(L_max=L, energy_spacing=0.010 * D.EV, min_kin_range=1.5 * D.EV)
+            kind = "mosfet"
+        else:
+            length = float(rng.integers(diode_length[0], diode_length[1] + 1))
+            L = int(rng.choice([1, 3, 5]))
+            de = float(rng.uniform(2.0, 10.0)) * 1e-3
+            bias = float(rng.uniform(0.0, 1.5))
+            dev = D.diode_1d(length_nm=length, vertices=int(length) + 1, bias=bias)
+            cfg = D.SheConfig(L_max=L, energy_spacing=de * D.EV, min_kin_range=1.0 * D.EV)
+            kind = "diode"
+        out.append(Request(i, t, kind, dev, cfg, seed=int(rng.integers(1 << 30)), steps=steps, sigma=sigma))
+    return out
+
+
+class Server:
+    """Slots (host threads) that admit arrived requests in arrival order while
+    the summed footprint estimate of the requests in flight stays within the
+    HBM budget."""
+
+    def __init__(self, slots=8, hbm_budget=200e9, tol=1e-10, device=0, autotune=True):
+        self.slots = slots
+        self.budget = hbm_budget
+        self.tol = tol
+        self.device = device
+        self.autotune = autotune
+
+    def run(self, requests, realtime=True):
+        """Serves `requests`.  realtime: requests become eligible at their
+        arrival time (else all at once).  Returns the results in request order."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        lock = threading.Condition()
+        state = {"in_use": 0.0, "next": 0, "peak": 0.0, "inflight": 0, "max_inflight": 0}
+        results = [None] * len(pending)
+        errors = []
+        t0 = time.perf_counter()
+        clock = lambda: time.perf_counter() - t0
+
+        def worker():
+            while True:
+                with lock:
+                    while True:
+                        if errors:
+                            return
+                        i = state["next"]
+                        if i >= len(pending):
+                            return
+                        req = pending[i]
+                        fp = footprint_estimate(req.dev, req.cfg)
+                        wait = req.arrival - clock() if realtime else 0.0
+                        fits = state["in_use"] + fp <= self.budget or state["inflight"] == 0
+                        if wait <= 0 and fits:
+                            state["next"] += 1
+                            state["in_use"] += fp
+                            state["inflight"] += 1
+                            state["peak"] = max(state["peak"], state["in_use"])
+                            state["max_inflight"] = max(state["max_inflight"], state["inflight"])
+                            break
+                        lock.wait(timeout=max(min(wait, 0.05), 0.001) if wait > 0 else 0.05)
+                try:
+                    res = run_request(req, tol=self.tol, device=self.device, clock=clock, autotune=self.autotune)
+                except Exception as e:  # noqa: BLE001 - surfaced to the caller below
+                    with lock:
+                        errors.append((req.rid, repr(e)))
+                        lock.notify_all()
+                    return
+                with lock:
+                    results[i] = res
+                    state["in_use"] -= fp
+                    state["inflight"] -= 1
+                    lock.notify_all()
+
+        threads = [threading.Thread(target=worker, daemon=True) for _ in range(self.slots)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise RuntimeError(f"requests failed: {errors}")
+        self.stats = {"peak_footprint": state["peak"], "max_inflight": state["max_inflight"],
+                      "makespan": clock()}
+        by_id = {r.rid: r for r in results}
+        return [by_id[r.rid] for r in requests]
+
+
+def summarize(results, makespan):
+    lat = np.array([r.latency for r in results])
+    solves = sum(len(r.reports) for r in results)
+    per_solve = []
+    for r in results:
+        per_solve.extend(rep["solve_ms"] for rep in r.reports)
+    return {
+        "requests": len(results),
+        "solves": solves,
+        "solves_per_s": solves / makespan if makespan > 0 else math.nan,
+        "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+        "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+        "solve_p50_ms": float(np.percentile(per_solve, 50)),
+        "solve_p99_ms": float(np.percentile(per_solve, 99)),
+        "max_rel_residual": max(rep["rel_residual"] for r in results for rep in r.reports),
+        "all_converged": all(rep["status"] == 0 for r in results for rep in r.reports),
+        "makespan_s": makespan,
+    }
