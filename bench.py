@@ -78,6 +78,8 @@ def cpu_baseline(rp, ci, val, b, fp64_iters, budget_s):
 
 
 def bench_batch(steps, count=96, seed=1234, fp64=True):
+    if count <= 0:
+        return {}
     """The batched workload: `count` 1D diodes drawn from a seed (length
     U[100,1000] nm, n+ 20 % each side, L in {1,3,5}, dE U[2,10] meV to 1.0 eV,
     bias U[0,1.5] V), solved together by the batched FP32+IR BiCGStab (one
@@ -135,7 +137,6 @@ def bench_batch(steps, count=96, seed=1234, fp64=True):
     log(f"[bench] batch of {count}: {info}, build {t_build:.1f} s, setup {t_setup:.1f} s")
     run(bs, 1)
     lat, reps = run(bs, steps)
-    bs.close()
     out = {
         "workload": f"{count} 1D diodes (seed {seed}): L in {{1,3,5}}, 100-1000 nm, dE 2-10 meV to 1.0 eV",
         "systems": count, "rows": info["rows"], "nnz": info["nnz"], "factor_nnz": info["factor_nnz"],
@@ -148,12 +149,12 @@ def bench_batch(steps, count=96, seed=1234, fp64=True):
         "setup_s": t_setup, "build_s": t_build, "precond": "ILU(0); ILU(1) for L=5",
     }
     if fp64:
-        bs, _ = make(shesolve.FP64)
+        bs.set_params(precision=shesolve.FP64)
         lat64, reps64 = run(bs, 1)
-        bs.close()
         out["fp64_ms_per_batch"] = 1e3 * lat64[0]
         out["fp64_iterations_max"] = max(r["iterations"] for r in reps64)
         out["speedup_vs_fp64"] = lat64[0] / (sum(lat) / len(lat))
+    bs.close()
     for p in dx:
         hip.hipFree(ctypes.c_void_p(p))
     for F in fills:
@@ -195,7 +196,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--profile-kernels", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-steps", type=int, default=1)
+    ap.add_argument("--batch-count", type=int, default=96)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=40)
     args = ap.parse_args()
@@ -335,7 +337,7 @@ def main():
     F.close()
     hip.hipFree(d_x)
     if not args.no_batch:
-        result["batch"] = bench_batch(args.batch_steps)
+        result["batch"] = bench_batch(args.batch_steps, count=args.batch_count)
         log(f"[bench] batch: {result['batch']}")
     if not args.no_serve:
         result["serving"] = bench_serving(args.serve_requests)

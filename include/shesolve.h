@@ -123,11 +123,13 @@ int shesolve_precond_apply(shesolve_handle h, const double *r, double *z);
  * by shesolve_factor_nnz. */
 int64_t shesolve_factor_nnz(shesolve_handle h);
 int shesolve_get_factors(shesolve_handle h, int64_t *row_ptr, int32_t *col, double *val);
-/* Launch shape the setup autotuner picked for the CSR-stream kernels: entries
- * per thread (row block = 256 * items entries), XCD-contiguous block mapping
- * (0/1), and the six timings it compared (FP32 SpMV, microseconds; order
- * items 4/8/16 x mapping strided/contiguous).  Any pointer may be NULL. */
-int shesolve_stream_config(shesolve_handle h, int32_t *items, int32_t *xcd, double *tune_us6);
+/* Launch shape the setup autotuner picked: entries per thread of the
+ * CSR-stream kernels (row block = 256 * items entries; the sweeps always use
+ * them), or -LPR when the vector-CSR SpMV with LPR lanes per row won the SpMV;
+ * XCD-contiguous block mapping (0/1); and the nine timings it compared (FP32
+ * SpMV, microseconds; items 4/8/16 x mapping strided/contiguous, then vector
+ * CSR with 4/8/16 lanes per row).  Any pointer may be NULL. */
+int shesolve_stream_config(shesolve_handle h, int32_t *items, int32_t *xcd, double *tune_us9);
 /* Row ordering of the factor (new -> old); the identity unless MC_ILU0. */
 int shesolve_get_ordering(shesolve_handle h, int32_t *perm);
 /* Level-schedule statistics: [0] forward levels, [1] backward levels. */

@@ -165,7 +165,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
                                                            const T *__restrict__ b, const int *__restrict__ perm,
                                                            const T *__restrict__ dinv, T *__restrict__ out2,
                                                            const T *__restrict__ d0, const T *__restrict__ d1,
-                                                           double *__restrict__ part) {
+                                                           double *__restrict__ part, const int *__restrict__ pairs) {
   constexpr int kStreamNnz = ITEMS * kStreamThreads;
   __shared__ T prod[kStreamNnz];
   double a0 = 0.0, a1 = 0.0;
@@ -177,7 +177,8 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
     bstep = per;
   }
   for (int bk = bk0; bk < bk1; bk += bstep) {
-    const int rs = blocks[bk], re = blocks[bk + 1];
+    // consecutive row blocks, or an explicit (first, end) pair per block
+    const int rs = pairs ? pairs[2 * bk] : blocks[bk], re = pairs ? pairs[2 * bk + 1] : blocks[bk + 1];
     const int k0 = rp[rs], k1 = rp[re];
     if (k1 - k0 <= kStreamNnz) {
       // all column indices and values first, then the gathers: 8 independent
@@ -559,7 +560,7 @@ __global__ __launch_bounds__(64) void k_ilu0_level(int cnt, const int *__restric
 // dot products, stopping test and a convergence mask per system, all on the
 // device, so that a sequence of iterations needs no host round trip.
 constexpr int kBatchParts = 32;  // partial sums per system and dot product
-constexpr int kStallWindow = 60;  // FP32 inner solves: iterations without a 1 % gain before giving up
+constexpr int kStallWindow = 100; // FP32 inner solves: iterations without a 1 % gain before giving up
 
 struct BSys {
   double rho, alpha, omega, beta;
@@ -728,38 +729,54 @@ __global__ void k_binit(int S, BSys *bs, const int *__restrict__ mask, double to
 
 // Triangular sweep of segment-local level schedules, one workgroup per
 // segment (one system of a batch): the workgroup walks its segment's levels in
-// order, a barrier between levels, so all communication stays inside one CU.
+// order, so all communication stays inside one CU.
 //
-// Storage is in level order: position q of the segment's schedule is row
-// rows[q], its entries lrp[q]..lrp[q+1], and each entry names its column by
-// the column's *position* cq (the schedule orders dependencies before their
-// users).  The outputs of the most recent positions are kept in an LDS window
-// of W slots (slot = position mod W): a dependency at position qc is read from
+// Everything the sweep touches is stored in level order -- position q of the
+// schedule: its entries lrp[q]..lrp[q+1] (each naming its column by the
+// column's position cq), right-hand side rhs[q], 1/diagonal dq[q], output
+// zq[q] -- so one CU streams contiguous memory level after level; the
+// permutations to and from natural order run as separate full-GPU passes.
+// The outputs of the most recent positions are also kept in an LDS window of
+// W slots (slot = position mod W): a dependency at position qc is read from
 // LDS when qc >= q1 - W (q1 = end of the current level, so no write of this
-// level can hit its slot) and from z[rows[qc]] otherwise.  A level's outputs
-// go to z and, for positions >= q1 - W, to the window.  The row data of the
-// next level (rows, entries, right-hand side, 1/diagonal) is loaded before
-// the current level is computed, so the only loads left on the critical path
-// are the window reads.
-//   lower: z[row] = in[row] - sum L[k] x(cq[k])
-//   upper: z[row] = dinv[row] (z[row] - sum U[k] x(cq[k]))
+// level can hit its slot) and from zq[qc] otherwise.
+//
+// Between levels only LDS has to be in order, so the levels are separated by
+// an LDS-only barrier (wait for this wave's LDS operations, then s_barrier);
+// the next level's prefetched row data and this level's stores keep flying.
+// A full barrier, which also drains global memory operations, is taken
+// whenever the next level could read from zq a position written after the
+// last full barrier (q_safe tracks what a full barrier has made visible), and
+// when a new chunk of level pointers is staged in LDS.
+//   lower: zq[q] = rhs[q] - sum L[k] x(cq[k])
+//   upper: zq[q] = dq[q] (rhs[q] - sum U[k] x(cq[k]))
 constexpr int kSweepThreads = 1024;
+constexpr int kLvlChunk = 256;
+__device__ inline void lds_barrier() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt left at their maximum (no wait)
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 template <int LPR, int R, int J, typename TV, typename T, bool UPPER>
 __global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict__ seg_lvl, const int *__restrict__ lvl_ptr,
-                                                           const int *__restrict__ rows, const int *__restrict__ lrp,
-                                                           const int *__restrict__ cq, const TV *__restrict__ v,
-                                                           const T *__restrict__ in, T *z, const TV *__restrict__ dinv,
-                                                           const BSys *__restrict__ bs, int W) {
+                                                           const int *__restrict__ lrp, const int *__restrict__ cq,
+                                                           const TV *__restrict__ v, const T *__restrict__ rhs, T *zq,
+                                                           const TV *__restrict__ dq, const BSys *__restrict__ bs,
+                                                           int W) {
   extern __shared__ unsigned char lds_raw[];
   T *win = reinterpret_cast<T *>(lds_raw);
+  __shared__ int lp[kLvlChunk + 1];
   const int s = blockIdx.x;
   if (bs && !bs[s].active) return;
   constexpr int G = kSweepThreads / LPR;
   const int Wm = W - 1;
   const int l0 = seg_lvl[s], l1 = seg_lvl[s + 1];
+  if (l0 >= l1) return;
   const int g = threadIdx.x / LPR, lane = threadIdx.x & (LPR - 1);
   // prefetched first chunk of a level: R rows per lane group, J entries per lane
-  int prow[R], pkb[R], pke[R], pc[R][J];
+  int pkb[R], pke[R], pc[R][J];
   T prhs[R], pd[R];
   TV pv[R][J];
   auto fetch = [&](int q0, int q1) {
@@ -767,11 +784,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict
     for (int r = 0; r < R; ++r) {
       const int q = q0 + g + r * G;
       const bool ok = q < q1;
-      prow[r] = ok ? rows[q] : 0;
       pkb[r] = ok ? lrp[q] : 0;
       pke[r] = ok ? lrp[q + 1] : 0;
-      prhs[r] = ok ? (UPPER ? z[prow[r]] : in[prow[r]]) : (T)0;
-      pd[r] = (UPPER && ok) ? (T)dinv[prow[r]] : (T)1;
+      prhs[r] = ok ? rhs[q] : (T)0;
+      pd[r] = (UPPER && ok) ? (T)dq[q] : (T)1;
 #pragma unroll
       for (int j = 0; j < J; ++j) {
         const int k = pkb[r] + lane + j * LPR;
@@ -781,22 +797,27 @@ __global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict
       }
     }
   };
-  if (l0 >= l1) return;
-  int q0 = lvl_ptr[l0], q1 = lvl_ptr[l0 + 1];
+  // level pointers of the segment, staged in LDS kLvlChunk levels at a time
+  int lc = l0;  // first level of the staged chunk
+  for (int t = threadIdx.x; t <= kLvlChunk && lc + t <= l1; t += kSweepThreads) lp[t] = lvl_ptr[lc + t];
+  __syncthreads();
+  int q0 = lp[0], q1 = lp[1];
+  int q_safe = q0;  // positions below q_safe are visible in zq to every wave
   fetch(q0, q1);
   for (int l = l0; l < l1; ++l) {
-    // this level's first chunk comes from the prefetch registers
-    int row[R], kb[R], ke[R], c[R][J];
-    T rhs[R], dd[R];
+    int kb[R], ke[R], c[R][J];
+    T rh[R], dd[R];
     TV vv[R][J];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      row[r] = prow[r]; kb[r] = pkb[r]; ke[r] = pke[r]; rhs[r] = prhs[r]; dd[r] = pd[r];
+      kb[r] = pkb[r]; ke[r] = pke[r]; rh[r] = prhs[r]; dd[r] = pd[r];
 #pragma unroll
       for (int j = 0; j < J; ++j) { c[r][j] = pc[r][j]; vv[r][j] = pv[r][j]; }
     }
-    const int nq1 = l + 1 < l1 ? lvl_ptr[l + 2] : q1;
-    if (l + 1 < l1) fetch(q1, nq1);
+    const bool more = l + 1 < l1;
+    const bool in_chunk = l + 2 - lc <= kLvlChunk;
+    const int nq1 = !more ? q1 : (in_chunk ? lp[l + 2 - lc] : lvl_ptr[l + 2]);
+    if (more) fetch(q1, nq1);
     const int lo = q1 - W;  // oldest position the window may serve during this level
     for (int base = q0 + g; base < q1; base += G * R) {
       if (base != q0 + g) {
@@ -805,11 +826,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict
         for (int r = 0; r < R; ++r) {
           const int q = base + r * G;
           const bool ok = q < q1;
-          row[r] = ok ? rows[q] : 0;
           kb[r] = ok ? lrp[q] : 0;
           ke[r] = ok ? lrp[q + 1] : 0;
-          rhs[r] = ok ? (UPPER ? z[row[r]] : in[row[r]]) : (T)0;
-          dd[r] = (UPPER && ok) ? (T)dinv[row[r]] : (T)1;
+          rh[r] = ok ? rhs[q] : (T)0;
+          dd[r] = (UPPER && ok) ? (T)dq[q] : (T)1;
 #pragma unroll
           for (int j = 0; j < J; ++j) {
             const int k = kb[r] + lane + j * LPR;
@@ -828,14 +848,14 @@ __global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict
           const int k = kb[r] + lane + j * LPR;
           if (k < ke[r]) {
             const int qc = c[r][j];
-            const T xv = qc >= lo ? win[qc & Wm] : z[rows[qc]];
+            const T xv = qc >= lo ? win[qc & Wm] : zq[qc];
             acc[r] += (T)vv[r][j] * xv;
           }
         }
         // entries beyond the prefetched J per lane
         for (int k = kb[r] + lane + J * LPR; k < ke[r]; k += LPR) {
           const int qc = cq[k];
-          const T xv = qc >= lo ? win[qc & Wm] : z[rows[qc]];
+          const T xv = qc >= lo ? win[qc & Wm] : zq[qc];
           acc[r] += (T)v[k] * xv;
         }
       }
@@ -846,16 +866,44 @@ __global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict
         for (int off = LPR / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, LPR);
         const int q = base + r * G;
         if (lane == 0 && q < q1) {
-          const T out = UPPER ? dd[r] * (rhs[r] - a) : rhs[r] - a;
-          z[row[r]] = out;
+          const T out = UPPER ? dd[r] * (rh[r] - a) : rh[r] - a;
+          zq[q] = out;
           if (q >= lo) win[q & Wm] = out;
         }
       }
     }
-    __syncthreads();
+    if (!more) break;
+    if (!in_chunk) {
+      // stage the next chunk of level pointers (levels l + 1 ..)
+      __syncthreads();
+      lc = l + 1;
+      for (int t = threadIdx.x; t <= kLvlChunk && lc + t <= l1; t += kSweepThreads) lp[t] = lvl_ptr[lc + t];
+      __syncthreads();
+      q_safe = q1;
+    } else if (nq1 - W > q_safe) {
+      // the next level may read positions >= q_safe from zq
+      __syncthreads();
+      q_safe = q1;
+    } else {
+      lds_barrier();
+    }
     q0 = q1;
     q1 = nq1;
   }
+}
+
+// level-order permutations of the segment sweeps: dst[i] = src[map[i]] / dst[map[i]] = src[i]
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kBlock) void k_qgather(int n, const int *__restrict__ map, const TI *__restrict__ src,
+                                                    TO *__restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (TO)src[map[i]];
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qscatter(int n, const int *__restrict__ map, const T *__restrict__ src,
+                                                     T *__restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[map[i]] = src[i];
 }
 
 // ---------------------------------------------------------------- host side
@@ -897,7 +945,7 @@ struct shesolve_ctx {
   // and columns), so that neither the SpMV nor the sweeps go through perm;
   // b and x are permuted once per solve
   bool solve_perm = false;
-  const int *s_rp = nullptr, *s_ci = nullptr, *s_blk = nullptr;
+  const int *s_rp = nullptr, *s_ci = nullptr, *s_blk = nullptr, *s_pairs = nullptr;
   int s_nblk = 0;
   double *d_sv64 = nullptr;
   float *d_sv32 = nullptr;
@@ -912,7 +960,8 @@ struct shesolve_ctx {
   int nablk = 0;
   int items = 8;   // entries per thread of a row block (row block = 256 * items entries)
   int xcd = 1;     // XCD-contiguous block mapping
-  double tune_us[6] = {0, 0, 0, 0, 0, 0};
+  double tune_us[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int spmv_lpr = 0;  // 0: CSR-stream SpMV; else lanes per row of the vector-CSR SpMV
   // split factor: strict lower / strict upper parts and 1/diagonal (factor space)
   int *d_Lrp = nullptr, *d_Lci = nullptr, *d_Urp = nullptr, *d_Uci = nullptr, *d_Lsrc = nullptr;
   double *d_L64 = nullptr, *d_U64 = nullptr, *d_dinv64 = nullptr;
@@ -932,6 +981,9 @@ struct shesolve_ctx {
   int *d_segL = nullptr, *d_lvlL = nullptr, *d_rowsL = nullptr;
   int *d_segU = nullptr, *d_lvlU = nullptr, *d_rowsU = nullptr;
   int *d_loLrp = nullptr, *d_loUrp = nullptr;  // level-ordered row pointers of the split factor
+  int *d_mapLU = nullptr;                      // U position -> L position of the same row
+  double *d_dq64 = nullptr, *d_tq64 = nullptr, *d_zq64 = nullptr;  // level-ordered 1/diag, rhs, output
+  float *d_dq32 = nullptr, *d_tq32 = nullptr, *d_zq32 = nullptr;
   std::vector<int> h_segrowsL, h_segrowsU;     // level-ordered rows (host copy during analysis)
   const BSys *seg_mask = nullptr;  // device per-segment state; inactive segments are skipped
   int seg_levels_max = 0;
@@ -1029,11 +1081,11 @@ inline int stream_grid(int nblk, bool fixed) {
 template <typename TV, typename T, int MODE>
 void launch_stream(shesolve_ctx *c, int g, int nblk, const int *blocks, const int *rp, const int *ci, const TV *v,
                    const T *x, T *y, const T *b, const int *perm, const T *dinv, T *out2, const T *d0, const T *d1,
-                   double *part) {
+                   double *part, const int *pairs = nullptr) {
 #define ST_CASE(IT)                                                                                                \
   case IT:                                                                                                         \
     hipLaunchKernelGGL((k_stream<TV, T, MODE, IT>), dim3(g), dim3(kStreamThreads), 0, c->stream, nblk, c->xcd,     \
-                       blocks, rp, ci, v, x, y, b, perm, dinv, out2, d0, d1, part);                                \
+                       blocks, rp, ci, v, x, y, b, perm, dinv, out2, d0, d1, part, pairs);                         \
     break;
   switch (c->items) { ST_CASE(4) ST_CASE(16) default: ST_CASE(8) }
 #undef ST_CASE
@@ -1044,17 +1096,30 @@ void launch_spmv(shesolve_ctx *c, int /*lpr*/, const TV *v, const T *x, T *y, co
                  double *part) {
   KTimer kt(c, sizeof(TV) == 8 ? 4 : 0);
   const int *rp = c->s_rp ? c->s_rp : c->d_rp, *ci = c->s_rp ? c->s_ci : c->d_ci;
+  if (c->spmv_lpr) {
+    // vector CSR: LPR lanes per row, grid-stride over rows; fused dots need the
+    // fixed grid (one partial per workgroup)
+    const int g = part ? kGrid : std::min(grid_for((int64_t)c->n * c->spmv_lpr, kBlock), 4 * kGrid);
+#define VS_CASE(LL)                                                                                              \
+  case LL:                                                                                                       \
+    hipLaunchKernelGGL((k_spmv<LL, TV, T, T>), dim3(g), dim3(kBlock), 0, c->stream, c->n, rp, ci, v, x, y, b, d0, d1, \
+                       part);                                                                                    \
+    break;
+    switch (c->spmv_lpr) { VS_CASE(4) VS_CASE(16) default: VS_CASE(8) }
+#undef VS_CASE
+    return;
+  }
   const int *blk = c->s_rp ? c->s_blk : c->d_ablk;
   const int nblk = c->s_rp ? c->s_nblk : c->nablk;
   launch_stream<TV, T, MODE_SPMV>(c, stream_grid(nblk, part != nullptr), nblk, blk, rp, ci, v, x, y, b,
-                                  (const int *)nullptr, (const T *)nullptr, (T *)nullptr, d0, d1, part);
+                                  (const int *)nullptr, (const T *)nullptr, (T *)nullptr, d0, d1, part,
+                                  c->s_rp ? c->s_pairs : nullptr);
 }
 
-// LDS window of the segment sweeps: 64 KiB (the per-workgroup limit without
-// an opt-in), as a power of two number of entries
+// LDS window of the segment sweeps, a power of two number of entries
 template <typename T>
 int sweep_window() {
-  return (int)(65536 / sizeof(T));
+  return (int)(32768 / sizeof(T));  // 32 KiB of LDS beside the level-pointer chunk
 }
 
 template <typename TV, typename T>
@@ -1076,17 +1141,26 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     KTimer kt(c, 1);
     const int W = sweep_window<T>();
     const size_t lds = sizeof(T) * (size_t)W;
-#define WG_CASE(LL)                                                                                               \
-  case LL:                                                                                                        \
-    hipLaunchKernelGGL((k_wgsweep<LL, 4, 2, TV, T, false>), dim3(c->nseg), dim3(kSweepThreads), lds, c->stream,   \
-                       c->d_segL, c->d_lvlL, c->d_rowsL, c->d_loLrp, c->d_Lci, Lv, in, out, (const TV *)nullptr,  \
-                       c->seg_mask, W);                                                                           \
-    hipLaunchKernelGGL((k_wgsweep<LL, 4, 2, TV, T, true>), dim3(c->nseg), dim3(kSweepThreads), lds, c->stream,    \
-                       c->d_segU, c->d_lvlU, c->d_rowsU, c->d_loUrp, c->d_Uci, Uv, (const T *)nullptr, out, dinv, \
-                       c->seg_mask, W);                                                                           \
+    const int n = c->n, g = grid_for(n, kBlock);
+    T *tq, *zq;
+    const TV *dq;
+    if constexpr (sizeof(T) == 8) { tq = (T *)c->d_tq64; zq = (T *)c->d_zq64; } else { tq = (T *)c->d_tq32; zq = (T *)c->d_zq32; }
+    if constexpr (sizeof(TV) == 8) dq = (const TV *)c->d_dq64; else dq = (const TV *)c->d_dq32;
+    // in (natural) -> forward schedule order; forward sweep; -> backward order;
+    // backward sweep; -> natural order
+    hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsL, in, tq);
+#define WG_CASE(LL)                                                                                                \
+  case LL:                                                                                                         \
+    hipLaunchKernelGGL((k_wgsweep<LL, 4, 2, TV, T, false>), dim3(c->nseg), dim3(kSweepThreads), lds, c->stream,    \
+                       c->d_segL, c->d_lvlL, c->d_loLrp, c->d_Lci, Lv, (const T *)tq, zq, (const TV *)nullptr,     \
+                       c->seg_mask, W);                                                                            \
+    hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_mapLU, (const T *)zq, tq);  \
+    hipLaunchKernelGGL((k_wgsweep<LL, 4, 2, TV, T, true>), dim3(c->nseg), dim3(kSweepThreads), lds, c->stream,     \
+                       c->d_segU, c->d_lvlU, c->d_loUrp, c->d_Uci, Uv, (const T *)tq, zq, dq, c->seg_mask, W);     \
     break;
     switch (L) { WG_CASE(2) WG_CASE(4) default: WG_CASE(8) }
 #undef WG_CASE
+    hipLaunchKernelGGL((k_qscatter<T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsU, (const T *)zq, out);
     return;
   }
   T *z = out;
@@ -1157,7 +1231,7 @@ void free_matrix(shesolve_ctx *c) {
   dfree(c->d_perm); dfree(c->z64); dfree(c->z32);
   dfree(c->d_sv64); dfree(c->d_sv32); dfree(c->d_sblk); dfree(c->pb64); dfree(c->px64);
   c->solve_perm = false;
-  c->s_rp = c->s_ci = c->s_blk = nullptr;
+  c->s_rp = c->s_ci = c->s_blk = c->s_pairs = nullptr;
   c->s_nblk = 0;
   dfree(c->d_ablk); c->nablk = 0;
   dfree(c->d_Lrp); dfree(c->d_Lci); dfree(c->d_Urp); dfree(c->d_Uci); dfree(c->d_Lsrc);
@@ -1168,7 +1242,8 @@ void free_matrix(shesolve_ctx *c) {
   c->lvL.ptr.clear(); c->lvU.ptr.clear();
   dfree(c->d_segL); dfree(c->d_lvlL); dfree(c->d_rowsL);
   dfree(c->d_segU); dfree(c->d_lvlU); dfree(c->d_rowsU);
-  dfree(c->d_loLrp); dfree(c->d_loUrp);
+  dfree(c->d_loLrp); dfree(c->d_loUrp); dfree(c->d_mapLU);
+  dfree(c->d_dq64); dfree(c->d_tq64); dfree(c->d_zq64); dfree(c->d_dq32); dfree(c->d_tq32); dfree(c->d_zq32);
   c->nseg = 0;
 }
 
@@ -1588,6 +1663,19 @@ int analyse_and_factor(shesolve_ctx *c) {
     };
     reorder(Lrp, Lci, c->h_segrowsL, loLrp);
     reorder(Urp, Uci, c->h_segrowsU, loUrp);
+    {
+      std::vector<int> posL(n), mapLU(n);
+      for (int q = 0; q < n; ++q) posL[c->h_segrowsL[q]] = q;
+      for (int q = 0; q < n; ++q) mapLU[q] = posL[c->h_segrowsU[q]];
+      HIPCHK(hipMalloc(&c->d_mapLU, sizeof(int) * n));
+      HIPCHK(hipMemcpy(c->d_mapLU, mapLU.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMalloc(&c->d_dq64, sizeof(double) * n));
+    HIPCHK(hipMalloc(&c->d_tq64, sizeof(double) * n));
+    HIPCHK(hipMalloc(&c->d_zq64, sizeof(double) * n));
+    HIPCHK(hipMalloc(&c->d_dq32, sizeof(float) * n));
+    HIPCHK(hipMalloc(&c->d_tq32, sizeof(float) * n));
+    HIPCHK(hipMalloc(&c->d_zq32, sizeof(float) * n));
     std::vector<int>().swap(c->h_segrowsL);
     std::vector<int>().swap(c->h_segrowsU);
     HIPCHK(hipMalloc(&c->d_loLrp, sizeof(int) * (n + 1)));
@@ -1676,6 +1764,14 @@ int refactor(shesolve_ctx *c) {
                        c->d_U64, c->d_U32);
   hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->n, kBlock * 4)), dim3(kBlock), 0, c->stream,
                      (int64_t)c->n, c->d_dinv64, c->d_dinv32);
+  if (c->nseg > 0) {
+    // 1/diagonal in the backward schedule's order
+    const int g = grid_for(c->n, kBlock);
+    hipLaunchKernelGGL((k_qgather<double, double>), dim3(g), dim3(kBlock), 0, c->stream, c->n, c->d_rowsU,
+                       c->d_dinv64, c->d_dq64);
+    hipLaunchKernelGGL((k_qgather<double, float>), dim3(g), dim3(kBlock), 0, c->stream, c->n, c->d_rowsU,
+                       c->d_dinv64, c->d_dq32);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
@@ -1711,14 +1807,38 @@ int setup_solve_space(shesolve_ctx *c) {
   HIPCHK(hipMalloc(&c->d_sv32, sizeof(float) * std::max<int64_t>(c->fnnz, 1)));
   HIPCHK(hipMalloc(&c->pb64, sizeof(double) * std::max(c->n, 1)));
   HIPCHK(hipMalloc(&c->px64, sizeof(double) * std::max(c->n, 1)));
-  std::vector<int> blk;
-  make_blocks(c->h_frp.data(), 0, c->n, c->items * kStreamThreads, blk);
-  HIPCHK(hipMalloc(&c->d_sblk, sizeof(int) * blk.size()));
-  HIPCHK(hipMemcpy(c->d_sblk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice));
+  // Row blocks of the permuted matrix, inside each colour, dealt in the order
+  // of their first row's natural position: the SpMV then walks all colours
+  // through the natural index space together, so the gathered x entries (the
+  // rows' neighbours, near them in natural order) stay in L2 instead of being
+  // fetched again for every colour.
+  std::vector<std::pair<int, int>> pr;
+  {
+    int start = 0;
+    for (int q = 1; q <= c->n; ++q) {
+      if (q < c->n && c->h_colour[c->h_perm[q]] == c->h_colour[c->h_perm[start]]) continue;
+      std::vector<int> blk;
+      make_blocks(c->h_frp.data(), start, q, c->items * kStreamThreads, blk);
+      for (size_t k = 0; k + 1 < blk.size(); ++k) pr.emplace_back(blk[k], blk[k + 1]);
+      start = q;
+    }
+  }
+  std::stable_sort(pr.begin(), pr.end(), [&](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+    return c->h_perm[a.first] < c->h_perm[b.first];
+  });
+  std::vector<int> flat(2 * pr.size());
+  for (size_t k = 0; k < pr.size(); ++k) {
+    flat[2 * k] = pr[k].first;
+    flat[2 * k + 1] = pr[k].second;
+  }
+  HIPCHK(hipMalloc(&c->d_sblk, sizeof(int) * std::max<size_t>(flat.size(), 2)));
+  HIPCHK(hipMemcpy(c->d_sblk, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
   c->s_rp = c->d_frp;
   c->s_ci = c->d_fci;
   c->s_blk = c->d_sblk;
-  c->s_nblk = (int)blk.size() - 1;
+  c->s_pairs = c->d_sblk;
+  c->spmv_lpr = 0;  // the interleaved block order needs the CSR-stream kernel
+  c->s_nblk = (int)pr.size();
   return 0;
 }
 
@@ -1895,6 +2015,16 @@ int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_r
   }
   if (int e = ensure_vectors(c)) return e;
   const int n = c->n;
+  if (warm) {
+    // an initial guess worse than zero (||b - A x0|| > ||b||) is dropped
+    double rn = 0, bn = 0;
+    if (int e = residual64(c, b, x, c->v64[8], &rn)) return e;
+    hipLaunchKernelGGL((k_copy_dot<double>), dim3(kGrid), dim3(kBlock), 0, c->stream, n, b, c->v64[0], c->d_part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, c->d_part, c->d_sc, c->d_norm);
+    HIPCHK(hipMemcpyAsync(&bn, c->d_norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!(rn <= bn)) warm = 0;
+  }
   if (!warm) hipLaunchKernelGGL((k_fill<double>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)n, x, 0.0);
   if (c->prm.precision == SHESOLVE_FP64) {
     int st = 1;
@@ -1957,7 +2087,7 @@ int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_r
         hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)n, dd, 0.0f);
         int st = 1;
         double ires = 0;
-        const int cap = first_it < 0 ? inner_cap : std::max(40, 3 * first_it);
+        const int cap = first_it < 0 ? inner_cap : std::max(100, 4 * first_it);
         int it = bicgstab_dev<float, float>(c, sv32(c), c->d_f32, r32, dd, c->prm.inner_tol,
                                             std::min(cap, c->prm.max_iters - r.iterations), &st, &ires, false);
         if (it < 0) return it;
@@ -2065,6 +2195,30 @@ int tune_stream(shesolve_ctx *c) {
     c->d_ablk = nullptr;
     HIPCHK(hipFree(d_ab));
   }
+  // vector-CSR candidates (lanes per row 4, 8, 16)
+  int best_lpr = 0;
+  if (c->prm.autotune) {
+    c->items = best_items;
+    c->xcd = best_xcd;
+    const int cand_lpr[3] = {4, 8, 16};
+    for (int li = 0; li < 3; ++li) {
+      c->spmv_lpr = cand_lpr[li];
+      launch_spmv<float, float, float>(c, 0, c->d_v32, c->v32[0], c->v32[1], nullptr, nullptr, nullptr, nullptr);
+      HIPCHK(hipEventRecord(c->ev0, c->stream));
+      for (int r = 0; r < 3; ++r)
+        launch_spmv<float, float, float>(c, 0, c->d_v32, c->v32[0], c->v32[1], nullptr, nullptr, nullptr, nullptr);
+      HIPCHK(hipEventRecord(c->ev1, c->stream));
+      HIPCHK(hipEventSynchronize(c->ev1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      c->tune_us[6 + li] = ms * 1e3 / 3;
+      if (ms < best) {
+        best = ms;
+        best_lpr = cand_lpr[li];
+      }
+    }
+  }
+  c->spmv_lpr = best_lpr;
   c->items = best_items;
   c->xcd = best_xcd;
   c->nablk = (int)best_blocks.size() - 1;
@@ -2280,8 +2434,18 @@ int batch_solve_internal(shesolve_batch_ctx *b, int warm) {
   std::vector<int> all(S, 1), act(S, 0), fb(S, 0);
   std::vector<double> bn, rn, prev(S, -1.0);
   if (int e = upload_mask(b, all)) return e;
-  if (!warm) bref(b, 3, b->x64, nullptr, nullptr);
   if (int e = bnorms<double>(b, b->b64, bn)) return e;
+  if (warm) {
+    // systems whose initial guess is worse than zero (||b - A x0|| > ||b||) start from zero
+    if (int e = bresidual(b, rn)) return e;
+    std::vector<int> drop(S, 0);
+    for (int s = 0; s < S; ++s) drop[s] = !(rn[s] <= bn[s]);
+    if (int e = upload_mask(b, drop)) return e;
+    bref(b, 3, b->x64, nullptr, nullptr);
+    if (int e = upload_mask(b, all)) return e;
+  } else {
+    bref(b, 3, b->x64, nullptr, nullptr);
+  }
   std::vector<int> zero_rhs(S, 0);
   bool anyz = false;
   for (int s = 0; s < S; ++s)
@@ -2360,10 +2524,10 @@ int batch_solve_internal(shesolve_batch_ctx *b, int warm) {
       bref(b, 4, nullptr, nullptr, b->dd);
       for (int s = 0; s < S; ++s)
         if (act[s]) { prev[s] = rn[s]; b->rep[s].ir_steps++; }
-      // later corrections get at most 3x the iterations the system's first
-      // FP32 solve needed (at least 40): FP32 solves of a correction that
+      // later corrections get at most 4x the iterations the system's first
+      // FP32 solve needed (at least 100): FP32 solves of a correction that
       // stop making progress are handed to FP64 early
-      for (int s = 0; s < S; ++s) caps[s] = first_it[s] < 0 ? inner_cap : std::max(40, 3 * first_it[s]);
+      for (int s = 0; s < S; ++s) caps[s] = first_it[s] < 0 ? inner_cap : std::max(100, 4 * first_it[s]);
       if (int e = bbicg_run<float, float>(b, r32, b->dd, prm.inner_tol, inner_cap, 1e4, act, &caps)) return e;
       for (int s = 0; s < S; ++s)
         if (act[s] && first_it[s] < 0) first_it[s] = b->h_bs[s].iters;
@@ -2836,12 +3000,12 @@ int shesolve_get_factors(shesolve_handle h, int64_t *row_ptr, int32_t *col, doub
   return 0;
 }
 
-int shesolve_stream_config(shesolve_handle h, int32_t *items, int32_t *xcd, double *tune_us6) {
+int shesolve_stream_config(shesolve_handle h, int32_t *items, int32_t *xcd, double *tune_us9) {
   if (!h) return 1;
-  if (items) *items = h->items;
+  if (items) *items = h->spmv_lpr ? -h->spmv_lpr : h->items;
   if (xcd) *xcd = h->xcd;
-  if (tune_us6)
-    for (int k = 0; k < 6; ++k) tune_us6[k] = h->tune_us[k];
+  if (tune_us9)
+    for (int k = 0; k < 9; ++k) tune_us9[k] = h->tune_us[k];
   return 0;
 }
 

@@ -218,7 +218,7 @@ class Solver:
 
     def stream_config(self):
         it, xc = C.c_int32(), C.c_int32()
-        t = np.zeros(6)
+        t = np.zeros(9)
         _check(lib().shesolve_stream_config(self._h, C.byref(it), C.byref(xc), t), "shesolve_stream_config")
         return {"items": it.value, "xcd": xc.value, "tune_us": [round(v, 1) for v in t]}
 

@@ -124,12 +124,16 @@ def footprint_estimate(dev: D.Device, cfg: D.SheConfig) -> float:
     return rows * (per_row * 52.0 + 220.0)
 
 
-def _preconditioner(dev, cfg):
+def _solver_params(dev, cfg):
+    """2D meshes: multicolour ILU0, FP32 + FP64 refinement (bandwidth-bound
+    sweeps over a few colours).  1D chains: natural-order ILU(0), ILU(1) where
+    ILU(0) diverges (L >= 5), in FP64 -- their sweeps are bound by the
+    dependency depth of the chain, not by bytes, and FP32 corrections of these
+    systems stall after the first refinement step."""
     import shesolve
     if dev.dim > 1:
-        return dict(precond=shesolve.PRECOND_MC_ILU0)
-    # natural order on 1D chains; ILU(1) where ILU(0) diverges (L >= 5)
-    return dict(precond=shesolve.PRECOND_ILUK, fill_level=1 if cfg.L_max >= 5 else 0)
+        return dict(precond=shesolve.PRECOND_MC_ILU0, precision=shesolve.FP32_IR)
+    return dict(precond=shesolve.PRECOND_ILUK, fill_level=1 if cfg.L_max >= 5 else 0, precision=shesolve.FP64)
 
 
 def _transfer(x_old, map_old, shift, map_new, n_new):
@@ -163,9 +167,8 @@ def run_request(req: Request, tol=1e-10, max_iters=3000, device=0, clock=None, a
     import shesolve
     start = clock() if clock else time.perf_counter()
     F = shesolve.Fill(device)
-    pc = _preconditioner(req.dev, req.cfg)
-    sv = shesolve.Solver(device, precision=shesolve.FP32_IR, tol=tol, max_iters=max_iters,
-                         autotune=1 if autotune else 0, **pc)
+    sv = shesolve.Solver(device, tol=tol, max_iters=max_iters, autotune=1 if autotune else 0,
+                         **_solver_params(req.dev, req.cfg))
     rng = np.random.default_rng(req.seed)
     dev = dataclasses.replace(req.dev, potential=np.array(req.dev.potential, np.float64))
     reps, ns = [], []
