@@ -77,9 +77,7 @@ def cpu_baseline(rp, ci, val, b, fp64_iters, budget_s):
     }
 
 
-def bench_batch(steps, count=96, seed=1234, fp64=True):
-    if count <= 0:
-        return {}
+def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0):
     """The batched workload: `count` 1D diodes drawn from a seed (length
     U[100,1000] nm, n+ 20 % each side, L in {1,3,5}, dE U[2,10] meV to 1.0 eV,
     bias U[0,1.5] V), solved together by the batched FP32+IR BiCGStab (one
@@ -87,6 +85,8 @@ def bench_batch(steps, count=96, seed=1234, fp64=True):
     iterations).  Preconditioner: ILU(0) per system, ILU(1) for L = 5 (ILU(0)
     diverges there).  A step = GPU refill of every system + refactorisation +
     batched solve to 1e-10."""
+    if count <= 0:
+        return {}
     import ctypes
     import shesolve
     from shesolve import devices as D
@@ -128,14 +128,13 @@ def bench_batch(steps, count=96, seed=1234, fp64=True):
             bad = [r for r in reps if r["status"] != 0 or not r["rel_residual"] <= 1e-10]
             log(f"[bench] batch step {step}: {lat[-1] * 1e3:.0f} ms, iterations max "
                 f"{max(r['iterations'] for r in reps)}, {len(bad)} not converged")
-            if bad:
-                raise SystemExit(f"batched solve did not reach 1e-10: {bad[:3]}")
         return lat, reps
 
     bs, t_setup = make(shesolve.FP32_IR)
     info = bs.info()
     log(f"[bench] batch of {count}: {info}, build {t_build:.1f} s, setup {t_setup:.1f} s")
-    run(bs, 1)
+    if warmup:
+        run(bs, warmup)
     lat, reps = run(bs, steps)
     out = {
         "workload": f"{count} 1D diodes (seed {seed}): L in {{1,3,5}}, 100-1000 nm, dE 2-10 meV to 1.0 eV",
@@ -143,6 +142,7 @@ def bench_batch(steps, count=96, seed=1234, fp64=True):
         "deepest_level_schedule": info["seg_levels_max"],
         "solves_per_s": count * len(lat) / sum(lat), "ms_per_batch": 1e3 * sum(lat) / len(lat),
         "p50_ms": 1e3 * percentile(lat, 50), "p99_ms": 1e3 * percentile(lat, 99),
+        "not_converged": sum(1 for r in reps if r["status"] != 0 or not r["rel_residual"] <= 1e-10),
         "iterations_max": max(r["iterations"] for r in reps),
         "iterations_mean": float(np.mean([r["iterations"] for r in reps])),
         "max_rel_residual": max(r["rel_residual"] for r in reps),
@@ -197,7 +197,9 @@ def main():
     ap.add_argument("--profile-kernels", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=1)
-    ap.add_argument("--batch-count", type=int, default=96)
+    ap.add_argument("--batch-count", type=int, default=24,
+                    help="systems of the batched workload (the full family is 96; the first N of the seeded draw)")
+    ap.add_argument("--batch-fp64", action="store_true", help="also time the batch in plain FP64")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=40)
     args = ap.parse_args()
@@ -337,7 +339,8 @@ def main():
     F.close()
     hip.hipFree(d_x)
     if not args.no_batch:
-        result["batch"] = bench_batch(args.batch_steps, count=args.batch_count)
+        os.environ.setdefault("SHESOLVE_VERBOSE", "1")  # refinement-step progress lines on stderr
+        result["batch"] = bench_batch(args.batch_steps, count=args.batch_count, fp64=args.batch_fp64)
         log(f"[bench] batch: {result['batch']}")
     if not args.no_serve:
         result["serving"] = bench_serving(args.serve_requests)

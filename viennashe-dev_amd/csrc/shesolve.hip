@@ -561,6 +561,7 @@ __global__ __launch_bounds__(64) void k_ilu0_level(int cnt, const int *__restric
 // device, so that a sequence of iterations needs no host round trip.
 constexpr int kBatchParts = 32;  // partial sums per system and dot product
 constexpr int kStallWindow = 100; // FP32 inner solves: iterations without a 1 % gain before giving up
+constexpr int kBatchStallWindow = 40;  // the same in batches, where a stalled system holds up the batch
 
 struct BSys {
   double rho, alpha, omega, beta;
@@ -775,69 +776,57 @@ __global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict
   const int l0 = seg_lvl[s], l1 = seg_lvl[s + 1];
   if (l0 >= l1) return;
   const int g = threadIdx.x / LPR, lane = threadIdx.x & (LPR - 1);
-  // prefetched first chunk of a level: R rows per lane group, J entries per lane
-  int pkb[R], pke[R], pc[R][J];
-  T prhs[R], pd[R];
-  TV pv[R][J];
-  auto fetch = [&](int q0, int q1) {
+  // Software pipeline over levels: the row data (entry range, right-hand side,
+  // 1/diagonal) of level l + 2 and the entries of level l + 1 are loaded while
+  // level l is computed; each lane group holds R rows, each lane J entries of a row.
+  struct Meta { int kb[R], ke[R]; T rh[R], d[R]; };
+  struct Ent { int c[R][J]; TV v[R][J]; };
+  auto meta = [&](Meta &M, int q0, int q1) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int q = q0 + g + r * G;
       const bool ok = q < q1;
-      pkb[r] = ok ? lrp[q] : 0;
-      pke[r] = ok ? lrp[q + 1] : 0;
-      prhs[r] = ok ? rhs[q] : (T)0;
-      pd[r] = (UPPER && ok) ? (T)dq[q] : (T)1;
+      M.kb[r] = ok ? lrp[q] : 0;
+      M.ke[r] = ok ? lrp[q + 1] : 0;
+      M.rh[r] = ok ? rhs[q] : (T)0;
+      M.d[r] = (UPPER && ok) ? (T)dq[q] : (T)1;
+    }
+  };
+  auto ents = [&](Ent &E, const Meta &M) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
 #pragma unroll
       for (int j = 0; j < J; ++j) {
-        const int k = pkb[r] + lane + j * LPR;
-        const bool kk = k < pke[r];
-        pc[r][j] = kk ? cq[k] : 0;
-        pv[r][j] = kk ? v[k] : (TV)0;
+        const int k = M.kb[r] + lane + j * LPR;
+        const bool kk = k < M.ke[r];
+        E.c[r][j] = kk ? cq[k] : 0;
+        E.v[r][j] = kk ? v[k] : (TV)0;
       }
-    }
   };
   // level pointers of the segment, staged in LDS kLvlChunk levels at a time
   int lc = l0;  // first level of the staged chunk
   for (int t = threadIdx.x; t <= kLvlChunk && lc + t <= l1; t += kSweepThreads) lp[t] = lvl_ptr[lc + t];
   __syncthreads();
-  int q0 = lp[0], q1 = lp[1];
-  int q_safe = q0;  // positions below q_safe are visible in zq to every wave
-  fetch(q0, q1);
+  auto qp = [&](int l) { return (l - lc <= kLvlChunk) ? lp[l - lc] : lvl_ptr[l]; };
+  int q_safe = lp[0];  // positions below q_safe are visible in zq to every wave
+  Meta cur, nxt, ahead;
+  Ent ecur, enxt;
+  meta(cur, qp(l0), qp(l0 + 1));
+  ents(ecur, cur);
+  if (l0 + 1 < l1) meta(nxt, qp(l0 + 1), qp(l0 + 2));
   for (int l = l0; l < l1; ++l) {
-    int kb[R], ke[R], c[R][J];
-    T rh[R], dd[R];
-    TV vv[R][J];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      kb[r] = pkb[r]; ke[r] = pke[r]; rh[r] = prhs[r]; dd[r] = pd[r];
-#pragma unroll
-      for (int j = 0; j < J; ++j) { c[r][j] = pc[r][j]; vv[r][j] = pv[r][j]; }
-    }
+    const int q0 = qp(l), q1 = qp(l + 1);
     const bool more = l + 1 < l1;
-    const bool in_chunk = l + 2 - lc <= kLvlChunk;
-    const int nq1 = !more ? q1 : (in_chunk ? lp[l + 2 - lc] : lvl_ptr[l + 2]);
-    if (more) fetch(q1, nq1);
+    if (l + 2 < l1) meta(ahead, qp(l + 2), qp(l + 3));
+    if (more) ents(enxt, nxt);
     const int lo = q1 - W;  // oldest position the window may serve during this level
     for (int base = q0 + g; base < q1; base += G * R) {
+      Meta m = cur;
+      Ent e = ecur;
       if (base != q0 + g) {
         // later chunks of a large level: load them now
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const int q = base + r * G;
-          const bool ok = q < q1;
-          kb[r] = ok ? lrp[q] : 0;
-          ke[r] = ok ? lrp[q + 1] : 0;
-          rh[r] = ok ? rhs[q] : (T)0;
-          dd[r] = (UPPER && ok) ? (T)dq[q] : (T)1;
-#pragma unroll
-          for (int j = 0; j < J; ++j) {
-            const int k = kb[r] + lane + j * LPR;
-            const bool kk = k < ke[r];
-            c[r][j] = kk ? cq[k] : 0;
-            vv[r][j] = kk ? v[k] : (TV)0;
-          }
-        }
+        meta(m, base - g, q1);
+        ents(e, m);
       }
       T acc[R];
 #pragma unroll
@@ -845,18 +834,16 @@ __global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict
         acc[r] = 0;
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-          const int k = kb[r] + lane + j * LPR;
-          if (k < ke[r]) {
-            const int qc = c[r][j];
-            const T xv = qc >= lo ? win[qc & Wm] : zq[qc];
-            acc[r] += (T)vv[r][j] * xv;
+          const int k = m.kb[r] + lane + j * LPR;
+          if (k < m.ke[r]) {
+            const int qc = e.c[r][j];
+            acc[r] += (T)e.v[r][j] * (qc >= lo ? win[qc & Wm] : zq[qc]);
           }
         }
         // entries beyond the prefetched J per lane
-        for (int k = kb[r] + lane + J * LPR; k < ke[r]; k += LPR) {
+        for (int k = m.kb[r] + lane + J * LPR; k < m.ke[r]; k += LPR) {
           const int qc = cq[k];
-          const T xv = qc >= lo ? win[qc & Wm] : zq[qc];
-          acc[r] += (T)v[k] * xv;
+          acc[r] += (T)v[k] * (qc >= lo ? win[qc & Wm] : zq[qc]);
         }
       }
 #pragma unroll
@@ -866,14 +853,15 @@ __global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict
         for (int off = LPR / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, LPR);
         const int q = base + r * G;
         if (lane == 0 && q < q1) {
-          const T out = UPPER ? dd[r] * (rh[r] - a) : rh[r] - a;
+          const T out = UPPER ? m.d[r] * (m.rh[r] - a) : m.rh[r] - a;
           zq[q] = out;
           if (q >= lo) win[q & Wm] = out;
         }
       }
     }
     if (!more) break;
-    if (!in_chunk) {
+    const int nq1 = qp(l + 2);
+    if (l + 3 - lc > kLvlChunk) {
       // stage the next chunk of level pointers (levels l + 1 ..)
       __syncthreads();
       lc = l + 1;
@@ -887,8 +875,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict
     } else {
       lds_barrier();
     }
-    q0 = q1;
-    q1 = nq1;
+    cur = nxt;
+    ecur = enxt;
+    nxt = ahead;
   }
 }
 
@@ -981,6 +970,7 @@ struct shesolve_ctx {
   int *d_segL = nullptr, *d_lvlL = nullptr, *d_rowsL = nullptr;
   int *d_segU = nullptr, *d_lvlU = nullptr, *d_rowsU = nullptr;
   int *d_loLrp = nullptr, *d_loUrp = nullptr;  // level-ordered row pointers of the split factor
+  int seg_lpr = 4;                             // lanes per row of the segment sweeps
   int *d_mapLU = nullptr;                      // U position -> L position of the same row
   double *d_dq64 = nullptr, *d_tq64 = nullptr, *d_zq64 = nullptr;  // level-ordered 1/diag, rhs, output
   float *d_dq32 = nullptr, *d_tq32 = nullptr, *d_zq32 = nullptr;
@@ -1116,6 +1106,11 @@ void launch_spmv(shesolve_ctx *c, int /*lpr*/, const TV *v, const T *x, T *y, co
                                   c->s_rp ? c->s_pairs : nullptr);
 }
 
+// rows in flight per lane group of the forward segment sweep (the pipeline's
+// registers must fit 128 VGPRs at 1024 threads); the backward sweep uses 2
+template <typename T>
+constexpr int kSweepRowsL = sizeof(T) == 8 ? 2 : 4;
+
 // LDS window of the segment sweeps, a power of two number of entries
 template <typename T>
 int sweep_window() {
@@ -1151,14 +1146,15 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsL, in, tq);
 #define WG_CASE(LL)                                                                                                \
   case LL:                                                                                                         \
-    hipLaunchKernelGGL((k_wgsweep<LL, 4, 2, TV, T, false>), dim3(c->nseg), dim3(kSweepThreads), lds, c->stream,    \
+    hipLaunchKernelGGL((k_wgsweep<LL, kSweepRowsL<T>, 2, TV, T, false>), dim3(c->nseg), dim3(kSweepThreads), lds,     \
+                       c->stream,                                                                                  \
                        c->d_segL, c->d_lvlL, c->d_loLrp, c->d_Lci, Lv, (const T *)tq, zq, (const TV *)nullptr,     \
                        c->seg_mask, W);                                                                            \
     hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_mapLU, (const T *)zq, tq);  \
-    hipLaunchKernelGGL((k_wgsweep<LL, 4, 2, TV, T, true>), dim3(c->nseg), dim3(kSweepThreads), lds, c->stream,     \
+    hipLaunchKernelGGL((k_wgsweep<LL, 2, 2, TV, T, true>), dim3(c->nseg), dim3(kSweepThreads), lds, c->stream,     \
                        c->d_segU, c->d_lvlU, c->d_loUrp, c->d_Uci, Uv, (const T *)tq, zq, dq, c->seg_mask, W);     \
     break;
-    switch (L) { WG_CASE(2) WG_CASE(4) default: WG_CASE(8) }
+    switch (c->seg_lpr) { WG_CASE(2) WG_CASE(4) default: WG_CASE(8) }
 #undef WG_CASE
     hipLaunchKernelGGL((k_qscatter<T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsU, (const T *)zq, out);
     return;
@@ -1663,6 +1659,12 @@ int analyse_and_factor(shesolve_ctx *c) {
     };
     reorder(Lrp, Lci, c->h_segrowsL, loLrp);
     reorder(Urp, Uci, c->h_segrowsU, loUrp);
+    {
+      // lanes per row: enough that 2 entries per lane cover the longest rows
+      int mx = 0;
+      for (int q = 0; q < n; ++q) mx = std::max(mx, std::max(loLrp[q + 1] - loLrp[q], loUrp[q + 1] - loUrp[q]));
+      c->seg_lpr = mx <= 4 ? 2 : (mx <= 8 ? 4 : 8);
+    }
     {
       std::vector<int> posL(n), mapLU(n);
       for (int q = 0; q < n; ++q) posL[c->h_segrowsL[q]] = q;
@@ -2379,7 +2381,7 @@ int bbicg_run(shesolve_batch_ctx *b, const T *rhs, T *x, double tol, int max_ite
   if (caps) HIPCHK(hipMemcpyAsync(b->d_caps, caps->data(), sizeof(int) * b->S, hipMemcpyHostToDevice, c->stream));
   // FP32 inner solves (finite divergence limit) also stop when they stall
   hipLaunchKernelGGL(k_binit, dim3((b->S + 63) / 64), dim3(64), 0, c->stream, b->S, b->d_bs, b->d_mask, tol, max_iters,
-                     div_limit, div_limit < 1e300 ? kStallWindow : 0, caps ? (const int *)b->d_caps : nullptr);
+                     div_limit, div_limit < 1e300 ? kBatchStallWindow : 0, caps ? (const int *)b->d_caps : nullptr);
   launch_spmv<TV, T, T>(c, c->lpr, Av, x, r, rhs, nullptr, nullptr, nullptr);
   bupd<T>(b, 3, r0, p, v, r, nullptr, nullptr, nullptr);
   bdot<T>(b, nullptr, r, r, rhs, rhs);
@@ -2524,10 +2526,11 @@ int batch_solve_internal(shesolve_batch_ctx *b, int warm) {
       bref(b, 4, nullptr, nullptr, b->dd);
       for (int s = 0; s < S; ++s)
         if (act[s]) { prev[s] = rn[s]; b->rep[s].ir_steps++; }
-      // later corrections get at most 4x the iterations the system's first
-      // FP32 solve needed (at least 100): FP32 solves of a correction that
-      // stop making progress are handed to FP64 early
-      for (int s = 0; s < S; ++s) caps[s] = first_it[s] < 0 ? inner_cap : std::max(100, 4 * first_it[s]);
+      // later corrections get at most 2x the iterations the system's first
+      // FP32 solve needed (at least 40): in a batch every iteration costs the
+      // whole batch, so FP32 solves of a correction that stop making progress
+      // are handed to FP64 early
+      for (int s = 0; s < S; ++s) caps[s] = first_it[s] < 0 ? inner_cap : std::max(40, 2 * first_it[s]);
       if (int e = bbicg_run<float, float>(b, r32, b->dd, prm.inner_tol, inner_cap, 1e4, act, &caps)) return e;
       for (int s = 0; s < S; ++s)
         if (act[s] && first_it[s] < 0) first_it[s] = b->h_bs[s].iters;
