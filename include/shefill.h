@@ -109,6 +109,20 @@ int shefill_recover_device(shefill_handle h, const double *d_x, double *d_x_full
 int shefill_postproc(shefill_handle h, const double *x_full, double *density, double *facet_current);
 int shefill_postproc_device(shefill_handle h, const double *d_x_full, double *d_density,
                             double *d_facet_current);
+/* Gummel update of the carrier density per cell (simulator.hpp:524-600,
+ * update_quantity with damping alpha in (0, 1]): density <- density (1 - alpha)
+ * + new alpha, or with logarithmic != 0, density^(1-alpha) *
+ * max(new, 1e-10 density)^alpha.  The reference's force_no_damping is a plain
+ * copy of new.  *update_norm (may be NULL) receives the relative update norm
+ * of simulator.hpp:286-320 between new and the density before the update:
+ * sqrt(sum ((new - old) / old)^2), with new in place of the quotient where
+ * old = 0.  (The reference's Gummel loop, simulator.hpp:1081-1086, evaluates
+ * that norm before x is written back, i.e. on the previous distribution.)
+ * _device: device buffers, n_cells values each. */
+int shefill_density_update(shefill_handle h, const double *new_density, double *density, double alpha,
+                           int logarithmic, double *update_norm);
+int shefill_density_update_device(shefill_handle h, const double *d_new, double *d_density, double alpha,
+                                  int logarithmic, double *update_norm);
 
 #ifdef __cplusplus
 }

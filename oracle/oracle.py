@@ -287,3 +287,24 @@ def bicgstab(A: CSR, b, x0=None, precond: Precond | None = None, tol=1e-10, max_
                                precond._h if precond is not None else None, tol, max_iters, max_seconds,
                                C.byref(res))
     return x, int(it), float(res.value)
+
+
+def density_update(new, cur, alpha, logarithmic=False):
+    """Damped Gummel update of a carrier density and its relative update norm.
+
+    update_quantity (simulator.hpp:524-600): linear cur (1 - a) + new a, or
+    logarithmic cur^(1-a) * max(new, 1e-10 cur)^a.  get_relative_update_norm
+    (simulator.hpp:286-320): sqrt(sum zw^2), zw = (new - old) / old, or new
+    where old is 0; the norm compares new with the density before the update.
+    Returns (updated density, norm)."""
+    new = np.asarray(new, np.float64)
+    cur = np.asarray(cur, np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        zw = np.where(np.abs(cur) > 0, (new - cur) / np.where(cur != 0, cur, 1.0), new)
+    norm = float(np.sqrt(np.sum(zw * zw)))
+    if logarithmic:
+        nv = np.where(new < cur * 1e-10, cur * 1e-10, new)
+        out = np.power(cur, 1.0 - alpha) * np.power(nv, alpha)
+    else:
+        out = cur * (1.0 - alpha) + new * alpha
+    return out, norm

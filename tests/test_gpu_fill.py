@@ -76,3 +76,38 @@ def test_recovery_and_postproc_match_oracle(name):
     if len(cur_ref):
         assert np.max(np.abs(cur - cur_ref)) <= 1e-11 * np.max(np.abs(cur_ref))
     F.close()
+
+
+@pytest.mark.parametrize("logarithmic", [False, True])
+@pytest.mark.parametrize("alpha", [1.0, 0.3])
+def test_density_update_matches_oracle(logarithmic, alpha):
+    """Damped Gummel update of the density and the relative update norm on the
+    GPU against the oracle: the oracle's converged density of a 2D device as
+    the new value, against a perturbed old density that has zeros (the norm
+    takes the new value there) and cells whose new value collapses below
+    1e-10 of the old one (the logarithmic form limits them).  Within 1e-14
+    relative per cell (pow and fused multiply-adds differ in the last bit),
+    1e-12 for the norm (summation order)."""
+    import shesolve
+    from oracle import oracle as O
+    s = oracle_system("nin2d_L3")
+    F = shesolve.Fill(0)
+    F.build(s.dev, s.cfg)
+    A, b, _, _ = s.reduced()
+    P = O.Precond(O.PC_ILUK, A, drop=0.0, fill=1)
+    x, _, res = O.bicgstab(A, b, precond=P, tol=1e-12, max_iters=3000)
+    xf, _ = s.finish(x)
+    new, _ = s.postproc(xf)
+    rng = np.random.default_rng(5)
+    old = new * np.exp(rng.uniform(-1.0, 1.0, new.shape))
+    old[::17] = 0.0
+    new = new.copy()
+    new[5::23] = old[5::23] * 1e-13
+    ref, ref_norm = O.density_update(new, old, alpha, logarithmic)
+    out, nrm = F.density_update(new, old, alpha, logarithmic)
+    assert out.shape == ref.shape
+    assert np.max(np.abs(out - ref) / np.maximum(np.abs(ref), 1e-300)) <= 1e-14
+    assert nrm == pytest.approx(ref_norm, rel=1e-12)
+    with pytest.raises(shesolve.SheSolveError):
+        F.density_update(new, old, 0.0)
+    F.close()

@@ -402,3 +402,31 @@ def test_current_conservation(name):
     # majority density in the contact cells is the doping (G/R contacts, tau = 1e-16 s)
     ct = np.nonzero(s.dev.contact)[0]
     assert np.allclose(n[ct], s.dev.doping_n[ct], rtol=2e-2)
+
+
+def test_density_update_closed_forms():
+    """The damped Gummel update: a = 1 takes the new density, a = 1/2 gives the
+    arithmetic (linear) or geometric (logarithmic) mean, the logarithmic form
+    limits a collapsing density to 1e-10 of the old one, and the norm is the
+    2-norm of the relative changes with the new value where the old is 0
+    (brute force over a small list)."""
+    from oracle import oracle as O
+    cur = np.array([1e10, 4.0, 2.5e16, 0.0, 3.0])
+    new = np.array([4e10, 9.0, 2.5e16, 7.0, -1.0])
+    out, _ = O.density_update(new, cur, 1.0)
+    assert np.array_equal(out, new)
+    out, _ = O.density_update(new, cur, 0.5)
+    assert np.allclose(out, (new + cur) / 2, rtol=1e-15)
+    pos = slice(0, 3)
+    out, _ = O.density_update(new[pos], cur[pos], 0.5, logarithmic=True)
+    assert np.allclose(out, np.sqrt(new[pos] * cur[pos]), rtol=1e-15)
+    out, _ = O.density_update(np.array([-5.0]), np.array([2.0]), 0.25, logarithmic=True)
+    assert np.isclose(out[0], 2.0 ** 0.75 * 2e-10 ** 0.25, rtol=1e-15)
+    out, _ = O.density_update(np.array([1e-20]), np.array([2.0]), 1.0, logarithmic=True)
+    assert out[0] == 2e-10
+    _, nrm = O.density_update(new, cur, 0.3)
+    brute = 0.0
+    for n_, c_ in zip(new, cur):
+        brute += ((n_ - c_) / c_) ** 2 if c_ != 0 else n_ ** 2
+    assert nrm == pytest.approx(brute ** 0.5, rel=1e-15)
+    assert nrm == pytest.approx(np.sqrt(3 ** 2 + 1.25 ** 2 + 0 + 49 + (4 / 3) ** 2), rel=1e-15)

@@ -344,6 +344,38 @@ __global__ void k_current(DevTables t, CurArgs a, const double *xf, double *cur)
   cur[f] = -pol * kQ * vn / Y00;
 }
 
+// Gummel update of the carrier density (simulator.hpp:524-600 update_quantity
+// with get_relative_update_norm, :286-313): the relative change of each cell
+// (new - old) / old (new where old is 0), squared, summed per block; then the
+// damped value: linear cur (1 - a) + new a, or logarithmic cur^(1-a) new^a
+// with new limited to >= 1e-10 cur.
+__global__ void k_density_update(int nc, const double *dnew, double *dcur, double alpha, int logarithmic,
+                                 double *part) {
+  __shared__ double red[kB / 64];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double z2 = 0.0;
+  if (c < nc) {
+    double nv = dnew[c];
+    const double cur = dcur[c];
+    const double zw = fabs(cur) > 0 ? (nv - cur) / cur : nv;
+    z2 = zw * zw;
+    if (logarithmic) {
+      if (nv < cur * 1e-10) nv = cur * 1e-10;
+      dcur[c] = pow(cur, 1.0 - alpha) * pow(nv, alpha);
+    } else {
+      dcur[c] = cur * (1.0 - alpha) + nv * alpha;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) z2 += __shfl_xor(z2, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = z2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kB / 64; ++w) t += red[w];
+    part[blockIdx.x] = t;
+  }
+}
+
 struct RowArgs {
   int n;
   const int *rp, *ci, *row_node;
@@ -1359,6 +1391,58 @@ int shefill_postproc(shefill_handle h, const double *x_full, double *density, do
     e = -10;
   hipFree(dxf);
   hipFree(dd);
+  hipFree(dc);
+  return e;
+}
+
+int shefill_density_update_device(shefill_handle h, const double *d_new, double *d_density, double alpha,
+                                  int logarithmic, double *update_norm) {
+  if (!h) { ferr("handle is NULL"); return 1; }
+  if (!h->built) { ferr("nothing built"); return -1; }
+  if (!d_new) { ferr("new density is NULL"); return 2; }
+  if (!d_density) { ferr("density is NULL"); return 3; }
+  if (!(alpha > 0.0)) { ferr("damping must be positive"); return 4; }
+  FHIP(hipSetDevice(h->device));
+  const int nb = grid(h->nc);
+  double *part = nullptr;
+  FHIP(hipMalloc(&part, sizeof(double) * nb));
+  hipLaunchKernelGGL(k_density_update, dim3(nb), dim3(kB), 0, h->stream, h->nc, d_new, d_density, alpha, logarithmic,
+                     part);
+  std::vector<double> hp(nb);
+  int e = 0;
+  if (hipMemcpyAsync(hp.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess) {
+    ferr("density update failed");
+    e = -10;
+  }
+  hipFree(part);
+  if (!e && update_norm) {
+    double t = 0.0;
+    for (double v : hp) t += v;
+    *update_norm = sqrt(t);
+  }
+  return e;
+}
+
+int shefill_density_update(shefill_handle h, const double *new_density, double *density, double alpha,
+                           int logarithmic, double *update_norm) {
+  if (!h) { ferr("handle is NULL"); return 1; }
+  if (!h->built) { ferr("nothing built"); return -1; }
+  if (!new_density) { ferr("new density is NULL"); return 2; }
+  if (!density) { ferr("density is NULL"); return 3; }
+  FHIP(hipSetDevice(h->device));
+  double *dn = nullptr, *dc = nullptr;
+  FHIP(hipMalloc(&dn, sizeof(double) * std::max(h->nc, 1)));
+  FHIP(hipMalloc(&dc, sizeof(double) * std::max(h->nc, 1)));
+  int e = 0;
+  if (hipMemcpy(dn, new_density, sizeof(double) * h->nc, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dc, density, sizeof(double) * h->nc, hipMemcpyHostToDevice) != hipSuccess) {
+    ferr("copy failed");
+    e = -10;
+  }
+  if (!e) e = shefill_density_update_device(h, dn, dc, alpha, logarithmic, update_norm);
+  if (!e && hipMemcpy(density, dc, sizeof(double) * h->nc, hipMemcpyDeviceToHost) != hipSuccess) e = -10;
+  hipFree(dn);
   hipFree(dc);
   return e;
 }

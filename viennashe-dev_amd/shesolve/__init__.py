@@ -374,7 +374,8 @@ class BatchSolver:
 FILL_EXPORTS = ("shefill_last_error", "shefill_create", "shefill_destroy", "shefill_build", "shefill_sizes",
                 "shefill_device_system", "shefill_get_system", "shefill_block_ranges", "shefill_mapping",
                 "shefill_last_fill_ms", "shefill_refill", "shefill_recover", "shefill_recover_device",
-                "shefill_postproc", "shefill_postproc_device", "shefill_pattern_hash")
+                "shefill_postproc", "shefill_postproc_device", "shefill_pattern_hash",
+                "shefill_density_update", "shefill_density_update_device")
 
 
 class FillDevice(C.Structure):
@@ -407,6 +408,9 @@ def _fill_lib():
     L.shefill_last_fill_ms.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.shefill_recover.argtypes = [C.c_void_p, _f64p, _f64p]
     L.shefill_pattern_hash.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+    L.shefill_density_update.argtypes = [C.c_void_p, _f64p, _f64p, C.c_double, C.c_int, C.POINTER(C.c_double)]
+    L.shefill_density_update_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
+                                                C.POINTER(C.c_double)]
     L.shefill_recover_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.shefill_postproc.argtypes = [C.c_void_p, _f64p, C.c_void_p, C.c_void_p]
     L.shefill_postproc_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -511,6 +515,16 @@ class Fill:
         xf = np.zeros(self.sizes["n"] + self.sizes["n_odd"])
         _fcheck(_fill_lib().shefill_recover(self._h, np.ascontiguousarray(x, np.float64), xf), "shefill_recover")
         return xf
+
+    def density_update(self, new_density, density, alpha=1.0, logarithmic=False):
+        """Damped Gummel update of the carrier density on the GPU; returns
+        (updated density, relative update norm)."""
+        cur = np.array(density, np.float64, copy=True)
+        nrm = C.c_double()
+        _fcheck(_fill_lib().shefill_density_update(self._h, np.ascontiguousarray(new_density, np.float64), cur,
+                                                   float(alpha), 1 if logarithmic else 0, C.byref(nrm)),
+                "shefill_density_update")
+        return cur, nrm.value
 
     def recover_device(self, d_x, d_x_full):
         _fcheck(_fill_lib().shefill_recover_device(self._h, d_x, d_x_full), "shefill_recover_device")
