@@ -171,7 +171,11 @@ def bench_serving(count, rate=200.0, slots=8, seed=2024):
     from shesolve import serving
     reqs = serving.make_requests(count, rate=rate, seed=seed)
     srv = serving.Server(slots=slots, hbm_budget=200e9)
-    res = srv.run(reqs, realtime=True)
+    def done(r, k, total):
+        log(f"[bench] serving: request {r.rid} ({r.desc}) done in {r.latency:.2f} s, "
+            f"{sum(rep['iterations'] for rep in r.reports)} iterations [{k}/{total}]")
+
+    res = srv.run(reqs, realtime=True, on_done=done)
     out = serving.summarize(res, srv.stats["makespan"])
     out.update({"workload": f"{count} requests at {rate:g}/s, 80% diodes / 20% 2D MOSFET-like, 5 solves each",
                 "slots": slots, "hbm_budget_gb": 200, "peak_admitted_gb": srv.stats["peak_footprint"] / 1e9,
@@ -205,7 +209,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the SHE solve of one device is not sharded "
-                                   "across GPUs here (see DESIGN.md for how it would shard); run with --gpus 1"}))
+                                   "across GPUs here; run with --gpus 1"}))
         sys.exit(2)
 
     import shesolve

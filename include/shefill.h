@@ -114,11 +114,22 @@ int shefill_postproc_device(shefill_handle h, const double *d_x_full, double *d_
  * + new alpha, or with logarithmic != 0, density^(1-alpha) *
  * max(new, 1e-10 density)^alpha.  The reference's force_no_damping is a plain
  * copy of new.  *update_norm (may be NULL) receives the relative update norm
- * of simulator.hpp:286-320 between new and the density before the update:
- * sqrt(sum ((new - old) / old)^2), with new in place of the quotient where
- * old = 0.  (The reference's Gummel loop, simulator.hpp:1081-1086, evaluates
- * that norm before x is written back, i.e. on the previous distribution.)
- * _device: device buffers, n_cells values each. */
+ * of simulator.hpp:286-320 between new and the updated density,
+ * sqrt(sum ((new - upd) / upd)^2), with new in place of the quotient where
+ * upd = 0: the value the reference's Gummel loop reports at its next
+ * iteration, where it evaluates the norm before writing the next solution
+ * back (simulator.hpp:1081-1086).  _device: device buffers, n_cells values
+ * each. */
+/* Terminal current (postproc/terminal_current.hpp:50-107, get_terminal_current):
+ * the facet currents of shefill_postproc summed over the interior facets with
+ * exactly one cell in the terminal (terminal_mask: one byte per cell, nonzero
+ * = the cell belongs to the terminal), times the facet area, positive where the
+ * terminal cell is the facet's first cell (left / bottom).  _device: device
+ * buffers. */
+int shefill_terminal_current(shefill_handle h, const double *facet_current, const unsigned char *terminal_mask,
+                             double *current);
+int shefill_terminal_current_device(shefill_handle h, const double *d_facet_current,
+                                    const unsigned char *d_terminal_mask, double *current);
 int shefill_density_update(shefill_handle h, const double *new_density, double *density, double alpha,
                            int logarithmic, double *update_norm);
 int shefill_density_update_device(shefill_handle h, const double *d_new, double *d_density, double alpha,

@@ -294,17 +294,17 @@ def density_update(new, cur, alpha, logarithmic=False):
 
     update_quantity (simulator.hpp:524-600): linear cur (1 - a) + new a, or
     logarithmic cur^(1-a) * max(new, 1e-10 cur)^a.  get_relative_update_norm
-    (simulator.hpp:286-320): sqrt(sum zw^2), zw = (new - old) / old, or new
-    where old is 0; the norm compares new with the density before the update.
+    (simulator.hpp:286-320): sqrt(sum zw^2), zw = (new - upd) / upd, or new
+    where upd is 0, between the new density and the updated one (the norm the
+    reference's loop reports at its next iteration, simulator.hpp:1081-1086).
     Returns (updated density, norm)."""
     new = np.asarray(new, np.float64)
     cur = np.asarray(cur, np.float64)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        zw = np.where(np.abs(cur) > 0, (new - cur) / np.where(cur != 0, cur, 1.0), new)
-    norm = float(np.sqrt(np.sum(zw * zw)))
     if logarithmic:
         nv = np.where(new < cur * 1e-10, cur * 1e-10, new)
         out = np.power(cur, 1.0 - alpha) * np.power(nv, alpha)
     else:
         out = cur * (1.0 - alpha) + new * alpha
-    return out, norm
+    with np.errstate(divide="ignore", invalid="ignore"):
+        zw = np.where(np.abs(out) > 0, (new - out) / np.where(out != 0, out, 1.0), new)
+    return out, float(np.sqrt(np.sum(zw * zw)))

@@ -75,6 +75,21 @@ def test_recovery_and_postproc_match_oracle(name):
     assert np.max(np.abs(dens - dens_ref)) <= 1e-12 * np.max(np.abs(dens_ref))
     if len(cur_ref):
         assert np.max(np.abs(cur - cur_ref)) <= 1e-11 * np.max(np.abs(cur_ref))
+        # terminal currents (postproc/terminal_current.hpp) on the GPU: against
+        # the same oriented sum over the same facet currents on the CPU (1e-12 of
+        # the sum of |terms|), and against the oracle's facet currents (1e-8:
+        # the facet currents agree to 1e-11 of their maximum, and a terminal
+        # current cancels terms of either sign)
+        from test_gpu_solver import terminal_currents, terminal_groups
+        I_same = terminal_currents(s, cur)
+        I_abs = terminal_currents(s, cur, absolute=True)
+        I_ref = terminal_currents(s, cur_ref)
+        for g, Is, Ia, Ir in zip(terminal_groups(s.dev), I_same, I_abs, I_ref):
+            mask = np.zeros(s.dev.ncells, bool)
+            mask[g] = True
+            I = F.terminal_current(cur, mask)
+            assert abs(I - Is) <= 1e-12 * abs(Ia), (I, Is)
+            assert abs(I - Ir) <= 1e-8 * np.max(np.abs(I_ref)), (I, Ir)
     F.close()
 
 

@@ -183,27 +183,29 @@ def test_solve_parity(name, precision, kind):
     check_postproc(s, x, x_ref)
 
 
-def terminal_currents(s, j):
+def terminal_groups(dev):
+    """Contact cells split by connectivity order (left/right in 1D;
+    source/drain/body in 2D)."""
+    ids = np.nonzero(np.asarray(dev.contact, bool))[0]
+    nx = dev.nx
+    if dev.dim == 1:
+        return [np.array([ids[0]]), np.array([ids[-1]])]
+    if dev.name.startswith("nin2d"):
+        return [ids[ids % nx == 0], ids[ids % nx == nx - 1]]
+    top = ids[ids // nx == dev.ny - 1]
+    bottom = ids[ids // nx == 0]
+    return [top[top % nx < nx // 2], top[top % nx >= nx // 2], bottom]
+
+
+def terminal_currents(s, j, absolute=False):
     """Net current out of each contact group: sum of facet currents (times the
     facet area) over facets with exactly one cell in the group, oriented from
     the contact into the device.  Groups: contact cells split by connectivity
     order (left/right in 1D; source/drain/body in 2D)."""
     dev = s.dev
     c1, c2, d = s.facets()
-    contact = np.asarray(dev.contact, bool)
-    ids = np.nonzero(contact)[0]
-    groups = []
-    nx = dev.nx
-    if dev.dim == 1:
-        groups = [np.array([ids[0]]), np.array([ids[-1]])]
-    elif dev.name.startswith("nin2d"):
-        groups = [ids[ids % nx == 0], ids[ids % nx == nx - 1]]
-    else:
-        top = ids[ids // nx == dev.ny - 1]
-        bottom = ids[ids // nx == 0]
-        groups = [top[top % nx < nx // 2], top[top % nx >= nx // 2], bottom]
     out = []
-    for g in groups:
+    for g in terminal_groups(dev):
         inside = np.zeros(dev.ncells, bool)
         inside[g] = True
         m1, m2 = inside[c1], inside[c2]
@@ -211,7 +213,8 @@ def terminal_currents(s, j):
         area = np.where(d == 0, np.diff(dev.yv)[c1 // dev.nx] if dev.dim > 1 else 1.0,
                         np.diff(dev.xv)[c1 % dev.nx])
         sign = np.where(m1, 1.0, -1.0)
-        out.append(float(np.sum((j * area * sign)[sel])))
+        terms = (j * area * sign)[sel]
+        out.append(float(np.sum(np.abs(terms) if absolute else terms)))
     return np.array(out)
 
 

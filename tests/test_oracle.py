@@ -405,18 +405,20 @@ def test_current_conservation(name):
 
 
 def test_density_update_closed_forms():
-    """The damped Gummel update: a = 1 takes the new density, a = 1/2 gives the
-    arithmetic (linear) or geometric (logarithmic) mean, the logarithmic form
-    limits a collapsing density to 1e-10 of the old one, and the norm is the
-    2-norm of the relative changes with the new value where the old is 0
-    (brute force over a small list)."""
+    """The damped Gummel update: a = 1 takes the new density (norm 0), a = 1/2
+    gives the arithmetic (linear) or geometric (logarithmic) mean, the
+    logarithmic form limits a collapsing density to 1e-10 of the old one, and
+    the norm is the 2-norm of (new - upd) / upd with the new value where the
+    update is 0: (new - old) / (new + old) per cell for the linear a = 1/2."""
     from oracle import oracle as O
     cur = np.array([1e10, 4.0, 2.5e16, 0.0, 3.0])
     new = np.array([4e10, 9.0, 2.5e16, 7.0, -1.0])
-    out, _ = O.density_update(new, cur, 1.0)
-    assert np.array_equal(out, new)
-    out, _ = O.density_update(new, cur, 0.5)
+    out, nrm = O.density_update(new, cur, 1.0)
+    assert np.array_equal(out, new) and nrm == 0.0
+    out, nrm = O.density_update(new, cur, 0.5)
     assert np.allclose(out, (new + cur) / 2, rtol=1e-15)
+    assert nrm == pytest.approx(np.sqrt(np.sum(((new - cur) / (new + cur)) ** 2)), rel=1e-14)
+    assert nrm == pytest.approx(np.sqrt(0.6 ** 2 + (5 / 13) ** 2 + 0 + 1 + 2 ** 2), rel=1e-14)
     pos = slice(0, 3)
     out, _ = O.density_update(new[pos], cur[pos], 0.5, logarithmic=True)
     assert np.allclose(out, np.sqrt(new[pos] * cur[pos]), rtol=1e-15)
@@ -424,9 +426,10 @@ def test_density_update_closed_forms():
     assert np.isclose(out[0], 2.0 ** 0.75 * 2e-10 ** 0.25, rtol=1e-15)
     out, _ = O.density_update(np.array([1e-20]), np.array([2.0]), 1.0, logarithmic=True)
     assert out[0] == 2e-10
-    _, nrm = O.density_update(new, cur, 0.3)
+    out, nrm = O.density_update(new, cur, 0.3)
     brute = 0.0
-    for n_, c_ in zip(new, cur):
-        brute += ((n_ - c_) / c_) ** 2 if c_ != 0 else n_ ** 2
+    for n_, u_ in zip(new, out):
+        brute += ((n_ - u_) / u_) ** 2 if u_ != 0 else n_ ** 2
     assert nrm == pytest.approx(brute ** 0.5, rel=1e-15)
-    assert nrm == pytest.approx(np.sqrt(3 ** 2 + 1.25 ** 2 + 0 + 49 + (4 / 3) ** 2), rel=1e-15)
+    out, nrm = O.density_update(np.array([0.0, 3.0]), np.array([0.0, 0.0]), 0.3)
+    assert np.allclose(out, [0.0, 0.9], rtol=1e-15) and nrm == pytest.approx(2.1 / 0.9, rel=1e-14)

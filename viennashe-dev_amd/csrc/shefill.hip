@@ -344,30 +344,55 @@ __global__ void k_current(DevTables t, CurArgs a, const double *xf, double *cur)
   cur[f] = -pol * kQ * vn / Y00;
 }
 
-// Gummel update of the carrier density (simulator.hpp:524-600 update_quantity
-// with get_relative_update_norm, :286-313): the relative change of each cell
-// (new - old) / old (new where old is 0), squared, summed per block; then the
-// damped value: linear cur (1 - a) + new a, or logarithmic cur^(1-a) new^a
-// with new limited to >= 1e-10 cur.
+// Gummel update of the carrier density (simulator.hpp:524-600 update_quantity):
+// the damped value, linear cur (1 - a) + new a, or logarithmic cur^(1-a) new^a
+// with new limited to >= 1e-10 cur; then the relative update norm of
+// simulator.hpp:286-320 between the new value and the updated density,
+// (new - upd) / upd (new where upd is 0), squared and summed per block.
 __global__ void k_density_update(int nc, const double *dnew, double *dcur, double alpha, int logarithmic,
                                  double *part) {
   __shared__ double red[kB / 64];
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double z2 = 0.0;
   if (c < nc) {
-    double nv = dnew[c];
+    const double nv = dnew[c];
     const double cur = dcur[c];
-    const double zw = fabs(cur) > 0 ? (nv - cur) / cur : nv;
-    z2 = zw * zw;
+    double upd;
     if (logarithmic) {
-      if (nv < cur * 1e-10) nv = cur * 1e-10;
-      dcur[c] = pow(cur, 1.0 - alpha) * pow(nv, alpha);
+      const double nl = nv < cur * 1e-10 ? cur * 1e-10 : nv;
+      upd = pow(cur, 1.0 - alpha) * pow(nl, alpha);
     } else {
-      dcur[c] = cur * (1.0 - alpha) + nv * alpha;
+      upd = cur * (1.0 - alpha) + nv * alpha;
     }
+    dcur[c] = upd;
+    const double zw = fabs(upd) > 0 ? (nv - upd) / upd : nv;
+    z2 = zw * zw;
   }
   for (int off = 32; off > 0; off >>= 1) z2 += __shfl_xor(z2, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = z2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kB / 64; ++w) t += red[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+// Terminal current (postproc/terminal_current.hpp:50-107): the facet currents
+// summed over the facets with exactly one cell in the terminal, times the
+// facet area (the weighted interface area of an orthogonal mesh), with +
+// where the terminal cell is the facet's first cell; per-block partial sums.
+__global__ void k_terminal_current(int nf, const int *fc1, const int *fc2, const double *fvol,
+                                   const unsigned char *mask, const double *jf, double *part) {
+  __shared__ double red[kB / 64];
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (f < nf) {
+    const bool m1 = mask[fc1[f]] != 0, m2 = mask[fc2[f]] != 0;
+    if (m1 != m2) v = m1 ? jf[f] * fvol[f] : -(jf[f] * fvol[f]);
+  }
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0.0;
@@ -1444,6 +1469,61 @@ int shefill_density_update(shefill_handle h, const double *new_density, double *
   if (!e && hipMemcpy(density, dc, sizeof(double) * h->nc, hipMemcpyDeviceToHost) != hipSuccess) e = -10;
   hipFree(dn);
   hipFree(dc);
+  return e;
+}
+
+int shefill_terminal_current_device(shefill_handle h, const double *d_facet_current,
+                                    const unsigned char *d_terminal_mask, double *current) {
+  if (!h) { ferr("handle is NULL"); return 1; }
+  if (!h->built) { ferr("nothing built"); return -1; }
+  if (!d_facet_current && h->nf > 0) { ferr("facet current is NULL"); return 2; }
+  if (!d_terminal_mask) { ferr("terminal mask is NULL"); return 3; }
+  if (!current) { ferr("current is NULL"); return 4; }
+  FHIP(hipSetDevice(h->device));
+  *current = 0.0;
+  if (h->nf == 0) return 0;
+  const int nb = grid(h->nf);
+  double *part = nullptr;
+  FHIP(hipMalloc(&part, sizeof(double) * nb));
+  hipLaunchKernelGGL(k_terminal_current, dim3(nb), dim3(kB), 0, h->stream, h->nf, h->dfc1.p, h->dfc2.p, h->dfvol.p,
+                     d_terminal_mask, d_facet_current, part);
+  std::vector<double> hp(nb);
+  int e = 0;
+  if (hipMemcpyAsync(hp.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess) {
+    ferr("terminal current failed");
+    e = -10;
+  }
+  hipFree(part);
+  if (!e) {
+    double t = 0.0;
+    for (double v : hp) t += v;
+    *current = t;
+  }
+  return e;
+}
+
+int shefill_terminal_current(shefill_handle h, const double *facet_current, const unsigned char *terminal_mask,
+                             double *current) {
+  if (!h) { ferr("handle is NULL"); return 1; }
+  if (!h->built) { ferr("nothing built"); return -1; }
+  if (!facet_current && h->nf > 0) { ferr("facet current is NULL"); return 2; }
+  if (!terminal_mask) { ferr("terminal mask is NULL"); return 3; }
+  if (!current) { ferr("current is NULL"); return 4; }
+  FHIP(hipSetDevice(h->device));
+  double *dj = nullptr;
+  unsigned char *dm = nullptr;
+  FHIP(hipMalloc(&dj, sizeof(double) * std::max(h->nf, 1)));
+  FHIP(hipMalloc(&dm, std::max(h->nc, 1)));
+  int e = 0;
+  if ((h->nf > 0 && hipMemcpy(dj, facet_current, sizeof(double) * h->nf, hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(dm, terminal_mask, h->nc, hipMemcpyHostToDevice) != hipSuccess) {
+    ferr("copy failed");
+    e = -10;
+  }
+  if (!e) e = shefill_terminal_current_device(h, dj, dm, current);
+  hipFree(dj);
+  hipFree(dm);
   return e;
 }
 

@@ -375,7 +375,8 @@ FILL_EXPORTS = ("shefill_last_error", "shefill_create", "shefill_destroy", "shef
                 "shefill_device_system", "shefill_get_system", "shefill_block_ranges", "shefill_mapping",
                 "shefill_last_fill_ms", "shefill_refill", "shefill_recover", "shefill_recover_device",
                 "shefill_postproc", "shefill_postproc_device", "shefill_pattern_hash",
-                "shefill_density_update", "shefill_density_update_device")
+                "shefill_density_update", "shefill_density_update_device",
+                "shefill_terminal_current", "shefill_terminal_current_device")
 
 
 class FillDevice(C.Structure):
@@ -408,6 +409,8 @@ def _fill_lib():
     L.shefill_last_fill_ms.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.shefill_recover.argtypes = [C.c_void_p, _f64p, _f64p]
     L.shefill_pattern_hash.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+    L.shefill_terminal_current.argtypes = [C.c_void_p, _f64p, C.c_void_p, C.POINTER(C.c_double)]
+    L.shefill_terminal_current_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
     L.shefill_density_update.argtypes = [C.c_void_p, _f64p, _f64p, C.c_double, C.c_int, C.POINTER(C.c_double)]
     L.shefill_density_update_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
                                                 C.POINTER(C.c_double)]
@@ -515,6 +518,25 @@ class Fill:
         xf = np.zeros(self.sizes["n"] + self.sizes["n_odd"])
         _fcheck(_fill_lib().shefill_recover(self._h, np.ascontiguousarray(x, np.float64), xf), "shefill_recover")
         return xf
+
+    def terminal_current(self, facet_current, terminal_mask):
+        """Net current of one terminal (cells where terminal_mask is true) from
+        the facet currents of postproc, on the GPU."""
+        m = np.ascontiguousarray(np.asarray(terminal_mask, bool).astype(np.uint8))
+        if m.shape != (self.ncells,):
+            raise ValueError(f"terminal mask has {m.size} entries for {self.ncells} cells")
+        out = C.c_double()
+        _fcheck(_fill_lib().shefill_terminal_current(self._h, np.ascontiguousarray(facet_current, np.float64),
+                                                     m.ctypes.data, C.byref(out)), "shefill_terminal_current")
+        return out.value
+
+    def density_update_device(self, d_new, d_density, alpha, logarithmic=False):
+        """In-place damped update of a device density; returns the relative update norm."""
+        nrm = C.c_double()
+        _fcheck(_fill_lib().shefill_density_update_device(self._h, d_new, d_density, float(alpha),
+                                                          1 if logarithmic else 0, C.byref(nrm)),
+                "shefill_density_update_device")
+        return nrm.value
 
     def density_update(self, new_density, density, alpha=1.0, logarithmic=False):
         """Damped Gummel update of the carrier density on the GPU; returns

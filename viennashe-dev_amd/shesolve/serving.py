@@ -80,6 +80,7 @@ class Request:
     seed: int
     steps: int = 5
     sigma: float = 5e-3       # potential perturbation per cell between steps [V]
+    damping: float = 0.3      # Gummel damping of the density (solvers/config.hpp:257 default)
 
 
 @dataclasses.dataclass
@@ -92,9 +93,10 @@ class Result:
     reports: list
     n: list
     x: np.ndarray             # last reduced solution
-    density: np.ndarray       # carrier density of the last step
+    density: np.ndarray       # damped carrier density after the last step
     footprint: float
     desc: str = ""
+    update_norms: list = dataclasses.field(default_factory=list)  # relative density update per step
 
     @property
     def latency(self):
@@ -171,10 +173,10 @@ def run_request(req: Request, tol=1e-10, max_iters=3000, device=0, clock=None, a
                          **_solver_params(req.dev, req.cfg))
     rng = np.random.default_rng(req.seed)
     dev = dataclasses.replace(req.dev, potential=np.array(req.dev.potential, np.float64))
-    reps, ns = [], []
+    reps, ns, norms = [], [], []
     prev = None
     pattern = None
-    dx = None
+    dx = dxf = dnew = dcur = dj = None
     try:
         for step in range(req.steps):
             if step > 0:
@@ -202,20 +204,37 @@ def run_request(req: Request, tol=1e-10, max_iters=3000, device=0, clock=None, a
             rep = sv.solve_device(d_rhs, dx.ptr, warm_start=warm)
             reps.append(rep)
             ns.append(n)
+            # the tail of the Gummel step on the device: full solution, carrier
+            # density and facet currents, then the damped density update and its
+            # relative update norm (simulator.hpp:1081-1086)
+            nfull = n + F.sizes["n_odd"]
+            if dxf is None or dxf.n != nfull:
+                if dxf is not None:
+                    dxf.close()
+                dxf = DeviceBuffer(nfull)
+            if dnew is None:
+                dnew, dcur = DeviceBuffer(F.sizes["cells"]), DeviceBuffer(F.sizes["cells"])
+                dj = DeviceBuffer(max(F.sizes["facets"], 1))
+            F.recover_device(dx.ptr, dxf.ptr)
+            F.postproc_device(dxf.ptr, dnew.ptr, dj.ptr)
+            if step == 0:
+                dcur.upload(dnew.download())
+            else:
+                norms.append(F.density_update_device(dnew.ptr, dcur.ptr, req.damping))
             x = dx.download()
             prev = (x, mapping, float(np.max(dev.potential)))
-        xf = F.recover(prev[0])
-        dens, _ = F.postproc(xf)
+        dens = dcur.download()
     finally:
-        if dx is not None:
-            dx.close()
+        for b in (dx, dxf, dnew, dcur, dj):
+            if b is not None:
+                b.close()
         sv.close()
         F.close()
     finish = clock() if clock else time.perf_counter()
     desc = (f"{req.kind} {req.dev.nx}x{req.dev.ny} L={req.cfg.L_max} "
             f"dE={req.cfg.energy_spacing / D.EV * 1e3:.2f} meV")
     return Result(req.rid, req.kind, req.arrival, start, finish, reps, ns, prev[0], dens,
-                  footprint_estimate(req.dev, req.cfg), desc)
+                  footprint_estimate(req.dev, req.cfg), desc, norms)
 
 
 _mos_cache = {}
@@ -274,12 +293,13 @@ class Server:
         self.device = device
         self.autotune = autotune
 
-    def run(self, requests, realtime=True):
+    def run(self, requests, realtime=True, on_done=None):
         """Serves `requests`.  realtime: requests become eligible at their
-        arrival time (else all at once).  Returns the results in request order."""
+        arrival time (else all at once).  on_done(result, finished, total) is
+        called as each request completes.  Returns the results in request order."""
         pending = sorted(requests, key=lambda r: r.arrival)
         lock = threading.Condition()
-        state = {"in_use": 0.0, "next": 0, "peak": 0.0, "inflight": 0, "max_inflight": 0}
+        state = {"in_use": 0.0, "next": 0, "peak": 0.0, "inflight": 0, "max_inflight": 0, "done": 0}
         results = [None] * len(pending)
         errors = []
         t0 = time.perf_counter()
@@ -317,6 +337,9 @@ class Server:
                     results[i] = res
                     state["in_use"] -= fp
                     state["inflight"] -= 1
+                    state["done"] += 1
+                    if on_done is not None:
+                        on_done(res, state["done"], len(pending))
                     lock.notify_all()
 
         threads = [threading.Thread(target=worker, daemon=True) for _ in range(self.slots)]
