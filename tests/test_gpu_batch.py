@@ -190,3 +190,55 @@ def test_batch_preconditioner_is_the_per_system_ilu(kind):
         assert np.linalg.norm(zs - zr) <= 1e-10 * np.linalg.norm(zr), nm
         off += A.n
     bs.close()
+
+
+def _wide(n, k, seed):
+    """A system whose ILU(0) level schedule is shallow and wide: row i couples
+    to i +- k (levels of k rows), diagonally dominant, with random values."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(n):
+        cols = [c for c in (i - k, i, i + k) if 0 <= c < n]
+        rows.append(cols)
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ci = np.concatenate([np.array(r, np.int32) for r in rows])
+    val = np.where(ci == np.repeat(np.arange(n), np.diff(rp)), 4.0, -rng.uniform(0.5, 1.5, len(ci)))
+    b = rng.standard_normal(n)
+    return rp, ci, val, b
+
+
+def test_batch_level_launch_sweeps():
+    """Shallow, wide schedules (levels of 30k-40k rows) are swept by one launch
+    per level over all systems instead of one workgroup per system.  That path
+    must apply each system's own ILU(0) (against the oracle's, 1e-10), solve
+    each system to the bar, and agree with the single solves of the same
+    systems."""
+    import scipy.sparse as sp
+    import shesolve
+    from oracle import oracle as O
+    systems = [_wide(400_000, 40_000, 1), _wide(300_000, 30_000, 2)]
+    bs = shesolve.BatchSolver(0, precond=shesolve.PRECOND_ILU0, precision=shesolve.FP64, tol=1e-10,
+                              max_iters=2000)
+    bs.set_systems([(rp, ci, v) for rp, ci, v, _ in systems])
+    assert bs.info()["seg_levels_max"] == 0  # no segment schedules: the level launches run
+    r = np.random.default_rng(3).standard_normal(sum(len(b) for *_, b in systems))
+    z = bs.precond_apply(r)
+    xs, reps = bs.solve([b for *_, b in systems])
+    off = 0
+    for (rp, ci, v, b), x, rep in zip(systems, xs, reps):
+        n = len(b)
+        A = O.CSR(rp, ci, v)
+        zr = O.Precond(O.PC_ILU0, A).apply(r[off:off + n])
+        assert np.linalg.norm(z[off:off + n] - zr) <= 1e-10 * np.linalg.norm(zr)
+        off += n
+        M = sp.csr_matrix((v, ci, rp), shape=(n, n))
+        assert rep["status"] == 0
+        assert np.linalg.norm(b - M @ x) <= 1e-10 * np.linalg.norm(b)
+        sv = shesolve.Solver(0, precond=shesolve.PRECOND_ILU0, precision=shesolve.FP64, tol=1e-10, max_iters=2000)
+        sv.set_matrix(rp, ci, v)
+        x1, rep1 = sv.solve(b)
+        sv.close()
+        assert rep1["status"] == 0
+        assert np.linalg.norm(x1 - x) <= 1e-8 * np.linalg.norm(x1)
+    bs.close()

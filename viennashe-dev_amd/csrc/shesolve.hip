@@ -135,6 +135,32 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ 
 // sweeps of every contiguous level of the triangular factors.
 constexpr int kStreamThreads = 256;
 constexpr int kStreamRows = 256;  // rows per block (rows without entries must not pile up)
+// Row blocks of the colour sweeps: entries per thread and rows per block
+// (8 and 256 measured fastest on the 2D L=3 device of 16.1 M rows against
+// 4-16 entries and 256-1024 rows).
+#ifndef SHESOLVE_SWEEP_ITEMS
+#define SHESOLVE_SWEEP_ITEMS 8
+#endif
+#ifndef SHESOLVE_SWEEP_ROWS
+#define SHESOLVE_SWEEP_ROWS 256
+#endif
+// Segment sweeps: entries per lane held in registers (J) and the largest
+// lane group per row.
+#ifndef SHESOLVE_SEG_J
+#define SHESOLVE_SEG_J 2
+#endif
+#ifndef SHESOLVE_SEG_LPR_MAX
+#define SHESOLVE_SEG_LPR_MAX 8
+#endif
+constexpr int kSegJ = SHESOLVE_SEG_J;
+constexpr int kSegLprMax = SHESOLVE_SEG_LPR_MAX;
+constexpr double kSegLevelUs = 1.4;      // segment sweep: per level ...
+constexpr double kSegEntryUs = 6e-4;     // ... plus per entry (and row) of the level
+constexpr double kLevelLaunchUs = 5.0;   // one launch per level
+constexpr int kSweepItems = SHESOLVE_SWEEP_ITEMS;
+constexpr int kSweepBlockRows = SHESOLVE_SWEEP_ROWS;
+static_assert(kSweepItems == 4 || kSweepItems == 8 || kSweepItems == 16, "sweep items: 4, 8 or 16");
+static_assert(kSweepBlockRows >= 1, "sweep rows per block");
 constexpr int kStreamMaxNnz = 16 * kStreamThreads;
 constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 
@@ -1072,12 +1098,14 @@ template <typename TV, typename T, int MODE>
 void launch_stream(shesolve_ctx *c, int g, int nblk, const int *blocks, const int *rp, const int *ci, const TV *v,
                    const T *x, T *y, const T *b, const int *perm, const T *dinv, T *out2, const T *d0, const T *d1,
                    double *part, const int *pairs = nullptr) {
+  // the sweeps' row blocks are built for kSweepItems entries per thread
+  const int items = MODE == MODE_SPMV ? c->items : kSweepItems;
 #define ST_CASE(IT)                                                                                                \
   case IT:                                                                                                         \
     hipLaunchKernelGGL((k_stream<TV, T, MODE, IT>), dim3(g), dim3(kStreamThreads), 0, c->stream, nblk, c->xcd,     \
                        blocks, rp, ci, v, x, y, b, perm, dinv, out2, d0, d1, part, pairs);                         \
     break;
-  switch (c->items) { ST_CASE(4) ST_CASE(16) default: ST_CASE(8) }
+  switch (items) { ST_CASE(4) ST_CASE(16) default: ST_CASE(8) }
 #undef ST_CASE
 }
 
@@ -1146,12 +1174,12 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsL, in, tq);
 #define WG_CASE(LL)                                                                                                \
   case LL:                                                                                                         \
-    hipLaunchKernelGGL((k_wgsweep<LL, kSweepRowsL<T>, 2, TV, T, false>), dim3(c->nseg), dim3(kSweepThreads), lds,     \
+    hipLaunchKernelGGL((k_wgsweep<LL, kSweepRowsL<T>, kSegJ, TV, T, false>), dim3(c->nseg), dim3(kSweepThreads), lds, \
                        c->stream,                                                                                  \
                        c->d_segL, c->d_lvlL, c->d_loLrp, c->d_Lci, Lv, (const T *)tq, zq, (const TV *)nullptr,     \
                        c->seg_mask, W);                                                                            \
     hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_mapLU, (const T *)zq, tq);  \
-    hipLaunchKernelGGL((k_wgsweep<LL, 2, 2, TV, T, true>), dim3(c->nseg), dim3(kSweepThreads), lds, c->stream,     \
+    hipLaunchKernelGGL((k_wgsweep<LL, 2, kSegJ, TV, T, true>), dim3(c->nseg), dim3(kSweepThreads), lds, c->stream, \
                        c->d_segU, c->d_lvlU, c->d_loUrp, c->d_Uci, Uv, (const T *)tq, zq, dq, c->seg_mask, W);     \
     break;
     switch (c->seg_lpr) { WG_CASE(2) WG_CASE(4) default: WG_CASE(8) }
@@ -1244,12 +1272,13 @@ void free_matrix(shesolve_ctx *c) {
 }
 
 // Row blocks over rows [r0, r1) of a CSR (host row pointer): consecutive rows
-// holding at most `cap` entries; a longer row forms a block of its own.
-void make_blocks(const int *rp, int r0, int r1, int cap, std::vector<int> &out) {
+// holding at most `cap` entries and `rowcap` rows; a longer row forms a block
+// of its own.
+void make_blocks(const int *rp, int r0, int r1, int cap, std::vector<int> &out, int rowcap = kStreamRows) {
   out.push_back(r0);
   int start = r0;
   for (int r = r0; r < r1; ++r) {
-    if (r > start && (rp[r + 1] - rp[start] > cap || r - start >= kStreamRows)) {
+    if (r > start && (rp[r + 1] - rp[start] > cap || r - start >= rowcap)) {
       out.push_back(r);
       start = r;
     }
@@ -1286,6 +1315,36 @@ void compute_levels(const std::vector<int> &rp, const std::vector<int> &ci, cons
     maxl = std::max(maxl, l);
   }
   bucket(maxl, Uptr, Urows);
+}
+
+// Whether the segment sweeps (one workgroup per segment, the segments side by
+// side) beat one launch per level of the whole matrix, estimated on the
+// forward triangle: a workgroup spends about 1.4 us + 0.6 ns per entry on a
+// level of its segment, a launch about 5 us per level.  Deep schedules of
+// narrow levels (1D devices) go to the segments; wide levels, where one CU
+// per segment runs out of bandwidth, to the launches.
+bool segment_sweeps_pay(const shesolve_ctx *c, const std::vector<int> &Lptr, const std::vector<int> &Lrows) {
+  const auto &sp = c->seg_ptr;
+  const int S = (int)sp.size() - 1, nl = (int)Lptr.size() - 1;
+  if (S < 1 || nl < 1) return false;
+  std::vector<double> cost(S, 0.0);
+  std::vector<int64_t> ent(S, 0);
+  std::vector<int> touched;
+  for (int l = 0; l < nl; ++l) {
+    touched.clear();
+    for (int q = Lptr[l]; q < Lptr[l + 1]; ++q) {
+      const int i = Lrows[q];
+      const int s = (int)(std::upper_bound(sp.begin(), sp.end(), i) - sp.begin()) - 1;
+      if (ent[s] == 0) touched.push_back(s);
+      ent[s] += 1 + (c->h_fdg[i] - c->h_frp[i]);
+    }
+    for (int s : touched) {
+      cost[s] += kSegLevelUs + kSegEntryUs * (double)ent[s];
+      ent[s] = 0;
+    }
+  }
+  const double seg = *std::max_element(cost.begin(), cost.end());
+  return seg < kLevelLaunchUs * nl;
 }
 
 // Segment-local level schedules of both triangles of the factor: for each
@@ -1560,8 +1619,16 @@ int analyse_and_factor(shesolve_ctx *c) {
   }
   std::vector<int> Lptr, Lrows, Uptr, Urows;
   compute_levels(c->h_frp, c->h_fci, c->h_fdg, n, Lptr, Lrows, Uptr, Urows);
-  if (!c->seg_ptr.empty() && perm.empty()) {
-    if (int e = segment_levels(c)) return e;
+  if (perm.empty()) {
+    // segments (the systems of a batch, or the whole matrix): one workgroup
+    // walks each segment's levels, unless one launch per level of the whole
+    // matrix is estimated faster
+    const bool batch = !c->seg_ptr.empty();
+    if (!batch) c->seg_ptr = {0, n};
+    int e = 0;
+    if (segment_sweeps_pay(c, Lptr, Lrows)) e = segment_levels(c);
+    if (!batch) c->seg_ptr.clear();
+    if (e) return e;
   }
   if (!perm.empty()) {
     // multicolour ordering: colour q occupies the contiguous range
@@ -1660,10 +1727,12 @@ int analyse_and_factor(shesolve_ctx *c) {
     reorder(Lrp, Lci, c->h_segrowsL, loLrp);
     reorder(Urp, Uci, c->h_segrowsU, loUrp);
     {
-      // lanes per row: enough that 2 entries per lane cover the longest rows
+      // lanes per row: enough that kSegJ entries per lane cover the longest
+      // rows, at most kSegLprMax (longer rows loop over the rest)
       int mx = 0;
       for (int q = 0; q < n; ++q) mx = std::max(mx, std::max(loLrp[q + 1] - loLrp[q], loUrp[q + 1] - loUrp[q]));
-      c->seg_lpr = mx <= 4 ? 2 : (mx <= 8 ? 4 : 8);
+      c->seg_lpr = mx <= 2 * kSegJ ? 2 : (mx <= 4 * kSegJ ? 4 : 8);
+      c->seg_lpr = std::min(c->seg_lpr, kSegLprMax);
     }
     {
       std::vector<int> posL(n), mapLU(n);
@@ -1700,7 +1769,7 @@ int analyse_and_factor(shesolve_ctx *c) {
         r.r1 = rows[e - 1] + 1;
         r.blk_off = (int)blk.size();
         std::vector<int> bb;
-        make_blocks(rp.data(), r.r0, r.r1, c->items * kStreamThreads, bb);
+        make_blocks(rp.data(), r.r0, r.r1, kSweepItems * kStreamThreads, bb, kSweepBlockRows);
         // stored as boundaries: blk_cnt blocks need blk_cnt + 1 entries
         for (int v : bb) blk.push_back(v);
         r.blk_cnt = (int)bb.size() - 1;
@@ -1841,6 +1910,31 @@ int setup_solve_space(shesolve_ctx *c) {
   c->s_pairs = c->d_sblk;
   c->spmv_lpr = 0;  // the interleaved block order needs the CSR-stream kernel
   c->s_nblk = (int)pr.size();
+  if (c->prm.autotune) {
+    // block-to-XCD mapping of the permuted matrix, timed on it (the natural
+    // matrix's choice does not carry over: contiguous eighths of the
+    // interleaved order give each XCD its own eighth of the natural index
+    // space in every colour)
+    hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)c->fnnz, c->d_sv32, 1.0f);
+    float best = 1e30f;
+    int best_xcd = c->xcd;
+    for (int xm = 0; xm < 2; ++xm) {
+      c->xcd = xm;
+      launch_spmv<float, float, float>(c, 0, c->d_sv32, c->v32[0], c->v32[1], nullptr, nullptr, nullptr, nullptr);
+      HIPCHK(hipEventRecord(c->ev0, c->stream));
+      for (int r = 0; r < 3; ++r)
+        launch_spmv<float, float, float>(c, 0, c->d_sv32, c->v32[0], c->v32[1], nullptr, nullptr, nullptr, nullptr);
+      HIPCHK(hipEventRecord(c->ev1, c->stream));
+      HIPCHK(hipEventSynchronize(c->ev1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      if (ms < best) {
+        best = ms;
+        best_xcd = xm;
+      }
+    }
+    c->xcd = best_xcd;
+  }
   return 0;
 }
 

@@ -371,5 +371,6 @@ def summarize(results, makespan):
         "solve_p99_ms": float(np.percentile(per_solve, 99)),
         "max_rel_residual": max(rep["rel_residual"] for r in results for rep in r.reports),
         "all_converged": all(rep["status"] == 0 for r in results for rep in r.reports),
+        "not_converged": sum(1 for r in results for rep in r.reports if rep["status"] != 0),
         "makespan_s": makespan,
     }
