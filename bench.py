@@ -215,7 +215,7 @@ def main():
     import shesolve
     from shesolve import devices as D
 
-    dev, cfg = D.cfg2(vx=args.vx, vy=args.vy, L=args.L)
+    dev, cfg = D.mosfet_200x100nm(vx=args.vx, vy=args.vy, L=args.L)
     log(f"[bench] building {dev.name} L={cfg.L_max}")
     t0 = time.perf_counter()
     F = shesolve.Fill(0)
@@ -299,7 +299,7 @@ def main():
     b32 = spmv_bytes(n, nnz, 4, 4)
     achieved = b32 / (spmv32_us * 1e-6) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_stream_spmv_cfg2_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "k_stream_spmv_mosfet2d_161x81_L3_traffic.json")
     if os.path.exists(tpath):
         tr = json.load(open(tpath))
         if tr.get("n") == n and tr.get("nnz") == nnz:
@@ -331,7 +331,7 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "avg_us": spmv32_us, "launches": spmv32_launches,
                      "algorithmic_bytes": b32,
-                     "traffic_source": "profiles/k_stream_spmv_cfg2_fp32ir_pmc.md" if traffic else None,
+                     "traffic_source": "profiles/k_stream_spmv_mosfet2d_161x81_L3_pmc.md" if traffic else None,
                      "plain_launch_us": plain32_us,
                      "plain_launch_gbs": b32 / (plain32_us * 1e-6) / 1e9,
                      "fp64_spmv_us": spmv64_us,

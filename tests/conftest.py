@@ -23,8 +23,8 @@ def oracle_system(name):
         return _cache[name]
     from oracle import oracle as O
     from shesolve import devices as D
-    if name == "cfg1":
-        dev, cfg = D.cfg1()
+    if name == "diode_L1":
+        dev, cfg = D.nin_diode_250nm()
     elif name == "diode_L3":
         dev = D.diode_1d(length_nm=100.0, vertices=101, bias=0.3)
         cfg = D.SheConfig(L_max=3, energy_spacing=0.01 * D.EV, min_kin_range=0.6 * D.EV)

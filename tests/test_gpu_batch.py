@@ -13,7 +13,7 @@ from conftest import oracle_system
 
 pytestmark = pytest.mark.gpu
 
-MIX = ["cfg1", "diode_L3", "nin2d_L3", "mos2d_L1", "mos2d_L3"]
+MIX = ["diode_L1", "diode_L3", "nin2d_L3", "mos2d_L1", "mos2d_L3"]
 
 
 def _ref(name):
@@ -101,7 +101,7 @@ def test_graph_replay_matches_direct_launches(precision):
     """The captured iteration, replayed many times with host checks in between,
     gives the same numbers as the kernels launched one by one."""
     import shesolve
-    names = ["cfg1", "nin2d_L3", "mos2d_L1"]
+    names = ["diode_L1", "nin2d_L3", "mos2d_L1"]
     sy = _systems(names)
     prec = shesolve.FP64 if precision == "fp64" else shesolve.FP32_IR
     out = []
@@ -128,7 +128,7 @@ def test_graph_replay_matches_direct_launches(precision):
 
 def test_batch_zero_rhs_warm_start_and_update_values():
     import shesolve
-    names = ["cfg1", "mos2d_L1"]
+    names = ["diode_L1", "mos2d_L1"]
     sy = _systems(names)
     bs = shesolve.BatchSolver(0, precond=shesolve.PRECOND_ILU0, precision=shesolve.FP32_IR, tol=1e-12,
                               max_iters=5000)
@@ -152,7 +152,7 @@ def test_batch_zero_rhs_warm_start_and_update_values():
 
 def test_batch_rejects_bad_input():
     import shesolve
-    A, b, _, _ = oracle_system("cfg1").reduced()
+    A, b, _, _ = oracle_system("diode_L1").reduced()
     bs = shesolve.BatchSolver(0)
     with pytest.raises(shesolve.SheSolveError):
         bs.set_systems([])
@@ -171,7 +171,7 @@ def test_batch_preconditioner_is_the_per_system_ilu(kind):
     that system alone (ILU(1) for the L=5 diode under iluk)."""
     import shesolve
     from oracle import oracle as O
-    names = ["cfg1", "diode_L3", "nin2d_L3", "mos2d_L1"] + (["diode_L5"] if kind == "iluk" else [])
+    names = ["diode_L1", "diode_L3", "nin2d_L3", "mos2d_L1"] + (["diode_L5"] if kind == "iluk" else [])
     sy = _systems(names)
     pc = shesolve.PRECOND_ILU0 if kind == "ilu0" else shesolve.PRECOND_ILUK
     bs = shesolve.BatchSolver(0, precond=pc, tol=1e-10, max_iters=5000)

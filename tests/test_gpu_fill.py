@@ -12,7 +12,7 @@ from conftest import oracle_system
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["cfg1", "diode_L3", "diode_L5", "nin2d_L3", "mos2d_L1", "mos2d_L3"]
+CASES = ["diode_L1", "diode_L3", "diode_L5", "nin2d_L3", "mos2d_L1", "mos2d_L3"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -45,7 +45,7 @@ def test_fill_matches_oracle(name):
     assert np.linalg.norm(rhs - b) <= 1e-11 * np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("name", ["cfg1", "diode_L3", "diode_L5", "nin2d_L3", "mos2d_L1", "mos2d_L3"])
+@pytest.mark.parametrize("name", ["diode_L1", "diode_L3", "diode_L5", "nin2d_L3", "mos2d_L1", "mos2d_L3"])
 def test_recovery_and_postproc_match_oracle(name):
     """Scale back + odd recovery and the post-processing (carrier density per
     cell, current per interior facet) on the GPU against the oracle, applied to

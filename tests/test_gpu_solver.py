@@ -16,8 +16,8 @@ from conftest import oracle_system
 
 pytestmark = pytest.mark.gpu
 
-SYSTEMS = ["cfg1", "diode_L3", "diode_L5", "nin2d_L3", "mos2d_L1", "mos2d_L3"]
-MC_WEAK = {"cfg1", "diode_L3", "nin2d_L3"}
+SYSTEMS = ["diode_L1", "diode_L3", "diode_L5", "nin2d_L3", "mos2d_L1", "mos2d_L3"]
+MC_WEAK = {"diode_L1", "diode_L3", "nin2d_L3"}
 # ILU(0) in any ordering diverges on the L=5 diode (the oracle's residual grows
 # to 1e5 in 2000 iterations as well); ILU(1) converges in 9 iterations
 ILU0_DIVERGES = {"diode_L5"}
@@ -61,7 +61,7 @@ def test_spmv_fp64_fp32(name):
     assert np.all(np.abs(y32.astype(np.float64) - y_ref) <= 2e-6 * bound + 1e-30)
 
 
-@pytest.mark.parametrize("name", ["cfg1", "mos2d_L3"])
+@pytest.mark.parametrize("name", ["diode_L1", "mos2d_L3"])
 @pytest.mark.parametrize("kind", ["ilu0", "block_ilu0"])
 def test_ilu0_factors_match_oracle(name, kind):
     import shesolve
@@ -91,7 +91,7 @@ def _pc(kind):
             "mc_ilu0": shesolve.PRECOND_MC_ILU0, "iluk": shesolve.PRECOND_ILUK}[kind]
 
 
-@pytest.mark.parametrize("name", ["cfg1", "diode_L5", "mos2d_L3"])
+@pytest.mark.parametrize("name", ["diode_L1", "diode_L5", "mos2d_L3"])
 @pytest.mark.parametrize("k,drop", [(1, 0.0), (2, 0.0), (2, 1e-3)])
 def test_iluk_factors_match_oracle(name, k, drop):
     """ILU(k): level-of-fill pattern bit-exact against the oracle, values within
@@ -115,7 +115,7 @@ def test_iluk_factors_match_oracle(name, k, drop):
     assert np.linalg.norm(z - z_ref) <= 1e-10 * np.linalg.norm(z_ref)
 
 
-@pytest.mark.parametrize("name", ["cfg1", "mos2d_L3"])
+@pytest.mark.parametrize("name", ["diode_L1", "mos2d_L3"])
 def test_mc_ilu0_factors_match_oracle(name):
     """Multicolour ILU0 = the oracle's ILU0 of the matrix permuted by the
     solver's own ordering (rows and columns)."""
@@ -156,7 +156,7 @@ def test_solve_parity(name, precision, kind):
                     "oracle's); ILU(k) is the preconditioner for it")
     if kind == "mc_ilu0" and name in MC_WEAK:
         pytest.skip("multicolour ILU0 on a 1D chain (or a thin 2D strip) decouples neighbouring cells across colours; "
-                    "BiCGStab then needs thousands of iterations (cfg1: 2541 vs 26 with natural ILU0), so natural "
+                    "BiCGStab then needs thousands of iterations (diode_L1: 2541 vs 26 with natural ILU0), so natural "
                     "ILU0 is the preconditioner for these devices")
     s = oracle_system(name)
     A, b, _, _ = s.reduced()
@@ -243,7 +243,7 @@ def check_postproc(s, x, x_ref):
 
 
 def test_zero_rhs_and_warm_start():
-    s = oracle_system("cfg1")
+    s = oracle_system("diode_L1")
     A, b, _, _ = s.reduced()
     sv = _solver(tol=1e-10)
     sv.set_matrix(A.row_ptr, A.col, A.val)
@@ -257,7 +257,7 @@ def test_zero_rhs_and_warm_start():
 
 def test_invalid_rhs_is_rejected():
     import shesolve
-    s = oracle_system("cfg1")
+    s = oracle_system("diode_L1")
     A, b, _, _ = s.reduced()
     sv = _solver()
     sv.set_matrix(A.row_ptr, A.col, A.val)

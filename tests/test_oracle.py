@@ -189,7 +189,7 @@ def test_iluk_level_of_fill_on_the_five_point_laplacian():
     assert np.allclose(P.apply(M @ x), x, rtol=1e-12, atol=1e-12)
 
 
-@pytest.mark.parametrize("name", ["cfg1", "diode_L5"])
+@pytest.mark.parametrize("name", ["diode_L1", "diode_L5"])
 def test_iluk_zero_level_is_ilu0_and_dropping_rules(name):
     """ILU(k) at k = 0 is ILU(0) entry for entry; ILU(k) keeps (L U)_ij = a_ij on
     A's pattern (its defining property on the extended pattern); with a drop
@@ -222,7 +222,7 @@ def test_iluk_zero_level_is_ilu0_and_dropping_rules(name):
     assert np.count_nonzero(Fd.val[off] == 0) > 0
 
 
-@pytest.mark.parametrize("name", ["cfg1", "diode_L3", "mos2d_L3"])
+@pytest.mark.parametrize("name", ["diode_L1", "diode_L3", "mos2d_L3"])
 def test_ilu0_reproduces_A_on_its_pattern(name):
     """ILU(0) defining property: (L U)_ij = a_ij for every (i, j) in the pattern of A."""
     from oracle import oracle as O
@@ -240,7 +240,7 @@ def test_ilu0_reproduces_A_on_its_pattern(name):
     assert np.max(np.abs(vals - ref)) <= 1e-12 * max(scale, 1.0)
 
 
-@pytest.mark.parametrize("name", ["cfg1", "diode_L3", "mos2d_L3"])
+@pytest.mark.parametrize("name", ["diode_L1", "diode_L3", "mos2d_L3"])
 def test_bicgstab_matches_direct_solve(name):
     from oracle import oracle as O
     s = oracle_system(name)
@@ -252,7 +252,7 @@ def test_bicgstab_matches_direct_solve(name):
 
 
 # ------------------------------------------------------------------ front end
-@pytest.mark.parametrize("name", ["cfg1", "diode_L3", "mos2d_L3"])
+@pytest.mark.parametrize("name", ["diode_L1", "diode_L3", "mos2d_L3"])
 def test_elimination_and_recovery_solve_the_full_system(name):
     """recover(eliminate(A)) gives the solution of the full even+odd system, and
     the odd-odd block is diagonal (she/eliminate.hpp:80-354)."""
@@ -269,7 +269,7 @@ def test_elimination_and_recovery_solve_the_full_system(name):
     assert np.linalg.norm(M @ xf - fb) <= 1e-10 * np.linalg.norm(fb)
 
 
-@pytest.mark.parametrize("name", ["cfg1", "mos2d_L3"])
+@pytest.mark.parametrize("name", ["diode_L1", "mos2d_L3"])
 def test_normalisation_identities(name):
     """row_normalize_system (math/linalg_util.hpp:200-282): A = R * A0 * S * C with
     S the unknown scaling exp(-E_kin/kT), C the column factors (geometric mean of
@@ -294,7 +294,7 @@ def test_normalisation_identities(name):
         assert abs(np.mean(np.log(np.abs(col)))) <= 1e-9
 
 
-@pytest.mark.parametrize("name", ["cfg1", "diode_L3", "nin2d_L3", "mos2d_L3"])
+@pytest.mark.parametrize("name", ["diode_L1", "diode_L3", "nin2d_L3", "mos2d_L3"])
 def test_mapping_rules(name):
     """Energy-major numbering (mapping.hpp:204-271): H outer, cells inner, no
     unknowns at H = 0 and N_H - 1, none where the averaged density of states is
@@ -319,7 +319,7 @@ def test_mapping_rules(name):
     assert np.all(ei[:, 1:-1][inside_gap] < 0)
 
 
-@pytest.mark.parametrize("name", ["cfg1", "mos2d_L3"])
+@pytest.mark.parametrize("name", ["diode_L1", "mos2d_L3"])
 def test_pattern_couples_only_facet_neighbours_and_phonon_partners(name):
     """Reduced rows couple a node to itself, its facet neighbours at the same H,
     and (harmonic 0 only) to the same cell at H +- hbar*omega/dE."""
@@ -387,7 +387,7 @@ def test_equilibrium_is_maxwellian(dim, T):
     assert np.max(np.abs(odd)) <= 1e-12 * np.max(np.abs(xf[:s.n_even]))
 
 
-@pytest.mark.parametrize("name", ["cfg1", "diode_L3", "nin2d_L3"])
+@pytest.mark.parametrize("name", ["diode_L1", "diode_L3", "nin2d_L3"])
 def test_current_conservation(name):
     """Terminal currents of the two contacts cancel to 1e-5 (ushape_2d.cpp:196-204)
     and the contact density follows the Maxwellian boundary normalisation."""

@@ -410,9 +410,11 @@ struct RowArgs {
   double *val, *rhs;
 };
 
-// One thread per row: value of every stored entry.
+// One thread per row: value of every stored entry.  (Lanes per row measured
+// slower: every lane repeats the row's facet set-up and the entry loops diverge.)
+constexpr int kRowLanes = 8;  // lanes per row of the normalisation
 __global__ __launch_bounds__(kB) void k_rows(DevTables t, RowArgs a) {
-  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= a.n) return;
   const int nd = a.row_node[row];
   const int c = a.nd_c[nd], base = a.nd_base[nd], Lc = a.nd_L[nd];
@@ -471,11 +473,11 @@ __global__ __launch_bounds__(kB) void k_rows(DevTables t, RowArgs a) {
   a.rhs[row] = (r == 0) ? a.rhs0[nd] : 0.0;
 }
 
-// unknown scaling of the columns (she/rescaling.hpp:89-107)
-__global__ void k_scale(int n, const int *rp, const int *ci, const double *scal, double *val) {
-  int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= n) return;
-  for (int j = rp[row]; j < rp[row + 1]; ++j) val[j] *= scal[ci[j]];
+// unknown scaling of the columns (she/rescaling.hpp:89-107): every entry
+// times its column's factor
+__global__ void k_scale(int64_t nnz, const int *ci, const double *scal, double *val) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nnz; j += (int64_t)gridDim.x * blockDim.x)
+    val[j] *= scal[ci[j]];
 }
 // log-magnitude sum of the structural entries of each column, in ascending
 // row order (the order of the transposed std::map rows)
@@ -495,20 +497,27 @@ __global__ void k_colnorm(int n, const int *colcount, const int *diag_pos, const
   logsum[i] = cn;  // reused as the column normaliser
   colfac[i] = 1.0 / cn;
 }
+// entries divided by their column normaliser, then the row by its 2-norm
+// (signed like the diagonal); kRowLanes lanes per row
 __global__ void k_rownorm(int n, const int *rp, const int *ci, const int *diag_pos, const double *cn, double *val,
                           double *rhs) {
-  int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= n) return;
-  double s = 0.0;
-  for (int j = rp[row]; j < rp[row + 1]; ++j) {
-    double v = val[j] / cn[ci[j]];
-    val[j] = v;
-    s += v * v;
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = (int)(gt / kRowLanes), lane = (int)(gt % kRowLanes);
+  const bool ok = row < n;
+  double s = 0.0, vd = 0.0;
+  if (ok) {
+    vd = val[diag_pos[row]] / cn[row];
+    for (int j = rp[row] + lane; j < rp[row + 1]; j += kRowLanes) {
+      const double v = val[j] / cn[ci[j]];
+      s += v * v;
+    }
   }
+  for (int off = kRowLanes / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kRowLanes);
+  if (!ok) return;
   double rn = sqrt(s);
-  if (val[diag_pos[row]] < 0.0) rn *= -1.0;
-  for (int j = rp[row]; j < rp[row + 1]; ++j) val[j] /= rn;
-  rhs[row] /= rn;
+  if (vd < 0.0) rn *= -1.0;
+  for (int j = rp[row] + lane; j < rp[row + 1]; j += kRowLanes) val[j] = (val[j] / cn[ci[j]]) / rn;
+  if (lane == 0) rhs[row] /= rn;
 }
 __global__ void k_node_scaling(int n, const int *row_node, const int *nd_c, const int *nd_H, DevTables t, int on,
                                double *scal) {
@@ -1176,14 +1185,14 @@ int launch_fill(shefill_ctx *c) {
     hipLaunchKernelGGL(k_rows, dim3(grid(n)), dim3(kB), 0, c->stream, t, a);
     hipLaunchKernelGGL(k_node_scaling, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->drow_node.p, c->dnd_c.p,
                        c->dnd_H.p, t, c->cfg.scale, c->dscal.p);
-    hipLaunchKernelGGL(k_scale, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->drp.p, c->dci.p, c->dscal.p,
-                       c->dval.p);
+    hipLaunchKernelGGL(k_scale, dim3(std::min(grid(c->nnz), 32768)), dim3(kB), 0, c->stream, (int64_t)c->nnz,
+                       c->dci.p, c->dscal.p, c->dval.p);
     hipLaunchKernelGGL(k_collog, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->dcptr.p, c->dcpos.p, c->dval.p,
                        c->dlog.p);
     hipLaunchKernelGGL(k_colnorm, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->dcolcount.p, c->ddiag_pos.p,
                        c->dval.p, c->dlog.p, c->dcolfac.p);
-    hipLaunchKernelGGL(k_rownorm, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->drp.p, c->dci.p, c->ddiag_pos.p,
-                       c->dlog.p, c->dval.p, c->drhs.p);
+    hipLaunchKernelGGL(k_rownorm, dim3(grid((int64_t)n * kRowLanes)), dim3(kB), 0, c->stream, (int)n, c->drp.p,
+                       c->dci.p, c->ddiag_pos.p, c->dlog.p, c->dval.p, c->drhs.p);
     hipLaunchKernelGGL(k_rp64, dim3(grid(n + 1)), dim3(kB), 0, c->stream, (int)n, c->drp.p, c->drp64.p);
   }
   FHIP(hipGetLastError());

@@ -53,6 +53,7 @@ void set_err(const char *fmt, ...) {
 
 constexpr int kBlock = 256;      // threads per block for streaming kernels
 constexpr int kGrid = 2048;      // persistent grid (8 workgroups per CU on 256 CUs)
+constexpr int kPartStride = 4 * kGrid;  // per-workgroup partial sums: two dot products, up to 4 kGrid workgroups
 constexpr int kMaxRowLDS = 512;  // longest row the LDS factorisation accepts
 
 // ---------------------------------------------------------------- device scalars
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ 
     block_sum<2>(vv);
     if (threadIdx.x == 0) {
       part[blockIdx.x] = vv[0];
-      part[kGrid + blockIdx.x] = vv[1];
+      part[kPartStride + blockIdx.x] = vv[1];
     }
   }
 }
@@ -161,7 +162,6 @@ constexpr int kSweepItems = SHESOLVE_SWEEP_ITEMS;
 constexpr int kSweepBlockRows = SHESOLVE_SWEEP_ROWS;
 static_assert(kSweepItems == 4 || kSweepItems == 8 || kSweepItems == 16, "sweep items: 4, 8 or 16");
 static_assert(kSweepBlockRows >= 1, "sweep rows per block");
-constexpr int kStreamMaxNnz = 16 * kStreamThreads;
 constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 
 enum { MODE_SPMV = 0, MODE_LSWEEP = 1, MODE_USWEEP = 2 };
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
     block_sum<2>(vv);
     if (threadIdx.x == 0) {
       part[blockIdx.x] = vv[0];
-      part[kGrid + blockIdx.x] = vv[1];
+      part[kPartStride + blockIdx.x] = vv[1];
     }
   }
 }
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_update_xr(int n, T *__restrict__ x, 
   block_sum<2>(vv);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = vv[0];
-    part[kGrid + blockIdx.x] = vv[1];
+    part[kPartStride + blockIdx.x] = vv[1];
   }
 }
 
@@ -437,10 +437,10 @@ __device__ inline double reduce_part(const double *part, int count) {
 //     2 = after t = A sh: omega = ts/tt
 //     3 = after x/r update: res, beta for next iteration
 //     4 = store norm of partial sum (sqrt) into slot
-__global__ __launch_bounds__(kBlock) void k_finalize(int op, const double *__restrict__ part, Scalars *sc,
+__global__ __launch_bounds__(kBlock) void k_finalize(int op, int cnt, const double *__restrict__ part, Scalars *sc,
                                                      double *__restrict__ out_norm) {
   if (op == 0) {
-    double rr = reduce_part(part, kGrid);
+    double rr = reduce_part(part, cnt);
     if (threadIdx.x == 0) {
       sc->rr = rr;
       sc->bnorm = sqrt(rr);
@@ -454,23 +454,23 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int op, const double *__res
       sc->rho = sc->r0r;
     }
   } else if (op == 1) {
-    double r0v = reduce_part(part, kGrid);
+    double r0v = reduce_part(part, cnt);
     if (threadIdx.x == 0) {
       sc->r0v = r0v;
       if (r0v == 0.0 || !isfinite(r0v)) { sc->breakdown = 1; sc->alpha = 0.0; }
       else sc->alpha = sc->rho / r0v;
     }
   } else if (op == 2) {
-    double ts = reduce_part(part, kGrid);
-    double tt = reduce_part(part + kGrid, kGrid);
+    double ts = reduce_part(part, cnt);
+    double tt = reduce_part(part + kPartStride, cnt);
     if (threadIdx.x == 0) {
       sc->ts = ts;
       sc->tt = tt;
       sc->omega = tt > 0.0 ? ts / tt : 0.0;
     }
   } else if (op == 3) {
-    double rr = reduce_part(part, kGrid);
-    double r0r = reduce_part(part + kGrid, kGrid);
+    double rr = reduce_part(part, cnt);
+    double r0r = reduce_part(part + kPartStride, cnt);
     if (threadIdx.x == 0) {
       sc->rr = rr;
       sc->r0r = r0r;
@@ -483,7 +483,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int op, const double *__res
       }
     }
   } else if (op == 4) {
-    double s = reduce_part(part, kGrid);
+    double s = reduce_part(part, cnt);
     if (threadIdx.x == 0) out_norm[0] = sqrt(s);
   }
 }
@@ -975,6 +975,7 @@ struct shesolve_ctx {
   int nablk = 0;
   int items = 8;   // entries per thread of a row block (row block = 256 * items entries)
   int xcd = 1;     // XCD-contiguous block mapping
+  int spmv_grid = kGrid;  // workgroups of the last SpMV with fused dot products
   double tune_us[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   int spmv_lpr = 0;  // 0: CSR-stream SpMV; else lanes per row of the vector-CSR SpMV
   // split factor: strict lower / strict upper parts and 1/diagonal (factor space)
@@ -1118,6 +1119,7 @@ void launch_spmv(shesolve_ctx *c, int /*lpr*/, const TV *v, const T *x, T *y, co
     // vector CSR: LPR lanes per row, grid-stride over rows; fused dots need the
     // fixed grid (one partial per workgroup)
     const int g = part ? kGrid : std::min(grid_for((int64_t)c->n * c->spmv_lpr, kBlock), 4 * kGrid);
+    if (part) c->spmv_grid = g;
 #define VS_CASE(LL)                                                                                              \
   case LL:                                                                                                       \
     hipLaunchKernelGGL((k_spmv<LL, TV, T, T>), dim3(g), dim3(kBlock), 0, c->stream, c->n, rp, ci, v, x, y, b, d0, d1, \
@@ -1129,7 +1131,11 @@ void launch_spmv(shesolve_ctx *c, int /*lpr*/, const TV *v, const T *x, T *y, co
   }
   const int *blk = c->s_rp ? c->s_blk : c->d_ablk;
   const int nblk = c->s_rp ? c->s_nblk : c->nablk;
-  launch_stream<TV, T, MODE_SPMV>(c, stream_grid(nblk, part != nullptr), nblk, blk, rp, ci, v, x, y, b,
+  // the fused dots leave one partial per workgroup of the persistent grid
+  // (a grid of up to 4 kGrid measured no faster)
+  const int g = stream_grid(nblk, part != nullptr);
+  if (part) c->spmv_grid = g;
+  launch_stream<TV, T, MODE_SPMV>(c, g, nblk, blk, rp, ci, v, x, y, b,
                                   (const int *)nullptr, (const T *)nullptr, (T *)nullptr, d0, d1, part,
                                   c->s_rp ? c->s_pairs : nullptr);
 }
@@ -1240,7 +1246,7 @@ int ensure_vectors(shesolve_ctx *c) {
   size_t n = (size_t)std::max(c->n, 1);
   for (auto &p : c->v64) HIPCHK(hipMalloc(&p, n * sizeof(double)));
   for (auto &p : c->v32) HIPCHK(hipMalloc(&p, n * sizeof(float)));
-  if (!c->d_part) HIPCHK(hipMalloc(&c->d_part, sizeof(double) * 2 * kGrid));
+  if (!c->d_part) HIPCHK(hipMalloc(&c->d_part, sizeof(double) * 2 * kPartStride));
   if (!c->d_sc) HIPCHK(hipMalloc(&c->d_sc, sizeof(Scalars)));
   if (!c->d_norm) HIPCHK(hipMalloc(&c->d_norm, sizeof(double) * 4));
   c->vec_n = c->n;
@@ -1962,10 +1968,10 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
   // bnorm comes from b, not from the initial residual
   hipLaunchKernelGGL((k_fill<T>), dim3(vg), dim3(kBlock), 0, c->stream, (int64_t)n, p, (T)0);
   hipLaunchKernelGGL((k_fill<T>), dim3(vg), dim3(kBlock), 0, c->stream, (int64_t)n, v, (T)0);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 0, part, c->d_sc, c->d_norm);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 0, kGrid, part, c->d_sc, c->d_norm);
   // |b|
   hipLaunchKernelGGL((k_copy_dot<T>), dim3(vg), dim3(kBlock), 0, c->stream, n, b, s, part);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, part, c->d_sc, c->d_norm);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, kGrid, part, c->d_sc, c->d_norm);
   double bn = 0;
   HIPCHK(hipMemcpyAsync(&hs, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(&bn, c->d_norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -2002,20 +2008,20 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
     }
     launch_precond<TV, T>(c, Mv, p, ph);
     launch_spmv<TV, T, T>(c, c->lpr, Av, ph, v, nullptr, r0, nullptr, part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 1, part, c->d_sc, c->d_norm);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 1, c->spmv_grid, part, c->d_sc, c->d_norm);
     {
       KTimer kt(c, 3);
       hipLaunchKernelGGL((k_update_s<T>), dim3(vg), dim3(kBlock), 0, c->stream, n, s, r, v, c->d_sc);
     }
     launch_precond<TV, T>(c, Mv, s, sh);
     launch_spmv<TV, T, T>(c, c->lpr, Av, sh, t, nullptr, s, t, part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 2, part, c->d_sc, c->d_norm);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 2, c->spmv_grid, part, c->d_sc, c->d_norm);
     {
       KTimer kt(c, 3);
       hipLaunchKernelGGL((k_update_xr<T>), dim3(vg), dim3(kBlock), 0, c->stream, n, x, r, ph, sh, s, t, r0, c->d_sc,
                          part);
     }
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 3, part, c->d_sc, c->d_norm);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 3, kGrid, part, c->d_sc, c->d_norm);
     HIPCHK(hipMemcpyAsync(&hs, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!std::isfinite(hs.res)) {
@@ -2045,7 +2051,7 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
       // confirm with the explicit residual r = b - A x
       launch_spmv<TV, T, T>(c, c->lpr, Av, x, r, b, nullptr, nullptr, nullptr);
       hipLaunchKernelGGL((k_copy_dot<T>), dim3(vg), dim3(kBlock), 0, c->stream, n, r, r0, part);
-      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, part, c->d_sc, c->d_norm);
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, kGrid, part, c->d_sc, c->d_norm);
       double rn = 0;
       HIPCHK(hipMemcpyAsync(&rn, c->d_norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
@@ -2061,7 +2067,7 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
       hipLaunchKernelGGL((k_copy_dot<T>), dim3(vg), dim3(kBlock), 0, c->stream, n, r, r0, part);
       hipLaunchKernelGGL((k_fill<T>), dim3(vg), dim3(kBlock), 0, c->stream, (int64_t)n, p, (T)0);
       hipLaunchKernelGGL((k_fill<T>), dim3(vg), dim3(kBlock), 0, c->stream, (int64_t)n, v, (T)0);
-      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 0, part, c->d_sc, c->d_norm);
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 0, kGrid, part, c->d_sc, c->d_norm);
       Scalars tmp{};
       HIPCHK(hipMemcpyAsync(&tmp, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
@@ -2079,7 +2085,7 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
 int residual64(shesolve_ctx *c, const double *b, const double *x, double *r, double *out) {
   launch_spmv<double, double, double>(c, c->lpr, sv64(c), x, r, b, nullptr, nullptr, nullptr);
   hipLaunchKernelGGL((k_copy_dot<double>), dim3(kGrid), dim3(kBlock), 0, c->stream, c->n, r, r, c->d_part);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, c->d_part, c->d_sc, c->d_norm);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, kGrid, c->d_part, c->d_sc, c->d_norm);
   HIPCHK(hipMemcpyAsync(out, c->d_norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
@@ -2116,7 +2122,7 @@ int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_r
     double rn = 0, bn = 0;
     if (int e = residual64(c, b, x, c->v64[8], &rn)) return e;
     hipLaunchKernelGGL((k_copy_dot<double>), dim3(kGrid), dim3(kBlock), 0, c->stream, n, b, c->v64[0], c->d_part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, c->d_part, c->d_sc, c->d_norm);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, kGrid, c->d_part, c->d_sc, c->d_norm);
     HIPCHK(hipMemcpyAsync(&bn, c->d_norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!(rn <= bn)) warm = 0;
@@ -2137,7 +2143,7 @@ int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_r
     double bn = 0;
     // ||b||
     hipLaunchKernelGGL((k_copy_dot<double>), dim3(kGrid), dim3(kBlock), 0, c->stream, n, b, c->v64[0], c->d_part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, c->d_part, c->d_sc, c->d_norm);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, kGrid, c->d_part, c->d_sc, c->d_norm);
     HIPCHK(hipMemcpyAsync(&bn, c->d_norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (bn == 0.0) {

@@ -216,12 +216,12 @@ def nin_2d(width_nm=100.0, height_nm=40.0, vx=41, vy=17, nplus_frac=0.2, n_plus=
     return Device(2, nx, ny, xv, yv, dn, dp, psi, contact.ravel().astype(np.int32), T, name=f"nin2d_{vx}x{vy}")
 
 
-def cfg1():
+def nin_diode_250nm():
     """1D n+ n n+ diode, 251 vertices on [0,250] nm, L=1, 3.1 meV to 1.0 eV."""
     return diode_1d(), SheConfig(L_max=1, energy_spacing=0.0031 * EV, min_kin_range=1.0 * EV)
 
 
-def cfg2(vx=161, vy=81, L=3):
+def mosfet_200x100nm(vx=161, vy=81, L=3):
     """2D MOSFET-like 200x100 nm, L=3, 10 meV to 1.5 eV."""
     return mosfet_2d(vx=vx, vy=vy), SheConfig(L_max=L, energy_spacing=0.010 * EV, min_kin_range=1.5 * EV)
 

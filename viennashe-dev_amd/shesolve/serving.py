@@ -251,7 +251,7 @@ def make_requests(count, rate=200.0, seed=2024, mos_frac=0.2, mesh=(41, 161), st
                   diode_length=(100, 1000)):
     """Poisson arrivals at `rate` per second.
 
-    - A fraction `mos_frac` are 2D MOSFET-like devices: cfg2 geometry, vx uniform in
+    - A fraction `mos_frac` are 2D MOSFET-like devices: the 200x100 nm geometry, vx uniform in
       `mesh`, vy = (vx - 1) / 2 + 1, L in {1, 3}, 10 meV to 1.5 eV.
     - The rest are 1D diodes drawn like the batched workload: length uniform in
       `diode_length` nm with 1 nm cells, n+ 20 % of the length each side, L in
