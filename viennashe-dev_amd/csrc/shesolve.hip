@@ -580,6 +580,89 @@ __global__ __launch_bounds__(64) void k_ilu0_level(int cnt, const int *__restric
   }
 }
 
+// The same factorisation for short rows, 8 lanes per row and 32 rows per
+// workgroup, everything in registers: lane k of a row's group holds the
+// row's positions k, k + 8, k + 16, k + 24, and for each L entry the U part of
+// that entry's row is loaded across the group and applied one entry at a time
+// (broadcast column, the lane owning that column updates it).  The updates
+// run in the same order as k_ilu0_level's, so the factor is the same.  Rows
+// and the U parts they read must have at most kShortRow entries.
+constexpr int kShortRow = 32;
+constexpr int kShortLanes = 8;
+__global__ __launch_bounds__(256) void k_ilu0_short(int cnt, const int *__restrict__ rows, const int *__restrict__ rp,
+                                                    const int *__restrict__ ci, const int *__restrict__ dg,
+                                                    double *__restrict__ fv, double drop) {
+  constexpr int E = kShortRow / kShortLanes;
+  const int gid = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kShortLanes);
+  const int lane = threadIdx.x & (kShortLanes - 1);
+  const bool ok = gid < cnt;
+  const int row = ok ? rows[gid] : 0;
+  const int beg = ok ? rp[row] : 0, len = ok ? rp[row + 1] - beg : 0, nl = ok ? dg[row] - beg : 0;
+  int col[E];
+  double val[E];
+  double sq = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int t = lane + kShortLanes * e;
+    col[e] = t < len ? ci[beg + t] : -1;
+    val[e] = t < len ? fv[beg + t] : 0.0;
+    sq += val[e] * val[e];
+  }
+  for (int off = kShortLanes / 2; off > 0; off >>= 1) sq += __shfl_xor(sq, off, kShortLanes);
+  const double tau = drop > 0.0 ? drop * sqrt(sq) : 0.0;
+  // groups run their own trip counts; every shuffle below stays inside the group
+  for (int t = 0; t < nl; ++t) {
+    const int owner = t % kShortLanes, et = t / kShortLanes;
+    int kc = 0;
+    double vt = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      if (e == et) { kc = col[e]; vt = val[e]; }
+    kc = __shfl(kc, owner, kShortLanes);
+    vt = __shfl(vt, owner, kShortLanes);
+    const double l = vt / fv[dg[kc]];
+    const bool dropped = drop > 0.0 && fabs(l) < tau;
+    if (lane == owner) {
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        if (e == et) val[e] = dropped ? 0.0 : l;
+    }
+    if (dropped) continue;
+    const int ub = dg[kc] + 1, m = rp[kc + 1] - ub;
+    int uj[E];
+    double uv[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int q = lane + kShortLanes * e;
+      uj[e] = q < m ? ci[ub + q] : -1;
+      uv[e] = q < m ? fv[ub + q] : 0.0;
+    }
+    for (int q = 0; q < m; ++q) {
+      const int qo = q % kShortLanes, qe = q / kShortLanes;
+      int j = 0;
+      double u = 0.0;
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        if (e == qe) { j = uj[e]; u = uv[e]; }
+      j = __shfl(j, qo, kShortLanes);
+      u = __shfl(u, qo, kShortLanes);
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        if (col[e] == j) val[e] -= l * u;
+    }
+  }
+  if (!ok) return;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int t = lane + kShortLanes * e;
+    if (t < len) {
+      double v = val[e];
+      if (drop > 0.0 && t > nl && fabs(v) < tau) v = 0.0;
+      fv[beg + t] = v;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- batched BiCGStab
 // Systems are concatenated block-diagonally; system s owns rows
 // [sys_ptr[s], sys_ptr[s+1]).  Every system runs its own recurrence: scalars,
@@ -951,7 +1034,7 @@ struct shesolve_ctx {
   int64_t fnnz = 0;
   int *d_frp = nullptr, *d_fci = nullptr, *d_fdg = nullptr, *d_fpos = nullptr;
   double *d_f64 = nullptr;
-  float *d_f32 = nullptr;
+  int fmaxlen = 0;  // longest factor row
   std::vector<int> h_frp, h_fci, h_fdg;
   std::vector<int> blk_of_row;
   int *d_perm = nullptr;            // multicolour ordering: new -> old row
@@ -1257,7 +1340,7 @@ void free_matrix(shesolve_ctx *c) {
   dfree(c->d_rp); dfree(c->d_ci); dfree(c->d_v64); dfree(c->d_v32);
   if (!c->factor_is_A) { dfree(c->d_frp); dfree(c->d_fci); }
   c->d_frp = c->d_fci = nullptr;
-  dfree(c->d_fdg); dfree(c->d_fpos); dfree(c->d_f64); dfree(c->d_f32);
+  dfree(c->d_fdg); dfree(c->d_fpos); dfree(c->d_f64);
   dfree(c->d_perm); dfree(c->z64); dfree(c->z32);
   dfree(c->d_sv64); dfree(c->d_sv32); dfree(c->d_sblk); dfree(c->pb64); dfree(c->px64);
   c->solve_perm = false;
@@ -1619,6 +1702,7 @@ int analyse_and_factor(shesolve_ctx *c) {
       return 2;
     }
   }
+  c->fmaxlen = maxlen;
   if (maxlen > kMaxRowLDS) {
     set_err("ILU0: row length %d exceeds %d", maxlen, kMaxRowLDS);
     return 2;
@@ -1692,7 +1776,6 @@ int analyse_and_factor(shesolve_ctx *c) {
   HIPCHK(hipMemcpyAsync(c->d_fdg, c->h_fdg.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMalloc(&c->d_f64, sizeof(double) * std::max<int64_t>(c->fnnz, 1)));
-  HIPCHK(hipMalloc(&c->d_f32, sizeof(float) * std::max<int64_t>(c->fnnz, 1)));
   c->flpr = pick_lpr((double)c->fnnz / std::max(n, 1) / 2.0);
   // split pattern of the factor
   std::vector<int> Lrp(n + 1, 0), Urp(n + 1, 0), Lci, Uci;
@@ -1822,15 +1905,18 @@ int refactor(shesolve_ctx *c) {
     hipLaunchKernelGGL(k_gather, dim3(grid_for(c->fnnz, kBlock * 4)), dim3(kBlock), 0, c->stream, c->fnnz, c->d_fpos,
                        c->d_v64, c->d_f64);
   }
+  const double drop = c->prm.precond == SHESOLVE_PRECOND_ILUK ? c->prm.drop_tol : 0.0;
+  const bool short_rows = c->fmaxlen <= kShortRow;
   for (int l = 0; l < c->lvL.count(); ++l) {
     int cnt = c->lvL.ptr[l + 1] - c->lvL.ptr[l];
-    hipLaunchKernelGGL(k_ilu0_level, dim3(cnt), dim3(64), 0, c->stream, cnt, c->lvL.d_rows + c->lvL.ptr[l],
-                       c->d_frp, c->d_fci, c->d_fdg, c->d_f64,
-                       c->prm.precond == SHESOLVE_PRECOND_ILUK ? c->prm.drop_tol : 0.0);
+    if (short_rows)
+      hipLaunchKernelGGL(k_ilu0_short, dim3(grid_for((int64_t)cnt * kShortLanes, 256)), dim3(256), 0, c->stream, cnt,
+                         c->lvL.d_rows + c->lvL.ptr[l], c->d_frp, c->d_fci, c->d_fdg, c->d_f64, drop);
+    else
+      hipLaunchKernelGGL(k_ilu0_level, dim3(cnt), dim3(64), 0, c->stream, cnt, c->lvL.d_rows + c->lvL.ptr[l],
+                         c->d_frp, c->d_fci, c->d_fdg, c->d_f64, drop);
   }
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->fnnz, kBlock * 4)), dim3(kBlock), 0, c->stream,
-                     c->fnnz, c->d_f64, c->d_f32);
   hipLaunchKernelGGL(k_split, dim3(grid_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->n, c->d_frp, c->d_fdg,
                      c->d_f64, c->d_Lrp, c->d_Urp, c->d_L64, c->d_U64, c->d_dinv64);
   if (c->nL)
@@ -2190,7 +2276,7 @@ int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_r
         int st = 1;
         double ires = 0;
         const int cap = first_it < 0 ? inner_cap : std::max(100, 4 * first_it);
-        int it = bicgstab_dev<float, float>(c, sv32(c), c->d_f32, r32, dd, c->prm.inner_tol,
+        int it = bicgstab_dev<float, float>(c, sv32(c), (const float *)nullptr, r32, dd, c->prm.inner_tol,
                                             std::min(cap, c->prm.max_iters - r.iterations), &st, &ires, false);
         if (it < 0) return it;
         if (first_it < 0) first_it = it;
@@ -2449,7 +2535,7 @@ void bbicg_iter(shesolve_batch_ctx *b, T *x) {
   T **V;
   const TV *Av, *Mv;
   if constexpr (sizeof(T) == 8) { V = (T **)c->v64; Av = (const TV *)c->d_v64; Mv = (const TV *)c->d_f64; }
-  else { V = (T **)c->v32; Av = (const TV *)c->d_v32; Mv = (const TV *)c->d_f32; }
+  else { V = (T **)c->v32; Av = (const TV *)c->d_v32; Mv = nullptr; }
   T *r = V[0], *r0 = V[1], *p = V[2], *v = V[3], *ph = V[4], *s = V[5], *sh = V[6], *t = V[7];
   bupd<T>(b, 0, p, nullptr, nullptr, r, v, nullptr, nullptr);
   launch_precond<TV, T>(c, Mv, p, ph);
