@@ -95,6 +95,9 @@ def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0):
     fills, Ls = [], []
     for dev, cfg in D.batch_diodes(count, seed):
         F = shesolve.Fill(0)
+        # the reference's numbering: with cells numbered from both ends the
+        # schedules are half as deep, but one system of the 24 then needed 759
+        # iterations instead of at most 342 and the batch took longer
         F.build(dev, cfg)
         fills.append(F)
         Ls.append(cfg.L_max)
@@ -172,7 +175,7 @@ def bench_serving(count, rate=200.0, slots=8, seed=2024):
     reqs = serving.make_requests(count, rate=rate, seed=seed)
     srv = serving.Server(slots=slots, hbm_budget=200e9)
     def done(r, k, total):
-        log(f"[bench] serving: request {r.rid} ({r.desc}) done in {r.latency:.2f} s, "
+        log(f"[bench] serving: request {r.rid} ({r.desc}) done in {r.latency:.2f} s (service {r.finish - r.start:.2f} s), "
             f"{sum(rep['iterations'] for rep in r.reports)} iterations [{k}/{total}]")
 
     res = srv.run(reqs, realtime=True, on_done=done)

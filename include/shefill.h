@@ -54,6 +54,10 @@ typedef struct {
   int acoustic;            /* acoustic phonon scattering on/off */
   int optical;             /* optical phonon scattering on/off */
   int scale;               /* kinetic-energy unknown scaling on/off */
+  int ordering;            /* numbering of the even unknowns within each energy level: 0 = cells in mesh
+                              order (mapping.hpp:249-271); 1 = 1D devices only, cells from both ends
+                              toward the middle (0, N-1, 1, N-2, ...): the same system permuted, with
+                              about half the ILU level depth */
 } shefill_config;
 
 const char *shefill_last_error(void);

@@ -126,3 +126,66 @@ def test_density_update_matches_oracle(logarithmic, alpha):
     with pytest.raises(shesolve.SheSolveError):
         F.density_update(new, old, 0.0)
     F.close()
+
+
+@pytest.mark.parametrize("name", ["diode_L1", "diode_L3", "diode_L5"])
+def test_two_sided_ordering_is_a_permutation(name):
+    """ordering = 1 numbers the cells of each energy level from both ends of a
+    1D device: the system must be the oracle's with rows and columns permuted
+    (pattern bit-exact, values within the fill's 1e-10), its ILU level
+    schedule must be shallower, and a solve in that numbering must
+    give the same carrier density (1e-6) as the oracle's solution."""
+    import dataclasses
+    import shesolve
+    from oracle import oracle as O
+    s = oracle_system(name)
+    A, b, _, _ = s.reduced()
+    oei, oeo, _, _ = s.mapping()
+    F = shesolve.Fill(0)
+    F.build(s.dev, dataclasses.replace(s.cfg, ordering=1))
+    ei, eo = F.mapping()
+    assert np.array_equal(eo, oeo)
+    assert np.array_equal(ei >= 0, oei >= 0) and not np.array_equal(ei, oei)
+    # new row -> old row: the rows of each (cell, energy) node keep their harmonic order
+    starts = np.sort(oei[oei >= 0])
+    cnt = np.diff(np.append(starts, A.n))
+    count_of = dict(zip(starts.tolist(), cnt.tolist()))
+    perm = np.empty(A.n, np.int64)
+    for e_new, e_old in zip(ei[oei >= 0], oei[oei >= 0]):
+        k = count_of[int(e_old)]
+        perm[e_new:e_new + k] = np.arange(e_old, e_old + k)
+    assert np.array_equal(np.sort(perm), np.arange(A.n))
+    iperm = np.empty_like(perm)
+    iperm[perm] = np.arange(A.n)
+    rp, ci, v, rhs, _, _ = F.system()
+    lens = np.diff(A.row_ptr)[perm]
+    assert np.array_equal(np.diff(rp), lens)
+    for r in range(0, A.n, max(1, A.n // 2000)):
+        o = perm[r]
+        cols = iperm[A.col[A.row_ptr[o]:A.row_ptr[o + 1]]]
+        order = np.argsort(cols)
+        assert np.array_equal(ci[rp[r]:rp[r + 1]], cols[order])
+        assert np.max(np.abs(v[rp[r]:rp[r + 1]] - A.val[A.row_ptr[o]:A.row_ptr[o + 1]][order])) <= 1e-10
+    assert np.linalg.norm(rhs - b[perm]) <= 1e-11 * np.linalg.norm(b)
+    kind = dict(precond=shesolve.PRECOND_ILUK, fill_level=1) if name == "diode_L5" else \
+        dict(precond=shesolve.PRECOND_ILU0)
+    deep = []
+    for rows_cols_vals in ((A.row_ptr, A.col, A.val), (rp, ci, v)):
+        sv = shesolve.Solver(0, precision=shesolve.FP64, tol=1e-12, max_iters=3000, **kind)
+        sv.set_matrix(*rows_cols_vals)
+        deep.append(max(sv.levels()))
+        sv.close()
+    # the cell chain sets most of the depth; the energy couplings add a part
+    # that does not shrink (60-100 cell test diodes: 0.53-0.73 of the depth)
+    assert deep[1] <= 0.75 * deep[0], deep
+    sv = shesolve.Solver(0, precision=shesolve.FP64, tol=1e-12, max_iters=3000, **kind)
+    sv.set_matrix(rp, ci, v)
+    x, rep = sv.solve(rhs)
+    sv.close()
+    assert rep["status"] == 0
+    dens, _ = F.postproc(F.recover(x))
+    P = O.Precond(O.PC_ILUK, A, drop=0.0, fill=1 if name == "diode_L5" else 0)
+    x_ref, _, res = O.bicgstab(A, b, precond=P, tol=1e-12, max_iters=3000)
+    dens_ref, _ = s.postproc(s.finish(x_ref)[0])
+    assert np.linalg.norm(dens - dens_ref) <= 1e-6 * np.linalg.norm(dens_ref)
+    F.close()

@@ -733,9 +733,16 @@ int build_energies_orders_mapping(shefill_ctx *c) {
   // mapping (mapping.hpp:97-329): energy-major, then odd unknowns on facets
   c->idxc.assign((size_t)c->nc * NH, -1);
   c->idxf.assign((size_t)c->nf * NH, -1);
+  // cells of each energy level in mesh order, or from both ends of a 1D
+  // device toward the middle
+  std::vector<int> cells(c->nc);
+  for (int k = 0; k < c->nc; ++k) cells[k] = k;
+  if (cf.ordering == 1 && c->dim == 1)
+    for (int k = 0; k < c->nc; ++k) cells[k] = (k % 2 == 0) ? k / 2 : c->nc - 1 - k / 2;
   int64_t u = 0;
   for (size_t H = 0; H < NH; ++H)
-    for (int k = 0; k < c->nc; ++k) {
+    for (int kk = 0; kk < c->nc; ++kk) {
+      const int k = cells[kk];
       const size_t q = (size_t)k * NH + H;
       if (H == 0 || H == NH - 1) { c->idxc[q] = -1; continue; }
       if (avg_dos_host(c, false, k, (int)H) > 0 && kin_c(c, k, (int)H) < cf.max_kin_range) {
@@ -927,6 +934,9 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
     for (int s = 2; s < 4; ++s)
       if (c->nd_fn[(size_t)k * 4 + s] >= 0) blks[nb++] = {c->nd_base[c->nd_nb[(size_t)k * 4 + s]], 2, s};
     if (c->nd_inn[2 * k + 1] >= 0) blks[nb++] = {c->nd_base[c->nd_inn[2 * k + 1]], 1, 1};
+    // (already ascending in mesh-order numbering; the two-sided numbering of
+    // 1D cells reorders the left and right neighbours)
+    std::sort(blks, blks + nb, [](const Blk &a, const Blk &b) { return a.base < b.base; });
     // self block pattern
     std::vector<std::vector<char>> selfm(e, std::vector<char>(e, 0));
     const bool optical_out = c->nd_out[2 * k] >= 0 || c->nd_out[2 * k + 1] >= 0;
@@ -1247,6 +1257,7 @@ int shefill_build(shefill_handle h, const shefill_device *d, const shefill_confi
   if (d->nx < 2 || (d->dim == 2 && (d->ny < 2 || !d->yv))) { ferr("mesh too small"); return 2; }
   if (!(cfg->energy_spacing > 0) || !(cfg->tau > 0)) { ferr("energy spacing and tau must be positive"); return 3; }
   if (cfg->carrier != 0 && cfg->carrier != 1) { ferr("carrier must be 0 or 1"); return 3; }
+  if (cfg->ordering != 0 && cfg->ordering != 1) { ferr("ordering must be 0 or 1"); return 3; }
   FHIP(hipSetDevice(h->device));
   h->cfg = *cfg;
   if (int e = build_geometry(h, d)) return e;

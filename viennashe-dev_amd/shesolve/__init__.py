@@ -388,7 +388,7 @@ class FillDevice(C.Structure):
 class FillConfig(C.Structure):
     _fields_ = [("carrier", C.c_int), ("L_max", C.c_int), ("energy_spacing", C.c_double),
                 ("min_kin_range", C.c_double), ("max_kin_range", C.c_double), ("tau", C.c_double),
-                ("acoustic", C.c_int), ("optical", C.c_int), ("scale", C.c_int)]
+                ("acoustic", C.c_int), ("optical", C.c_int), ("scale", C.c_int), ("ordering", C.c_int)]
 
 
 def _fill_lib():
@@ -457,7 +457,7 @@ class Fill:
                        keep["dn"].ctypes.data, keep["dp"].ctypes.data, keep["psi"].ctypes.data,
                        keep["ct"].ctypes.data, dev.temperature)
         c = FillConfig(cfg.carrier, cfg.L_max, cfg.energy_spacing, cfg.min_kin_range, cfg.max_kin_range, cfg.tau,
-                       cfg.acoustic, cfg.optical, cfg.scale)
+                       cfg.acoustic, cfg.optical, cfg.scale, int(getattr(cfg, "ordering", 0)))
         _fcheck(_fill_lib().shefill_build(self._h, C.byref(d), C.byref(c)), "shefill_build")
         s = np.zeros(7, np.int64)
         _fcheck(_fill_lib().shefill_sizes(self._h, s), "shefill_sizes")

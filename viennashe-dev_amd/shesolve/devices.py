@@ -54,6 +54,7 @@ class SheConfig:
     acoustic: int = 1
     optical: int = 1
     scale: int = 1
+    ordering: int = 0                       # 1: cells of a 1D device from both ends (the fill's numbering)
 
 
 def _band_weight(T, ct):
