@@ -498,21 +498,25 @@ __global__ void k_colnorm(int n, const int *colcount, const int *diag_pos, const
   colfac[i] = 1.0 / cn;
 }
 // entries divided by their column normaliser, then the row by its 2-norm
-// (signed like the diagonal); kRowLanes lanes per row
+// (signed like the diagonal); kRowLanes lanes per row load and scale, the
+// squares are summed in column order as linalg_util.hpp:200-282 does
 __global__ void k_rownorm(int n, const int *rp, const int *ci, const int *diag_pos, const double *cn, double *val,
                           double *rhs) {
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int row = (int)(gt / kRowLanes), lane = (int)(gt % kRowLanes);
   const bool ok = row < n;
+  const int b = ok ? rp[row] : 0, len = ok ? rp[row + 1] - b : 0;
   double s = 0.0, vd = 0.0;
-  if (ok) {
-    vd = val[diag_pos[row]] / cn[row];
-    for (int j = rp[row] + lane; j < rp[row + 1]; j += kRowLanes) {
-      const double v = val[j] / cn[ci[j]];
-      s += v * v;
+  if (ok) vd = val[diag_pos[row]] / cn[row];
+  for (int j0 = 0; j0 < len; j0 += kRowLanes) {
+    const int j = j0 + lane;
+    const double v = j < len ? val[b + j] / cn[ci[b + j]] : 0.0;
+    const int m = min(kRowLanes, len - j0);
+    for (int q = 0; q < m; ++q) {
+      const double w = __shfl(v, q, kRowLanes);
+      s += w * w;
     }
   }
-  for (int off = kRowLanes / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kRowLanes);
   if (!ok) return;
   double rn = sqrt(s);
   if (vd < 0.0) rn *= -1.0;

@@ -163,9 +163,10 @@ def _transfer(x_old, map_old, shift, map_new, n_new):
     return x0
 
 
-def run_request(req: Request, tol=1e-10, max_iters=3000, device=0, clock=None, autotune=True):
+def run_request(req: Request, tol=1e-10, max_iters=1000, device=0, clock=None, autotune=True):
     """One request, start to finish, on the calling thread.  autotune=False
-    pins the launch shape, so that the result cannot depend on timings."""
+    pins the launch shape, so that the result cannot depend on timings.
+    max_iters: the reference's default iteration cap (solvers/config.hpp:62-63)."""
     import shesolve
     start = clock() if clock else time.perf_counter()
     F = shesolve.Fill(device)
