@@ -413,13 +413,32 @@ struct RowArgs {
 // One thread per row: value of every stored entry.  (Lanes per row measured
 // slower: every lane repeats the row's facet set-up and the entry loops diverge.)
 constexpr int kRowLanes = 8;  // lanes per row of the normalisation
+// With K <= kRowsLdsK the four coupling tables (4 directions each) are staged
+// in LDS first: their lookups in the elimination sums below go to LDS instead
+// of scattered cached loads.  The arithmetic is the same either way.
+constexpr int kRowsLdsK = 16;
 __global__ __launch_bounds__(kB) void k_rows(DevTables t, RowArgs a) {
+  extern __shared__ double tab[];
+  const int K = t.K;
+  if (K <= kRowsLdsK) {
+    const int kk = 4 * K * K;
+    for (int i = threadIdx.x; i < kk; i += blockDim.x) {
+      tab[i] = t.Ae[i];
+      tab[kk + i] = t.Be[i];
+      tab[2 * kk + i] = t.Ao[i];
+      tab[3 * kk + i] = t.Bo[i];
+    }
+    __syncthreads();
+    t.Ae = tab;
+    t.Be = tab + kk;
+    t.Ao = tab + 2 * kk;
+    t.Bo = tab + 3 * kk;
+  }
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= a.n) return;
   const int nd = a.row_node[row];
   const int c = a.nd_c[nd], base = a.nd_base[nd], Lc = a.nd_L[nd];
   const int r = row - base;
-  const int K = t.K;
   const int hr = t.ev_h[t.ev_off[Lc] + r];
   // facets with odd unknowns: slot data
   int fnk[4], nbk[4], dirk[4], side_c[4];
@@ -1196,7 +1215,8 @@ int launch_fill(shefill_ctx *c) {
   a.odiag = c->dodiag.p; a.diag0 = c->ddiag0.p; a.diagr = c->ddiagr.p; a.rhs0 = c->drhs0.p; a.inco = c->dinco.p;
   a.val = c->dval.p; a.rhs = c->drhs.p;
   if (n > 0) {
-    hipLaunchKernelGGL(k_rows, dim3(grid(n)), dim3(kB), 0, c->stream, t, a);
+    const size_t lds = t.K <= kRowsLdsK ? sizeof(double) * 16 * t.K * t.K : 0;
+    hipLaunchKernelGGL(k_rows, dim3(grid(n)), dim3(kB), lds, c->stream, t, a);
     hipLaunchKernelGGL(k_node_scaling, dim3(grid(n)), dim3(kB), 0, c->stream, (int)n, c->drow_node.p, c->dnd_c.p,
                        c->dnd_H.p, t, c->cfg.scale, c->dscal.p);
     hipLaunchKernelGGL(k_scale, dim3(std::min(grid(c->nnz), 32768)), dim3(kB), 0, c->stream, (int64_t)c->nnz,
