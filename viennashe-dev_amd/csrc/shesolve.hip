@@ -180,7 +180,11 @@ __device__ inline T block_sum_t(T v) {
 
 // MODE_SPMV:   y[row] = (b ? b[row] - s : s), dots of y with d0, d1 (per persistent block)
 // MODE_LSWEEP: y[row] = b[perm ? perm[row] : row] - s           (y == x: the factor-space vector)
-// MODE_USWEEP: y[row] = dinv[row] * (y[row] - s), out2[perm[row]] = y[row] when perm
+// MODE_USWEEP: y[row] = dinv[row] * ((row < c0 ? b[row] : y[row]) - s), out2[perm[row]] = y[row] when perm
+// c0: the first level of the forward sweep, stored as the contiguous range
+// [0, c0), has no entries, so its output would be b itself; that launch is
+// skipped, and the forward sweep's gathers and the backward sweep's own rows
+// below c0 read b instead.
 // xcd: 0 = row blocks strided over the whole grid; 1 = each XCD (blockIdx % 8)
 // works through its own contiguous eighth of the row blocks, so the gathered
 // vector entries of neighbouring rows are reused in that XCD's L2.
@@ -191,7 +195,8 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
                                                            const T *__restrict__ b, const int *__restrict__ perm,
                                                            const T *__restrict__ dinv, T *__restrict__ out2,
                                                            const T *__restrict__ d0, const T *__restrict__ d1,
-                                                           double *__restrict__ part, const int *__restrict__ pairs) {
+                                                           double *__restrict__ part, const int *__restrict__ pairs,
+                                                           int c0) {
   constexpr int kStreamNnz = ITEMS * kStreamThreads;
   __shared__ T prod[kStreamNnz];
   double a0 = 0.0, a1 = 0.0;
@@ -224,7 +229,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
 #pragma unroll
       for (int u = 0; u < kItems; ++u) {
         const int k = (int)threadIdx.x + u * kStreamThreads;
-        if (k < cnt) prod[k] = (T)vv[u] * x[cc[u]];
+        if (k < cnt) prod[k] = (T)vv[u] * ((MODE == MODE_LSWEEP && cc[u] < c0) ? b[cc[u]] : x[cc[u]]);
       }
       __syncthreads();
       for (int row = rs + (int)threadIdx.x; row < re; row += kStreamThreads) {
@@ -239,7 +244,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
         } else if (MODE == MODE_LSWEEP) {
           y[row] = (perm ? b[perm[row]] : b[row]) - acc;
         } else {
-          const T val = dinv[row] * (y[row] - acc);
+          const T val = dinv[row] * ((row < c0 ? b[row] : y[row]) - acc);
           y[row] = val;
           if (perm) out2[perm[row]] = val;
         }
@@ -248,7 +253,10 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
     } else {
       // one long row
       T acc = 0;
-      for (int k = k0 + (int)threadIdx.x; k < k1; k += kStreamThreads) acc += (T)v[k] * x[ci[k]];
+      for (int k = k0 + (int)threadIdx.x; k < k1; k += kStreamThreads) {
+        const int cc = ci[k];
+        acc += (T)v[k] * ((MODE == MODE_LSWEEP && cc < c0) ? b[cc] : x[cc]);
+      }
       acc = block_sum_t<T>(acc);
       if (threadIdx.x == 0) {
         const int row = rs;
@@ -260,7 +268,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
         } else if (MODE == MODE_LSWEEP) {
           y[row] = (perm ? b[perm[row]] : b[row]) - acc;
         } else {
-          const T val = dinv[row] * (y[row] - acc);
+          const T val = dinv[row] * ((row < c0 ? b[row] : y[row]) - acc);
           y[row] = val;
           if (perm) out2[perm[row]] = val;
         }
@@ -1068,6 +1076,7 @@ struct shesolve_ctx {
   int64_t nL = 0, nU = 0;
   struct LevelRun { int r0, r1, blk_off, blk_cnt; };  // contiguous level: rows [r0,r1), row blocks
   std::vector<LevelRun> runL, runU;
+  int c0 = 0;  // first forward level [0, c0) skipped by the sweeps (its output is the right-hand side)
   int *d_lblk = nullptr, *d_ublk = nullptr;
   Levels lvL, lvU;
   int lpr = 8, flpr = 8;
@@ -1181,13 +1190,13 @@ inline int stream_grid(int nblk, bool fixed) {
 template <typename TV, typename T, int MODE>
 void launch_stream(shesolve_ctx *c, int g, int nblk, const int *blocks, const int *rp, const int *ci, const TV *v,
                    const T *x, T *y, const T *b, const int *perm, const T *dinv, T *out2, const T *d0, const T *d1,
-                   double *part, const int *pairs = nullptr) {
+                   double *part, const int *pairs = nullptr, int c0 = 0) {
   // the sweeps' row blocks are built for kSweepItems entries per thread
   const int items = MODE == MODE_SPMV ? c->items : kSweepItems;
 #define ST_CASE(IT)                                                                                                \
   case IT:                                                                                                         \
     hipLaunchKernelGGL((k_stream<TV, T, MODE, IT>), dim3(g), dim3(kStreamThreads), 0, c->stream, nblk, c->xcd,     \
-                       blocks, rp, ci, v, x, y, b, perm, dinv, out2, d0, d1, part, pairs);                         \
+                       blocks, rp, ci, v, x, y, b, perm, dinv, out2, d0, d1, part, pairs, c0);                     \
     break;
   switch (items) { ST_CASE(4) ST_CASE(16) default: ST_CASE(8) }
 #undef ST_CASE
@@ -1280,13 +1289,16 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
   if (perm) {
     if constexpr (sizeof(T) == 8) z = (T *)c->z64; else z = (T *)c->z32;
   }
-  for (int l = 0; l < c->lvL.count(); ++l) {
+  // the first forward level, [0, c0), would copy the right-hand side: the
+  // later launches read it from `in` directly
+  const int c0 = perm ? 0 : c->c0;
+  for (int l = c0 > 0 ? 1 : 0; l < c->lvL.count(); ++l) {
     KTimer kt(c, 1);
     const auto &run = c->runL[l];
     if (run.blk_cnt > 0) {
       launch_stream<TV, T, MODE_LSWEEP>(c, stream_grid(run.blk_cnt, false), run.blk_cnt, c->d_lblk + run.blk_off,
                                         c->d_Lrp, c->d_Lci, Lv, z, z, in, perm, (const T *)nullptr, (T *)nullptr,
-                                        (const T *)nullptr, (const T *)nullptr, (double *)nullptr);
+                                        (const T *)nullptr, (const T *)nullptr, (double *)nullptr, nullptr, c0);
       continue;
     }
     int cnt = c->lvL.ptr[l + 1] - c->lvL.ptr[l];
@@ -1305,8 +1317,9 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     const auto &run = c->runU[l];
     if (run.blk_cnt > 0) {
       launch_stream<TV, T, MODE_USWEEP>(c, stream_grid(run.blk_cnt, false), run.blk_cnt, c->d_ublk + run.blk_off,
-                                        c->d_Urp, c->d_Uci, Uv, z, z, (const T *)nullptr, perm, (const T *)dinv, out,
-                                        (const T *)nullptr, (const T *)nullptr, (double *)nullptr);
+                                        c->d_Urp, c->d_Uci, Uv, z, z, c0 > 0 ? in : (const T *)nullptr, perm,
+                                        (const T *)dinv, out, (const T *)nullptr, (const T *)nullptr,
+                                        (double *)nullptr, nullptr, c0);
       continue;
     }
     int cnt = c->lvU.ptr[l + 1] - c->lvU.ptr[l];
@@ -1867,9 +1880,14 @@ int analyse_and_factor(shesolve_ctx *c) {
     }
   };
   std::vector<int> lblk, ublk;
+  c->c0 = 0;
   if (c->nseg == 0) {
     runs(c->lvL, Lrows, Lrp, c->runL, lblk);
     runs(c->lvU, Urows, Urp, c->runU, ublk);
+    bool all_runs = !c->runL.empty() && c->runL[0].r0 == 0 && Lrp[c->runL[0].r1] == 0;
+    for (const auto &r : c->runL) all_runs = all_runs && r.blk_cnt > 0;
+    for (const auto &r : c->runU) all_runs = all_runs && r.blk_cnt > 0;
+    if (all_runs && c->runL.size() > 1) c->c0 = c->runL[0].r1;
   }
   HIPCHK(hipMalloc(&c->d_Lrp, sizeof(int) * (n + 1)));
   HIPCHK(hipMalloc(&c->d_Urp, sizeof(int) * (n + 1)));
