@@ -95,9 +95,8 @@ def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0):
     fills, Ls = [], []
     for dev, cfg in D.batch_diodes(count, seed):
         F = shesolve.Fill(0)
-        # the reference's numbering: with cells numbered from both ends the
-        # schedules are half as deep, but one system of the 24 then needed 759
-        # iterations instead of at most 342 and the batch took longer
+        # the reference's numbering: numbered from both ends, the L = 5 systems
+        # (which set the batch time) converge worse with ILU(1)
         F.build(dev, cfg)
         fills.append(F)
         Ls.append(cfg.L_max)
