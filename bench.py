@@ -141,7 +141,7 @@ def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0):
     out = {
         "workload": f"{count} 1D diodes (seed {seed}): L in {{1,3,5}}, 100-1000 nm, dE 2-10 meV to 1.0 eV",
         "systems": count, "rows": info["rows"], "nnz": info["nnz"], "factor_nnz": info["factor_nnz"],
-        "deepest_level_schedule": info["seg_levels_max"],
+        "deepest_level_schedule": max(info["seg_levels_max"], info["levels"]),
         "solves_per_s": count * len(lat) / sum(lat), "ms_per_batch": 1e3 * sum(lat) / len(lat),
         "p50_ms": 1e3 * percentile(lat, 50), "p99_ms": 1e3 * percentile(lat, 99),
         "not_converged": sum(1 for r in reps if r["status"] != 0 or not r["rel_residual"] <= 1e-10),

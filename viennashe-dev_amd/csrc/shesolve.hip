@@ -497,48 +497,6 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int op, int cnt, const doub
 }
 
 // ---------------------------------------------------------------- level-scheduled ILU(0)
-// Forward sweep (unit lower triangle): out[row] = rhs[row] - sum_{k<diag} L[k] out[col[k]]
-// With a permutation (multicolour ordering) the sweeps run in the permuted
-// index space: the right-hand side is gathered through perm (new -> old).
-template <int LPR, typename TV, typename T>
-__global__ __launch_bounds__(kBlock) void k_lsolve(int cnt, const int *__restrict__ rows, const int *__restrict__ rp,
-                                                   const int *__restrict__ ci, const int *__restrict__ dg,
-                                                   const TV *__restrict__ fv, const T *__restrict__ rhs,
-                                                   T *__restrict__ out, const int *__restrict__ perm) {
-  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = gtid / LPR, lane = gtid & (LPR - 1);
-  if (g >= cnt) return;
-  const int row = rows[g];
-  T s = 0;
-  for (int k = rp[row] + lane; k < dg[row]; k += LPR) s += (T)fv[k] * out[ci[k]];
-#pragma unroll
-  for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
-  if (lane == 0) out[row] = (perm ? rhs[perm[row]] : rhs[row]) - s;
-}
-// Backward sweep (upper triangle with diagonal), in place on out.
-// Backward sweep; with a permutation the result is also scattered to the
-// natural order (out2[perm[row]]).
-template <int LPR, typename TV, typename T>
-__global__ __launch_bounds__(kBlock) void k_usolve(int cnt, const int *__restrict__ rows, const int *__restrict__ rp,
-                                                   const int *__restrict__ ci, const int *__restrict__ dg,
-                                                   const TV *__restrict__ fv, T *__restrict__ out,
-                                                   const int *__restrict__ perm, T *__restrict__ out2) {
-  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = gtid / LPR, lane = gtid & (LPR - 1);
-  if (g >= cnt) return;
-  const int row = rows[g];
-  const int d = dg[row];
-  T s = 0;
-  for (int k = d + 1 + lane; k < rp[row + 1]; k += LPR) s += (T)fv[k] * out[ci[k]];
-#pragma unroll
-  for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
-  if (lane == 0) {
-    const T v = (out[row] - s) / (T)fv[d];
-    out[row] = v;
-    if (perm) out2[perm[row]] = v;
-  }
-}
-
 // ILU(0) of the rows of one level; one 64-lane block per row, row kept in LDS.
 // Row-wise IKJ order (Saad, Iterative Methods, 2nd ed., Alg. 10.4).
 // drop > 0: threshold rule of ILUT inside the fixed pattern -- multipliers with
@@ -679,6 +637,7 @@ __global__ __launch_bounds__(256) void k_ilu0_short(int cnt, const int *__restri
 constexpr int kBatchParts = 32;  // partial sums per system and dot product
 constexpr int kStallWindow = 100; // FP32 inner solves: iterations without a 1 % gain before giving up
 constexpr int kBatchStallWindow = 40;  // the same in batches, where a stalled system holds up the batch
+constexpr double kStallKeep = 0.1;     // batches: a stalled FP32 correction that cut the residual 10x keeps FP32
 
 struct BSys {
   double rho, alpha, omega, beta;
@@ -2713,9 +2672,23 @@ int batch_solve_internal(shesolve_batch_ctx *b, int warm) {
           continue;
         }
         b->rep[s].rel_residual = rn[s] / bn[s];
-        if (b->rep[s].rel_residual <= prm.tol) { b->rep[s].status = 0; act[s] = 0; }
-        else if (inner_failed[s]) { fb[s] = 1; act[s] = 0; }
-        else if (k == prm.max_ir_steps || b->rep[s].iterations >= prm.max_iters) act[s] = 0;
+        if (b->rep[s].rel_residual <= prm.tol) {
+          b->rep[s].status = 0;
+          act[s] = 0;
+        } else if (inner_failed[s]) {
+          // a stalled FP32 solve whose correction still cut the FP64 residual
+          // tenfold keeps refining in FP32 (the next step restarts from the
+          // new residual); otherwise the system moves to FP64
+          if (rn[s] < kStallKeep * prev[s]) {
+            inner_failed[s] = 0;
+          } else {
+            fb[s] = 1;
+            act[s] = 0;
+          }
+        } else if (k == prm.max_ir_steps || b->rep[s].iterations >= prm.max_iters) {
+          if (b->rep[s].iterations < prm.max_iters) fb[s] = 1;  // refinement steps used up
+          act[s] = 0;
+        }
         any |= act[s] != 0;
       }
       if (anyrev) {
