@@ -4,8 +4,11 @@ sequences on one GPU.
 Each request's result must be the same alone and in the mix:
 - With the launch shape pinned (autotune off), the final iterate and carrier
   density are bit-identical: the requests share the device but no state.
-- With the autotuner on (the serving default), they agree within 1e-8
-  (solution, 2-norm relative) and 1e-6 (density), the solve bar.
+- With the autotuner on (the serving default), the launch shapes and with
+  them the rounding can differ, so the two runs are two solves to 1e-10 along
+  different paths: the solutions agree within 1e-7 (2-norm relative; a
+  condition number of ~1e3 times the 1e-10 residual bar) and the carrier
+  densities within the 1e-6 bar.
 - Every solve of every request reaches FP64 relative residual 1e-10.
 """
 import numpy as np
@@ -40,7 +43,7 @@ def test_results_same_alone_and_in_the_mix(autotune):
         if not autotune:
             assert np.array_equal(a.x, m.x) and np.array_equal(a.density, m.density)
         else:
-            assert np.linalg.norm(a.x - m.x) <= 1e-8 * np.linalg.norm(a.x)
+            assert np.linalg.norm(a.x - m.x) <= 1e-7 * np.linalg.norm(a.x)
             assert np.linalg.norm(a.density - m.density) <= 1e-6 * np.linalg.norm(a.density)
 
 
