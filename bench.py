@@ -115,7 +115,8 @@ def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0):
     dx = []
     for n in ns:
         p = ctypes.c_void_p()
-        assert hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(8 * n)) == 0
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(8 * n)) != 0:
+            raise SystemExit(f"hipMalloc of {8 * n} bytes failed")
         dx.append(p.value)
 
     def run(bs, k):
@@ -234,7 +235,8 @@ def main():
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
     d_x = ctypes.c_void_p()
-    assert hip.hipMalloc(ctypes.byref(d_x), ctypes.c_size_t(8 * n)) == 0
+    if hip.hipMalloc(ctypes.byref(d_x), ctypes.c_size_t(8 * n)) != 0:
+        raise SystemExit(f"hipMalloc of {8 * n} bytes failed")
 
     def run(precision, steps, warmup, live=False):
         sv = shesolve.Solver(0, precond=pc, precision=precision, tol=1e-10, max_iters=5000)

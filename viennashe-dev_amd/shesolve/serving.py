@@ -56,12 +56,17 @@ class DeviceBuffer:
 
     def upload(self, a):
         a = np.ascontiguousarray(a, np.float64)
-        assert a.size == self.n
-        assert _hiplib().hipMemcpy(self.ptr, a.ctypes.data, 8 * self.n, 1) == 0
+        if a.size != self.n:
+            raise ValueError(f"upload of {a.size} values into a buffer of {self.n}")
+        rc = _hiplib().hipMemcpy(self.ptr, a.ctypes.data, 8 * self.n, 1)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy host to device failed ({rc})")
 
     def download(self):
         a = np.empty(self.n)
-        assert _hiplib().hipMemcpy(a.ctypes.data, self.ptr, 8 * self.n, 2) == 0
+        rc = _hiplib().hipMemcpy(a.ctypes.data, self.ptr, 8 * self.n, 2)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy device to host failed ({rc})")
         return a
 
     def close(self):
