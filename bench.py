@@ -281,12 +281,32 @@ def main():
         sv32.set_profiling(False)
         log(f"[bench] kernel stats {prof}")
     fp64 = None
+    ir_history = None
     ref_iters = None
     if not args.no_fp64:
         sv64, lat64, reps64 = run(shesolve.FP64, max(1, min(args.steps, 3)), 1)
         fp64 = {"ms_per_solve": 1e3 * statistics.mean(lat64), "iterations": reps64[-1]["iterations"],
                 "precond": args.precond}
+        # error of every refinement step of the FP32+IR solve against the FP64
+        # solution of the same system at 1e-12 (untimed)
+        d_xref = ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(d_xref), ctypes.c_size_t(8 * n)) != 0:
+            raise SystemExit(f"hipMalloc of {8 * n} bytes failed")
+        sv64.set_params(tol=1e-12)
+        F.refill()
+        sv64.update_values_device(d_val)
+        rep_ref = sv64.solve_device(d_rhs, d_xref.value)
         sv64.close()
+        sv32.set_reference_solution(device_ptr=d_xref.value)
+        F.refill()
+        sv32.update_values_device(d_val)
+        rep_h = sv32.solve_device(d_rhs, d_x.value)
+        ir_history = {"reference": {"precision": "fp64", "precond": args.precond, "tol": 1e-12,
+                                    "iterations": rep_ref["iterations"], "rel_residual": rep_ref["rel_residual"]},
+                      "iterations": rep_h["iterations"], "steps": sv32.ir_history()}
+        sv32.set_reference_solution(None)
+        hip.hipFree(d_xref)
+        log(f"[bench] FP32+IR refinement history against the FP64 solution: {ir_history}")
         # the reference algorithm (natural-order ILU0 + BiCGStab, FP64) on the GPU: its
         # iteration count sizes the CPU baseline's extrapolation
         svr = shesolve.Solver(0, precond=shesolve.PRECOND_ILU0, precision=shesolve.FP64, tol=1e-10, max_iters=5000)
@@ -333,6 +353,7 @@ def main():
         "iterations": reps32[-1]["iterations"],
         "ir_steps": reps32[-1]["ir_steps"],
         "fp64": fp64,
+        "fp32_ir_history": ir_history,
         "speedup_vs_fp64": (fp64["ms_per_solve"] / (1e3 * total / len(lat32))) if fp64 else None,
         "roofline": {"bound": "hbm", "kernel": "k_stream<float,float,0,8> (CSR-stream SpMV, FP32)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -135,6 +135,31 @@ int shesolve_get_ordering(shesolve_handle h, int32_t *perm);
 /* Level-schedule statistics: [0] forward levels, [1] backward levels. */
 int shesolve_levels(shesolve_handle h, int64_t *out2);
 
+/* Refinement history of the last solve on this handle.  The reference solves
+ * in FP64 only (serial_linear_solver.hpp:129-136) and has no counterpart: this
+ * reports the FP32 + FP64-refinement path against an FP64 solution.
+ * One record per FP64 residual evaluation of an FP32_IR solve: the initial
+ * guess (step 0), then the iterate after each refinement step, then, when
+ * the refinement handed over to FP64 BiCGStab, the final iterate (kind 2).
+ * An FP64 solve leaves one record of kind 2. */
+typedef struct {
+  int32_t step;              /* refinement steps applied before this record */
+  int32_t inner_iterations;  /* FP32 BiCGStab iterations of that step (FP64 ones for kind 2) */
+  int32_t kind;              /* 0 initial guess or accepted FP32 correction, 1 correction that did
+                                not reduce the FP64 residual (undone), 2 after the FP64 solve */
+  int32_t reserved;
+  double rel_residual;       /* FP64 ||b - A x||_2 / ||b||_2 of the iterate */
+  double rel_error;          /* ||x - x_ref||_2 / ||x_ref||_2, NaN without a reference solution */
+} shesolve_ir_step;
+
+/* Reference solution x_ref (n values, host or device memory as on_device says)
+ * against which the history records the solution error; NULL clears it.  It
+ * is kept until cleared, a new matrix is set or the handle is destroyed; the
+ * error costs one extra vector pass per record, ordinary solves none. */
+int shesolve_set_reference_solution(shesolve_handle h, int64_t n, const double *x_ref, int on_device);
+/* Copies up to cap records of the last solve to out and their total number to *count. */
+int shesolve_ir_history(shesolve_handle h, int32_t cap, shesolve_ir_step *out, int32_t *count);
+
 /* Timing of one kernel class over the solves since profiling was switched on:
  * average device time per launch in microseconds and the launch count, for
  * the FP32 SpMV (kind 0), forward sweep level (1), backward sweep level (2),

@@ -319,3 +319,44 @@ def test_update_values_refactorises():
     x2, rep = sv.solve(b)
     assert rep["status"] == 0
     assert np.linalg.norm(2.0 * x2 - x1) <= 1e-8 * np.linalg.norm(x1)
+
+
+@pytest.mark.parametrize("name,kind", [("diode_L1", "ilu0"), ("diode_L5", "iluk"), ("mos2d_L3", "mc_ilu0")])
+def test_refinement_history_against_the_oracle(name, kind):
+    """Per-refinement-step report of the FP32 + FP64-refinement solve: each
+    record holds the FP64 residual of the iterate and, with a reference
+    solution set, its error against the oracle's FP64 solution (1e-13)."""
+    import shesolve
+    s = oracle_system(name)
+    A, b, _, _ = s.reduced()
+    x_ref = _ref_solution(name)
+    sv = _solver(precond=_pc(kind), precision=shesolve.FP32_IR, tol=1e-10, max_iters=5000, fill_level=1)
+    sv.set_matrix(A.row_ptr, A.col, A.val)
+    x0, rep0 = sv.solve(b)
+    h0 = sv.ir_history()
+    assert all(np.isnan(r["rel_error"]) for r in h0)
+    sv.set_reference_solution(x_ref)
+    x, rep = sv.solve(b)
+    h = sv.ir_history()
+    print(name, kind, rep, h)
+    # the reference only adds the error evaluation: same iterates, same residuals
+    assert np.array_equal(x, x0)
+    assert [r["rel_residual"] for r in h] == [r["rel_residual"] for r in h0]
+    # one record for the initial guess (x = 0: residual and error exactly 1),
+    # then one per refinement step
+    assert h[0]["step"] == 0 and h[0]["inner_iterations"] == 0 and h[0]["kind"] == 0
+    assert abs(h[0]["rel_residual"] - 1.0) <= 1e-12 and abs(h[0]["rel_error"] - 1.0) <= 1e-12
+    assert [r["step"] for r in h if r["kind"] != 2] == list(range(len([r for r in h if r["kind"] != 2])))
+    assert sum(r["inner_iterations"] for r in h) == rep["iterations"]
+    assert h[-1]["rel_residual"] == rep["rel_residual"] <= 1e-10
+    # the last error is the host's ||x - x_ref|| / ||x_ref||
+    err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    assert abs(h[-1]["rel_error"] - err) <= 1e-9 * err + 1e-15, (h[-1]["rel_error"], err)
+    assert err <= 1e-6
+    # accepted corrections reduce the FP64 residual
+    acc = [r["rel_residual"] for r in h if r["kind"] == 0]
+    assert all(r2 < r1 for r1, r2 in zip(acc, acc[1:]))
+    # cleared: errors are NaN again
+    sv.set_reference_solution(None)
+    sv.solve(b)
+    assert all(np.isnan(r["rel_error"]) for r in sv.ir_history())

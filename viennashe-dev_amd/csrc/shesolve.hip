@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <queue>
 #include <thread>
@@ -348,6 +349,23 @@ __global__ __launch_bounds__(kBlock) void k_copy_dot(int n, const T *__restrict_
   double vv[1] = {s};
   block_sum<1>(vv);
   if (threadIdx.x == 0) part[blockIdx.x] = vv[0];
+}
+
+// per-workgroup partial sums of (x - xr, x - xr) and (xr, xr)
+__global__ __launch_bounds__(kBlock) void k_err_partials(int n, const double *__restrict__ x,
+                                                         const double *__restrict__ xr, double *__restrict__ part) {
+  double a0 = 0, a1 = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double d = x[i] - xr[i];
+    a0 += d * d;
+    a1 += xr[i] * xr[i];
+  }
+  double vv[2] = {a0, a1};
+  block_sum<2>(vv);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = vv[0];
+    part[kPartStride + blockIdx.x] = vv[1];
+  }
 }
 
 template <typename T>
@@ -1016,6 +1034,11 @@ struct shesolve_ctx {
   float *d_sv32 = nullptr;
   int *d_sblk = nullptr;
   double *pb64 = nullptr, *px64 = nullptr;
+  // reference solution of the refinement history: natural order, its
+  // solve-space copy for a permuted matrix, and the one the running solve uses
+  double *d_xref = nullptr, *d_xref_s = nullptr;
+  const double *xref_cur = nullptr;
+  std::vector<shesolve_ir_step> ir_hist;  // records of the last solve
   double *z64 = nullptr;            // permuted-space work vectors of the sweeps
   float *z32 = nullptr;
   int ncolors = 0;
@@ -1315,6 +1338,8 @@ void free_matrix(shesolve_ctx *c) {
   dfree(c->d_fdg); dfree(c->d_fpos); dfree(c->d_f64);
   dfree(c->d_perm); dfree(c->z64); dfree(c->z32);
   dfree(c->d_sv64); dfree(c->d_sv32); dfree(c->d_sblk); dfree(c->pb64); dfree(c->px64);
+  dfree(c->d_xref); dfree(c->d_xref_s);
+  c->xref_cur = nullptr;
   c->solve_perm = false;
   c->s_rp = c->s_ci = c->s_blk = c->s_pairs = nullptr;
   c->s_nblk = 0;
@@ -2156,11 +2181,47 @@ int residual64(shesolve_ctx *c, const double *b, const double *x, double *r, dou
 
 int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_report *rep);
 
+// One record of the refinement history; the relative error against the
+// reference solution when one is set (partial sums added on the host in a
+// fixed order).
+int ir_record(shesolve_ctx *c, const double *x, int step, int inner, int kind, double rel_res) {
+  shesolve_ir_step rec{};
+  rec.step = step;
+  rec.inner_iterations = inner;
+  rec.kind = kind;
+  rec.rel_residual = rel_res;
+  rec.rel_error = std::numeric_limits<double>::quiet_NaN();
+  if (c->xref_cur) {
+    hipLaunchKernelGGL(k_err_partials, dim3(kGrid), dim3(kBlock), 0, c->stream, c->n, x, c->xref_cur, c->d_part);
+    std::vector<double> h(2 * kGrid);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_part, sizeof(double) * kGrid, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(h.data() + kGrid, c->d_part + kPartStride, sizeof(double) * kGrid, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double dd = 0, rr = 0;
+    for (int i = 0; i < kGrid; ++i) {
+      dd += h[i];
+      rr += h[kGrid + i];
+    }
+    rec.rel_error = rr > 0 ? std::sqrt(dd / rr) : (dd > 0 ? std::numeric_limits<double>::infinity() : 0.0);
+  }
+  c->ir_hist.push_back(rec);
+  return 0;
+}
+
 // Solve in the solve space: b and x go through the multicolour permutation
 // once per solve when the matrix is stored permuted.
 int solve_device(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_report *rep) {
+  c->ir_hist.clear();
+  c->xref_cur = c->d_xref;
   if (!c->solve_perm) return solve_core(c, b, x, warm, rep);
   const int g = grid_for(c->n, kBlock);
+  if (c->d_xref) {
+    if (!c->d_xref_s) HIPCHK(hipMalloc(&c->d_xref_s, sizeof(double) * c->n));
+    hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(kBlock), 0, c->stream, c->n, c->d_perm, c->d_xref,
+                       c->d_xref_s, 0);
+    c->xref_cur = c->d_xref_s;
+  }
   hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(kBlock), 0, c->stream, c->n, c->d_perm, b, c->pb64, 0);
   if (warm)
     hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(kBlock), 0, c->stream, c->n, c->d_perm, x, c->px64, 0);
@@ -2199,6 +2260,7 @@ int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_r
     r.iterations = it;
     r.status = st;
     r.rel_residual = res;
+    if (int e = ir_record(c, x, 0, it, 2, res)) return e;
   } else {
     // FP32 BiCGStab inside FP64 iterative refinement
     double *r64 = c->v64[8];
@@ -2220,10 +2282,13 @@ int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_r
       const int inner_cap = std::min(c->prm.max_iters, 1000);
       int first_it = -1;
       double *xprev = c->v64[6];  // idle during the FP32 inner solves
+      int last_inner = 0;
       for (int k = 0; k <= c->prm.max_ir_steps; ++k) {
         double rn = 0;
         if (int e = residual64(c, b, x, r64, &rn)) return e;
-        if (prev_rn >= 0 && !(rn < prev_rn)) {
+        const bool rejected = prev_rn >= 0 && !(rn < prev_rn);
+        if (int e = ir_record(c, x, r.ir_steps, last_inner, rejected ? 1 : 0, rn / bn)) return e;
+        if (rejected) {
           // the last correction did not reduce the FP64 residual: undo it and
           // finish with FP64 BiCGStab from the best iterate
           HIPCHK(hipMemcpyAsync(x, xprev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
@@ -2257,6 +2322,7 @@ int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_r
                                             std::min(cap, c->prm.max_iters - r.iterations), &st, &ires, false);
         if (it < 0) return it;
         if (first_it < 0) first_it = it;
+        last_inner = it;
         r.iterations += it;
         if (st != 0) inner_failed = true;
         HIPCHK(hipMemcpyAsync(c->d_norm, &rn, sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -2271,6 +2337,7 @@ int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_r
         r.iterations += it;
         r.status = st;
         r.rel_residual = res;
+        if (int e = ir_record(c, x, r.ir_steps, it, 2, res)) return e;
       }
     }
   }
@@ -3201,6 +3268,34 @@ int shesolve_levels(shesolve_handle h, int64_t *out2) {
   if (!out2) return 2;
   out2[0] = h->lvL.count();
   out2[1] = h->lvU.count();
+  return 0;
+}
+
+int shesolve_set_reference_solution(shesolve_handle h, int64_t n, const double *x_ref, int on_device) {
+  if (!h) { set_err("handle is NULL"); return 1; }
+  HIPCHK(hipSetDevice(h->device));
+  if (!x_ref) {
+    dfree(h->d_xref);
+    dfree(h->d_xref_s);
+    h->xref_cur = nullptr;
+    return 0;
+  }
+  if (!h->d_rp) { set_err("no matrix set"); return -1; }
+  if (n != h->n) { set_err("reference solution has %lld values, the matrix %d rows", (long long)n, h->n); return 2; }
+  if (!h->d_xref) HIPCHK(hipMalloc(&h->d_xref, sizeof(double) * h->n));
+  HIPCHK(hipMemcpyAsync(h->d_xref, x_ref, sizeof(double) * h->n,
+                        on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int shesolve_ir_history(shesolve_handle h, int32_t cap, shesolve_ir_step *out, int32_t *count) {
+  if (!h) { set_err("handle is NULL"); return 1; }
+  if (cap < 0) { set_err("cap must not be negative"); return 2; }
+  if (cap > 0 && !out) { set_err("out is NULL"); return 3; }
+  const int m = std::min<int>(cap, (int)h->ir_hist.size());
+  for (int i = 0; i < m; ++i) out[i] = h->ir_hist[i];
+  if (count) *count = (int32_t)h->ir_hist.size();
   return 0;
 }
 

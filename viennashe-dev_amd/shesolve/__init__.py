@@ -24,7 +24,7 @@ EXPORTS = (
     "shesolve_update_values_device", "shesolve_solve", "shesolve_solve_device", "shesolve_spmv",
     "shesolve_spmv_f32", "shesolve_precond_apply", "shesolve_factor_nnz", "shesolve_get_factors",
     "shesolve_levels", "shesolve_set_profiling", "shesolve_kernel_stats", "shesolve_time_spmv",
-    "shesolve_get_ordering", "shesolve_stream_config",
+    "shesolve_get_ordering", "shesolve_stream_config", "shesolve_set_reference_solution", "shesolve_ir_history",
     "shesolve_batch_create", "shesolve_batch_destroy", "shesolve_batch_set_params", "shesolve_batch_set_systems",
     "shesolve_batch_set_systems_device", "shesolve_batch_update_values", "shesolve_batch_update_values_device",
     "shesolve_batch_solve", "shesolve_batch_solve_device", "shesolve_batch_options", "shesolve_batch_info",
@@ -44,6 +44,15 @@ class Report(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class IRStep(C.Structure):
+    """One record of the refinement history (shesolve_ir_step)."""
+    _fields_ = [("step", C.c_int32), ("inner_iterations", C.c_int32), ("kind", C.c_int32), ("reserved", C.c_int32),
+                ("rel_residual", C.c_double), ("rel_error", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class SheSolveError(RuntimeError):
@@ -89,6 +98,8 @@ def lib():
     L.shesolve_set_profiling.argtypes = [C.c_void_p, C.c_int]
     L.shesolve_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.shesolve_time_spmv.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.shesolve_set_reference_solution.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int]
+    L.shesolve_ir_history.argtypes = [C.c_void_p, C.c_int32, C.POINTER(IRStep), C.POINTER(C.c_int32)]
     VP = C.POINTER(C.c_void_p)
     L.shesolve_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(Params)]
     L.shesolve_batch_destroy.argtypes = [C.c_void_p]
@@ -221,6 +232,26 @@ class Solver:
         t = np.zeros(9)
         _check(lib().shesolve_stream_config(self._h, C.byref(it), C.byref(xc), t), "shesolve_stream_config")
         return {"items": it.value, "xcd": xc.value, "tune_us": [round(v, 1) for v in t]}
+
+    def set_reference_solution(self, x_ref=None, device_ptr=None):
+        """Reference solution for the refinement history: a host array, or the
+        address of n FP64 values in device memory; neither clears it."""
+        if device_ptr is not None:
+            rc = lib().shesolve_set_reference_solution(self._h, self.n, device_ptr, 1)
+        elif x_ref is not None:
+            self._xref = np.ascontiguousarray(x_ref, np.float64)
+            rc = lib().shesolve_set_reference_solution(self._h, len(self._xref), self._xref.ctypes.data, 0)
+        else:
+            rc = lib().shesolve_set_reference_solution(self._h, 0, None, 0)
+        _check(rc, "shesolve_set_reference_solution")
+
+    def ir_history(self):
+        """Records of the last solve: initial guess, each refinement step, FP64 hand-over."""
+        cnt = C.c_int32()
+        _check(lib().shesolve_ir_history(self._h, 0, None, C.byref(cnt)), "shesolve_ir_history")
+        out = (IRStep * max(cnt.value, 1))()
+        _check(lib().shesolve_ir_history(self._h, cnt.value, out, C.byref(cnt)), "shesolve_ir_history")
+        return [out[i].as_dict() for i in range(cnt.value)]
 
     def ordering(self):
         p = np.zeros(self.n, np.int32)
