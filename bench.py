@@ -197,7 +197,7 @@ def main():
     ap.add_argument("--vx", type=int, default=161)
     ap.add_argument("--vy", type=int, default=81)
     ap.add_argument("--L", type=int, default=3)
-    ap.add_argument("--precond", default="mc_ilu0", choices=["ilu0", "block_ilu0", "mc_ilu0"])
+    ap.add_argument("--precond", default="mc_ilu0", choices=["ilu0", "block_ilu0", "mc_ilu0", "bmc_ilu0"])
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -229,7 +229,7 @@ def main():
     d_rp, d_ci, d_val, d_rhs = F.device_system()
     ranges = F.block_ranges()
     pc = {"ilu0": shesolve.PRECOND_ILU0, "block_ilu0": shesolve.PRECOND_BLOCK_ILU0,
-          "mc_ilu0": shesolve.PRECOND_MC_ILU0}[args.precond]
+          "mc_ilu0": shesolve.PRECOND_MC_ILU0, "bmc_ilu0": shesolve.PRECOND_BMC_ILU0}[args.precond]
     blk = ranges if args.precond == "block_ilu0" else None
 
     import ctypes

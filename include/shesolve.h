@@ -45,11 +45,16 @@ enum {
   SHESOLVE_PRECOND_MC_ILU0 = 4,     /* ILU(0) of the matrix in a greedy multicolour row
                                        ordering: a few dependency levels instead of
                                        hundreds; not in the reference, same stopping rule */
-  SHESOLVE_PRECOND_ILUK = 5         /* ILU(k) on the level-of-fill pattern, k = fill_level,
+  SHESOLVE_PRECOND_ILUK = 5,        /* ILU(k) on the level-of-fill pattern, k = fill_level,
                                        natural order, level-scheduled; drop_tol > 0 adds
                                        ILUT's threshold rule inside that pattern.  The
                                        GPU form of the ILUT(drop, fill) slot of the
                                        reference config (solvers/config.hpp:166-186) */
+  SHESOLVE_PRECOND_BMC_ILU0 = 6     /* ILU(0) of the matrix in a block multicolour ordering:
+                                       blocks of 32 consecutive rows coloured greedily, a
+                                       block's rows kept together in natural order; one
+                                       sweep launch per colour, each block solved by half
+                                       a wavefront.  Not in the reference, same stopping rule */
 };
 
 enum {

@@ -88,7 +88,8 @@ def test_ilu0_factors_match_oracle(name, kind):
 def _pc(kind):
     import shesolve
     return {"ilu0": shesolve.PRECOND_ILU0, "block_ilu0": shesolve.PRECOND_BLOCK_ILU0,
-            "mc_ilu0": shesolve.PRECOND_MC_ILU0, "iluk": shesolve.PRECOND_ILUK}[kind]
+            "mc_ilu0": shesolve.PRECOND_MC_ILU0, "iluk": shesolve.PRECOND_ILUK,
+            "bmc_ilu0": shesolve.PRECOND_BMC_ILU0}[kind]
 
 
 @pytest.mark.parametrize("name", ["diode_L1", "diode_L5", "mos2d_L3"])
@@ -115,22 +116,30 @@ def test_iluk_factors_match_oracle(name, k, drop):
     assert np.linalg.norm(z - z_ref) <= 1e-10 * np.linalg.norm(z_ref)
 
 
-def _colour_order(A):
+def _colour_order(A, block=1, with_colours=False):
     """Greedy colouring of the symmetrised pattern (explicit zeros included)
     in natural order, smallest free colour; rows sorted by colour (stable).
-    Returns (new -> old, the matrix permuted into that order as an oracle CSR)."""
+    With block > 1 the units coloured are blocks of `block` consecutive rows,
+    two blocks conflicting when an entry links them.
+    Returns (new -> old, the matrix permuted into that order as an oracle CSR
+    [, the colour of each natural row])."""
     import scipy.sparse as sp
     from oracle import oracle as O
     n = A.n
     pattern = sp.csr_matrix((np.ones(A.nnz), A.col, A.row_ptr), shape=(n, n))
     S = (pattern + pattern.T).tocsr()
-    colour = np.full(n, -1)
-    for i in range(n):
-        used = set(colour[S.indices[S.indptr[i]:S.indptr[i + 1]]].tolist())
+    nb = (n + block - 1) // block
+    R = sp.csr_matrix((np.ones(n), (np.arange(n) // block, np.arange(n))), shape=(nb, n))
+    SB = (R @ S @ R.T).tocsr()
+    bcol = np.full(nb, -1)
+    for i in range(nb):
+        nbrs = SB.indices[SB.indptr[i]:SB.indptr[i + 1]]
+        used = set(bcol[nbrs[nbrs != i]].tolist())
         q = 0
         while q in used:
             q += 1
-        colour[i] = q
+        bcol[i] = q
+    colour = bcol[np.arange(n) // block]
     perm = np.argsort(colour, kind="stable")
     iperm = np.empty(n, np.int64)
     iperm[perm] = np.arange(n)
@@ -143,6 +152,8 @@ def _colour_order(A):
         order = np.argsort(cols)
         ci[rp[p]:rp[p + 1]] = cols[order]
         va[rp[p]:rp[p + 1]] = A.val[A.row_ptr[i]:A.row_ptr[i + 1]][order]
+    if with_colours:
+        return perm, O.CSR(rp, ci, va), colour
     return perm, O.CSR(rp, ci, va)
 
 
@@ -188,9 +199,45 @@ def test_mc_ilu0_factors_match_oracle(name):
     assert np.linalg.norm(z - z_ref) <= 1e-10 * np.linalg.norm(z_ref)
 
 
+@pytest.mark.parametrize("name", ["diode_L1", "mos2d_L1", "mos2d_L3", "nin2d_L3"])
+def test_bmc_ilu0_factors_match_oracle(name):
+    """Block multicolour ILU0 is ILU(0) in the order of a greedy colouring of
+    blocks of 32 consecutive rows (natural block order, smallest free colour,
+    rows sorted by colour, stable).  The device stores exactly that order; its
+    factor must equal the oracle's ILU(0) of the reordered matrix (same entries,
+    values within 1e-10), blocks of one colour must be independent (no entry
+    links two of them), and M^-1 r must agree to 1e-10."""
+    from oracle import oracle as O
+    s = oracle_system(name)
+    A, b, _, _ = s.reduced()
+    sv = _solver(precond=_pc("bmc_ilu0"))
+    sv.set_matrix(A.row_ptr, A.col, A.val)
+    perm_s = sv.ordering().astype(np.int64)
+    perm_c, Ap, colour = _colour_order(A, block=32, with_colours=True)
+    assert np.array_equal(perm_s, perm_c)
+    # independence: an entry between two rows of one colour stays inside one
+    # block, and a block is one consecutive run of the order
+    rows = perm_c[np.repeat(np.arange(A.n), np.diff(Ap.row_ptr))]
+    cols = perm_c[Ap.col]
+    same_colour = colour[rows] == colour[cols]
+    assert np.all((rows // 32 == cols // 32)[same_colour])
+    blk_in_order = perm_c // 32
+    assert len(np.unique(blk_in_order)) == 1 + np.count_nonzero(np.diff(blk_in_order))
+    P = O.Precond(O.PC_ILU0, Ap)
+    F = P.factors()
+    rp, ci, v = sv.factors()
+    assert np.array_equal(rp, F.row_ptr) and np.array_equal(ci, F.col)
+    assert np.max(np.abs(v - F.val) / np.maximum(np.abs(F.val), 1e-300)) <= 1e-10
+    r = np.random.default_rng(6).standard_normal(A.n)
+    z = sv.precond_apply(r)
+    z_ref = np.empty(A.n)
+    z_ref[perm_c] = P.apply(r[perm_c])
+    assert np.linalg.norm(z - z_ref) <= 1e-10 * np.linalg.norm(z_ref)
+
+
 @pytest.mark.parametrize("name", SYSTEMS)
 @pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
-@pytest.mark.parametrize("kind", ["ilu0", "block_ilu0", "mc_ilu0", "iluk"])
+@pytest.mark.parametrize("kind", ["ilu0", "block_ilu0", "mc_ilu0", "iluk", "bmc_ilu0"])
 def test_solve_parity(name, precision, kind):
     import shesolve
     if kind != "iluk" and name in ILU0_DIVERGES:

@@ -164,6 +164,11 @@ constexpr int kSweepBlockRows = SHESOLVE_SWEEP_ROWS;
 static_assert(kSweepItems == 4 || kSweepItems == 8 || kSweepItems == 16, "sweep items: 4, 8 or 16");
 static_assert(kSweepBlockRows >= 1, "sweep rows per block");
 constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+// Block multicolour ordering: blocks of kBmcRows consecutive natural rows are
+// coloured, and a sweep gives each block half a wavefront (one lane per row)
+constexpr int kBmcRows = 32;
+constexpr int kBmcThreads = 128;   // 4 blocks per workgroup
+static_assert(kBmcRows == 32, "one block per half wavefront");
 
 enum { MODE_SPMV = 0, MODE_LSWEEP = 1, MODE_USWEEP = 2 };
 
@@ -284,6 +289,110 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
       part[blockIdx.x] = vv[0];
       part[kPartStride + blockIdx.x] = vv[1];
     }
+  }
+}
+
+// Value of `x` in lane j of this lane's half wavefront (j a compile-time
+// constant inside the unrolled step loops below).
+template <typename T>
+__device__ inline T half_bcast(T x, int j) {
+  const bool upper = (threadIdx.x & 32) != 0;
+  if constexpr (sizeof(T) == 4) {
+    const int a = __builtin_amdgcn_readlane(__float_as_int(x), j);
+    const int b = __builtin_amdgcn_readlane(__float_as_int(x), 32 + j);
+    return __int_as_float(upper ? b : a);
+  } else {
+    const long long bits = __double_as_longlong(x);
+    const int lo = (int)(bits & 0xffffffffll), hi = (int)(bits >> 32);
+    const int alo = __builtin_amdgcn_readlane(lo, j), ahi = __builtin_amdgcn_readlane(hi, j);
+    const int blo = __builtin_amdgcn_readlane(lo, 32 + j), bhi = __builtin_amdgcn_readlane(hi, 32 + j);
+    const unsigned long long r = upper ? (((unsigned long long)(unsigned)bhi << 32) | (unsigned)blo)
+                                       : (((unsigned long long)(unsigned)ahi << 32) | (unsigned)alo);
+    return __longlong_as_double((long long)r);
+  }
+}
+
+// One colour of a block multicolour sweep.  Blocks [b0, b1) of the solve-space
+// block starts bstart, each at most kBmcRows rows, independent of every other
+// block of the colour.  Half a wavefront per block, lane hl owns row r0 + hl:
+//   forward (unit L):  z[row] = in[row] - sum_k L[k] z[ci[k]]
+//   backward:          z[row] = dinv[row] * (z[row] - sum_k U[k] z[ci[k]])
+// A row's entries are loaded kBmcLoad at a time, all loads of a batch in
+// flight together; entries outside the block (other colours, already final)
+// are subtracted in column order.  Entries inside the block go to the lane's
+// own slots of a packed strict triangle in LDS; then all lanes walk the
+// block's rows in elimination order, the finished value of row j broadcast
+// from lane j and subtracted by the rows that couple to it.
+constexpr int kBmcLoad = 8;
+constexpr int kBmcTri = kBmcRows * (kBmcRows - 1) / 2;  // strict triangle of a block
+// slot of (column j, row r) in the packed triangle: forward rows r > j,
+// backward rows r < j; consecutive rows of one column are adjacent (the lanes
+// of one step read consecutive words)
+__device__ constexpr int bmc_slot_fwd(int j, int r) { return j * (kBmcRows - 1) - j * (j - 1) / 2 + (r - j - 1); }
+__device__ constexpr int bmc_slot_bwd(int j, int r) { return j * (j - 1) / 2 + r; }
+
+template <bool BACKWARD, typename TV, typename T>
+__global__ __launch_bounds__(kBmcThreads) void k_bmc_sweep(int b0, int b1, const int *__restrict__ bstart,
+                                                           const int *__restrict__ rp, const int *__restrict__ ci,
+                                                           const TV *__restrict__ v, const TV *__restrict__ dinv,
+                                                           const T *__restrict__ in, T *z) {
+  __shared__ T tile[(kBmcThreads / kBmcRows) * kBmcTri];
+  const int sub = (int)threadIdx.x / kBmcRows, hl = (int)threadIdx.x % kBmcRows;
+  T *tl = tile + sub * kBmcTri;
+  const int blk = b0 + (int)blockIdx.x * (kBmcThreads / kBmcRows) + sub;
+  int r0 = 0, nr = 0;
+  if (blk < b1) {
+    r0 = bstart[blk];
+    nr = bstart[blk + 1] - r0;
+  }
+  // zero the lane's own slots
+#pragma unroll
+  for (int j = 0; j < kBmcRows; ++j) {
+    if (!BACKWARD && hl > j) tl[bmc_slot_fwd(j, hl)] = (T)0;
+    if (BACKWARD && hl < j) tl[bmc_slot_bwd(j, hl)] = (T)0;
+  }
+  const int row = r0 + hl;
+  T acc = 0, d = 1;
+  if (hl < nr) {
+    acc = BACKWARD ? z[row] : in[row];
+    if (BACKWARD) d = (T)dinv[row];
+    const int k1 = rp[row + 1];
+    for (int kb = rp[row]; kb < k1; kb += kBmcLoad) {
+      int cc[kBmcLoad];
+      TV vv[kBmcLoad];
+      T zz[kBmcLoad];
+#pragma unroll
+      for (int u = 0; u < kBmcLoad; ++u)
+        if (kb + u < k1) {
+          cc[u] = ci[kb + u];
+          vv[u] = v[kb + u];
+        }
+#pragma unroll
+      for (int u = 0; u < kBmcLoad; ++u)
+        if (kb + u < k1) zz[u] = ((unsigned)(cc[u] - r0) < (unsigned)nr) ? (T)0 : z[cc[u]];
+#pragma unroll
+      for (int u = 0; u < kBmcLoad; ++u)
+        if (kb + u < k1) {
+          const int off = cc[u] - r0;
+          if ((unsigned)off < (unsigned)nr) tl[BACKWARD ? bmc_slot_bwd(off, hl) : bmc_slot_fwd(off, hl)] = (T)vv[u];
+          else acc -= (T)vv[u] * zz[u];
+        }
+    }
+  }
+  if (!BACKWARD) {
+#pragma unroll
+    for (int j = 0; j < kBmcRows; ++j) {
+      const T zj = half_bcast(acc, j);
+      if (hl > j) acc -= tl[bmc_slot_fwd(j, hl)] * zj;
+    }
+    if (hl < nr) z[row] = acc;
+  } else {
+#pragma unroll
+    for (int j = kBmcRows - 1; j >= 0; --j) {
+      const T zj = half_bcast(acc * d, j);
+      if (hl < j) acc -= tl[bmc_slot_bwd(j, hl)] * zj;
+    }
+    if (hl < nr) z[row] = acc * d;
   }
 }
 
@@ -1043,6 +1152,10 @@ struct shesolve_ctx {
   float *z32 = nullptr;
   int ncolors = 0;
   std::vector<int> h_perm, h_colour;
+  // block multicolour factor: solve-space block starts, and per colour the
+  // range [bmc_cptr[q], bmc_cptr[q+1]) of its blocks
+  int *d_bmc_start = nullptr;
+  std::vector<int> bmc_cptr;
   // CSR-stream row blocks of A and the tuned launch shape
   int *d_ablk = nullptr;
   int nablk = 0;
@@ -1240,6 +1353,24 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
   }
   const int L = c->flpr;
   const int *perm = c->solve_perm ? nullptr : c->d_perm;
+  if (!c->bmc_cptr.empty() && !perm) {
+    // block multicolour factor in the solve space: one launch per colour and
+    // direction, forward in ascending and backward in descending colours
+    const int nq = (int)c->bmc_cptr.size() - 1, per = kBmcThreads / kBmcRows;
+    for (int q = 0; q < nq; ++q) {
+      KTimer kt(c, 1);
+      const int b0 = c->bmc_cptr[q], b1 = c->bmc_cptr[q + 1];
+      hipLaunchKernelGGL((k_bmc_sweep<false, TV, T>), dim3((b1 - b0 + per - 1) / per), dim3(kBmcThreads), 0,
+                         c->stream, b0, b1, c->d_bmc_start, c->d_Lrp, c->d_Lci, Lv, dinv, in, out);
+    }
+    for (int q = nq - 1; q >= 0; --q) {
+      KTimer kt(c, 2);
+      const int b0 = c->bmc_cptr[q], b1 = c->bmc_cptr[q + 1];
+      hipLaunchKernelGGL((k_bmc_sweep<true, TV, T>), dim3((b1 - b0 + per - 1) / per), dim3(kBmcThreads), 0,
+                         c->stream, b0, b1, c->d_bmc_start, c->d_Urp, c->d_Uci, Uv, dinv, in, out);
+    }
+    return;
+  }
   if (c->nseg > 0 && !perm) {
     KTimer kt(c, 1);
     const int W = sweep_window<T>();
@@ -1340,6 +1471,8 @@ void free_matrix(shesolve_ctx *c) {
   dfree(c->d_sv64); dfree(c->d_sv32); dfree(c->d_sblk); dfree(c->pb64); dfree(c->px64);
   dfree(c->d_xref); dfree(c->d_xref_s);
   c->xref_cur = nullptr;
+  dfree(c->d_bmc_start);
+  c->bmc_cptr.clear();
   c->solve_perm = false;
   c->s_rp = c->s_ci = c->s_blk = c->s_pairs = nullptr;
   c->s_nblk = 0;
@@ -1619,11 +1752,14 @@ int analyse_and_factor(shesolve_ctx *c) {
         }
       c->h_frp[i + 1] = (int)c->h_fci.size();
     }
-  } else if (c->prm.precond == SHESOLVE_PRECOND_MC_ILU0) {
+  } else if (c->prm.precond == SHESOLVE_PRECOND_MC_ILU0 || c->prm.precond == SHESOLVE_PRECOND_BMC_ILU0) {
     // greedy colouring of the rows in natural order on the symmetrised
     // pattern: rows of one colour are mutually independent in both
     // triangles, so each colour is one level of both sweeps; rows sorted by
-    // colour (stable) give the multicolour ordering new -> old
+    // colour (stable) give the multicolour ordering new -> old.  The block
+    // form colours blocks of kBmcRows consecutive rows instead (two blocks
+    // conflict when an entry links them); a block keeps its rows together and
+    // in natural order, and the blocks of one colour are independent.
     std::vector<int> trp(n + 1, 0), tci(c->h_ci.size());
     for (int i = 0; i < n; ++i)
       for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) trp[c->h_ci[k] + 1]++;
@@ -1636,19 +1772,40 @@ int analyse_and_factor(shesolve_ctx *c) {
     std::vector<int> colour(n, -1);
     std::vector<int> mark(1024, -1);
     int ncol = 0;
-    for (int i = 0; i < n; ++i) {
-      for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) {
-        const int cj = colour[c->h_ci[k]];
-        if (cj >= 0) mark[cj] = i;
+    if (c->prm.precond == SHESOLVE_PRECOND_MC_ILU0) {
+      for (int i = 0; i < n; ++i) {
+        for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) {
+          const int cj = colour[c->h_ci[k]];
+          if (cj >= 0) mark[cj] = i;
+        }
+        for (int k = trp[i]; k < trp[i + 1]; ++k) {
+          const int cj = colour[tci[k]];
+          if (cj >= 0) mark[cj] = i;
+        }
+        int q = 0;
+        while (q < 1023 && mark[q] == i) ++q;
+        colour[i] = q;
+        ncol = std::max(ncol, q + 1);
       }
-      for (int k = trp[i]; k < trp[i + 1]; ++k) {
-        const int cj = colour[tci[k]];
-        if (cj >= 0) mark[cj] = i;
+    } else {
+      const int nb = (n + kBmcRows - 1) / kBmcRows;
+      for (int bi = 0; bi < nb; ++bi) {
+        const int i0 = bi * kBmcRows, i1 = std::min(n, i0 + kBmcRows);
+        for (int i = i0; i < i1; ++i) {
+          for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) {
+            const int j = c->h_ci[k];
+            if ((j < i0 || j >= i1) && colour[j] >= 0) mark[colour[j]] = bi;
+          }
+          for (int k = trp[i]; k < trp[i + 1]; ++k) {
+            const int j = tci[k];
+            if ((j < i0 || j >= i1) && colour[j] >= 0) mark[colour[j]] = bi;
+          }
+        }
+        int q = 0;
+        while (q < 1023 && mark[q] == bi) ++q;
+        for (int i = i0; i < i1; ++i) colour[i] = q;
+        ncol = std::max(ncol, q + 1);
       }
-      int q = 0;
-      while (q < 1023 && mark[q] == i) ++q;
-      colour[i] = q;
-      ncol = std::max(ncol, q + 1);
     }
     c->ncolors = ncol;
     c->h_colour = colour;
@@ -1744,6 +1901,26 @@ int analyse_and_factor(shesolve_ctx *c) {
         Uptr.push_back((int)Urows.size());
       }
     }
+  }
+  if (!perm.empty() && c->prm.precond == SHESOLVE_PRECOND_BMC_ILU0) {
+    // blocks in the solve space: a block's rows stay consecutive (one colour,
+    // natural order), so a block starts where the natural block changes
+    std::vector<int> bst;
+    c->bmc_cptr.assign(1, 0);
+    for (int p = 0; p < n; ++p) {
+      if (p > 0 && perm[p] / kBmcRows == perm[p - 1] / kBmcRows) continue;
+      if (p > 0 && c->h_colour[perm[p]] != c->h_colour[perm[p - 1]]) c->bmc_cptr.push_back((int)bst.size());
+      bst.push_back(p);
+    }
+    c->bmc_cptr.push_back((int)bst.size());
+    bst.push_back(n);
+    for (size_t k = 0; k + 1 < bst.size(); ++k)
+      if (bst[k + 1] - bst[k] > kBmcRows) {
+        set_err("block multicolour ordering: block %zu holds %d rows", k, bst[k + 1] - bst[k]);
+        return -1;
+      }
+    HIPCHK(hipMalloc(&c->d_bmc_start, sizeof(int) * bst.size()));
+    HIPCHK(hipMemcpyAsync(c->d_bmc_start, bst.data(), sizeof(int) * bst.size(), hipMemcpyHostToDevice, c->stream));
   }
   c->lvL.ptr = Lptr;
   c->lvU.ptr = Uptr;
@@ -2830,8 +3007,8 @@ int batch_set_systems(shesolve_batch_ctx *b, bool device_src, int count, const i
   HIPCHK(hipSetDevice(c->device));
   if (count <= 0) { set_err("count must be positive"); return 2; }
   if (!n) { set_err("n is NULL"); return 3; }
-  if (c->prm.precond == SHESOLVE_PRECOND_MC_ILU0) {
-    set_err("the multicolour ordering is not available for batches (use ILU0, BLOCK_ILU0 or ILUK)");
+  if (c->prm.precond == SHESOLVE_PRECOND_MC_ILU0 || c->prm.precond == SHESOLVE_PRECOND_BMC_ILU0) {
+    set_err("the multicolour orderings are not available for batches (use ILU0, BLOCK_ILU0 or ILUK)");
     return -3;
   }
   if (!rp) { set_err("row_ptr is NULL"); return 4; }
@@ -3012,7 +3189,7 @@ static int check_params(const shesolve_params *p) {
   if (p->max_iters <= 0) { set_err("max_iters must be positive"); return 1; }
   if (p->precond != SHESOLVE_PRECOND_NONE && p->precond != SHESOLVE_PRECOND_ILU0 &&
       p->precond != SHESOLVE_PRECOND_BLOCK_ILU0 && p->precond != SHESOLVE_PRECOND_MC_ILU0 &&
-      p->precond != SHESOLVE_PRECOND_ILUK) {
+      p->precond != SHESOLVE_PRECOND_BMC_ILU0 && p->precond != SHESOLVE_PRECOND_ILUK) {
     set_err("unknown preconditioner %d", p->precond);
     return 1;
   }

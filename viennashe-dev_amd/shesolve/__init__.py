@@ -14,7 +14,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBPATH = os.path.join(PKG, "libshesolve.so")
 
-PRECOND_NONE, PRECOND_ILU0, PRECOND_BLOCK_ILU0, PRECOND_MC_ILU0, PRECOND_ILUK = 0, 1, 3, 4, 5
+PRECOND_NONE, PRECOND_ILU0, PRECOND_BLOCK_ILU0, PRECOND_MC_ILU0, PRECOND_ILUK, PRECOND_BMC_ILU0 = 0, 1, 3, 4, 5, 6
 FP64, FP32_IR = 0, 1
 
 # symbols declared in include/shesolve.h
