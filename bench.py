@@ -197,7 +197,7 @@ def main():
     ap.add_argument("--vx", type=int, default=161)
     ap.add_argument("--vy", type=int, default=81)
     ap.add_argument("--L", type=int, default=3)
-    ap.add_argument("--precond", default="mc_ilu0", choices=["ilu0", "block_ilu0", "mc_ilu0", "bmc_ilu0"])
+    ap.add_argument("--precond", default="bmc_ilu0", choices=["ilu0", "block_ilu0", "mc_ilu0", "bmc_ilu0"])
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -326,10 +326,10 @@ def main():
     b32_solve = b32 + n * 4
     achieved = b32_solve / (spmv32_us * 1e-6) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_stream_spmv_mosfet2d_161x81_L3_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"k_stream_spmv_mosfet2d_161x81_L3_{args.precond}_traffic.json")
     if os.path.exists(tpath):
         tr = json.load(open(tpath))
-        if tr.get("n") == n and tr.get("nnz") == nnz:
+        if tr.get("n") == n and tr.get("nnz") == nnz and tr.get("precond") == args.precond:
             traffic = tr["traffic_bytes"]
     result = {
         "metric": "SHE solves/s at rel. residual 1e-10 (FP32+IR), p50/p99 latency",
@@ -359,7 +359,8 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "avg_us": spmv32_us, "launches": spmv32_launches,
                      "algorithmic_bytes": b32_solve,
-                     "traffic_source": "profiles/k_stream_spmv_mosfet2d_161x81_L3_pmc.md" if traffic else None,
+                     "traffic_source": (f"profiles/k_stream_spmv_mosfet2d_161x81_L3_{args.precond}_pmc.md"
+                                        if traffic else None),
                      "plain_launch_us": plain32_us,
                      "plain_launch_gbs": b32 / (plain32_us * 1e-6) / 1e9,
                      "fp64_spmv_us": spmv64_us,

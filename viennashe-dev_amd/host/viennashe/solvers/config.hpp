@@ -49,7 +49,7 @@ class linear_solver_config : public linear_solver_ids {
  public:
   typedef std::vector<std::pair<std::size_t, std::size_t> > block_preconditioner_boundaries_container;
   enum gpu_precision_type { gpu_fp64 = 0, gpu_fp32_refined = 1 };
-  enum gpu_preconditioner_type { gpu_ilu0 = 1, gpu_block_ilu0 = 3, gpu_mc_ilu0 = 4, gpu_iluk = 5 };
+  enum gpu_preconditioner_type { gpu_ilu0 = 1, gpu_block_ilu0 = 3, gpu_mc_ilu0 = 4, gpu_iluk = 5, gpu_bmc_ilu0 = 6 };
 
   linear_solver_config()
       : id_(serial_linear_solver), tol_(1e-13), max_iters_(1000), ilut_entries_(60), ilut_drop_tol_(1e-4),

@@ -92,6 +92,8 @@ std::vector<double> solve(viennashe::math::sparse_matrix<NumericT> const &A, std
     p.drop_tol = config.gpu_drop_tolerance();
   } else if (config.gpu_preconditioner() == linear_solver_config::gpu_mc_ilu0) {
     p.precond = SHESOLVE_PRECOND_MC_ILU0;
+  } else if (config.gpu_preconditioner() == linear_solver_config::gpu_bmc_ilu0) {
+    p.precond = SHESOLVE_PRECOND_BMC_ILU0;
   } else {
     p.precond = SHESOLVE_PRECOND_ILU0;
   }

@@ -132,14 +132,14 @@ def footprint_estimate(dev: D.Device, cfg: D.SheConfig) -> float:
 
 
 def _solver_params(dev, cfg):
-    """2D meshes: multicolour ILU0, FP32 + FP64 refinement (bandwidth-bound
-    sweeps over a few colours).  1D chains: natural-order ILU(0), ILU(1) where
+    """2D meshes: block multicolour ILU0, FP32 + FP64 refinement (one sweep
+    launch per colour, blocks of consecutive rows solved inside).  1D chains: natural-order ILU(0), ILU(1) where
     ILU(0) diverges (L >= 5), in FP64 -- their sweeps are bound by the
     dependency depth of the chain, not by bytes, and FP32 corrections of these
     systems stall after the first refinement step."""
     import shesolve
     if dev.dim > 1:
-        return dict(precond=shesolve.PRECOND_MC_ILU0, precision=shesolve.FP32_IR)
+        return dict(precond=shesolve.PRECOND_BMC_ILU0, precision=shesolve.FP32_IR)
     return dict(precond=shesolve.PRECOND_ILUK, fill_level=1 if cfg.L_max >= 5 else 0, precision=shesolve.FP64)
 
 
