@@ -325,32 +325,36 @@ __device__ inline T half_bcast(T x, int j) {
 // from lane j and subtracted by the rows that couple to it.
 constexpr int kBmcLoad = 8;
 constexpr int kBmcTri = kBmcRows * (kBmcRows - 1) / 2;  // strict triangle of a block
+// slots per block: the triangle plus one slot that always holds zero, read by
+// the lanes that own no slot of a step (so the steps need no branches)
+constexpr int kBmcSlots = kBmcTri + 1;
 // slot of (column j, row r) in the packed triangle: forward rows r > j,
 // backward rows r < j; consecutive rows of one column are adjacent (the lanes
 // of one step read consecutive words)
 __device__ constexpr int bmc_slot_fwd(int j, int r) { return j * (kBmcRows - 1) - j * (j - 1) / 2 + (r - j - 1); }
 __device__ constexpr int bmc_slot_bwd(int j, int r) { return j * (j - 1) / 2 + r; }
+// lane r's slot of step j, or the zero slot kBmcTri when it owns none
+__device__ constexpr int bmc_own(bool backward, int j, int r) {
+  return backward ? (r < j ? bmc_slot_bwd(j, r) : kBmcTri) : (r > j ? bmc_slot_fwd(j, r) : kBmcTri);
+}
 
 template <bool BACKWARD, typename TV, typename T>
 __global__ __launch_bounds__(kBmcThreads) void k_bmc_sweep(int b0, int b1, const int *__restrict__ bstart,
                                                            const int *__restrict__ rp, const int *__restrict__ ci,
                                                            const TV *__restrict__ v, const TV *__restrict__ dinv,
                                                            const T *__restrict__ in, T *z) {
-  __shared__ T tile[(kBmcThreads / kBmcRows) * kBmcTri];
+  __shared__ T tile[(kBmcThreads / kBmcRows) * kBmcSlots];
   const int sub = (int)threadIdx.x / kBmcRows, hl = (int)threadIdx.x % kBmcRows;
-  T *tl = tile + sub * kBmcTri;
+  T *tl = tile + sub * kBmcSlots;
   const int blk = b0 + (int)blockIdx.x * (kBmcThreads / kBmcRows) + sub;
   int r0 = 0, nr = 0;
   if (blk < b1) {
     r0 = bstart[blk];
     nr = bstart[blk + 1] - r0;
   }
-  // zero the lane's own slots
+  // zero the lane's own slots (the zero slot where the lane owns none)
 #pragma unroll
-  for (int j = 0; j < kBmcRows; ++j) {
-    if (!BACKWARD && hl > j) tl[bmc_slot_fwd(j, hl)] = (T)0;
-    if (BACKWARD && hl < j) tl[bmc_slot_bwd(j, hl)] = (T)0;
-  }
+  for (int j = 0; j < kBmcRows; ++j) tl[bmc_own(BACKWARD, j, hl)] = (T)0;
   const int row = r0 + hl;
   T acc = 0, d = 1;
   if (hl < nr) {
@@ -383,14 +387,14 @@ __global__ __launch_bounds__(kBmcThreads) void k_bmc_sweep(int b0, int b1, const
 #pragma unroll
     for (int j = 0; j < kBmcRows; ++j) {
       const T zj = half_bcast(acc, j);
-      if (hl > j) acc -= tl[bmc_slot_fwd(j, hl)] * zj;
+      acc -= tl[bmc_own(false, j, hl)] * zj;
     }
     if (hl < nr) z[row] = acc;
   } else {
 #pragma unroll
     for (int j = kBmcRows - 1; j >= 0; --j) {
       const T zj = half_bcast(acc * d, j);
-      if (hl < j) acc -= tl[bmc_slot_bwd(j, hl)] * zj;
+      acc -= tl[bmc_own(true, j, hl)] * zj;
     }
     if (hl < nr) z[row] = acc * d;
   }
