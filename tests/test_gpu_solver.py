@@ -407,3 +407,25 @@ def test_refinement_history_against_the_oracle(name, kind):
     sv.set_reference_solution(None)
     sv.solve(b)
     assert all(np.isnan(r["rel_error"]) for r in sv.ir_history())
+
+
+def test_failed_refinement_returns_its_best_iterate():
+    """ILU(0) diverges on the L=5 diode (so does the oracle's): the FP32 inner
+    solves blow up, the refinement hands over to FP64 BiCGStab, which diverges
+    as well.  The solve reports the failure and returns the best iterate it
+    reached: the reported residual is the true residual of the returned x, no
+    larger than that of x = 0 or of the best accepted refinement step."""
+    import shesolve
+    s = oracle_system("diode_L5")
+    A, b, _, _ = s.reduced()
+    sv = _solver(precond=_pc("ilu0"), precision=shesolve.FP32_IR, tol=1e-10, max_iters=2000)
+    sv.set_matrix(A.row_ptr, A.col, A.val)
+    x, rep = sv.solve(b)
+    h = sv.ir_history()
+    print(rep, h)
+    assert rep["status"] != 0
+    res = np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b)
+    assert np.isfinite(res) and res <= 1.0
+    assert abs(res - rep["rel_residual"]) <= 1e-6 * res
+    assert h[-1]["kind"] == 2 and h[-1]["rel_residual"] == rep["rel_residual"]
+    assert rep["rel_residual"] <= min(r["rel_residual"] for r in h if r["kind"] == 0)

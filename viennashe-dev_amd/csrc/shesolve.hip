@@ -2510,7 +2510,12 @@ int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_r
         hipLaunchKernelGGL(k_axpy_f32, dim3(kGrid), dim3(kBlock), 0, c->stream, n, x, dd, c->d_norm);
       }
       if (fallback) {
-        // the FP64 solve is the safety net: it gets the full iteration budget
+        // the FP64 solve is the safety net: it gets the full iteration budget.
+        // x is the best iterate so far (residual r.rel_residual); it is kept in
+        // r64 (idle from here on) and returned instead should the FP64 solve
+        // stop unconverged with a larger true residual
+        const double best_rel = r.rel_residual;
+        HIPCHK(hipMemcpyAsync(r64, x, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
         int st = 1;
         double res = 0;
         int it = bicgstab_dev<double, double>(c, sv64(c), c->d_f64, b, x, c->prm.tol, c->prm.max_iters, &st, &res);
@@ -2518,7 +2523,16 @@ int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_r
         r.iterations += it;
         r.status = st;
         r.rel_residual = res;
-        if (int e = ir_record(c, x, r.ir_steps, it, 2, res)) return e;
+        if (st != 0) {
+          double rn_true = 0;
+          if (int e = residual64(c, b, x, c->v64[0], &rn_true)) return e;
+          r.rel_residual = rn_true / bn;
+          if (!(r.rel_residual <= best_rel)) {
+            HIPCHK(hipMemcpyAsync(x, r64, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+            r.rel_residual = best_rel;
+          }
+        }
+        if (int e = ir_record(c, x, r.ir_steps, it, 2, r.rel_residual)) return e;
       }
     }
   }
