@@ -242,3 +242,27 @@ def test_batch_level_launch_sweeps():
         assert rep1["status"] == 0
         assert np.linalg.norm(x1 - x) <= 1e-8 * np.linalg.norm(x1)
     bs.close()
+
+
+def test_batch_failed_refinement_returns_its_best_iterate():
+    """One system of the batch (the L=5 diode under ILU(0), which diverges in
+    both precisions, the oracle's too) fails; the others still converge.  The
+    failed system reports the failure and gets back its best iterate, whose true
+    residual is the reported one and no larger than that of x = 0."""
+    import shesolve
+    names = ["diode_L1", "diode_L5", "mos2d_L1"]
+    sy = _systems(names)
+    bs = shesolve.BatchSolver(0, precond=shesolve.PRECOND_ILU0, precision=shesolve.FP32_IR, tol=1e-10,
+                              max_iters=2000)
+    bs.set_systems([(A.row_ptr, A.col, A.val) for A, _ in sy])
+    xs, reps = bs.solve([b for _, b in sy])
+    for nm, (A, b), x, rep in zip(names, sy, xs, reps):
+        print(nm, rep)
+        res = np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b)
+        if nm == "diode_L5":
+            assert rep["status"] != 0
+            assert np.isfinite(res) and res <= 1.0
+            assert abs(res - rep["rel_residual"]) <= 1e-6 * res
+        else:
+            assert rep["status"] == 0 and res <= 1e-10, (nm, rep, res)
+    bs.close()

@@ -2860,6 +2860,9 @@ int batch_solve_internal(shesolve_batch_ctx *b, int warm) {
   b->rep.assign(S, shesolve_report{});
   std::vector<int> all(S, 1), act(S, 0), fb(S, 0);
   std::vector<double> bn, rn, prev(S, -1.0);
+  // systems handed to the FP64 solve: relative residual of their best iterate,
+  // which xprev keeps while that solve runs
+  std::vector<double> best(S, -1.0);
   if (int e = upload_mask(b, all)) return e;
   if (int e = bnorms<double>(b, b->b64, bn)) return e;
   if (warm) {
@@ -2996,6 +2999,10 @@ int batch_solve_internal(shesolve_batch_ctx *b, int warm) {
       fprintf(stderr, "[shesolve batch] FP64 solve for %d systems\n", nfb);
     }
     if (anyfb) {
+      if (int e = upload_mask(b, fb)) return e;
+      bref(b, 2, b->xprev, b->x64, nullptr);
+      for (int s = 0; s < S; ++s)
+        if (fb[s]) best[s] = b->rep[s].rel_residual;
       if (int e = bbicg_run<double, double>(b, b->b64, b->x64, prm.tol, prm.max_iters, 1e300, fb)) return e;
       for (int s = 0; s < S; ++s)
         if (fb[s]) {
@@ -3008,11 +3015,23 @@ int batch_solve_internal(shesolve_batch_ctx *b, int warm) {
   for (int s = 0; s < S; ++s) act[s] = !zero_rhs[s];
   if (int e = upload_mask(b, act)) return e;
   if (int e = bresidual(b, rn)) return e;
+  std::vector<int> restore(S, 0);
+  bool anyrestore = false;
   for (int s = 0; s < S; ++s) {
     if (zero_rhs[s]) { b->rep[s].rel_residual = 0; b->rep[s].status = 0; continue; }
     b->rep[s].rel_residual = rn[s] / bn[s];
     if (b->rep[s].rel_residual <= prm.tol) b->rep[s].status = 0;
     else if (b->rep[s].status == 0) b->rep[s].status = 1;
+    if (b->rep[s].status != 0 && best[s] >= 0 && !(b->rep[s].rel_residual <= best[s])) {
+      // the FP64 solve ended worse than the refinement: its best iterate is returned
+      restore[s] = 1;
+      anyrestore = true;
+      b->rep[s].rel_residual = best[s];
+    }
+  }
+  if (anyrestore) {
+    if (int e = upload_mask(b, restore)) return e;
+    bref(b, 2, b->x64, b->xprev, nullptr);
   }
   return 0;
 }
