@@ -433,3 +433,58 @@ def test_density_update_closed_forms():
     assert nrm == pytest.approx(brute ** 0.5, rel=1e-15)
     out, nrm = O.density_update(np.array([0.0, 3.0]), np.array([0.0, 0.0]), 0.3)
     assert np.allclose(out, [0.0, 0.9], rtol=1e-15) and nrm == pytest.approx(2.1 / 0.9, rel=1e-14)
+
+
+def _greedy_block_colours(A, block):
+    """Greedy colouring of blocks of `block` consecutive rows in natural order
+    (smallest free colour; two blocks conflict when an entry of the symmetrised
+    pattern links them); the colour of each row."""
+    n = A.n
+    P = sp.csr_matrix((np.ones(A.nnz), A.col, A.row_ptr), shape=(n, n))
+    S = (P + P.T).tocsr()
+    nb = (n + block - 1) // block
+    R = sp.csr_matrix((np.ones(n), (np.arange(n) // block, np.arange(n))), shape=(nb, n))
+    SB = (R @ S @ R.T).tocsr()
+    col = np.full(nb, -1)
+    for i in range(nb):
+        nbrs = SB.indices[SB.indptr[i]:SB.indptr[i + 1]]
+        used = set(col[nbrs[nbrs != i]].tolist())
+        q = 0
+        while q in used:
+            q += 1
+        col[i] = q
+    return col[np.arange(n) // block]
+
+
+def _bicgstab_iterations_in_order(A, b, perm):
+    from oracle import oracle as O
+    M = sp.csr_matrix((A.val, A.col, A.row_ptr), shape=(A.n, A.n))[perm][:, perm].tocsr()
+    M.sort_indices()
+    Ap = O.CSR(M.indptr.astype(np.int64), M.indices.astype(np.int32), M.data.copy())
+    x, it, res = O.bicgstab(Ap, b[perm], precond=O.Precond(O.PC_ILU0, Ap), tol=1e-10, max_iters=3000)
+    assert res <= 1e-10, res
+    return it
+
+
+@pytest.mark.parametrize("name", ["mos2d_L3", "nin2d_L3"])
+def test_block_multicolour_ordering(name):
+    """The block multicolour ordering the GPU solver uses for 2D devices (blocks
+    of 32 consecutive rows coloured greedily, rows sorted by colour): blocks of
+    one colour are independent, and ILU(0)-preconditioned BiCGStab in that order
+    converges in no more iterations than in the point multicolour order (block
+    size 1).  A colouring that let two blocks of one colour touch fails the
+    first check."""
+    s = oracle_system(name)
+    A, b, _, _ = s.reduced()
+    colour = _greedy_block_colours(A, 32)
+    rows = np.repeat(np.arange(A.n), np.diff(A.row_ptr))
+    same = colour[rows] == colour[A.col]
+    assert np.all((rows // 32 == A.col // 32)[same])
+    bad = colour.copy()
+    bad[:32] = colour[32]
+    same_bad = bad[rows] == bad[A.col]
+    assert not np.all((rows // 32 == A.col // 32)[same_bad])
+    it_block = _bicgstab_iterations_in_order(A, b, np.argsort(colour, kind="stable"))
+    it_point = _bicgstab_iterations_in_order(A, b, np.argsort(_greedy_block_colours(A, 1), kind="stable"))
+    print(name, "colours", colour.max() + 1, "iterations block", it_block, "point", it_point)
+    assert it_block <= it_point
