@@ -137,7 +137,10 @@ int shesolve_get_factors(shesolve_handle h, int64_t *row_ptr, int32_t *col, doub
 int shesolve_stream_config(shesolve_handle h, int32_t *items, int32_t *xcd, double *tune_us9);
 /* Row ordering of the factor (new -> old); the identity unless MC_ILU0. */
 int shesolve_get_ordering(shesolve_handle h, int32_t *perm);
-/* Level-schedule statistics: [0] forward levels, [1] backward levels. */
+/* Level-schedule statistics: [0] forward levels, [1] backward levels of the
+ * factor's dependency graph (the factorisation's launches).  The sweeps of
+ * MC_ILU0 take one launch per colour on these levels; those of BMC_ILU0 take
+ * one launch per colour of blocks, fewer than these levels. */
 int shesolve_levels(shesolve_handle h, int64_t *out2);
 
 /* Refinement history of the last solve on this handle.  The reference solves
