@@ -75,11 +75,20 @@ static int cpu_checks() {
   conf.gpu_drop_tolerance(1e-3);
   conf.gpu_drop_tolerance(2.0);
   CHECK(conf.gpu_fill_level() == 2 && conf.gpu_drop_tolerance() == 1e-3);
+  // the preconditioner knob carries the C ABI's values
+  CHECK(linear_solver_config::gpu_ilu0 == SHESOLVE_PRECOND_ILU0);
+  CHECK(linear_solver_config::gpu_block_ilu0 == SHESOLVE_PRECOND_BLOCK_ILU0);
+  CHECK(linear_solver_config::gpu_mc_ilu0 == SHESOLVE_PRECOND_MC_ILU0);
+  CHECK(linear_solver_config::gpu_iluk == SHESOLVE_PRECOND_ILUK);
+  CHECK(linear_solver_config::gpu_bmc_ilu0 == SHESOLVE_PRECOND_BMC_ILU0);
+  CHECK(conf.gpu_preconditioner() == linear_solver_config::gpu_ilu0);
+  conf.gpu_preconditioner(linear_solver_config::gpu_bmc_ilu0);
+  CHECK(conf.gpu_preconditioner() == linear_solver_config::gpu_bmc_ilu0);
   std::printf("cpu checks passed\n");
   return 0;
 }
 
-static int gpu_solve(const char *in, const char *out) {
+static int gpu_solve(const char *in, const char *out, const char *precond) {
   FILE *f = std::fopen(in, "rb");
   if (!f) return 2;
   int64_t n = 0, nnz = 0;
@@ -98,6 +107,7 @@ static int gpu_solve(const char *in, const char *out) {
   conf.set(std::string("gpu_parallel_linear_solver"));
   conf.tolerance(1e-12);
   conf.max_iters(5000);
+  if (precond && std::strcmp(precond, "bmc_ilu0") == 0) conf.gpu_preconditioner(linear_solver_config::gpu_bmc_ilu0);
   std::vector<double> x = solve(A, b, conf);
   FILE *g = std::fopen(out, "wb");
   std::fwrite(x.data(), 8, n, g);
@@ -108,7 +118,7 @@ static int gpu_solve(const char *in, const char *out) {
 
 int main(int argc, char **argv) {
   if (argc >= 2 && std::strcmp(argv[1], "cpu") == 0) return cpu_checks();
-  if (argc >= 4 && std::strcmp(argv[1], "gpu") == 0) return gpu_solve(argv[2], argv[3]);
+  if (argc >= 4 && std::strcmp(argv[1], "gpu") == 0) return gpu_solve(argv[2], argv[3], argc >= 5 ? argv[4] : nullptr);
   std::fprintf(stderr, "usage: %s cpu | gpu in.bin out.bin\n", argv[0]);
   return 2;
 }

@@ -27,9 +27,10 @@ def test_cpu_semantics():
 
 
 @pytest.mark.gpu
-def test_gpu_solve_through_mirror(tmp_path):
+@pytest.mark.parametrize("name,precond", [("diode_L3", "ilu0"), ("mos2d_L3", "bmc_ilu0")])
+def test_gpu_solve_through_mirror(tmp_path, name, precond):
     exe = _build()
-    s = oracle_system("diode_L3")
+    s = oracle_system(name)
     A, b, _, _ = s.reduced()
     fin, fout = tmp_path / "sys.bin", tmp_path / "x.bin"
     with open(fin, "wb") as f:
@@ -38,7 +39,7 @@ def test_gpu_solve_through_mirror(tmp_path):
         f.write(A.col.tobytes())
         f.write(A.val.tobytes())
         f.write(np.ascontiguousarray(b).tobytes())
-    r = subprocess.run([exe, "gpu", str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "gpu", str(fin), str(fout), precond], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     x = np.fromfile(fout, np.float64)
     res = np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b)

@@ -407,6 +407,15 @@ def test_refinement_history_against_the_oracle(name, kind):
     sv.set_reference_solution(None)
     sv.solve(b)
     assert all(np.isnan(r["rel_error"]) for r in sv.ir_history())
+    # a plain FP64 solve leaves one record, kind 2
+    sv.set_reference_solution(x_ref)
+    sv.set_params(precision=shesolve.FP64)
+    x64, rep64 = sv.solve(b)
+    h64 = sv.ir_history()
+    assert len(h64) == 1 and h64[0]["kind"] == 2 and h64[0]["inner_iterations"] == rep64["iterations"]
+    assert h64[0]["rel_residual"] == rep64["rel_residual"]
+    err64 = np.linalg.norm(x64 - x_ref) / np.linalg.norm(x_ref)
+    assert abs(h64[0]["rel_error"] - err64) <= 1e-9 * err64 + 1e-15
 
 
 def test_failed_refinement_returns_its_best_iterate():
