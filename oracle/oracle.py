@@ -84,6 +84,11 @@ def lib():
     L.oracle_bicgstab.restype = C.c_int
     L.oracle_bicgstab.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, _f64p, C.c_void_p, C.c_double,
                                   C.c_int, C.c_double, C.POINTER(C.c_double)]
+    L.oracle_bicgstab_parallel.restype = C.c_int
+    L.oracle_bicgstab_parallel.argtypes = L.oracle_bicgstab.argtypes
+    L.oracle_threads.restype = C.c_int
+    L.oracle_precond_blocks.restype = C.c_int
+    L.oracle_precond_blocks.argtypes = [C.c_void_p]
     _lib = L
     return L
 
@@ -261,6 +266,10 @@ class Precond:
             raise RuntimeError("precond failed: " + lib().oracle_last_error().decode())
         self._h = h
 
+    def parallel_blocks(self):
+        """Blocks the parallel baseline applies concurrently (0 = sequential)."""
+        return int(lib().oracle_precond_blocks(self._h))
+
     def __del__(self):
         if getattr(self, "_h", None):
             lib().oracle_precond_free(self._h)
@@ -287,6 +296,18 @@ def bicgstab(A: CSR, b, x0=None, precond: Precond | None = None, tol=1e-10, max_
                                precond._h if precond is not None else None, tol, max_iters, max_seconds,
                                C.byref(res))
     return x, int(it), float(res.value)
+
+
+def bicgstab_parallel(A: CSR, b, x0=None, precond: Precond | None = None, tol=1e-10, max_iters=1000,
+                      max_seconds=0.0):
+    """bicgstab on all OpenMP threads (oracle_bicgstab_parallel); returns
+    (x, iterations, relative residual, threads)."""
+    x = np.zeros(A.n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    res = C.c_double(0.0)
+    it = lib().oracle_bicgstab_parallel(A.n, A.row_ptr, A.col, A.val, np.ascontiguousarray(b, np.float64), x,
+                                        precond._h if precond is not None else None, tol, max_iters, max_seconds,
+                                        C.byref(res))
+    return x, int(it), float(res.value), int(lib().oracle_threads())
 
 
 def density_update(new, cur, alpha, logarithmic=False):

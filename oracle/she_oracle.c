@@ -5,6 +5,9 @@
  * Every function names the reference file:line it restates.  Reference
  * paths are relative to the ViennaSHE 1.3.0 source tree.
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "she_oracle.h"
 
 #include <math.h>
@@ -1102,11 +1105,25 @@ struct oracle_precond {
   int32_t *ci;
   double *v;
   int64_t *diag;   /* position of the diagonal in each row */
+  int nseg;        /* block ILU0 whose ranges tile [0, n): the blocks' starts seg[0..nseg), seg[nseg] = n */
+  int64_t *seg;
 };
 
 void oracle_dispersion(int carrier, double eps, double *Z, double *v) {
   *Z = dos(carrier, eps);
   *v = velocity(carrier, eps);
+}
+
+/* Rows are independent: with par the row loop runs on all OpenMP threads (the
+ * parallel CPU baseline), otherwise in order on the calling thread. */
+static void spmv_par(int par, int64_t n, const int64_t *rp, const int32_t *ci, const double *v, const double *x,
+                     double *y) {
+#pragma omp parallel for if (par) schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) s += v[k] * x[ci[k]];
+    y[i] = s;
+  }
 }
 
 void oracle_spmv(int64_t n, const int64_t *rp, const int32_t *ci, const double *v, const double *x, double *y) {
@@ -1363,7 +1380,43 @@ oracle_precond *oracle_precond_create(int kind, int64_t n, const int64_t *rp, co
   }
   free(blk);
   if (ilu0_factor(p)) { oracle_precond_free(p); return NULL; }
+  if (kind == ORACLE_PC_BLOCK_ILU0 && nblocks > 0) {
+    /* nonempty ranges that tile [0, n) in order (empty ones, energy levels
+     * without unknowns, are skipped): the blocks apply independently
+     * (parallel_linear_solver.hpp:79-93 applies them with OpenMP) */
+    int64_t *st = (int64_t *)malloc((size_t)(nblocks + 1) * sizeof(int64_t));
+    int m = 0, tiled = 1;
+    int64_t at = 0;
+    for (int b = 0; b < nblocks && tiled; ++b) {
+      const int64_t f = ranges[2 * b], l = ranges[2 * b + 1];
+      if (l == f) continue;
+      if (f != at || l < f) tiled = 0;
+      else { st[m++] = f; at = l; }
+    }
+    if (tiled && m > 0 && at == n) {
+      st[m] = n;
+      p->nseg = m;
+      p->seg = st;
+    } else {
+      free(st);
+    }
+  }
   return p;
+}
+
+/* Forward then backward sweep over rows [r0, r1) of a factor with no entries
+ * leaving that range (a block of block ILU0). */
+static void precond_apply_range(const oracle_precond *p, int64_t r0, int64_t r1, const double *in, double *out) {
+  for (int64_t i = r0; i < r1; ++i) {
+    double s = in[i];
+    for (int64_t k = p->rp[i]; k < p->diag[i]; ++k) s -= p->v[k] * out[p->ci[k]];
+    out[i] = s;
+  }
+  for (int64_t i = r1 - 1; i >= r0; --i) {
+    double s = out[i];
+    for (int64_t k = p->diag[i] + 1; k < p->rp[i + 1]; ++k) s -= p->v[k] * out[p->ci[k]];
+    out[i] = s / p->v[p->diag[i]];
+  }
 }
 
 static void precond_apply_n(const oracle_precond *p, int64_t n, const double *in, double *out) {
@@ -1389,7 +1442,7 @@ void oracle_precond_factors(const oracle_precond *p, int64_t *rp, int32_t *ci, d
 }
 void oracle_precond_free(oracle_precond *p) {
   if (!p) return;
-  free(p->rp); free(p->ci); free(p->v); free(p->diag);
+  free(p->rp); free(p->ci); free(p->v); free(p->diag); free(p->seg);
   free(p);
 }
 
@@ -1398,20 +1451,53 @@ static double dot(int64_t n, const double *a, const double *b) {
   for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
   return s;
 }
+static double dot_par(int par, int64_t n, const double *a, const double *b) {
+  if (!par) return dot(n, a, b);
+  double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+/* the preconditioner application of the parallel baseline: the blocks of a
+ * block ILU0 in parallel (same values as the sequential sweep, no entry leaves
+ * a block); anything else in order */
+static void precond_apply_par(int par, const oracle_precond *p, int64_t n, const double *in, double *out) {
+  if (par && p && p->kind == ORACLE_PC_BLOCK_ILU0 && p->nseg > 0) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < p->nseg; ++b) precond_apply_range(p, p->seg[b], p->seg[b + 1], in, out);
+    return;
+  }
+  precond_apply_n(p, n, in, out);
+}
+
+int oracle_precond_blocks(const oracle_precond *p) { return p ? p->nseg : 0; }
+
+int oracle_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 static double now_s(void) {
   struct timespec t;
   clock_gettime(CLOCK_MONOTONIC, &t);
   return t.tv_sec + 1e-9 * t.tv_nsec;
 }
 
-int oracle_bicgstab(int64_t n, const int64_t *rp, const int32_t *ci, const double *val, const double *b, double *x,
-                    const oracle_precond *p, double tol, int max_iters, double max_seconds, double *rel_res) {
+/* Right-preconditioned BiCGStab (van der Vorst 1992) as the reference calls it
+ * (serial_linear_solver.hpp:129-136).  With par the vector work, the SpMV and
+ * the block ILU0 application run on all OpenMP threads (the parallel CPU
+ * baseline; the same steps, dot products summed in another order). */
+static int bicgstab_impl(int par, int64_t n, const int64_t *rp, const int32_t *ci, const double *val, const double *b,
+                         double *x, const oracle_precond *p, double tol, int max_iters, double max_seconds,
+                         double *rel_res) {
   double t0 = now_s();
   size_t sz = (size_t)(n > 0 ? n : 1) * sizeof(double);
   double *r = (double *)malloc(sz), *r0 = (double *)malloc(sz), *pp = (double *)calloc(1, sz),
          *v = (double *)calloc(1, sz), *ph = (double *)malloc(sz), *sv = (double *)malloc(sz),
          *sh = (double *)malloc(sz), *t = (double *)malloc(sz);
-  double nb = sqrt(dot(n, b, b));
+  double nb = sqrt(dot_par(par, n, b, b));
   int it = 0;
   if (nb == 0.0) {
     /* solve_impl: zero right-hand side gives the zero solution (src/solvers/solve.hpp:61-66) */
@@ -1419,55 +1505,58 @@ int oracle_bicgstab(int64_t n, const int64_t *rp, const int32_t *ci, const doubl
     *rel_res = 0.0;
     goto done;
   }
-  oracle_spmv(n, rp, ci, val, x, r);
+  spmv_par(par, n, rp, ci, val, x, r);
   for (int64_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
   memcpy(r0, r, sz);
   double rho = 1.0, alpha = 1.0, omega = 1.0;
-  double res = sqrt(dot(n, r, r)) / nb;
+  double res = sqrt(dot_par(par, n, r, r)) / nb;
   while (res > tol && it < max_iters) {
     if (max_seconds > 0 && now_s() - t0 > max_seconds) break;
     ++it;
-    double rho_new = dot(n, r0, r);
+    double rho_new = dot_par(par, n, r0, r);
     if (rho_new == 0.0 || omega == 0.0) {
       /* breakdown: restart from the current residual */
       memcpy(r0, r, sz);
       for (int64_t i = 0; i < n; ++i) { pp[i] = 0.0; v[i] = 0.0; }
       rho = alpha = omega = 1.0;
-      rho_new = dot(n, r0, r);
+      rho_new = dot_par(par, n, r0, r);
     }
     double beta = (rho_new / rho) * (alpha / omega);
+#pragma omp parallel for if (par) schedule(static)
     for (int64_t i = 0; i < n; ++i) pp[i] = r[i] + beta * (pp[i] - omega * v[i]);
-    precond_apply_n(p, n, pp, ph);
-    oracle_spmv(n, rp, ci, val, ph, v);
-    alpha = rho_new / dot(n, r0, v);
+    precond_apply_par(par, p, n, pp, ph);
+    spmv_par(par, n, rp, ci, val, ph, v);
+    alpha = rho_new / dot_par(par, n, r0, v);
+#pragma omp parallel for if (par) schedule(static)
     for (int64_t i = 0; i < n; ++i) sv[i] = r[i] - alpha * v[i];
-    double sres = sqrt(dot(n, sv, sv)) / nb;
+    double sres = sqrt(dot_par(par, n, sv, sv)) / nb;
     if (sres <= tol) {
       for (int64_t i = 0; i < n; ++i) x[i] += alpha * ph[i];
-      oracle_spmv(n, rp, ci, val, x, r);
+      spmv_par(par, n, rp, ci, val, x, r);
       for (int64_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
-      res = sqrt(dot(n, r, r)) / nb;
+      res = sqrt(dot_par(par, n, r, r)) / nb;
       if (res <= tol) break;
       memcpy(r0, r, sz);
       for (int64_t i = 0; i < n; ++i) { pp[i] = 0.0; v[i] = 0.0; }
       rho = alpha = omega = 1.0;
       continue;
     }
-    precond_apply_n(p, n, sv, sh);
-    oracle_spmv(n, rp, ci, val, sh, t);
-    double tt = dot(n, t, t);
-    omega = tt > 0 ? dot(n, t, sv) / tt : 0.0;
+    precond_apply_par(par, p, n, sv, sh);
+    spmv_par(par, n, rp, ci, val, sh, t);
+    double tt = dot_par(par, n, t, t);
+    omega = tt > 0 ? dot_par(par, n, t, sv) / tt : 0.0;
+#pragma omp parallel for if (par) schedule(static)
     for (int64_t i = 0; i < n; ++i) {
       x[i] += alpha * ph[i] + omega * sh[i];
       r[i] = sv[i] - omega * t[i];
     }
     rho = rho_new;
-    res = sqrt(dot(n, r, r)) / nb;
+    res = sqrt(dot_par(par, n, r, r)) / nb;
     if (res <= tol) {
       /* confirm with the explicit residual */
-      oracle_spmv(n, rp, ci, val, x, r);
+      spmv_par(par, n, rp, ci, val, x, r);
       for (int64_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
-      res = sqrt(dot(n, r, r)) / nb;
+      res = sqrt(dot_par(par, n, r, r)) / nb;
       if (res > tol) {
         memcpy(r0, r, sz);
         for (int64_t i = 0; i < n; ++i) { pp[i] = 0.0; v[i] = 0.0; }
@@ -1475,10 +1564,21 @@ int oracle_bicgstab(int64_t n, const int64_t *rp, const int32_t *ci, const doubl
       }
     }
   }
-  oracle_spmv(n, rp, ci, val, x, r);
+  spmv_par(par, n, rp, ci, val, x, r);
   for (int64_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
-  *rel_res = sqrt(dot(n, r, r)) / nb;
+  *rel_res = sqrt(dot_par(par, n, r, r)) / nb;
 done:
   free(r); free(r0); free(pp); free(v); free(ph); free(sv); free(sh); free(t);
   return it;
+}
+
+int oracle_bicgstab(int64_t n, const int64_t *rp, const int32_t *ci, const double *val, const double *b, double *x,
+                    const oracle_precond *p, double tol, int max_iters, double max_seconds, double *rel_res) {
+  return bicgstab_impl(0, n, rp, ci, val, b, x, p, tol, max_iters, max_seconds, rel_res);
+}
+
+int oracle_bicgstab_parallel(int64_t n, const int64_t *rp, const int32_t *ci, const double *val, const double *b,
+                             double *x, const oracle_precond *p, double tol, int max_iters, double max_seconds,
+                             double *rel_res) {
+  return bicgstab_impl(1, n, rp, ci, val, b, x, p, tol, max_iters, max_seconds, rel_res);
 }

@@ -149,6 +149,17 @@ void oracle_spmv(int64_t n, const int64_t *row_ptr, const int32_t *col, const do
 int oracle_bicgstab(int64_t n, const int64_t *row_ptr, const int32_t *col, const double *val,
                     const double *b, double *x, const oracle_precond *p, double tol,
                     int max_iters, double max_seconds, double *rel_res);
+/* The same iteration on all OpenMP threads: SpMV, dot products and vector
+ * updates in parallel, the blocks of a block ILU0 (ranges tiling the rows)
+ * applied in parallel, as parallel_linear_solver does
+ * (src/solvers/viennacl/parallel_linear_solver.hpp:64-114).  The CPU baseline
+ * over the host's cores; oracle_threads() is the thread count it uses. */
+int oracle_bicgstab_parallel(int64_t n, const int64_t *row_ptr, const int32_t *col, const double *val,
+                             const double *b, double *x, const oracle_precond *p, double tol,
+                             int max_iters, double max_seconds, double *rel_res);
+int oracle_threads(void);
+/* Blocks a block ILU0 applies in parallel (0: its ranges do not tile the rows). */
+int oracle_precond_blocks(const oracle_precond *p);
 
 #ifdef __cplusplus
 }

@@ -488,3 +488,29 @@ def test_block_multicolour_ordering(name):
     it_point = _bicgstab_iterations_in_order(A, b, np.argsort(_greedy_block_colours(A, 1), kind="stable"))
     print(name, "colours", colour.max() + 1, "iterations block", it_block, "point", it_point)
     assert it_block <= it_point
+
+
+@pytest.mark.parametrize("name", ["mos2d_L3", "nin2d_L3", "diode_L3"])
+def test_parallel_baseline_matches_the_sequential_solver(name):
+    """The parallel CPU baseline (oracle_bicgstab_parallel) runs the same
+    BiCGStab with per-energy block ILU0 on all OpenMP threads.  Its block
+    application is the sequential one restricted to each block (no factor entry
+    leaves a block), so one iteration gives the sequential iterate up to the
+    summation order of the dot products, and the solve reaches the same
+    solution."""
+    from oracle import oracle as O
+    s = oracle_system(name)
+    A, b, _, _ = s.reduced()
+    ranges = s.block_ranges()
+    P = O.Precond(O.PC_BLOCK_ILU0, A, ranges=ranges)
+    # every nonempty energy slab is one parallel block
+    assert P.parallel_blocks() == int(np.count_nonzero(ranges[:, 1] > ranges[:, 0])) > 1
+    x1, _, _ = O.bicgstab(A, b, precond=P, tol=1e-300, max_iters=1)
+    x2, _, _, threads = O.bicgstab_parallel(A, b, precond=P, tol=1e-300, max_iters=1)
+    assert threads >= 1
+    assert np.linalg.norm(x2 - x1) <= 1e-12 * np.linalg.norm(x1)
+    x1, it1, r1 = O.bicgstab(A, b, precond=P, tol=1e-12, max_iters=5000)
+    x2, it2, r2, _ = O.bicgstab_parallel(A, b, precond=P, tol=1e-12, max_iters=5000)
+    print(name, "iterations", it1, it2, "threads", threads)
+    assert r1 <= 1e-12 and r2 <= 1e-12
+    assert np.linalg.norm(x2 - x1) <= 1e-9 * np.linalg.norm(x1)
