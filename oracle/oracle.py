@@ -87,6 +87,8 @@ def lib():
     L.oracle_bicgstab_parallel.restype = C.c_int
     L.oracle_bicgstab_parallel.argtypes = L.oracle_bicgstab.argtypes
     L.oracle_threads.restype = C.c_int
+    L.oracle_precond_create_parallel.restype = C.c_void_p
+    L.oracle_precond_create_parallel.argtypes = L.oracle_precond_create.argtypes
     L.oracle_precond_blocks.restype = C.c_int
     L.oracle_precond_blocks.argtypes = [C.c_void_p]
     _lib = L
@@ -251,7 +253,7 @@ PC_NONE, PC_ILU0, PC_ILUT, PC_BLOCK_ILU0, PC_ILUK = 0, 1, 2, 3, 5
 
 
 class Precond:
-    def __init__(self, kind, A: CSR, drop=1e-3, fill=20, ranges=None):
+    def __init__(self, kind, A: CSR, drop=1e-3, fill=20, ranges=None, parallel=False):
         self.kind = kind
         self.n = A.n
         rr = None
@@ -260,8 +262,8 @@ class Precond:
             rr = np.ascontiguousarray(np.asarray(ranges, dtype=np.int64).reshape(-1))
             nb = len(rr) // 2
         self._keep = rr
-        h = lib().oracle_precond_create(kind, A.n, A.row_ptr, A.col, A.val, drop, fill, nb,
-                                        rr.ctypes.data if rr is not None else None)
+        create = lib().oracle_precond_create_parallel if parallel else lib().oracle_precond_create
+        h = create(kind, A.n, A.row_ptr, A.col, A.val, drop, fill, nb, rr.ctypes.data if rr is not None else None)
         if not h:
             raise RuntimeError("precond failed: " + lib().oracle_last_error().decode())
         self._h = h

@@ -1142,12 +1142,11 @@ void oracle_spmv(int64_t n, const int64_t *rp, const int32_t *ci, const double *
  * done; dropped entries stay in the pattern as zeros. */
 static int ilu0_factor_drop(oracle_precond *p, double drop);
 static int ilu0_factor(oracle_precond *p) { return ilu0_factor_drop(p, 0.0); }
-static int ilu0_factor_drop(oracle_precond *p, double drop) {
-  int64_t n = p->n;
-  int64_t *pos = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
-  for (int64_t i = 0; i < n; ++i) pos[i] = -1;
-  p->diag = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
-  for (int64_t i = 0; i < n; ++i) {
+/* Rows [r0, r1) of the factorisation; pos (column -> position in the current
+ * row, -1 elsewhere) is left as it was found.  Rows of different blocks of a
+ * block ILU0 touch disjoint columns, so blocks may run concurrently on one pos. */
+static int ilu0_rows(oracle_precond *p, int64_t r0, int64_t r1, double drop, int64_t *pos) {
+  for (int64_t i = r0; i < r1; ++i) {
     double rn = 0.0;
     for (int64_t k = p->rp[i]; k < p->rp[i + 1]; ++k) {
       pos[p->ci[k]] = k;
@@ -1155,7 +1154,10 @@ static int ilu0_factor_drop(oracle_precond *p, double drop) {
     }
     const double tau = drop * sqrt(rn);
     p->diag[i] = pos[i];
-    if (pos[i] < 0) { free(pos); set_err("ILU0: missing diagonal in row %lld", (long long)i); return -1; }
+    if (pos[i] < 0) {
+      for (int64_t k = p->rp[i]; k < p->rp[i + 1]; ++k) pos[p->ci[k]] = -1;
+      return (int)-1;
+    }
     for (int64_t k = p->rp[i]; k < p->rp[i + 1]; ++k) {
       int64_t kc = p->ci[k];
       if (kc >= i) break;
@@ -1173,8 +1175,35 @@ static int ilu0_factor_drop(oracle_precond *p, double drop) {
         if (fabs(p->v[k]) < tau) p->v[k] = 0.0;
     for (int64_t k = p->rp[i]; k < p->rp[i + 1]; ++k) pos[p->ci[k]] = -1;
   }
-  free(pos);
   return 0;
+}
+
+static int64_t *ilu0_begin(oracle_precond *p) {
+  const int64_t n = p->n;
+  int64_t *pos = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+  for (int64_t i = 0; i < n; ++i) pos[i] = -1;
+  p->diag = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+  return pos;
+}
+
+static int ilu0_factor_drop(oracle_precond *p, double drop) {
+  int64_t *pos = ilu0_begin(p);
+  int e = ilu0_rows(p, 0, p->n, drop, pos);
+  free(pos);
+  if (e) set_err("ILU0: a row has no diagonal entry");
+  return e;
+}
+
+/* The same factorisation with the blocks of a block ILU0 (p->seg) on all
+ * OpenMP threads; every row computes exactly as in ilu0_factor_drop. */
+static int ilu0_factor_blocks_parallel(oracle_precond *p) {
+  int64_t *pos = ilu0_begin(p);
+  int err = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+  for (int b = 0; b < p->nseg; ++b) err |= ilu0_rows(p, p->seg[b], p->seg[b + 1], 0.0, pos) != 0;
+  free(pos);
+  if (err) set_err("ILU0: a row has no diagonal entry");
+  return err ? -1 : 0;
 }
 
 /* Symbolic ILU(k): level-of-fill pattern (Saad 2nd ed. Alg. 10.5).  A's
@@ -1330,8 +1359,9 @@ static int ilut_factor(oracle_precond *p, int64_t n, const int64_t *rp, const in
   return 0;
 }
 
-oracle_precond *oracle_precond_create(int kind, int64_t n, const int64_t *rp, const int32_t *ci, const double *val,
-                                      double drop, int fill, int nblocks, const int64_t *ranges) {
+static oracle_precond *precond_create_impl(int parallel, int kind, int64_t n, const int64_t *rp, const int32_t *ci,
+                                           const double *val, double drop, int fill, int nblocks,
+                                           const int64_t *ranges) {
   oracle_precond *p = (oracle_precond *)calloc(1, sizeof(oracle_precond));
   p->kind = kind;
   p->n = n;
@@ -1379,7 +1409,6 @@ oracle_precond *oracle_precond_create(int kind, int64_t n, const int64_t *rp, co
     p->rp[i + 1] = q;
   }
   free(blk);
-  if (ilu0_factor(p)) { oracle_precond_free(p); return NULL; }
   if (kind == ORACLE_PC_BLOCK_ILU0 && nblocks > 0) {
     /* nonempty ranges that tile [0, n) in order (empty ones, energy levels
      * without unknowns, are skipped): the blocks apply independently
@@ -1401,6 +1430,10 @@ oracle_precond *oracle_precond_create(int kind, int64_t n, const int64_t *rp, co
       free(st);
     }
   }
+  if (parallel && p->nseg > 0 ? ilu0_factor_blocks_parallel(p) : ilu0_factor(p)) {
+    oracle_precond_free(p);
+    return NULL;
+  }
   return p;
 }
 
@@ -1417,6 +1450,16 @@ static void precond_apply_range(const oracle_precond *p, int64_t r0, int64_t r1,
     for (int64_t k = p->diag[i] + 1; k < p->rp[i + 1]; ++k) s -= p->v[k] * out[p->ci[k]];
     out[i] = s / p->v[p->diag[i]];
   }
+}
+
+oracle_precond *oracle_precond_create(int kind, int64_t n, const int64_t *rp, const int32_t *ci, const double *val,
+                                      double drop, int fill, int nblocks, const int64_t *ranges) {
+  return precond_create_impl(0, kind, n, rp, ci, val, drop, fill, nblocks, ranges);
+}
+oracle_precond *oracle_precond_create_parallel(int kind, int64_t n, const int64_t *rp, const int32_t *ci,
+                                               const double *val, double drop, int fill, int nblocks,
+                                               const int64_t *ranges) {
+  return precond_create_impl(1, kind, n, rp, ci, val, drop, fill, nblocks, ranges);
 }
 
 static void precond_apply_n(const oracle_precond *p, int64_t n, const double *in, double *out) {

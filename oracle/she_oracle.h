@@ -158,6 +158,12 @@ int oracle_bicgstab_parallel(int64_t n, const int64_t *row_ptr, const int32_t *c
                              const double *b, double *x, const oracle_precond *p, double tol,
                              int max_iters, double max_seconds, double *rel_res);
 int oracle_threads(void);
+/* oracle_precond_create with the blocks of a block ILU0 (ranges tiling the
+ * rows) factorised on all OpenMP threads; the factor is the same bit for bit. */
+oracle_precond *oracle_precond_create_parallel(int kind, int64_t n, const int64_t *row_ptr,
+                                               const int32_t *col, const double *val,
+                                               double ilut_drop, int ilut_fill,
+                                               int nblocks, const int64_t *ranges);
 /* Blocks a block ILU0 applies in parallel (0: its ranges do not tile the rows). */
 int oracle_precond_blocks(const oracle_precond *p);
 

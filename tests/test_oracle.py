@@ -505,6 +505,11 @@ def test_parallel_baseline_matches_the_sequential_solver(name):
     P = O.Precond(O.PC_BLOCK_ILU0, A, ranges=ranges)
     # every nonempty energy slab is one parallel block
     assert P.parallel_blocks() == int(np.count_nonzero(ranges[:, 1] > ranges[:, 0])) > 1
+    # the blocks factorised on all threads give the same factor bit for bit
+    Pp = O.Precond(O.PC_BLOCK_ILU0, A, ranges=ranges, parallel=True)
+    F, Fp = P.factors(), Pp.factors()
+    assert np.array_equal(F.row_ptr, Fp.row_ptr) and np.array_equal(F.col, Fp.col)
+    assert np.array_equal(F.val, Fp.val)
     x1, _, _ = O.bicgstab(A, b, precond=P, tol=1e-300, max_iters=1)
     x2, _, _, threads = O.bicgstab_parallel(A, b, precond=P, tol=1e-300, max_iters=1)
     assert threads >= 1
