@@ -17,7 +17,9 @@
  *                                         she/rescaling.hpp, math/linalg_util.hpp:200-282
  *   - ILU0 / ILUT / block-ILU0 + BiCGStab (ViennaCL, not in the reference tree;
  *     restated from Saad, "Iterative Methods for Sparse Linear Systems", 2nd ed.,
- *     Alg. 10.4 (ILU0), Alg. 10.6 (ILUT) and van der Vorst 1992 (BiCGStab))
+ *     Alg. 10.4 (ILU0), Alg. 10.6 (ILUT) and van der Vorst 1992 (BiCGStab)),
+ *     and the same block-ILU0 + BiCGStab over all OpenMP threads, the CPU
+ *     baseline              src/solvers/viennacl/parallel_linear_solver.hpp:64-114
  *   - odd recovery, carrier density, facet current
  *                                         she/eliminate.hpp:305-354,
  *                                         she/postproc/carrier_density.hpp:68-131,
