@@ -91,6 +91,7 @@ def lib():
     L.oracle_precond_create_parallel.argtypes = L.oracle_precond_create.argtypes
     L.oracle_precond_blocks.restype = C.c_int
     L.oracle_precond_blocks.argtypes = [C.c_void_p]
+    L.oracle_set_plant.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
     _lib = L
     return L
 
@@ -107,6 +108,25 @@ def coupling_tables():
 
 
 NAMES = ("a_x", "a_y", "a_z", "b_x", "b_y", "b_z")
+
+
+class planted:
+    """Context manager of the fill's test-only scale factors (oracle_set_plant):
+    the diffusion and drift coefficients and the acoustic and optical rates of
+    every system built inside the block are multiplied by them; all go back to
+    1 on exit.  Used by the physics pins to switch a term off and to plant a
+    mistake their checks must catch."""
+
+    def __init__(self, diffusion=1.0, drift=1.0, acoustic=1.0, optical=1.0):
+        self.f = (float(diffusion), float(drift), float(acoustic), float(optical))
+
+    def __enter__(self):
+        lib().oracle_set_plant(*self.f)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_plant(1.0, 1.0, 1.0, 1.0)
+        return False
 
 
 def dispersion(carrier, eps):

@@ -27,6 +27,17 @@ static const double M0_ = 9.10938291e-31;
 static double hbar_(void) { return 6.62606896e-34 / (2.0 * 3.1415926535897932384626433832795); }
 
 static char g_err[512];
+
+/* Test-only scale factors of the fill (oracle_set_plant): 1 everywhere except
+ * in the physics pins, which switch a term off or plant a factor to show that
+ * their check catches it. */
+static double g_plant_diff = 1.0, g_plant_drift = 1.0, g_plant_ac = 1.0, g_plant_op = 1.0;
+void oracle_set_plant(double diffusion, double drift, double acoustic, double optical) {
+  g_plant_diff = diffusion;
+  g_plant_drift = drift;
+  g_plant_ac = acoustic;
+  g_plant_op = optical;
+}
 const char *oracle_last_error(void) { return g_err; }
 static void set_err(const char *fmt, ...) {
   va_list ap;
@@ -400,7 +411,7 @@ static int scattering_events(const oracle_system *s, double E, sevent *ev) {
     const double E1 = ct == 0 ? 8.9 * Q_ : 5.12 * Q_;
     const double rate = 2.0 * pi * KB_ * T * E1 * E1 / (hbar * ul * ul * rho);
     const double fit = ct == 0 ? 1.05 : 0.875;
-    ev[n].ei = E; ev[n].ef = E; ev[n].rate = fit * rate; ev[n].gen = 0; ++n;
+    ev[n].ei = E; ev[n].ef = E; ev[n].rate = g_plant_ac * fit * rate; ev[n].gen = 0; ++n;
   }
   if (s->cfg.optical) {
     double hw = 1.24 * (Q_ / 20.0);
@@ -413,7 +424,7 @@ static int scattering_events(const oracle_system *s, double E, sevent *ev) {
     }
     const double Nop = optical_phonon_number(hw, T);
     const double rho = 2330.0, DtK = 5e10 * Q_;
-    const double r = 0.7 * (pi * DtK * DtK * hbar / (2.0 * rho * hw));
+    const double r = g_plant_op * 0.7 * (pi * DtK * DtK * hbar / (2.0 * rho * hw));
     ev[n].ei = E;      ev[n].ef = E - hw; ev[n].rate = r * (Nop + 1.0); ev[n].gen = 0; ++n;
     ev[n].ei = E + hw; ev[n].ef = E;      ev[n].rate = r * (Nop + 1.0); ev[n].gen = 0; ++n;
     ev[n].ei = E;      ev[n].ef = E + hw; ev[n].rate = r * Nop;         ev[n].gen = 0; ++n;
@@ -497,7 +508,7 @@ static int assemble_streaming(oracle_system *s, int c, int f, int H, const cmat 
   int Lr = odd ? s->ord_f[(int64_t)f * s->NH + H] : s->ord_c[(int64_t)c * s->NH + H];
   int Lcol = odd ? s->ord_c[(int64_t)c * s->NH + H] : s->ord_f[(int64_t)f * s->NH + H];
   double lo = lower_kin_facet(s, f, H), up = upper_kin_facet(s, f, H);
-  double cdiff = integral_vZ(s, 1, f, H) * area;
+  double cdiff = g_plant_diff * integral_vZ(s, 1, f, H) * area;
   add_block_matrix(&s->full, row, col, cdiff, diff, Lr, odd, Lcol, !odd, 0);
   double izhk;
   if (integral_Z_over_hk(s->cfg.carrier, lo, up, &izhk)) return -1;
@@ -505,7 +516,7 @@ static int assemble_streaming(oracle_system *s, int c, int f, int H, const cmat 
   double ddx = s->cx[other] - s->cx[c], ddy = s->dim > 1 ? s->cy[other] - s->cy[c] : 0.0;
   double dist = sqrt(ddx * ddx + ddy * ddy);
   double force = (kin_c(s, c, H) - kin_c(s, other, H)) / dist;
-  double cdrift = izhk * force * area * conn_len / 2.0;
+  double cdrift = g_plant_drift * izhk * force * area * conn_len / 2.0;
   add_block_matrix(&s->full, row, col, cdrift, drift, Lr, odd, Lcol, !odd, 0);
   return 0;
 }

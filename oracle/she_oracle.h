@@ -119,6 +119,13 @@ double oracle_finish(const oracle_system *s, const double *x_reduced, double *x_
 void oracle_postproc(const oracle_system *s, const double *x_full, double *density,
                      double *facet_current);
 
+/* TEST ONLY: scale factors of the fill's diffusion coefficient (integral_vZ *
+ * area, assemble_streaming.hpp:127), drift coefficient (assemble_streaming.hpp:241-251),
+ * acoustic rate and optical rate (scattering/config.hpp:50-116); 1 by default.
+ * The physics pins use them to switch the drift term off and to plant a factor
+ * that their checks must catch.  Applies to every later oracle_build. */
+void oracle_set_plant(double diffusion, double drift, double acoustic, double optical);
+
 /* Modena density of states [1/(J m^3)] and velocity [m/s] at kinetic energy eps [J]. */
 void oracle_dispersion(int carrier, double eps, double *Z, double *v);
 

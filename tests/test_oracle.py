@@ -92,12 +92,11 @@ def test_coupling_fixture_is_the_coupling_integral(L):
 
 
 def test_dispersion_known_values():
-    """Modena band at 0.1 eV: density of states and velocity recorded from the
-    compiled reference headers (1.3717e45, 2.783e5 m/s), and the closed form."""
+    """Modena band at 0.1 eV: density of states and velocity against the closed
+    forms of physics/dispersion.hpp:233-270 (eps (1 + alpha eps) = hbar^2 k^2 / 2 m_d);
+    their magnitudes in the fill are fixed by tests/test_physics_pins.py."""
     from oracle import oracle as O
     Z, v = O.dispersion(0, 0.1 * Q)
-    assert abs(Z / 1.3717e45 - 1) < 1e-4
-    assert abs(v / 2.783e5 - 1) < 2e-4
     hbar = 6.62606896e-34 / (2 * math.pi)
     m0 = 9.10938291e-31
     md, ml, mt = 0.32 * m0, 0.98 * m0, 0.19 * m0
