@@ -16,7 +16,9 @@
  *
  * Conventions (mirroring libviennashe, libviennashe/src/viennashe_all.hpp:72-77):
  *   every function returns int: 0 on success, >0 = 1-based position of an
- *   invalid argument, <0 = internal / HIP error / no convergence.
+ *   invalid argument, <0 = internal / HIP error.  Non-convergence is not an
+ *   error: the solve returns 0 with report.status != 0 (as the reference's
+ *   solvers::solve returns its last iterate without throwing).
  *   shesolve_last_error() returns a message for the last failure of the calling thread.
  *   All calls are synchronous at the ABI (results are complete on return).
  *   A handle is not thread-safe; use one handle per host thread.

@@ -356,6 +356,50 @@ def test_invalid_rhs_is_rejected():
         sv.solve(bb)
 
 
+def test_failed_setup_leaves_no_matrix():
+    """A matrix the analysis rejects (a row without a diagonal entry, a row
+    pointer that decreases) leaves the handle without a matrix: a later solve
+    reports "no matrix set" instead of running on a half-built factor."""
+    import shesolve
+    s = oracle_system("diode_L1")
+    A, b, _, _ = s.reduced()
+    sv = _solver(precond=shesolve.PRECOND_ILU0)
+    sv.set_matrix(A.row_ptr, A.col, A.val)
+    # drop the diagonal entry of row 3
+    rp, ci, v = A.row_ptr.copy(), A.col.copy(), A.val.copy()
+    k = int(np.nonzero(ci[rp[3]:rp[4]] == 3)[0][0]) + int(rp[3])
+    ci, v = np.delete(ci, k), np.delete(v, k)
+    rp[4:] -= 1
+    with pytest.raises(shesolve.SheSolveError, match="no diagonal"):
+        sv.set_matrix(rp, ci, v)
+    with pytest.raises(shesolve.SheSolveError, match="no matrix set"):
+        sv.solve(b)
+    bad = A.row_ptr.copy()
+    bad[5], bad[6] = bad[6], bad[5]
+    with pytest.raises(shesolve.SheSolveError, match="decreases"):
+        sv.set_matrix(bad, A.col, A.val)
+    with pytest.raises(shesolve.SheSolveError, match="no matrix set"):
+        sv.solve(b)
+    # the handle takes a good matrix again
+    sv.set_matrix(A.row_ptr, A.col, A.val)
+    x, rep = sv.solve(b)
+    assert rep["status"] == 0
+
+
+def test_unconverged_fp64_reports_the_true_residual():
+    """An FP64 solve stopped by its iteration cap reports ||b - A x|| / ||b||
+    of the returned iterate, not the BiCGStab recurrence residual."""
+    import shesolve
+    s = oracle_system("mos2d_L3")
+    A, b, _, _ = s.reduced()
+    sv = _solver(precond=shesolve.PRECOND_ILU0, precision=shesolve.FP64, max_iters=3, tol=1e-14)
+    sv.set_matrix(A.row_ptr, A.col, A.val)
+    x, rep = sv.solve(b)
+    assert rep["status"] != 0
+    true = np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b)
+    assert rep["rel_residual"] == pytest.approx(true, rel=1e-8)
+
+
 def test_update_values_refactorises():
     s = oracle_system("diode_L3")
     A, b, _, _ = s.reduced()

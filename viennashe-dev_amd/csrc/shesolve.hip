@@ -2441,7 +2441,17 @@ int solve_core(shesolve_ctx *c, const double *b, double *x, int warm, shesolve_r
     r.iterations = it;
     r.status = st;
     r.rel_residual = res;
-    if (int e = ir_record(c, x, 0, it, 2, res)) return e;
+    if (st != 0) {
+      // the recurrence residual of an unconverged solve is not ||b - A x||: recompute it
+      double rn = 0, bn = 0;
+      if (int e = residual64(c, b, x, c->v64[8], &rn)) return e;
+      hipLaunchKernelGGL((k_copy_dot<double>), dim3(kGrid), dim3(kBlock), 0, c->stream, n, b, c->v64[0], c->d_part);
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 4, kGrid, c->d_part, c->d_sc, c->d_norm);
+      HIPCHK(hipMemcpyAsync(&bn, c->d_norm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      r.rel_residual = bn > 0 ? rn / bn : 0.0;
+    }
+    if (int e = ir_record(c, x, 0, it, 2, r.rel_residual)) return e;
   } else {
     // FP32 BiCGStab inside FP64 iterative refinement
     double *r64 = c->v64[8];
@@ -2548,6 +2558,8 @@ int set_pattern_host(shesolve_ctx *c, int64_t n, int64_t nnz, const int64_t *rp,
   if (n <= 0 || n > 0x7fffffff) { set_err("n out of range"); return 2; }
   if (nnz < 0 || nnz > 0x7fffffff) { set_err("nnz must fit a 32-bit index"); return 3; }
   if (rp[0] != 0 || rp[n] != nnz) { set_err("row_ptr inconsistent with nnz"); return 4; }
+  for (int64_t i = 0; i < n; ++i)
+    if (rp[i + 1] < rp[i]) { set_err("row_ptr decreases at row %lld", (long long)i); return 4; }
   c->h_rp.resize(n + 1);
   for (int64_t i = 0; i <= n; ++i) c->h_rp[i] = (int)rp[i];
   c->h_ci.assign(ci, ci + nnz);
@@ -2656,19 +2668,37 @@ int tune_stream(shesolve_ctx *c) {
   return 0;
 }
 
+int set_matrix_body(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, const int64_t *rp,
+                    const int32_t *ci, const double *val, int nblocks, const int64_t *ranges, int val_device);
+
+// Stores a matrix and prepares the solver; on any failure the handle is left
+// without a matrix (every solve then reports "no matrix set"), never with a
+// half-built one.
 int set_matrix_common(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, const int64_t *rp,
                       const int32_t *ci, const double *val, int nblocks, const int64_t *ranges,
                       int val_device = -1) {
   if (val_device < 0) val_device = device_src ? 1 : 0;
-  auto t0 = std::chrono::steady_clock::now();
   HIPCHK(hipSetDevice(c->device));
   free_matrix(c);
+  if (n <= 0 || n > 0x7fffffff) { set_err("n out of range"); return 2; }
+  const int e = set_matrix_body(c, device_src, n, nnz, rp, ci, val, nblocks, ranges, val_device);
+  if (e) {
+    hipStreamSynchronize(c->stream);
+    free_matrix(c);
+  }
+  return e;
+}
+
+int set_matrix_body(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, const int64_t *rp,
+                    const int32_t *ci, const double *val, int nblocks, const int64_t *ranges, int val_device) {
+  auto t0 = std::chrono::steady_clock::now();
   std::vector<int64_t> hrp;
   std::vector<int32_t> hci;
   if (device_src) {
     hrp.resize(n + 1);
     HIPCHK(hipMemcpy(hrp.data(), rp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
     nnz = hrp[n];
+    if (hrp[0] != 0 || nnz < 0 || nnz > 0x7fffffff) { set_err("row_ptr inconsistent: nnz %lld", (long long)nnz); return 4; }
     hci.resize(nnz);
     HIPCHK(hipMemcpy(hci.data(), ci, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
     rp = hrp.data();
