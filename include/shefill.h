@@ -134,6 +134,18 @@ int shefill_terminal_current(shefill_handle h, const double *facet_current, cons
                              double *current);
 int shefill_terminal_current_device(shefill_handle h, const double *d_facet_current,
                                     const unsigned char *d_terminal_mask, double *current);
+/* The device update above without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and error codes as shefill_density_update_device (the
+ * density bit for bit, the update norm bit for bit); its partial sums live in
+ * a fixed device array of the library, one per GPU, held until the process
+ * ends.  At most SHEFILL_NOALLOC_MAX_CELLS cells (returns 1 beyond).  All calls
+ * on one GPU share that array: calls that overlap in time on one GPU (two
+ * host threads, two streams) race and must be serialised by the caller. */
+#define SHEFILL_NOALLOC_MAX_CELLS (65536 * 256)
+int shefill_density_update_device_noalloc(shefill_handle h, const double *d_new, double *d_density, double alpha,
+                                          int logarithmic, double *update_norm);
 int shefill_density_update(shefill_handle h, const double *new_density, double *density, double alpha,
                            int logarithmic, double *update_norm);
 int shefill_density_update_device(shefill_handle h, const double *d_new, double *d_density, double alpha,

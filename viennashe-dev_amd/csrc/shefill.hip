@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -28,6 +29,14 @@
 #include "shefill.h"
 
 using namespace shephys;
+
+// Device scratch of shefill_density_update_device_noalloc: kNoAllocParts
+// per-workgroup partial sums of the update norm, then the norm itself.  Every
+// device the library runs on holds its own copy for the life of the process;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device (noalloc_scratch), never through this name.
+constexpr int kNoAllocParts = 65536;
+static __device__ __align__(256) double scratch[kNoAllocParts + 1];
 
 namespace {
 
@@ -399,6 +408,16 @@ __global__ void k_terminal_current(int nf, const int *fc1, const int *fc2, const
     for (int w = 0; w < kB / 64; ++w) t += red[w];
     part[blockIdx.x] = t;
   }
+}
+
+// Sums part[0..cnt) in index order in one thread -- the order of the host
+// loop of shefill_density_update_device, so the norm is the same bit for bit --
+// and stores the square root in *out.
+__global__ void k_sum_sqrt_ordered(int cnt, const double *__restrict__ part, double *__restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double t = 0.0;
+  for (int i = 0; i < cnt; ++i) t += part[i];
+  *out = sqrt(t);
 }
 
 struct RowArgs {
@@ -1238,6 +1257,30 @@ int launch_fill(shefill_ctx *c) {
   return 0;
 }
 
+// Address of this device's scratch, looked up once per device with that
+// device current (shefill_create); nullptr until then.
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_ptr;
+
+int noalloc_scratch_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (g_scratch_ptr[device]) return 0;
+  double *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    ferr("hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+    return -10;
+  }
+  g_scratch_ptr[device] = p;
+  return 0;
+}
+
+double *noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1248,6 +1291,7 @@ int shefill_create(shefill_handle *h, int device) {
   if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) { ferr("no HIP device available"); return -2; }
   if (device < 0 || device >= cnt) { ferr("device %d out of range", device); return 2; }
   FHIP(hipSetDevice(device));
+  if (int e = noalloc_scratch_setup(device)) return e;
   auto *c = new shefill_ctx();
   c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&c->e0) != hipSuccess ||
@@ -1491,6 +1535,34 @@ int shefill_density_update_device(shefill_handle h, const double *d_new, double 
     *update_norm = sqrt(t);
   }
   return e;
+}
+
+int shefill_density_update_device_noalloc(shefill_handle h, const double *d_new, double *d_density, double alpha,
+                                          int logarithmic, double *update_norm) {
+  if (!h) { ferr("handle is NULL"); return 1; }
+  if (!h->built) { ferr("nothing built"); return -1; }
+  if (!d_new) { ferr("new density is NULL"); return 2; }
+  if (!d_density) { ferr("density is NULL"); return 3; }
+  if (!(alpha > 0.0)) { ferr("damping must be positive"); return 4; }
+  const int nb = grid(h->nc);
+  if (nb > kNoAllocParts) {
+    ferr("%d cells exceed the %lld of the allocation-free update", h->nc, (long long)kNoAllocParts * kB);
+    return 1;
+  }
+  double *part = noalloc_scratch(h->device);
+  if (!part) { ferr("no scratch address for device %d", h->device); return -1; }
+  FHIP(hipSetDevice(h->device));
+  // every partial the sum reads is written by this launch: nothing to clear
+  hipLaunchKernelGGL(k_density_update, dim3(nb), dim3(kB), 0, h->stream, h->nc, d_new, d_density, alpha, logarithmic,
+                     part);
+  if (update_norm) {
+    hipLaunchKernelGGL(k_sum_sqrt_ordered, dim3(1), dim3(64), 0, h->stream, nb, (const double *)part,
+                       part + kNoAllocParts);
+    FHIP(hipMemcpyAsync(update_norm, part + kNoAllocParts, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  }
+  FHIP(hipGetLastError());
+  FHIP(hipStreamSynchronize(h->stream));
+  return 0;
 }
 
 int shefill_density_update(shefill_handle h, const double *new_density, double *density, double alpha,

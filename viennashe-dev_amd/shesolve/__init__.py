@@ -407,6 +407,7 @@ FILL_EXPORTS = ("shefill_last_error", "shefill_create", "shefill_destroy", "shef
                 "shefill_last_fill_ms", "shefill_refill", "shefill_recover", "shefill_recover_device",
                 "shefill_postproc", "shefill_postproc_device", "shefill_pattern_hash",
                 "shefill_density_update", "shefill_density_update_device",
+                "shefill_density_update_device_noalloc",
                 "shefill_terminal_current", "shefill_terminal_current_device")
 
 
@@ -445,6 +446,7 @@ def _fill_lib():
     L.shefill_density_update.argtypes = [C.c_void_p, _f64p, _f64p, C.c_double, C.c_int, C.POINTER(C.c_double)]
     L.shefill_density_update_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
                                                 C.POINTER(C.c_double)]
+    L.shefill_density_update_device_noalloc.argtypes = L.shefill_density_update_device.argtypes
     L.shefill_recover_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.shefill_postproc.argtypes = [C.c_void_p, _f64p, C.c_void_p, C.c_void_p]
     L.shefill_postproc_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -567,6 +569,16 @@ class Fill:
         _fcheck(_fill_lib().shefill_density_update_device(self._h, d_new, d_density, float(alpha),
                                                           1 if logarithmic else 0, C.byref(nrm)),
                 "shefill_density_update_device")
+        return nrm.value
+
+    def density_update_device_noalloc(self, d_new, d_density, alpha, logarithmic=False):
+        """The same update through the allocation-free entry point (its partial
+        sums in the library's fixed device scratch; calls on one GPU must not
+        overlap)."""
+        nrm = C.c_double()
+        _fcheck(_fill_lib().shefill_density_update_device_noalloc(self._h, d_new, d_density, float(alpha),
+                                                                  1 if logarithmic else 0, C.byref(nrm)),
+                "shefill_density_update_device_noalloc")
         return nrm.value
 
     def density_update(self, new_density, density, alpha=1.0, logarithmic=False):
