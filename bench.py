@@ -110,14 +110,16 @@ def cpu_baseline_parallel(rp, ci, val, b, ranges, iters, budget_s):
     }
 
 
-def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0):
+def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0, ordering=0):
     """The batched workload: `count` 1D diodes drawn from a seed (length
     U[100,1000] nm, n+ 20 % each side, L in {1,3,5}, dE U[2,10] meV to 1.0 eV,
     bias U[0,1.5] V), solved together by the batched FP32+IR BiCGStab (one
     launch per phase over all systems, per-system masks, graph-replayed
-    iterations).  Preconditioner: ILU(0) per system, ILU(1) for L = 5 (ILU(0)
-    diverges there).  A step = GPU refill of every system + refactorisation +
-    batched solve to 1e-10."""
+    iterations).  Preconditioner: ILU(k) of each per-energy slab (the block
+    ranges of she::solve, she/linear_solver.hpp:42-83), k = 0, and k = 1 for
+    L = 5 (ILU(0) diverges there); the slabs of all systems are swept side by
+    side, one workgroup each.  A step = GPU refill of every system +
+    refactorisation + batched solve to 1e-10."""
     if count <= 0:
         return {}
     import ctypes
@@ -128,21 +130,22 @@ def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0):
     fills, Ls = [], []
     for dev, cfg in D.batch_diodes(count, seed):
         F = shesolve.Fill(0)
-        # the reference's numbering: numbered from both ends, the L = 5 systems
-        # (which set the batch time) converge worse with ILU(1)
+        cfg.ordering = ordering
         F.build(dev, cfg)
         fills.append(F)
         Ls.append(cfg.L_max)
     t_build = time.perf_counter() - t0
     ns = [F.n for F in fills]
     ptr = [F.device_system() for F in fills]
+    ranges = [F.block_ranges() for F in fills]
     levels = [1 if L >= 5 else 0 for L in Ls]
 
     def make(precision):
-        bs = shesolve.BatchSolver(0, precond=shesolve.PRECOND_ILUK, precision=precision, tol=1e-10, max_iters=3000)
+        bs = shesolve.BatchSolver(0, precond=shesolve.PRECOND_BLOCK_ILUK, precision=precision, tol=1e-10,
+                                  max_iters=3000)
         bs.set_fill_levels(levels)
         t = time.perf_counter()
-        bs.set_systems_device(ns, [p[0] for p in ptr], [p[1] for p in ptr], [p[2] for p in ptr])
+        bs.set_systems_device(ns, [p[0] for p in ptr], [p[1] for p in ptr], [p[2] for p in ptr], block_ranges=ranges)
         return bs, time.perf_counter() - t
 
     dx = []
@@ -182,7 +185,11 @@ def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0):
         "iterations_max": max(r["iterations"] for r in reps),
         "iterations_mean": float(np.mean([r["iterations"] for r in reps])),
         "max_rel_residual": max(r["rel_residual"] for r in reps),
-        "setup_s": t_setup, "build_s": t_build, "precond": "ILU(0); ILU(1) for L=5",
+        "slowest": [{"system": i, "rows": ns[i], "L": Ls[i], "iterations": reps[i]["iterations"],
+                     "ir_steps": reps[i]["ir_steps"], "rel_residual": reps[i]["rel_residual"]}
+                    for i in sorted(range(count), key=lambda i: -reps[i]["iterations"])[:6]],
+        "setup_s": t_setup, "build_s": t_build,
+        "precond": "per-energy block ILU(k): k=0, k=1 for L=5; one workgroup per slab",
     }
     if fp64:
         bs.set_params(precision=shesolve.FP64)
@@ -237,8 +244,8 @@ def main():
     ap.add_argument("--profile-kernels", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=1)
-    ap.add_argument("--batch-count", type=int, default=24,
-                    help="systems of the batched workload (the full family is 96; the first N of the seeded draw)")
+    ap.add_argument("--batch-count", type=int, default=96,
+                    help="systems of the batched workload (the seeded family of 96; fewer takes the first N)")
     ap.add_argument("--batch-fp64", action="store_true", help="also time the batch in plain FP64")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=40)

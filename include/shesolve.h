@@ -52,6 +52,11 @@ enum {
                                        ILUT's threshold rule inside that pattern.  The
                                        GPU form of the ILUT(drop, fill) slot of the
                                        reference config (solvers/config.hpp:166-186) */
+  SHESOLVE_PRECOND_BLOCK_ILUK = 7,  /* ILU(k) of the diagonal blocks (per-energy slabs), k =
+                                       fill_level (per system in batches); k = 0 is BLOCK_ILU0.
+                                       The reference's block_ilu_precond with ILU(k) in place of
+                                       its ilu0_tag (parallel_linear_solver.hpp:79-93); the
+                                       ranges must be contiguous row ranges */
   SHESOLVE_PRECOND_BMC_ILU0 = 6     /* ILU(0) of the matrix in a block multicolour ordering:
                                        blocks of 32 consecutive rows coloured greedily, a
                                        block's rows kept together in natural order; one

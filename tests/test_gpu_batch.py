@@ -38,17 +38,17 @@ def _systems(names):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
-@pytest.mark.parametrize("kind", ["ilu0", "block_ilu0", "iluk"])
+@pytest.mark.parametrize("kind", ["ilu0", "block_ilu0", "iluk", "block_iluk"])
 def test_batch_parity(precision, kind):
     import shesolve
-    # ILU(1) also takes the L=5 diode, on which ILU(0) diverges
-    names = MIX + ["diode_L5"] if kind == "iluk" else MIX
+    # ILU(1), whole or per energy slab, also takes the L=5 diode, on which ILU(0) diverges
+    names = MIX + ["diode_L5"] if kind in ("iluk", "block_iluk") else MIX
     sy = _systems(names)
     pc = {"ilu0": shesolve.PRECOND_ILU0, "block_ilu0": shesolve.PRECOND_BLOCK_ILU0,
-          "iluk": shesolve.PRECOND_ILUK}[kind]
+          "iluk": shesolve.PRECOND_ILUK, "block_iluk": shesolve.PRECOND_BLOCK_ILUK}[kind]
     prec = shesolve.FP64 if precision == "fp64" else shesolve.FP32_IR
     bs = shesolve.BatchSolver(0, precond=pc, precision=prec, tol=1e-10, max_iters=5000, fill_level=1)
-    ranges = [oracle_system(nm).block_ranges() for nm in names] if kind == "block_ilu0" else None
+    ranges = [oracle_system(nm).block_ranges() for nm in names] if kind.startswith("block") else None
     bs.set_systems([(A.row_ptr, A.col, A.val) for A, _ in sy], block_ranges=ranges)
     info = bs.info()
     assert info["systems"] == len(names) and info["rows"] == sum(len(b) for _, b in sy)
@@ -188,6 +188,31 @@ def test_batch_preconditioner_is_the_per_system_ilu(kind):
         zr = P.apply(r[off:off + A.n])
         zs = z[off:off + A.n]
         assert np.linalg.norm(zs - zr) <= 1e-10 * np.linalg.norm(zr), nm
+        off += A.n
+    bs.close()
+
+
+def test_batch_block_iluk_is_the_per_slab_ilu():
+    """Per-energy block ILU(k) in a batch, one workgroup per slab of every
+    system: M^-1 r equals, system by system, the oracle's ILU(k) of that
+    system's block-diagonal part (k = 1 for the L=5 diode, 0 otherwise)."""
+    import shesolve
+    from oracle import oracle as O
+    from test_gpu_solver import block_diagonal
+    names = ["diode_L1", "diode_L3", "nin2d_L3", "mos2d_L1", "diode_L5"]
+    sy = _systems(names)
+    ranges = [oracle_system(nm).block_ranges() for nm in names]
+    levels = [1 if nm == "diode_L5" else 0 for nm in names]
+    bs = shesolve.BatchSolver(0, precond=shesolve.PRECOND_BLOCK_ILUK, tol=1e-10, max_iters=5000)
+    bs.set_fill_levels(levels)
+    bs.set_systems([(A.row_ptr, A.col, A.val) for A, _ in sy], block_ranges=ranges)
+    r = np.random.default_rng(11).standard_normal(sum(len(b) for _, b in sy))
+    z = bs.precond_apply(r)
+    off = 0
+    for nm, (A, b), rg, k in zip(names, sy, ranges, levels):
+        P = O.Precond(O.PC_ILUK, block_diagonal(A, rg), drop=0.0, fill=k)
+        zr = P.apply(r[off:off + A.n])
+        assert np.linalg.norm(z[off:off + A.n] - zr) <= 1e-10 * np.linalg.norm(zr), nm
         off += A.n
     bs.close()
 

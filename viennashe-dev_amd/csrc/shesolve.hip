@@ -129,6 +129,20 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ 
   }
 }
 
+// Per-system state of a batched BiCGStab (see the batched kernels below).
+struct BSys {
+  double rho, alpha, omega, beta;
+  double bnorm, res, tol, div_limit;
+  int active;     // 1 while the system iterates
+  int status;     // 0 converged, 1 iteration cap, 2 breakdown / divergence
+  int iters;      // iterations of the current solve
+  int max_iters;
+  int bd, best_it;
+  double best;    // best residual so far (stall test of FP32 inner solves)
+  int stall;      // iterations without a 1 % gain before giving up (0 = off)
+  int pad;
+};
+
 // ---------------------------------------------------------------- CSR-stream kernels
 // A row block is a run of consecutive rows holding at most kStreamNnz entries
 // (a single longer row forms a block of its own).  The block's entries are
@@ -156,7 +170,9 @@ constexpr int kStreamRows = 256;  // rows per block (rows without entries must n
 #endif
 constexpr int kSegJ = SHESOLVE_SEG_J;
 constexpr int kSegLprMax = SHESOLVE_SEG_LPR_MAX;
-constexpr double kSegLevelUs = 1.4;      // segment sweep: per level ...
+constexpr double kSegLevelUs = 1.4;      // segment sweep of a 1024-thread workgroup: per level ...
+constexpr double kSegLevelUs256 = 1.0;   // ... of a 256-thread workgroup
+constexpr double kSegLevelUs64 = 0.8;    // ... of a 64-thread workgroup
 constexpr double kSegEntryUs = 6e-4;     // ... plus per entry (and row) of the level
 constexpr double kLevelLaunchUs = 5.0;   // one launch per level
 constexpr int kSweepItems = SHESOLVE_SWEEP_ITEMS;
@@ -202,7 +218,8 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
                                                            const T *__restrict__ dinv, T *__restrict__ out2,
                                                            const T *__restrict__ d0, const T *__restrict__ d1,
                                                            double *__restrict__ part, const int *__restrict__ pairs,
-                                                           int c0) {
+                                                           int c0, const int *__restrict__ blk_sys,
+                                                           const BSys *__restrict__ skip) {
   constexpr int kStreamNnz = ITEMS * kStreamThreads;
   __shared__ T prod[kStreamNnz];
   double a0 = 0.0, a1 = 0.0;
@@ -214,6 +231,8 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
     bstep = per;
   }
   for (int bk = bk0; bk < bk1; bk += bstep) {
+    // batched iterations: row blocks of converged systems are left alone
+    if (skip && !skip[blk_sys[bk]].active) continue;
     // consecutive row blocks, or an explicit (first, end) pair per block
     const int rs = pairs ? pairs[2 * bk] : blocks[bk], re = pairs ? pairs[2 * bk + 1] : blocks[bk + 1];
     const int k0 = rp[rs], k1 = rp[re];
@@ -770,18 +789,6 @@ constexpr int kStallWindow = 100; // FP32 inner solves: iterations without a 1 %
 constexpr int kBatchStallWindow = 40;  // the same in batches, where a stalled system holds up the batch
 constexpr double kStallKeep = 0.1;     // batches: a stalled FP32 correction that cut the residual 10x keeps FP32
 
-struct BSys {
-  double rho, alpha, omega, beta;
-  double bnorm, res, tol, div_limit;
-  int active;     // 1 while the system iterates
-  int status;     // 0 converged, 1 iteration cap, 2 breakdown / divergence
-  int iters;      // iterations of the current solve
-  int max_iters;
-  int bd, best_it;
-  double best;    // best residual so far (stall test of FP32 inner solves)
-  int stall;      // iterations without a 1 % gain before giving up (0 = off)
-  int pad;
-};
 
 // NV dot products per system into part[(q * S + s) * kBatchParts + j]
 // (systems with mask[s] == 0, or with bs[s].active == 0 when mask is NULL, are skipped)
@@ -958,7 +965,6 @@ __global__ void k_binit(int S, BSys *bs, const int *__restrict__ mask, double to
 // when a new chunk of level pointers is staged in LDS.
 //   lower: zq[q] = rhs[q] - sum L[k] x(cq[k])
 //   upper: zq[q] = dq[q] (rhs[q] - sum U[k] x(cq[k]))
-constexpr int kSweepThreads = 1024;
 constexpr int kLvlChunk = 256;
 __device__ inline void lds_barrier() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -967,18 +973,18 @@ __device__ inline void lds_barrier() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-template <int LPR, int R, int J, typename TV, typename T, bool UPPER>
-__global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict__ seg_lvl, const int *__restrict__ lvl_ptr,
-                                                           const int *__restrict__ lrp, const int *__restrict__ cq,
-                                                           const TV *__restrict__ v, const T *__restrict__ rhs, T *zq,
-                                                           const TV *__restrict__ dq, const BSys *__restrict__ bs,
-                                                           int W) {
+template <int NT, int LPR, int R, int J, typename TV, typename T, bool UPPER>
+__global__ __launch_bounds__(NT) void k_wgsweep(const int *__restrict__ seg_lvl, const int *__restrict__ lvl_ptr,
+                                                const int *__restrict__ lrp, const int *__restrict__ cq,
+                                                const TV *__restrict__ v, const T *__restrict__ rhs, T *zq,
+                                                const TV *__restrict__ dq, const BSys *__restrict__ bs,
+                                                const int *__restrict__ seg_sys, int W) {
   extern __shared__ unsigned char lds_raw[];
   T *win = reinterpret_cast<T *>(lds_raw);
   __shared__ int lp[kLvlChunk + 1];
   const int s = blockIdx.x;
-  if (bs && !bs[s].active) return;
-  constexpr int G = kSweepThreads / LPR;
+  if (bs && !bs[seg_sys ? seg_sys[s] : s].active) return;
+  constexpr int G = NT / LPR;
   const int Wm = W - 1;
   const int l0 = seg_lvl[s], l1 = seg_lvl[s + 1];
   if (l0 >= l1) return;
@@ -1012,7 +1018,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict
   };
   // level pointers of the segment, staged in LDS kLvlChunk levels at a time
   int lc = l0;  // first level of the staged chunk
-  for (int t = threadIdx.x; t <= kLvlChunk && lc + t <= l1; t += kSweepThreads) lp[t] = lvl_ptr[lc + t];
+  for (int t = threadIdx.x; t <= kLvlChunk && lc + t <= l1; t += NT) lp[t] = lvl_ptr[lc + t];
   __syncthreads();
   auto qp = [&](int l) { return (l - lc <= kLvlChunk) ? lp[l - lc] : lvl_ptr[l]; };
   int q_safe = lp[0];  // positions below q_safe are visible in zq to every wave
@@ -1072,7 +1078,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_wgsweep(const int *__restrict
       // stage the next chunk of level pointers (levels l + 1 ..)
       __syncthreads();
       lc = l + 1;
-      for (int t = threadIdx.x; t <= kLvlChunk && lc + t <= l1; t += kSweepThreads) lp[t] = lvl_ptr[lc + t];
+      for (int t = threadIdx.x; t <= kLvlChunk && lc + t <= l1; t += NT) lp[t] = lvl_ptr[lc + t];
       __syncthreads();
       q_safe = q1;
     } else if (nq1 - W > q_safe) {
@@ -1160,9 +1166,12 @@ struct shesolve_ctx {
   // range [bmc_cptr[q], bmc_cptr[q+1]) of its blocks
   int *d_bmc_start = nullptr;
   std::vector<int> bmc_cptr;
-  // CSR-stream row blocks of A and the tuned launch shape
+  // CSR-stream row blocks of A and the tuned launch shape; in a batch no block
+  // straddles two systems, and d_blk_sys holds each block's system
   int *d_ablk = nullptr;
   int nablk = 0;
+  int *d_blk_sys = nullptr;
+  const BSys *spmv_skip = nullptr;  // set during a batched iteration: SpMV skips converged systems
   int items = 8;   // entries per thread of a row block (row block = 256 * items entries)
   int xcd = 1;     // XCD-contiguous block mapping
   int spmv_grid = kGrid;  // workgroups of the last SpMV with fused dot products
@@ -1183,7 +1192,13 @@ struct shesolve_ctx {
   // rows, and per segment its levels (seg -> level range, level -> row range
   // of the level-ordered row list) for the one-workgroup-per-segment sweeps
   std::vector<int> seg_ptr;
-  std::vector<int> seg_fill;  // ILU(k) level per segment (batches; empty = prm.fill_level)
+  std::vector<int> seg_fill;  // ILU(k) level per system of a batch (empty = prm.fill_level)
+  // segments of the factor the sweeps walk: the systems of a batch, or for the
+  // block preconditioners the per-energy slabs (of every system); fseg_sys[s]
+  // is the system of segment s (its convergence mask)
+  std::vector<int> fseg_ptr, fseg_sys;
+  int *d_fseg_sys = nullptr;
+  int seg_nt = 1024;  // threads of a segment sweep's workgroup
   int nseg = 0;
   int *d_segL = nullptr, *d_lvlL = nullptr, *d_rowsL = nullptr;
   int *d_segU = nullptr, *d_lvlU = nullptr, *d_rowsU = nullptr;
@@ -1295,7 +1310,8 @@ void launch_stream(shesolve_ctx *c, int g, int nblk, const int *blocks, const in
 #define ST_CASE(IT)                                                                                                \
   case IT:                                                                                                         \
     hipLaunchKernelGGL((k_stream<TV, T, MODE, IT>), dim3(g), dim3(kStreamThreads), 0, c->stream, nblk, c->xcd,     \
-                       blocks, rp, ci, v, x, y, b, perm, dinv, out2, d0, d1, part, pairs, c0);                     \
+                       blocks, rp, ci, v, x, y, b, perm, dinv, out2, d0, d1, part, pairs, c0, c->d_blk_sys,        \
+                       MODE == MODE_SPMV ? c->spmv_skip : (const BSys *)nullptr);                                 \
     break;
   switch (items) { ST_CASE(4) ST_CASE(16) default: ST_CASE(8) }
 #undef ST_CASE
@@ -1336,10 +1352,13 @@ void launch_spmv(shesolve_ctx *c, int /*lpr*/, const TV *v, const T *x, T *y, co
 template <typename T>
 constexpr int kSweepRowsL = sizeof(T) == 8 ? 2 : 4;
 
-// LDS window of the segment sweeps, a power of two number of entries
+// LDS window of the segment sweeps, a power of two number of entries: 32 KiB
+// beside the level-pointer chunk for a 1024-thread workgroup, less for the
+// smaller ones so that several fit a CU
 template <typename T>
-int sweep_window() {
-  return (int)(32768 / sizeof(T));  // 32 KiB of LDS beside the level-pointer chunk
+int sweep_window(int nt) {
+  const int bytes = nt >= 1024 ? 32768 : (nt >= 256 ? 16384 : 4096);
+  return (int)(bytes / sizeof(T));
 }
 
 template <typename TV, typename T>
@@ -1377,7 +1396,7 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
   }
   if (c->nseg > 0 && !perm) {
     KTimer kt(c, 1);
-    const int W = sweep_window<T>();
+    const int W = sweep_window<T>(c->seg_nt);
     const size_t lds = sizeof(T) * (size_t)W;
     const int n = c->n, g = grid_for(n, kBlock);
     T *tq, *zq;
@@ -1387,17 +1406,24 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     // in (natural) -> forward schedule order; forward sweep; -> backward order;
     // backward sweep; -> natural order
     hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsL, in, tq);
-#define WG_CASE(LL)                                                                                                \
+#define WG_CASE(NT, LL)                                                                                            \
   case LL:                                                                                                         \
-    hipLaunchKernelGGL((k_wgsweep<LL, kSweepRowsL<T>, kSegJ, TV, T, false>), dim3(c->nseg), dim3(kSweepThreads), lds, \
+    hipLaunchKernelGGL((k_wgsweep<NT, LL, kSweepRowsL<T>, kSegJ, TV, T, false>), dim3(c->nseg), dim3(NT), lds,      \
                        c->stream,                                                                                  \
                        c->d_segL, c->d_lvlL, c->d_loLrp, c->d_Lci, Lv, (const T *)tq, zq, (const TV *)nullptr,     \
-                       c->seg_mask, W);                                                                            \
+                       c->seg_mask, c->d_fseg_sys, W);                                                             \
     hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_mapLU, (const T *)zq, tq);  \
-    hipLaunchKernelGGL((k_wgsweep<LL, 2, kSegJ, TV, T, true>), dim3(c->nseg), dim3(kSweepThreads), lds, c->stream, \
-                       c->d_segU, c->d_lvlU, c->d_loUrp, c->d_Uci, Uv, (const T *)tq, zq, dq, c->seg_mask, W);     \
+    hipLaunchKernelGGL((k_wgsweep<NT, LL, 2, kSegJ, TV, T, true>), dim3(c->nseg), dim3(NT), lds, c->stream,        \
+                       c->d_segU, c->d_lvlU, c->d_loUrp, c->d_Uci, Uv, (const T *)tq, zq, dq, c->seg_mask,         \
+                       c->d_fseg_sys, W);                                                                          \
     break;
-    switch (c->seg_lpr) { WG_CASE(2) WG_CASE(4) default: WG_CASE(8) }
+    if (c->seg_nt >= 1024) {
+      switch (c->seg_lpr) { WG_CASE(1024, 2) WG_CASE(1024, 4) default: WG_CASE(1024, 8) }
+    } else if (c->seg_nt >= 256) {
+      switch (c->seg_lpr) { WG_CASE(256, 2) WG_CASE(256, 4) default: WG_CASE(256, 8) }
+    } else {
+      switch (c->seg_lpr) { WG_CASE(64, 2) WG_CASE(64, 4) default: WG_CASE(64, 8) }
+    }
 #undef WG_CASE
     hipLaunchKernelGGL((k_qscatter<T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsU, (const T *)zq, out);
     return;
@@ -1481,6 +1507,8 @@ void free_matrix(shesolve_ctx *c) {
   c->s_rp = c->s_ci = c->s_blk = c->s_pairs = nullptr;
   c->s_nblk = 0;
   dfree(c->d_ablk); c->nablk = 0;
+  dfree(c->d_blk_sys);
+  c->spmv_skip = nullptr;
   dfree(c->d_Lrp); dfree(c->d_Lci); dfree(c->d_Urp); dfree(c->d_Uci); dfree(c->d_Lsrc);
   dfree(c->d_L64); dfree(c->d_U64); dfree(c->d_dinv64); dfree(c->d_L32); dfree(c->d_U32); dfree(c->d_dinv32);
   dfree(c->d_lblk); dfree(c->d_ublk);
@@ -1491,6 +1519,9 @@ void free_matrix(shesolve_ctx *c) {
   dfree(c->d_segU); dfree(c->d_lvlU); dfree(c->d_rowsU);
   dfree(c->d_loLrp); dfree(c->d_loUrp); dfree(c->d_mapLU);
   dfree(c->d_dq64); dfree(c->d_tq64); dfree(c->d_zq64); dfree(c->d_dq32); dfree(c->d_tq32); dfree(c->d_zq32);
+  dfree(c->d_fseg_sys);
+  c->fseg_ptr.clear();
+  c->fseg_sys.clear();
   c->nseg = 0;
 }
 
@@ -1542,31 +1573,51 @@ void compute_levels(const std::vector<int> &rp, const std::vector<int> &ci, cons
 
 // Whether the segment sweeps (one workgroup per segment, the segments side by
 // side) beat one launch per level of the whole matrix, estimated on the
-// forward triangle: a workgroup spends about 1.4 us + 0.6 ns per entry on a
-// level of its segment, a launch about 5 us per level.  Deep schedules of
-// narrow levels (1D devices) go to the segments; wide levels, where one CU
-// per segment runs out of bandwidth, to the launches.
-bool segment_sweeps_pay(const shesolve_ctx *c, const std::vector<int> &Lptr, const std::vector<int> &Lrows) {
-  const auto &sp = c->seg_ptr;
+// forward triangle, and the workgroup size of the segment sweeps.  A
+// workgroup spends a fixed time per level of its segment (kSegLevelUs for
+// 1024 threads, less for the smaller groups, whose barriers are cheaper) plus
+// kSegEntryUs per entry; the segments run side by side up to the number of
+// workgroups the chip holds at once.  One launch per level costs about
+// kLevelLaunchUs.  Narrow levels (1D devices, per-energy slabs) take small
+// workgroups; wide levels, where one CU runs out of bandwidth, the launches.
+bool segment_sweeps_pay(shesolve_ctx *c, const std::vector<int> &Lptr, const std::vector<int> &Lrows) {
+  const auto &sp = c->fseg_ptr;
   const int S = (int)sp.size() - 1, nl = (int)Lptr.size() - 1;
   if (S < 1 || nl < 1) return false;
-  std::vector<double> cost(S, 0.0);
-  std::vector<int64_t> ent(S, 0);
+  std::vector<double> ent(S, 0.0);
+  std::vector<int> lev(S, 0), rows(S, 0);
   std::vector<int> touched;
+  std::vector<int64_t> cur(S, 0);
+  std::vector<double> entries_cost(S, 0.0);
   for (int l = 0; l < nl; ++l) {
     touched.clear();
     for (int q = Lptr[l]; q < Lptr[l + 1]; ++q) {
       const int i = Lrows[q];
-      const int s = (int)(std::upper_bound(sp.begin(), sp.end(), i) - sp.begin()) - 1;
-      if (ent[s] == 0) touched.push_back(s);
-      ent[s] += 1 + (c->h_fdg[i] - c->h_frp[i]);
+      const int sg = (int)(std::upper_bound(sp.begin(), sp.end(), i) - sp.begin()) - 1;
+      if (cur[sg] == 0) touched.push_back(sg);
+      cur[sg] += 1 + (c->h_fdg[i] - c->h_frp[i]);
+      rows[sg] += 1;
     }
-    for (int s : touched) {
-      cost[s] += kSegLevelUs + kSegEntryUs * (double)ent[s];
-      ent[s] = 0;
+    for (int sg : touched) {
+      lev[sg] += 1;
+      entries_cost[sg] += kSegEntryUs * (double)cur[sg];
+      cur[sg] = 0;
     }
   }
-  const double seg = *std::max_element(cost.begin(), cost.end());
+  // workgroup size from the mean level width (rows per level of a segment)
+  int64_t tot_rows = 0, tot_lev = 0;
+  for (int sg = 0; sg < S; ++sg) { tot_rows += rows[sg]; tot_lev += lev[sg]; }
+  const double width = tot_lev ? (double)tot_rows / (double)tot_lev : 1.0;
+  c->seg_nt = width <= 16.0 ? 64 : (width <= 128.0 ? 256 : 1024);
+  const double level_us = c->seg_nt >= 1024 ? kSegLevelUs : (c->seg_nt >= 256 ? kSegLevelUs256 : kSegLevelUs64);
+  const double slots = 256.0 * (c->seg_nt >= 1024 ? 2 : (c->seg_nt >= 256 ? 8 : 16));
+  double mx = 0.0, sum = 0.0;
+  for (int sg = 0; sg < S; ++sg) {
+    const double t = level_us * lev[sg] + entries_cost[sg];
+    mx = std::max(mx, t);
+    sum += t;
+  }
+  const double seg = std::max(mx, sum / slots);
   return seg < kLevelLaunchUs * nl;
 }
 
@@ -1575,14 +1626,15 @@ bool segment_sweeps_pay(const shesolve_ctx *c, const std::vector<int> &Lptr, con
 // ordered by level, ascending row within a level.
 int segment_levels(shesolve_ctx *c) {
   const int n = c->n;
-  const int S = (int)c->seg_ptr.size() - 1;
+  const std::vector<int> &sp = c->fseg_ptr;
+  const int S = (int)sp.size() - 1;
   const auto &rp = c->h_frp, &ci = c->h_fci, &dg = c->h_fdg;
   std::vector<int> lev(n);
   for (int s = 0; s < S; ++s)
-    for (int i = c->seg_ptr[s]; i < c->seg_ptr[s + 1]; ++i)
+    for (int i = sp[s]; i < sp[s + 1]; ++i)
       for (int k = rp[i]; k < rp[i + 1]; ++k)
-        if (ci[k] < c->seg_ptr[s] || ci[k] >= c->seg_ptr[s + 1]) {
-          set_err("factor entry (%d, %d) couples two systems", i, ci[k]);
+        if (ci[k] < sp[s] || ci[k] >= sp[s + 1]) {
+          set_err("factor entry (%d, %d) couples two segments", i, ci[k]);
           return 2;
         }
   c->seg_levels_max = 0;
@@ -1590,7 +1642,7 @@ int segment_levels(shesolve_ctx *c) {
     std::vector<int> seg(S + 1, 0), lvl(1, 0), rows;
     rows.reserve(n);
     for (int s = 0; s < S; ++s) {
-      const int r0 = c->seg_ptr[s], r1 = c->seg_ptr[s + 1];
+      const int r0 = sp[s], r1 = sp[s + 1];
       int maxl = -1;
       if (!upper) {
         for (int i = r0; i < r1; ++i) {
@@ -1629,6 +1681,11 @@ int segment_levels(shesolve_ctx *c) {
     HIPCHK(hipMemcpy(drows, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice));
     (upper ? c->h_segrowsU : c->h_segrowsL).swap(rows);
   }
+  if (!c->seg_ptr.empty()) {
+    // batch: the segment's system holds its convergence mask
+    HIPCHK(hipMalloc(&c->d_fseg_sys, sizeof(int) * S));
+    HIPCHK(hipMemcpy(c->d_fseg_sys, c->fseg_sys.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+  }
   c->nseg = S;
   return 0;
 }
@@ -1641,7 +1698,7 @@ int segment_levels(shesolve_ctx *c) {
 // Output row lengths in len[i - r0], columns and the A-entry of each factor
 // entry (-1 for fill) appended to fci / pos.
 int iluk_rows(const std::vector<int> &rp, const std::vector<int> &ci, int r0, int r1, int k, std::vector<int> &len,
-              std::vector<int> &fci, std::vector<int> &pos) {
+              std::vector<int> &fci, std::vector<int> &pos, bool drop_outside = false) {
   const int m = r1 - r0;
   std::vector<int> frp(m + 1, 0), udg(m);
   std::vector<unsigned char> flev;
@@ -1653,7 +1710,10 @@ int iluk_rows(const std::vector<int> &rp, const std::vector<int> &ci, int r0, in
     const int gi = r0 + i;
     for (int q = rp[gi]; q < rp[gi + 1]; ++q) {
       const int cl = ci[q] - r0;
-      if (cl < 0 || cl >= m) return -1;
+      if (cl < 0 || cl >= m) {
+        if (drop_outside) continue;  // block ILU(k): only the diagonal block is factorised
+        return -1;
+      }
       lev[cl] = 0;
       src[cl] = q;
       touched.push_back(cl);
@@ -1692,16 +1752,18 @@ int iluk_rows(const std::vector<int> &rp, const std::vector<int> &ci, int r0, in
   return 0;
 }
 
-// ILU(k) pattern of the whole matrix: one diagonal block per segment (systems
-// of a batch, built in parallel, each with its own level) or the whole matrix.
-// pos maps factor entries to A's entries, -1 for fill.
-int iluk_pattern(shesolve_ctx *c, std::vector<int> &pos) {
+// ILU(k) pattern of the whole matrix: one diagonal block per factor segment
+// (systems of a batch, or per-energy slabs for BLOCK_ILUK, built in parallel,
+// each with its own level), entries outside a segment dropped when
+// drop_outside.  pos maps factor entries to A's entries, -1 for fill.
+int iluk_pattern(shesolve_ctx *c, std::vector<int> &pos, bool drop_outside) {
   const int n = c->n;
-  std::vector<int> seg = c->seg_ptr;
-  if (seg.empty()) seg = {0, n};
+  const std::vector<int> &seg = c->fseg_ptr;
   const int S = (int)seg.size() - 1;
   std::vector<int> kseg(S, c->prm.fill_level);
-  if ((int)c->seg_fill.size() == S) kseg = c->seg_fill;
+  const int nsys = c->seg_ptr.empty() ? 1 : (int)c->seg_ptr.size() - 1;
+  if ((int)c->seg_fill.size() == nsys)
+    for (int q = 0; q < S; ++q) kseg[q] = c->seg_fill[c->fseg_sys[q]];
   std::vector<std::vector<int>> lens(S), fcis(S), poss(S);
   std::vector<int> err(S, 0);
   std::atomic<int> next{0};
@@ -1709,7 +1771,7 @@ int iluk_pattern(shesolve_ctx *c, std::vector<int> &pos) {
     for (;;) {
       const int s = next.fetch_add(1);
       if (s >= S) break;
-      err[s] = iluk_rows(c->h_rp, c->h_ci, seg[s], seg[s + 1], kseg[s], lens[s], fcis[s], poss[s]);
+      err[s] = iluk_rows(c->h_rp, c->h_ci, seg[s], seg[s + 1], kseg[s], lens[s], fcis[s], poss[s], drop_outside);
     }
   };
   const int nt = std::max(1, std::min<int>(S, (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
@@ -1744,7 +1806,43 @@ int analyse_and_factor(shesolve_ctx *c) {
   // factor pattern (in the factor's own index space) and the gather map from A
   if (c->prm.precond == SHESOLVE_PRECOND_NONE) return 0;
   std::vector<int> pos, perm;
-  if (c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILU0 && !c->blk_of_row.empty()) {
+  // Segments of the factor: the systems of a batch (or the whole matrix), cut
+  // further into the blocks of the block preconditioners when every block is
+  // one contiguous row range inside one system (the per-energy slabs): the
+  // block factor never couples two blocks, so each is swept on its own.
+  {
+    std::vector<int> sys = c->seg_ptr.empty() ? std::vector<int>{0, n} : c->seg_ptr;
+    c->fseg_ptr = sys;
+    const bool block = (c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILU0 ||
+                        c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILUK) && !c->blk_of_row.empty();
+    if (block) {
+      std::vector<int> runs(1, 0);
+      std::vector<char> seen;
+      bool ok = true;
+      for (int i = 1; i <= n && ok; ++i) {
+        if (i < n && c->blk_of_row[i] == c->blk_of_row[i - 1]) continue;
+        const int b = c->blk_of_row[i - 1];
+        if (b >= (int)seen.size()) seen.resize(b + 1, 0);
+        if (seen[b]) ok = false;  // a block in two pieces
+        seen[b] = 1;
+        runs.push_back(i);
+      }
+      // every system boundary must be a block boundary
+      for (int b : sys) ok = ok && std::binary_search(runs.begin(), runs.end(), b);
+      if (ok) c->fseg_ptr = runs;
+      else if (c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILUK) {
+        set_err("BLOCK_ILUK needs block ranges that are contiguous row ranges inside one system");
+        return 2;
+      }
+    }
+    const int S = (int)c->fseg_ptr.size() - 1;
+    c->fseg_sys.assign(S, 0);
+    for (int q = 0; q < S; ++q)
+      c->fseg_sys[q] = (int)(std::upper_bound(sys.begin(), sys.end(), c->fseg_ptr[q]) - sys.begin()) - 1;
+  }
+  if (c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILUK) {
+    if (int e = iluk_pattern(c, pos, true)) return e;
+  } else if (c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILU0 && !c->blk_of_row.empty()) {
     c->h_frp.assign(n + 1, 0);
     c->h_fci.clear();
     c->h_fci.reserve(c->h_ci.size());
@@ -1841,7 +1939,7 @@ int analyse_and_factor(shesolve_ctx *c) {
       c->h_frp[p + 1] = w;
     }
   } else if (c->prm.precond == SHESOLVE_PRECOND_ILUK && (c->prm.fill_level > 0 || !c->seg_fill.empty())) {
-    if (int e = iluk_pattern(c, pos)) return e;
+    if (int e = iluk_pattern(c, pos, false)) return e;
   } else {
     c->h_frp = c->h_rp;
     c->h_fci = c->h_ci;
@@ -1868,15 +1966,11 @@ int analyse_and_factor(shesolve_ctx *c) {
   std::vector<int> Lptr, Lrows, Uptr, Urows;
   compute_levels(c->h_frp, c->h_fci, c->h_fdg, n, Lptr, Lrows, Uptr, Urows);
   if (perm.empty()) {
-    // segments (the systems of a batch, or the whole matrix): one workgroup
-    // walks each segment's levels, unless one launch per level of the whole
-    // matrix is estimated faster
-    const bool batch = !c->seg_ptr.empty();
-    if (!batch) c->seg_ptr = {0, n};
-    int e = 0;
-    if (segment_sweeps_pay(c, Lptr, Lrows)) e = segment_levels(c);
-    if (!batch) c->seg_ptr.clear();
-    if (e) return e;
+    // factor segments (systems of a batch, per-energy slabs or the whole
+    // matrix): one workgroup walks each segment's levels, unless one launch
+    // per level of the whole matrix is estimated faster
+    if (segment_sweeps_pay(c, Lptr, Lrows))
+      if (int e = segment_levels(c)) return e;
   }
   if (!perm.empty()) {
     // multicolour ordering: colour q occupies the contiguous range
@@ -2571,7 +2665,7 @@ int set_pattern_host(shesolve_ctx *c, int64_t n, int64_t nnz, const int64_t *rp,
       }
     }
   c->blk_of_row.clear();
-  if (c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILU0) {
+  if (c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILU0 || c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILUK) {
     if (nblocks > 0 && !ranges) { set_err("block ranges missing"); return 7; }
     c->blk_of_row.assign(n, -1);
     for (int bk = 0; bk < nblocks; ++bk)
@@ -2586,6 +2680,22 @@ int set_pattern_host(shesolve_ctx *c, int64_t n, int64_t nnz, const int64_t *rp,
 }
 
 int ensure_vectors(shesolve_ctx *c);
+
+// Row blocks of the whole matrix for `cap` entries per block: per system in a
+// batch (no block straddles two systems), else over all rows.
+void matrix_blocks(const shesolve_ctx *c, int cap, std::vector<int> &out) {
+  out.clear();
+  if (c->seg_ptr.empty()) {
+    make_blocks(c->h_rp.data(), 0, c->n, cap, out);
+    return;
+  }
+  std::vector<int> one;
+  for (size_t s = 0; s + 1 < c->seg_ptr.size(); ++s) {
+    one.clear();
+    make_blocks(c->h_rp.data(), c->seg_ptr[s], c->seg_ptr[s + 1], cap, one);
+    out.insert(out.end(), out.empty() ? one.begin() : one.begin() + 1, one.end());
+  }
+}
 
 // Autotuner of the CSR-stream launch shape: entries per thread (4, 8, 16) x
 // block-to-XCD mapping (strided, contiguous), each timed on three FP32 SpMV
@@ -2605,7 +2715,7 @@ int tune_stream(shesolve_ctx *c) {
   for (int ii = 0; ii < 3; ++ii) {
     if (!c->prm.autotune && cand_items[ii] != 8) continue;
     std::vector<int> ab;
-    make_blocks(c->h_rp.data(), 0, n, cand_items[ii] * kStreamThreads, ab);
+    matrix_blocks(c, cand_items[ii] * kStreamThreads, ab);
     int *d_ab = nullptr;
     HIPCHK(hipMalloc(&d_ab, sizeof(int) * ab.size()));
     HIPCHK(hipMemcpyAsync(d_ab, ab.data(), sizeof(int) * ab.size(), hipMemcpyHostToDevice, c->stream));
@@ -2664,6 +2774,13 @@ int tune_stream(shesolve_ctx *c) {
   HIPCHK(hipMalloc(&c->d_ablk, sizeof(int) * best_blocks.size()));
   HIPCHK(hipMemcpyAsync(c->d_ablk, best_blocks.data(), sizeof(int) * best_blocks.size(), hipMemcpyHostToDevice,
                         c->stream));
+  if (!c->seg_ptr.empty()) {
+    std::vector<int> bsys(std::max(c->nablk, 1), 0);
+    for (int k = 0; k < c->nablk; ++k)
+      bsys[k] = (int)(std::upper_bound(c->seg_ptr.begin(), c->seg_ptr.end(), best_blocks[k]) - c->seg_ptr.begin()) - 1;
+    HIPCHK(hipMalloc(&c->d_blk_sys, sizeof(int) * bsys.size()));
+    HIPCHK(hipMemcpy(c->d_blk_sys, bsys.data(), sizeof(int) * bsys.size(), hipMemcpyHostToDevice));
+  }
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -2806,6 +2923,8 @@ void bbicg_iter(shesolve_batch_ctx *b, T *x) {
   if constexpr (sizeof(T) == 8) { V = (T **)c->v64; Av = (const TV *)c->d_v64; Mv = (const TV *)c->d_f64; }
   else { V = (T **)c->v32; Av = (const TV *)c->d_v32; Mv = nullptr; }
   T *r = V[0], *r0 = V[1], *p = V[2], *v = V[3], *ph = V[4], *s = V[5], *sh = V[6], *t = V[7];
+  // the SpMVs of the iteration skip the row blocks of converged systems
+  c->spmv_skip = c->d_blk_sys ? b->d_bs : nullptr;
   bupd<T>(b, 0, p, nullptr, nullptr, r, v, nullptr, nullptr);
   launch_precond<TV, T>(c, Mv, p, ph);
   launch_spmv<TV, T, T>(c, c->lpr, Av, ph, v, nullptr, nullptr, nullptr, nullptr);
@@ -2819,6 +2938,7 @@ void bbicg_iter(shesolve_batch_ctx *b, T *x) {
   bupd<T>(b, 2, x, r, nullptr, ph, sh, s, t);
   bdot<T>(b, nullptr, r, r, r0, r);
   bfin(b, 3);
+  c->spmv_skip = nullptr;
 }
 
 // Batched BiCGStab on the systems selected by mask (host vector), from the
@@ -2918,32 +3038,45 @@ int batch_solve_internal(shesolve_batch_ctx *b, int warm) {
     b->rep[s].status = zero_rhs[s] ? 0 : 1;
     act[s] = !zero_rhs[s];
   }
-  if (prm.precision == SHESOLVE_FP64) {
+  // FP64 BiCGStab on the systems in `sel` (cleared as they finish): a system
+  // whose recurrence converges is confirmed with its explicit residual and
+  // restarted from it when that is still above the tolerance, at most four
+  // runs, within the iteration budget.  iters_base[s] counts iterations spent
+  // before (the FP32 refinement of a handed-over system): the budget of the
+  // FP64 runs is max_iters of their own.
+  auto fp64_runs = [&](std::vector<int> &sel, const std::vector<int> &iters_base) -> int {
     for (int attempt = 0; attempt < 4; ++attempt) {
       bool any = false;
-      for (int s = 0; s < S; ++s) any |= act[s] != 0;
+      for (int s = 0; s < S; ++s) any |= sel[s] != 0;
       if (!any) break;
       int cap = 0;
+      std::vector<int> caps(S, 0);
       for (int s = 0; s < S; ++s)
-        if (act[s]) cap = std::max(cap, prm.max_iters - b->rep[s].iterations);
+        if (sel[s]) {
+          caps[s] = prm.max_iters - (b->rep[s].iterations - iters_base[s]);
+          cap = std::max(cap, caps[s]);
+        }
       if (cap <= 0) break;
-      if (int e = bbicg_run<double, double>(b, b->b64, b->x64, prm.tol, cap, 1e300, act)) return e;
+      if (int e = bbicg_run<double, double>(b, b->b64, b->x64, prm.tol, cap, 1e300, sel, &caps)) return e;
       for (int s = 0; s < S; ++s)
-        if (act[s]) {
+        if (sel[s]) {
           b->rep[s].iterations += b->h_bs[s].iters;
           b->rep[s].status = b->h_bs[s].status;
         }
-      // confirm converged systems with the explicit residual; restart the others
-      if (int e = upload_mask(b, act)) return e;
+      if (int e = upload_mask(b, sel)) return e;
       if (int e = bresidual(b, rn)) return e;
       for (int s = 0; s < S; ++s) {
-        if (!act[s]) continue;
+        if (!sel[s]) continue;
         b->rep[s].rel_residual = rn[s] / bn[s];
-        if (b->rep[s].rel_residual <= prm.tol) { b->rep[s].status = 0; act[s] = 0; }
-        else if (b->rep[s].status != 0 || b->rep[s].iterations >= prm.max_iters) act[s] = 0;
+        if (b->rep[s].rel_residual <= prm.tol) { b->rep[s].status = 0; sel[s] = 0; }
+        else if (b->rep[s].status != 0 || b->rep[s].iterations - iters_base[s] >= prm.max_iters) sel[s] = 0;
         else b->rep[s].status = 1;  // recurrence converged but the true residual did not: restart
       }
     }
+    return 0;
+  };
+  if (prm.precision == SHESOLVE_FP64) {
+    if (int e = fp64_runs(act, std::vector<int>(S, 0))) return e;
   } else {
     float *r32 = c->v32[8];
     // an FP32 inner solve that stalls (iteration cap) or breaks down hands its
@@ -2999,9 +3132,10 @@ int batch_solve_internal(shesolve_batch_ctx *b, int warm) {
       for (int s = 0; s < S; ++s)
         if (act[s]) { prev[s] = rn[s]; b->rep[s].ir_steps++; }
       // later corrections get at most 2x the iterations the system's first
-      // FP32 solve needed (at least 40): in a batch every iteration costs the
-      // whole batch, so FP32 solves of a correction that stop making progress
-      // are handed to FP64 early
+      // FP32 solve needed (at least 40): a correction that stops making
+      // progress goes to FP64 early (on the 96-diode family a cap of
+      // max(100, 4x) took 41.8 s against 28.7 s, the FP32 solves that stall
+      // running longer before the same hand-over)
       for (int s = 0; s < S; ++s) caps[s] = first_it[s] < 0 ? inner_cap : std::max(40, 2 * first_it[s]);
       if (int e = bbicg_run<float, float>(b, r32, b->dd, prm.inner_tol, inner_cap, 1e4, act, &caps)) return e;
       for (int s = 0; s < S; ++s)
@@ -3033,12 +3167,10 @@ int batch_solve_internal(shesolve_batch_ctx *b, int warm) {
       bref(b, 2, b->xprev, b->x64, nullptr);
       for (int s = 0; s < S; ++s)
         if (fb[s]) best[s] = b->rep[s].rel_residual;
-      if (int e = bbicg_run<double, double>(b, b->b64, b->x64, prm.tol, prm.max_iters, 1e300, fb)) return e;
-      for (int s = 0; s < S; ++s)
-        if (fb[s]) {
-          b->rep[s].iterations += b->h_bs[s].iters;
-          b->rep[s].status = b->h_bs[s].status;
-        }
+      std::vector<int> base(S, 0);
+      for (int s = 0; s < S; ++s) base[s] = b->rep[s].iterations;
+      std::vector<int> sel = fb;
+      if (int e = fp64_runs(sel, base)) return e;
     }
   }
   // final explicit residuals of every system with a nonzero right-hand side
@@ -3256,7 +3388,8 @@ static int check_params(const shesolve_params *p) {
   if (p->max_iters <= 0) { set_err("max_iters must be positive"); return 1; }
   if (p->precond != SHESOLVE_PRECOND_NONE && p->precond != SHESOLVE_PRECOND_ILU0 &&
       p->precond != SHESOLVE_PRECOND_BLOCK_ILU0 && p->precond != SHESOLVE_PRECOND_MC_ILU0 &&
-      p->precond != SHESOLVE_PRECOND_BMC_ILU0 && p->precond != SHESOLVE_PRECOND_ILUK) {
+      p->precond != SHESOLVE_PRECOND_BMC_ILU0 && p->precond != SHESOLVE_PRECOND_ILUK &&
+      p->precond != SHESOLVE_PRECOND_BLOCK_ILUK) {
     set_err("unknown preconditioner %d", p->precond);
     return 1;
   }

@@ -15,6 +15,7 @@ PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBPATH = os.path.join(PKG, "libshesolve.so")
 
 PRECOND_NONE, PRECOND_ILU0, PRECOND_BLOCK_ILU0, PRECOND_MC_ILU0, PRECOND_ILUK, PRECOND_BMC_ILU0 = 0, 1, 3, 4, 5, 6
+PRECOND_BLOCK_ILUK = 7  # ILU(k) of the per-energy slabs (k = fill_level)
 FP64, FP32_IR = 0, 1
 
 # symbols declared in include/shesolve.h
