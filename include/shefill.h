@@ -113,6 +113,19 @@ int shefill_recover_device(shefill_handle h, const double *d_x, double *d_x_full
 int shefill_postproc(shefill_handle h, const double *x_full, double *density, double *facet_current);
 int shefill_postproc_device(shefill_handle h, const double *d_x_full, double *d_density,
                             double *d_facet_current);
+/* Initial guess for the current build from the solution of the previous
+ * build on the same mesh (the step before a potential change), on the device:
+ * she::transfer_to_new_h_space (she/transfer_to_new_h_space.hpp:58-196) for
+ * the even unknowns -- per cell and level the old level of nearest kinetic
+ * energy, values rescaled per cell so that the cell keeps the previous
+ * density -- then converted to the reduced, scaled unknowns of the current
+ * system.  d_x_full_prev: full solution of the previous build (as
+ * shefill_recover_device wrote it), d_density_prev: its density per cell (as
+ * shefill_postproc_device wrote it); d_x: n values.  Odd unknowns are
+ * eliminated, so the reference's facet part has no counterpart here.  Returns
+ * -1 when the previous build was on another mesh or there was none. */
+int shefill_transfer_device(shefill_handle h, const double *d_x_full_prev, const double *d_density_prev,
+                            double *d_x);
 /* Gummel update of the carrier density per cell (simulator.hpp:524-600,
  * update_quantity with damping alpha in (0, 1]): density <- density (1 - alpha)
  * + new alpha, or with logarithmic != 0, density^(1-alpha) *

@@ -410,6 +410,72 @@ __global__ void k_terminal_current(int nf, const int *fc1, const int *fc2, const
   }
 }
 
+// Previous iterate on the unknowns of a new build (she/transfer_to_new_h_space.hpp:58-196,
+// the cell part): each new (cell, level) takes the values of the old level of
+// the same cell whose kinetic energy is nearest (energy_index_H,
+// she/assemble_scattering.hpp:64-105, walked on the old grid from the same
+// index), moving up past old levels without unknowns (below the band edge);
+// harmonics the old node lacks are zero, as are levels the old grid does not
+// reach.  f_new holds the even part of a full solution of the new build.
+struct TransferArgs {
+  int nc, NH, NH_old, carrier, L_max;
+  const double *H, *shc, *H_old, *shc_old;
+  const int *idx, *ord, *idx_old, *ord_old;
+  const double *x_old;  // full solution of the old build (even unknowns first)
+  double *f_new;
+};
+__device__ inline int even_of(int L) { return L * (L + 1) / 2; }  // even harmonics up to odd L
+__global__ void k_transfer_even(TransferArgs a) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= (int64_t)a.nc * a.NH) return;
+  const int c = (int)(k / a.NH), H = (int)(k % a.NH);
+  const int u = a.idx[k];
+  if (u < 0) return;
+  const int e_new = even_of(a.ord[k]);
+  for (int i = 0; i < e_new; ++i) a.f_new[u + i] = 0.0;
+  if (H < 1 || H > a.NH - 2) return;
+  const double E = a.carrier == 0 ? a.H[H] - a.shc[c] : a.shc[c] - a.H[H];
+  auto kold = [&](long h) { return a.carrier == 0 ? a.H_old[h] - a.shc_old[c] : a.shc_old[c] - a.H_old[h]; };
+  long r = H < a.NH_old - 1 ? H : a.NH_old - 1;
+  if (E < kold(r) || E > kold(r)) {
+    long inc = 1;
+    double diff = E - kold(r), diff0 = diff;
+    if (diff < 0) inc = -1;
+    while (r < (long)(a.NH_old - 1) && r > 0 && diff * diff0 > 0) {
+      r += inc;
+      diff = E - kold(r);
+    }
+    if (diff * diff0 > 0) return;  // the old grid does not reach this energy
+    double e1 = fabs(diff), e2 = e1;
+    if (r - inc >= 0 && r - inc < (long)a.NH_old) e2 = fabs(E - kold(r - inc));
+    if (e2 < e1) r -= inc;
+  }
+  const int64_t ko = (int64_t)c * a.NH_old;
+  while (a.ord_old[ko + r] == 0 && r < a.NH_old - 1) ++r;
+  const int uo = a.idx_old[ko + r];
+  if (a.ord_old[ko + r] <= 0 || uo < 0) return;
+  const int e_old = even_of(a.ord_old[ko + r]);
+  for (int i = 0; i < e_new && i < e_old; ++i) a.f_new[u + i] = a.x_old[uo + i];
+}
+// Density rescaling of the transferred values per cell (density_old /
+// density_new where both are positive), then the reduced, scaled unknowns
+// x = f / (scaling * column factor) of the new system.
+__global__ void k_transfer_scale(int nc, int NH, const int *idx, const int *ord, const double *dens_old,
+                                 const double *dens_new, const double *f_new, const double *scal,
+                                 const double *colfac, int scaled, double *x) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= (int64_t)nc * NH) return;
+  const int u = idx[k];
+  if (u < 0) return;
+  const int c = (int)(k / NH);
+  const double s = (dens_new[c] > 0 && dens_old[c] > 0) ? dens_old[c] / dens_new[c] : 1.0;
+  const int e = even_of(ord[k]);
+  for (int i = 0; i < e; ++i) {
+    const double f = f_new[u + i] * s;
+    x[u + i] = scaled ? f / (scal[u + i] * colfac[u + i]) : f;
+  }
+}
+
 // Sums part[0..cnt) in index order in one thread -- the order of the host
 // loop of shefill_density_update_device, so the norm is the same bit for bit --
 // and stores the square root in *out.
@@ -623,7 +689,12 @@ struct shefill_ctx {
   DBuf<double> dcorr, dncont, dcdiff, dcdrift, dodiag, ddiag0, ddiagr, drhs0, dinco;
   DBuf<double> dval, drhs, dscal, dcolfac, dlog;
   DBuf<int64_t> drp64;
-  DBuf<int> dfdir, dfn_base, didxc, dfnode_of, dnode_of;
+  DBuf<int> dfdir, dfn_base, didxc, dfnode_of, dnode_of, dordc;
+  // the previous build on the same mesh, for shefill_transfer_device: its
+  // energies, band-edge shifts and even mapping (prev_nc < 0: none)
+  int prev_nc = -1, prev_NH = 0;
+  DBuf<double> dprev_H, dprev_shc, dtr_f, dtr_dens;
+  DBuf<int> dprev_idx, dprev_ord;
   DBuf<double> dcx, dcy;
   DevTables t{};
   bool built = false;
@@ -1161,6 +1232,7 @@ int fill_values(shefill_ctx *c, Builder &B) {
   e = e ? e : upload(c, c->dfdir, c->fdir);
   e = e ? e : upload(c, c->dfn_base, c->fn_base);
   e = e ? e : upload(c, c->didxc, c->idxc);
+  e = e ? e : upload(c, c->dordc, c->ordc);
   e = e ? e : upload(c, c->dfnode_of, c->fnode_of);
   e = e ? e : upload(c, c->dnode_of, c->node_of);
   e = e ? e : upload(c, c->dcx, c->cx);
@@ -1327,6 +1399,21 @@ int shefill_build(shefill_handle h, const shefill_device *d, const shefill_confi
   if (cfg->carrier != 0 && cfg->carrier != 1) { ferr("carrier must be 0 or 1"); return 3; }
   if (cfg->ordering != 0 && cfg->ordering != 1) { ferr("ordering must be 0 or 1"); return 3; }
   FHIP(hipSetDevice(h->device));
+  // keep what a transfer of this build's solution to the next build needs
+  h->prev_nc = -1;
+  if (h->built && h->nc == d->nx * (d->dim == 2 ? d->ny : 1)) {
+    const size_t m = (size_t)h->nc * h->NH;
+    if (h->dprev_H.alloc(h->NH) || h->dprev_shc.alloc(h->nc) || h->dprev_idx.alloc(m) || h->dprev_ord.alloc(m))
+      return -11;
+    FHIP(hipMemcpyAsync(h->dprev_H.p, h->dHval.p, sizeof(double) * h->NH, hipMemcpyDeviceToDevice, h->stream));
+    FHIP(hipMemcpyAsync(h->dprev_shc.p, h->dshc.p, sizeof(double) * h->nc, hipMemcpyDeviceToDevice, h->stream));
+    FHIP(hipMemcpyAsync(h->dprev_idx.p, h->didxc.p, sizeof(int) * m, hipMemcpyDeviceToDevice, h->stream));
+    FHIP(hipMemcpyAsync(h->dprev_ord.p, h->dordc.p, sizeof(int) * m, hipMemcpyDeviceToDevice, h->stream));
+    FHIP(hipStreamSynchronize(h->stream));
+    h->prev_nc = h->nc;
+    h->prev_NH = h->NH;
+  }
+  h->built = false;
   h->cfg = *cfg;
   if (int e = build_geometry(h, d)) return e;
   if (int e = build_energies_orders_mapping(h)) return e;
@@ -1535,6 +1622,35 @@ int shefill_density_update_device(shefill_handle h, const double *d_new, double 
     *update_norm = sqrt(t);
   }
   return e;
+}
+
+int shefill_transfer_device(shefill_handle h, const double *d_x_full_prev, const double *d_density_prev,
+                            double *d_x) {
+  if (!h) { ferr("handle is NULL"); return 1; }
+  if (!h->built) { ferr("nothing built"); return -1; }
+  if (!d_x_full_prev) { ferr("previous solution is NULL"); return 2; }
+  if (!d_density_prev) { ferr("previous density is NULL"); return 3; }
+  if (!d_x) { ferr("x is NULL"); return 4; }
+  if (h->prev_nc != h->nc) { ferr("no previous build on this mesh"); return -1; }
+  FHIP(hipSetDevice(h->device));
+  const int64_t m = (int64_t)h->nc * h->NH;
+  const int64_t N = h->n + h->n_odd;
+  if (h->dtr_f.alloc(std::max<int64_t>(N, 1)) || h->dtr_dens.alloc(std::max(h->nc, 1))) return -11;
+  TransferArgs a{};
+  a.nc = h->nc; a.NH = h->NH; a.NH_old = h->prev_NH; a.carrier = h->cfg.carrier; a.L_max = h->cfg.L_max;
+  a.H = h->dHval.p; a.shc = h->dshc.p; a.H_old = h->dprev_H.p; a.shc_old = h->dprev_shc.p;
+  a.idx = h->didxc.p; a.ord = h->dordc.p; a.idx_old = h->dprev_idx.p; a.ord_old = h->dprev_ord.p;
+  a.x_old = d_x_full_prev; a.f_new = h->dtr_f.p;
+  const int g = (int)std::max<int64_t>(1, (m + kB - 1) / kB);
+  hipLaunchKernelGGL(k_transfer_even, dim3(g), dim3(kB), 0, h->stream, a);
+  // density of the transferred values on the new grid (the even part is all k_density reads)
+  hipLaunchKernelGGL(k_density, dim3(grid(h->nc)), dim3(kB), 0, h->stream, h->t, h->nc, h->didxc.p, h->dtr_f.p,
+                     h->dtr_dens.p);
+  hipLaunchKernelGGL(k_transfer_scale, dim3(g), dim3(kB), 0, h->stream, h->nc, h->NH, h->didxc.p, h->dordc.p,
+                     d_density_prev, h->dtr_dens.p, h->dtr_f.p, h->dscal.p, h->dcolfac.p, h->cfg.scale, d_x);
+  FHIP(hipGetLastError());
+  FHIP(hipStreamSynchronize(h->stream));
+  return 0;
 }
 
 int shefill_density_update_device_noalloc(shefill_handle h, const double *d_new, double *d_density, double alpha,

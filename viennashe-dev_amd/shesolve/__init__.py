@@ -408,7 +408,7 @@ FILL_EXPORTS = ("shefill_last_error", "shefill_create", "shefill_destroy", "shef
                 "shefill_last_fill_ms", "shefill_refill", "shefill_recover", "shefill_recover_device",
                 "shefill_postproc", "shefill_postproc_device", "shefill_pattern_hash",
                 "shefill_density_update", "shefill_density_update_device",
-                "shefill_density_update_device_noalloc",
+                "shefill_density_update_device_noalloc", "shefill_transfer_device",
                 "shefill_terminal_current", "shefill_terminal_current_device")
 
 
@@ -448,6 +448,7 @@ def _fill_lib():
     L.shefill_density_update_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
                                                 C.POINTER(C.c_double)]
     L.shefill_density_update_device_noalloc.argtypes = L.shefill_density_update_device.argtypes
+    L.shefill_transfer_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.shefill_recover_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.shefill_postproc.argtypes = [C.c_void_p, _f64p, C.c_void_p, C.c_void_p]
     L.shefill_postproc_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -571,6 +572,13 @@ class Fill:
                                                           1 if logarithmic else 0, C.byref(nrm)),
                 "shefill_density_update_device")
         return nrm.value
+
+    def transfer_device(self, d_x_full_prev, d_density_prev, d_x):
+        """Initial guess for the current build from the previous build's full
+        solution and density (device addresses), written to d_x (n values):
+        the reference's transfer_to_new_h_space on the device."""
+        _fcheck(_fill_lib().shefill_transfer_device(self._h, d_x_full_prev, d_density_prev, d_x),
+                "shefill_transfer_device")
 
     def density_update_device_noalloc(self, d_new, d_density, alpha, logarithmic=False):
         """The same update through the allocation-free entry point (its partial

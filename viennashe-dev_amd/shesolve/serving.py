@@ -5,8 +5,11 @@ simulator::run does per nonlinear iteration (simulator.hpp:895-1096):
 1. set up energies, expansion orders and mapping again (the pattern follows the potential);
 2. assemble;
 3. solve, warm-started from the previous iterate transferred to the new energy grid
-   (she::transfer_to_new_h_space plays this part in the reference);
-4. perturb the potential before the next step (here: N(0, sigma) per cell).
+   on the device (shefill_transfer_device, the reference's
+   she::transfer_to_new_h_space);
+4. perturb the potential before the next step (here: the request's potential plus
+   N(0, sigma) per mesh vertex drawn afresh each step, each cell taking the mean of
+   its corner vertices).
 
 Serving is continuous:
 - A request is admitted as soon as it has arrived, a slot is free and its estimated
@@ -84,7 +87,7 @@ class Request:
     cfg: D.SheConfig
     seed: int
     steps: int = 5
-    sigma: float = 5e-3       # potential perturbation per cell between steps [V]
+    sigma: float = 5e-3       # potential perturbation per mesh vertex between steps [V]
     damping: float = 0.3      # Gummel damping of the density (solvers/config.hpp:257 default)
 
 
@@ -131,6 +134,17 @@ def footprint_estimate(dev: D.Device, cfg: D.SheConfig) -> float:
     return rows * (per_row * 52.0 + 220.0)
 
 
+def vertex_perturbation(dev: D.Device, rng, sigma):
+    """Potential perturbation per cell from N(0, sigma) drawn at every mesh
+    vertex: each cell takes the mean of its corner vertices (2 in 1D, 4 in 2D),
+    the cell value of the piecewise (bi)linear vertex field."""
+    if dev.dim == 1:
+        v = rng.normal(0.0, sigma, dev.nx + 1)
+        return 0.5 * (v[:-1] + v[1:])
+    v = rng.normal(0.0, sigma, (dev.ny + 1, dev.nx + 1))
+    return (0.25 * (v[:-1, :-1] + v[:-1, 1:] + v[1:, :-1] + v[1:, 1:])).ravel()
+
+
 def _solver_params(dev, cfg):
     """2D meshes: block multicolour ILU0, FP32 + FP64 refinement (one sweep
     launch per colour, blocks of consecutive rows solved inside).  1D chains: natural-order ILU(0), ILU(1) where
@@ -143,31 +157,6 @@ def _solver_params(dev, cfg):
     return dict(precond=shesolve.PRECOND_ILUK, fill_level=1 if cfg.L_max >= 5 else 0, precision=shesolve.FP64)
 
 
-def _transfer(x_old, map_old, shift, map_new, n_new):
-    """Previous iterate onto the new unknowns: the same cell, the same energy
-    (new level h holds the energy of old level h + shift), the harmonics both
-    orders share."""
-    ei_o, eo_o = map_old
-    ei_n, eo_n = map_new
-    x0 = np.zeros(n_new)
-    nh_o, nh_n = ei_o.shape[1], ei_n.shape[1]
-    h_n = np.arange(nh_n)
-    h_o = h_n + shift
-    ok = (h_o >= 0) & (h_o < nh_o)
-    h_n, h_o = h_n[ok], h_o[ok]
-    a_n = ei_n[:, h_n]
-    a_o = ei_o[:, h_o]
-    k_n = np.where(a_n >= 0, eo_n[:, h_n] * (eo_n[:, h_n] + 1) // 2, 0)
-    k_o = np.where(a_o >= 0, eo_o[:, h_o] * (eo_o[:, h_o] + 1) // 2, 0)
-    k = np.minimum(k_n, k_o)
-    sel = (a_n >= 0) & (a_o >= 0) & (k > 0)
-    dst, src, cnt = a_n[sel], a_o[sel], k[sel]
-    for j in range(int(cnt.max()) if cnt.size else 0):
-        m = cnt > j
-        x0[dst[m] + j] = x_old[src[m] + j]
-    return x0
-
-
 def run_request(req: Request, tol=1e-10, max_iters=1000, device=0, clock=None, autotune=True):
     """One request, start to finish, on the calling thread.  autotune=False
     pins the launch shape, so that the result cannot depend on timings.
@@ -178,15 +167,21 @@ def run_request(req: Request, tol=1e-10, max_iters=1000, device=0, clock=None, a
     sv = shesolve.Solver(device, tol=tol, max_iters=max_iters, autotune=1 if autotune else 0,
                          **_solver_params(req.dev, req.cfg))
     rng = np.random.default_rng(req.seed)
-    dev = dataclasses.replace(req.dev, potential=np.array(req.dev.potential, np.float64))
+    base = np.array(req.dev.potential, np.float64)
+    dev = dataclasses.replace(req.dev, potential=base.copy())
     reps, ns, norms = [], [], []
-    prev = None
     pattern = None
     dx = dxf = dnew = dcur = dj = None
     try:
         for step in range(req.steps):
             if step > 0:
-                dev = dataclasses.replace(dev, potential=dev.potential + rng.normal(0.0, req.sigma, dev.ncells))
+                # each step perturbs the request's potential afresh: successive
+                # potentials differ by sqrt(2) sigma per vertex.  A random walk
+                # (each step perturbing the last) grows as sqrt(step) sigma; on
+                # the 154x77 L=3 device its 4th system is not solved to 1e-10 in
+                # 3000 FP64 iterations by natural ILU(0), ILU(1) or block
+                # multicolour ILU(0), while the fresh draw converges in 117-495
+                dev = dataclasses.replace(dev, potential=base + vertex_perturbation(dev, rng, req.sigma))
             sz = F.build(dev, req.cfg)
             n, nnz = sz["n"], sz["nnz"]
             d_rp, d_ci, d_val, d_rhs = F.device_system()
@@ -196,17 +191,15 @@ def run_request(req: Request, tol=1e-10, max_iters=1000, device=0, clock=None, a
             else:
                 sv.set_matrix_device(n, nnz, d_rp, d_ci, d_val)
                 pattern = h
-            mapping = F.mapping()
             if dx is None or dx.n != n:
                 if dx is not None:
                     dx.close()
                 dx = DeviceBuffer(n)
-            warm = prev is not None
+            warm = step > 0
             if warm:
-                x_old, map_old, psimax_old = prev
-                # new level h has the energy of old level h + shift (edge = -q max(psi) - dE + Eg/2)
-                shift = int(round(-(float(np.max(dev.potential)) - psimax_old) * D.Q / req.cfg.energy_spacing))
-                dx.upload(_transfer(x_old, map_old, shift, mapping, n))
+                # the previous full solution and its density, still resident, onto
+                # the new unknowns (transfer_to_new_h_space, on the device)
+                F.transfer_device(dxf.ptr, dnew.ptr, dx.ptr)
             rep = sv.solve_device(d_rhs, dx.ptr, warm_start=warm)
             reps.append(rep)
             ns.append(n)
@@ -227,9 +220,8 @@ def run_request(req: Request, tol=1e-10, max_iters=1000, device=0, clock=None, a
                 dcur.upload(dnew.download())
             else:
                 norms.append(F.density_update_device(dnew.ptr, dcur.ptr, req.damping))
-            x = dx.download()
-            prev = (x, mapping, float(np.max(dev.potential)))
         dens = dcur.download()
+        x_last = dx.download()
     finally:
         for b in (dx, dxf, dnew, dcur, dj):
             if b is not None:
@@ -239,7 +231,7 @@ def run_request(req: Request, tol=1e-10, max_iters=1000, device=0, clock=None, a
     finish = clock() if clock else time.perf_counter()
     desc = (f"{req.kind} {req.dev.nx}x{req.dev.ny} L={req.cfg.L_max} "
             f"dE={req.cfg.energy_spacing / D.EV * 1e3:.2f} meV")
-    return Result(req.rid, req.kind, req.arrival, start, finish, reps, ns, prev[0], dens,
+    return Result(req.rid, req.kind, req.arrival, start, finish, reps, ns, x_last, dens,
                   footprint_estimate(req.dev, req.cfg), desc, norms)
 
 
@@ -296,12 +288,13 @@ class Server:
     the summed footprint estimate of the requests in flight stays within the
     HBM budget."""
 
-    def __init__(self, slots=8, hbm_budget=200e9, tol=1e-10, device=0, autotune=True):
+    def __init__(self, slots=8, hbm_budget=200e9, tol=1e-10, device=0, autotune=True, max_iters=5000):
         self.slots = slots
         self.budget = hbm_budget
         self.tol = tol
         self.device = device
         self.autotune = autotune
+        self.max_iters = max_iters
 
     def run(self, requests, realtime=True, on_done=None):
         """Serves `requests`.  realtime: requests become eligible at their
@@ -337,7 +330,8 @@ class Server:
                             break
                         lock.wait(timeout=max(min(wait, 0.05), 0.001) if wait > 0 else 0.05)
                 try:
-                    res = run_request(req, tol=self.tol, device=self.device, clock=clock, autotune=self.autotune)
+                    res = run_request(req, tol=self.tol, max_iters=self.max_iters, device=self.device, clock=clock,
+                                      autotune=self.autotune)
                 except Exception as e:  # noqa: BLE001 - surfaced to the caller below
                     with lock:
                         errors.append((req.rid, repr(e)))
