@@ -5,9 +5,13 @@ devices (the matrix the reference hands to solvers::solve).  Tolerances:
   - SpMV FP64: |y - y_ref| <= 1e-13 * (|A| |x|) per row (summation order differs);
   - SpMV FP32: |y - y_ref| <= 2e-6 * (|A| |x|) per row;
   - ILU0 factors: pattern bit-exact, values within 1e-10 relative;
-  - solves: FP64 true relative residual <= 1e-10, solution
-    within 1e-8 relative (2-norm) of the oracle's FP64 solution, carrier density
-    and facet current within 1e-6 relative.
+  - solves stopped at FP64 relative residual 1e-10 (the metric's stop): true
+    residual <= 1e-10, solution within 2e-7 of the oracle's FP64 solution
+    (condition number ~1e3 times the residual), carrier density within 1e-6
+    (2-norm);
+  - solves stopped at 1e-12: solution within 1e-8 relative (2-norm), carrier
+    density per cell and terminal currents within 1e-6 relative -- the parity
+    bars need this stop on these systems.
 """
 import numpy as np
 import pytest
@@ -296,20 +300,27 @@ def test_solve_parity(name, precision, kind):
     sv.set_matrix(A.row_ptr, A.col, A.val, block_ranges=s.block_ranges() if kind.startswith("block") else None)
     # FP64 relative residual <= 1e-10
     x, rep = sv.solve(b)
-    print(name, precision, kind, rep)
     assert rep["status"] == 0
     res = np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b)
     assert res <= 1e-10, res
     assert rep["rel_residual"] <= 1e-10
-    # solution parity: a 1e-10 residual bounds the error only up to the condition
-    # number (about 1e2 here), so the 1e-8 solution check solves to 1e-12
+    # At the metric's 1e-10 stop the solution error is bounded by the condition
+    # number (about 1e3 on these systems) times the residual: measured up to
+    # 1.2e-7 (the 1D diodes), so 2e-7 here; the density holds the 1e-6 bar in the
+    # 2-norm, per cell it and the terminal currents miss it by up to 3x.
+    x_ref = _ref_solution(name)
+    err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    print(name, precision, kind, rep, "error at the 1e-10 stop", err)
+    assert err <= 2e-7, err
+    assert density_2norm(s, x, x_ref) <= 1e-6
+    # The parity bars -- solution within 1e-8, density per cell and terminal
+    # currents within 1e-6 -- hold with the stop at 1e-12.
     sv.set_params(tol=1e-12)
     x, rep = sv.solve(b)
     assert rep["status"] == 0 and rep["rel_residual"] <= 1e-12
-    x_ref = _ref_solution(name)
-    err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
-    assert err <= 1e-8, err
-    # post-processed quantities: carrier density and terminal currents
+    err12 = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    print(name, precision, kind, "error at the 1e-12 stop", err12)
+    assert err12 <= 1e-8, err12
     check_postproc(s, x, x_ref)
 
 
@@ -346,6 +357,13 @@ def terminal_currents(s, j, absolute=False):
         terms = (j * area * sign)[sel]
         out.append(float(np.sum(np.abs(terms) if absolute else terms)))
     return np.array(out)
+
+
+def density_2norm(s, x, x_ref):
+    """Relative 2-norm difference of the carrier densities of two reduced solutions."""
+    n, _ = s.postproc(s.finish(x)[0])
+    n_ref, _ = s.postproc(s.finish(x_ref)[0])
+    return np.linalg.norm(n - n_ref) / np.linalg.norm(n_ref)
 
 
 def check_postproc(s, x, x_ref):
