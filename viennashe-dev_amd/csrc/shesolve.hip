@@ -210,7 +210,7 @@ __device__ inline T block_sum_t(T v) {
 // xcd: 0 = row blocks strided over the whole grid; 1 = each XCD (blockIdx % 8)
 // works through its own contiguous eighth of the row blocks, so the gathered
 // vector entries of neighbouring rows are reused in that XCD's L2.
-template <typename TV, typename T, int MODE, int ITEMS>
+template <typename TV, typename T, int MODE, int ITEMS, bool SKIP = false>
 __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, const int *__restrict__ blocks,
                                                            const int *__restrict__ rp, const int *__restrict__ ci,
                                                            const TV *__restrict__ v, const T *x, T *y,
@@ -231,8 +231,12 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
     bstep = per;
   }
   for (int bk = bk0; bk < bk1; bk += bstep) {
-    // batched iterations: row blocks of converged systems are left alone
-    if (skip && !skip[blk_sys[bk]].active) continue;
+    // batched iterations: row blocks of converged systems are left alone (a
+    // separate instantiation: the check in the loop of every SpMV measured
+    // 416 -> 598 us on the fused-dot launch of the 161x81 L=3 device)
+    if constexpr (SKIP) {
+      if (!skip[blk_sys[bk]].active) continue;
+    }
     // consecutive row blocks, or an explicit (first, end) pair per block
     const int rs = pairs ? pairs[2 * bk] : blocks[bk], re = pairs ? pairs[2 * bk + 1] : blocks[bk + 1];
     const int k0 = rp[rs], k1 = rp[re];
@@ -1307,13 +1311,17 @@ void launch_stream(shesolve_ctx *c, int g, int nblk, const int *blocks, const in
                    double *part, const int *pairs = nullptr, int c0 = 0) {
   // the sweeps' row blocks are built for kSweepItems entries per thread
   const int items = MODE == MODE_SPMV ? c->items : kSweepItems;
-#define ST_CASE(IT)                                                                                                \
+#define ST_CASE(IT, SK)                                                                                            \
   case IT:                                                                                                         \
-    hipLaunchKernelGGL((k_stream<TV, T, MODE, IT>), dim3(g), dim3(kStreamThreads), 0, c->stream, nblk, c->xcd,     \
+    hipLaunchKernelGGL((k_stream<TV, T, MODE, IT, SK>), dim3(g), dim3(kStreamThreads), 0, c->stream, nblk, c->xcd, \
                        blocks, rp, ci, v, x, y, b, perm, dinv, out2, d0, d1, part, pairs, c0, c->d_blk_sys,        \
-                       MODE == MODE_SPMV ? c->spmv_skip : (const BSys *)nullptr);                                 \
+                       c->spmv_skip);                                                                              \
     break;
-  switch (items) { ST_CASE(4) ST_CASE(16) default: ST_CASE(8) }
+  if (MODE == MODE_SPMV && c->spmv_skip) {
+    switch (items) { ST_CASE(4, true) ST_CASE(16, true) default: ST_CASE(8, true) }
+  } else {
+    switch (items) { ST_CASE(4, false) ST_CASE(16, false) default: ST_CASE(8, false) }
+  }
 #undef ST_CASE
 }
 
@@ -2252,58 +2260,80 @@ int setup_solve_space(shesolve_ctx *c) {
   // through the natural index space together, so the gathered x entries (the
   // rows' neighbours, near them in natural order) stay in L2 instead of being
   // fetched again for every colour.
-  std::vector<std::pair<int, int>> pr;
-  {
+  auto build_pairs = [&](int items, std::vector<int> &flat) {
+    std::vector<std::pair<int, int>> pr;
     int start = 0;
     for (int q = 1; q <= c->n; ++q) {
       if (q < c->n && c->h_colour[c->h_perm[q]] == c->h_colour[c->h_perm[start]]) continue;
       std::vector<int> blk;
-      make_blocks(c->h_frp.data(), start, q, c->items * kStreamThreads, blk);
+      make_blocks(c->h_frp.data(), start, q, items * kStreamThreads, blk);
       for (size_t k = 0; k + 1 < blk.size(); ++k) pr.emplace_back(blk[k], blk[k + 1]);
       start = q;
     }
-  }
-  std::stable_sort(pr.begin(), pr.end(), [&](const std::pair<int, int> &a, const std::pair<int, int> &b) {
-    return c->h_perm[a.first] < c->h_perm[b.first];
-  });
-  std::vector<int> flat(2 * pr.size());
-  for (size_t k = 0; k < pr.size(); ++k) {
-    flat[2 * k] = pr[k].first;
-    flat[2 * k + 1] = pr[k].second;
-  }
-  HIPCHK(hipMalloc(&c->d_sblk, sizeof(int) * std::max<size_t>(flat.size(), 2)));
-  HIPCHK(hipMemcpy(c->d_sblk, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
+    std::stable_sort(pr.begin(), pr.end(), [&](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+      return c->h_perm[x.first] < c->h_perm[y.first];
+    });
+    flat.resize(2 * pr.size());
+    for (size_t k = 0; k < pr.size(); ++k) {
+      flat[2 * k] = pr[k].first;
+      flat[2 * k + 1] = pr[k].second;
+    }
+  };
+  auto install = [&](const std::vector<int> &flat) -> int {
+    dfree(c->d_sblk);
+    HIPCHK(hipMalloc(&c->d_sblk, sizeof(int) * std::max<size_t>(flat.size(), 2)));
+    HIPCHK(hipMemcpy(c->d_sblk, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
+    c->s_blk = c->d_sblk;
+    c->s_pairs = c->d_sblk;
+    c->s_nblk = (int)(flat.size() / 2);
+    return 0;
+  };
   c->s_rp = c->d_frp;
   c->s_ci = c->d_fci;
-  c->s_blk = c->d_sblk;
-  c->s_pairs = c->d_sblk;
   c->spmv_lpr = 0;  // the interleaved block order needs the CSR-stream kernel
-  c->s_nblk = (int)pr.size();
-  if (c->prm.autotune) {
-    // block-to-XCD mapping of the permuted matrix, timed on it (the natural
-    // matrix's choice does not carry over: contiguous eighths of the
-    // interleaved order give each XCD its own eighth of the natural index
-    // space in every colour)
-    hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)c->fnnz, c->d_sv32, 1.0f);
-    float best = 1e30f;
-    int best_xcd = c->xcd;
+  std::vector<int> flat;
+  if (!c->prm.autotune) {
+    build_pairs(c->items, flat);
+    return install(flat);
+  }
+  // Entries per thread and block-to-XCD mapping of the permuted matrix, timed
+  // on it as the solve launches it (fused dot product, persistent grid).  The
+  // natural matrix's choice does not carry over (contiguous eighths of the
+  // interleaved order give each XCD its own eighth of the natural index space
+  // in every colour), and plain launches do not tell: on the 161x81 L=3
+  // device two mappings ran 362 and 375 us plain but 414 and 560 us fused.
+  hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)c->fnnz, c->d_sv32, 1.0f);
+  hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)c->n, c->v32[2], 1.0f);
+  float best = 1e30f;
+  int best_items = c->items, best_xcd = c->xcd;
+  std::vector<int> best_flat;
+  const int cand[3] = {4, 8, 16};
+  for (int ii = 0; ii < 3; ++ii) {
+    build_pairs(cand[ii], flat);
+    if (int e = install(flat)) return e;
+    c->items = cand[ii];
     for (int xm = 0; xm < 2; ++xm) {
       c->xcd = xm;
-      launch_spmv<float, float, float>(c, 0, c->d_sv32, c->v32[0], c->v32[1], nullptr, nullptr, nullptr, nullptr);
+      launch_spmv<float, float, float>(c, 0, c->d_sv32, c->v32[0], c->v32[1], nullptr, c->v32[2], nullptr, c->d_part);
       HIPCHK(hipEventRecord(c->ev0, c->stream));
-      for (int r = 0; r < 3; ++r)
-        launch_spmv<float, float, float>(c, 0, c->d_sv32, c->v32[0], c->v32[1], nullptr, nullptr, nullptr, nullptr);
+      for (int r = 0; r < 5; ++r)
+        launch_spmv<float, float, float>(c, 0, c->d_sv32, c->v32[0], c->v32[1], nullptr, c->v32[2], nullptr,
+                                         c->d_part);
       HIPCHK(hipEventRecord(c->ev1, c->stream));
       HIPCHK(hipEventSynchronize(c->ev1));
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
       if (ms < best) {
         best = ms;
+        best_items = cand[ii];
         best_xcd = xm;
+        best_flat = flat;
       }
     }
-    c->xcd = best_xcd;
   }
+  c->items = best_items;
+  c->xcd = best_xcd;
+  if (int e = install(best_flat)) return e;
   return 0;
 }
 
@@ -2712,6 +2742,9 @@ int tune_stream(shesolve_ctx *c) {
                      c->d_v64, c->d_v32);
   if (int e = ensure_vectors(c)) return e;
   hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)n, c->v32[0], 1.0f);
+  hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)n, c->v32[2], 1.0f);
+  // every candidate is timed as the solve launches it: with the fused dot
+  // product on the persistent grid (plain launches rank the mappings differently)
   for (int ii = 0; ii < 3; ++ii) {
     if (!c->prm.autotune && cand_items[ii] != 8) continue;
     std::vector<int> ab;
@@ -2725,10 +2758,11 @@ int tune_stream(shesolve_ctx *c) {
       c->xcd = xm;
       c->d_ablk = d_ab;
       c->nablk = (int)ab.size() - 1;
-      launch_spmv<float, float, float>(c, 0, c->d_v32, c->v32[0], c->v32[1], nullptr, nullptr, nullptr, nullptr);
+      launch_spmv<float, float, float>(c, 0, c->d_v32, c->v32[0], c->v32[1], nullptr, c->v32[2], nullptr, c->d_part);
       HIPCHK(hipEventRecord(c->ev0, c->stream));
       for (int r = 0; r < 3; ++r)
-        launch_spmv<float, float, float>(c, 0, c->d_v32, c->v32[0], c->v32[1], nullptr, nullptr, nullptr, nullptr);
+        launch_spmv<float, float, float>(c, 0, c->d_v32, c->v32[0], c->v32[1], nullptr, c->v32[2], nullptr,
+                                         c->d_part);
       HIPCHK(hipEventRecord(c->ev1, c->stream));
       HIPCHK(hipEventSynchronize(c->ev1));
       float ms = 0;
@@ -2752,10 +2786,11 @@ int tune_stream(shesolve_ctx *c) {
     const int cand_lpr[3] = {4, 8, 16};
     for (int li = 0; li < 3; ++li) {
       c->spmv_lpr = cand_lpr[li];
-      launch_spmv<float, float, float>(c, 0, c->d_v32, c->v32[0], c->v32[1], nullptr, nullptr, nullptr, nullptr);
+      launch_spmv<float, float, float>(c, 0, c->d_v32, c->v32[0], c->v32[1], nullptr, c->v32[2], nullptr, c->d_part);
       HIPCHK(hipEventRecord(c->ev0, c->stream));
       for (int r = 0; r < 3; ++r)
-        launch_spmv<float, float, float>(c, 0, c->d_v32, c->v32[0], c->v32[1], nullptr, nullptr, nullptr, nullptr);
+        launch_spmv<float, float, float>(c, 0, c->d_v32, c->v32[0], c->v32[1], nullptr, c->v32[2], nullptr,
+                                         c->d_part);
       HIPCHK(hipEventRecord(c->ev1, c->stream));
       HIPCHK(hipEventSynchronize(c->ev1));
       float ms = 0;
