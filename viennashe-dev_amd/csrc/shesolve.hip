@@ -27,6 +27,7 @@
 #include <queue>
 #include <thread>
 #include <atomic>
+#include <utility>
 #include <vector>
 
 #include "shesolve.h"
@@ -183,8 +184,7 @@ constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dispatched roun
 // Block multicolour ordering: blocks of kBmcRows consecutive natural rows are
 // coloured, and a sweep gives each block half a wavefront (one lane per row)
 constexpr int kBmcRows = 32;
-constexpr int kBmcThreads = 128;   // 4 blocks per workgroup
-static_assert(kBmcRows == 32, "one block per half wavefront");
+static_assert(kBmcRows == 32, "one block per half wavefront, two DPP rows");
 
 enum { MODE_SPMV = 0, MODE_LSWEEP = 1, MODE_USWEEP = 2 };
 
@@ -315,111 +315,256 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
   }
 }
 
-// Value of `x` in lane j of this lane's half wavefront (j a compile-time
-// constant inside the unrolled step loops below).
-template <typename T>
-__device__ inline T half_bcast(T x, int j) {
-  const bool upper = (threadIdx.x & 32) != 0;
-  if constexpr (sizeof(T) == 4) {
-    const int a = __builtin_amdgcn_readlane(__float_as_int(x), j);
-    const int b = __builtin_amdgcn_readlane(__float_as_int(x), 32 + j);
-    return __int_as_float(upper ? b : a);
-  } else {
-    const long long bits = __double_as_longlong(x);
-    const int lo = (int)(bits & 0xffffffffll), hi = (int)(bits >> 32);
-    const int alo = __builtin_amdgcn_readlane(lo, j), ahi = __builtin_amdgcn_readlane(hi, j);
-    const int blo = __builtin_amdgcn_readlane(lo, 32 + j), bhi = __builtin_amdgcn_readlane(hi, 32 + j);
-    const unsigned long long r = upper ? (((unsigned long long)(unsigned)bhi << 32) | (unsigned)blo)
-                                       : (((unsigned long long)(unsigned)ahi << 32) | (unsigned)alo);
-    return __longlong_as_double((long long)r);
-  }
+// Compile-time unrolled loop: f(std::integral_constant<int, J>) for J = 0..N-1
+// (the DPP lane selects below must be constants).
+template <typename F, int... J>
+__device__ inline void unroll_seq(F &&f, std::integer_sequence<int, J...>) {
+  (f(std::integral_constant<int, J>{}), ...);
+}
+template <int N, typename F>
+__device__ inline void unroll(F &&f) {
+  unroll_seq(f, std::make_integer_sequence<int, N>{});
+}
+
+// Value of `x` in lane L of this lane's row of 16 lanes (DPP row_newbcast).
+template <int L>
+__device__ inline float row_bcast(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x150 + L, 0xF, 0xF, false));
+}
+template <int L>
+__device__ inline double row_bcast(double x) {
+  const long long bits = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)bits, 0x150 + L, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), 0x150 + L, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// Rows of 16 lanes exchanged in pairs (v_permlane16_swap): lanes 16-31 (and
+// 48-63) receive the value of lanes 0-15 (32-47) when `upper` is false, and
+// lanes 0-15 (32-47) that of lanes 16-31 (48-63) when it is true.
+template <bool UPPER>
+__device__ inline float row_swap(float x) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float(UPPER ? r[1] : r[0]);
+}
+template <bool UPPER>
+__device__ inline double row_swap(double x) {
+  const long long bits = __double_as_longlong(x);
+  const unsigned lo = (unsigned)bits, hi = (unsigned)(bits >> 32);
+  const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  const unsigned rl = UPPER ? a[1] : a[0], rh = UPPER ? b[1] : b[0];
+  return __longlong_as_double((long long)(((unsigned long long)rh << 32) | rl));
 }
 
 // One colour of a block multicolour sweep.  Blocks [b0, b1) of the solve-space
-// block starts bstart, each at most kBmcRows rows, independent of every other
-// block of the colour.  Half a wavefront per block, lane hl owns row r0 + hl:
+// block starts bstart (at most kBmcRows rows each; blocks of one colour are
+// independent); each triangle is split at set-up into the entries that couple
+// to other blocks (columns of other colours, already final), a CSR by row
+// (orp, oci, ov; bK[blk] = orp at the block's first row), and the entries
+// inside each block as (LDS slot, value) lists per block (iptr, icode, iv).
 //   forward (unit L):  z[row] = in[row] - sum_k L[k] z[ci[k]]
 //   backward:          z[row] = dinv[row] * (z[row] - sum_k U[k] z[ci[k]])
-// A row's entries are loaded kBmcLoad at a time, all loads of a batch in
-// flight together; entries outside the block (other colours, already final)
-// are subtracted in column order.  Entries inside the block go to the lane's
-// own slots of a packed strict triangle in LDS; then all lanes walk the
-// block's rows in elimination order, the finished value of row j broadcast
-// from lane j and subtracted by the rows that couple to it.
-constexpr int kBmcLoad = 8;
-constexpr int kBmcTri = kBmcRows * (kBmcRows - 1) / 2;  // strict triangle of a block
-// slots per block: the triangle plus one slot that always holds zero, read by
-// the lanes that own no slot of a step (so the steps need no branches)
-constexpr int kBmcSlots = kBmcTri + 1;
-// slot of (column j, row r) in the packed triangle: forward rows r > j,
-// backward rows r < j; consecutive rows of one column are adjacent (the lanes
-// of one step read consecutive words)
-__device__ constexpr int bmc_slot_fwd(int j, int r) { return j * (kBmcRows - 1) - j * (j - 1) / 2 + (r - j - 1); }
-__device__ constexpr int bmc_slot_bwd(int j, int r) { return j * (j - 1) / 2 + r; }
-// lane r's slot of step j, or the zero slot kBmcTri when it owns none
-__device__ constexpr int bmc_own(bool backward, int j, int r) {
-  return backward ? (r < j ? bmc_slot_bwd(j, r) : kBmcTri) : (r > j ? bmc_slot_fwd(j, r) : kBmcTri);
+// A workgroup takes kBmcGroupOf<T> consecutive blocks of the colour (groups
+// counted from the colour's first block; with xcd set each XCD, workgroup id
+// mod 8, takes a contiguous eighth of them), half a wavefront per block and
+// one lane per row.  It streams the group's coupling entries as one contiguous
+// range (consecutive lanes, consecutive entries), multiplies each by its
+// gathered z into LDS and sums each row's run, while the in-block lists fill
+// each block's coefficient tile.  Then each block is eliminated in registers:
+// in the elimination order (forward: rows ascending; backward: descending,
+// written m = 31 - row below) the block's first 16 rows form one DPP row of the
+// half wavefront and the last 16 the other.  The first row's lanes eliminate
+// among themselves, each step broadcasting the finished value of lane j to its
+// row (row_bcast) and adding coefficient times value; the finished values are
+// then handed to the second row's lanes (row_swap), which add their couplings
+// to all 16 and eliminate among themselves the same way.  The tile holds, for
+// lane m and step j (both in elimination order), the negated coefficient:
+// -L(row, col) forward, -U(row, col) / diag(col) backward (so that the value
+// broadcast is the unscaled one); first-row lanes at j*16 + m, second-row lanes
+// at kBmcSecond + j*16 + (m - 16), each step's reads being 16 consecutive words
+// per row of lanes, the two rows in disjoint banks.
+constexpr int kBmcHalf = 16;                                // lanes of a DPP row
+constexpr int kBmcSecond = 272;                             // second-row region (16 mod 32 words)
+constexpr int kBmcTile = kBmcSecond + kBmcRows * kBmcHalf;  // words per block
+constexpr int kBmcGroupMax = 8;                             // blocks per workgroup (FP32)
+constexpr int kBmcItems = 4;                                // coupling entries per thread held per group
+constexpr int kBmcIn = 2;                                   // in-block entries per thread held per group
+static_assert(kBmcGroupMax * kBmcTile < 65536, "in-block slot codes are 16-bit");
+static_assert(kBmcSecond >= kBmcHalf * kBmcHalf && kBmcSecond % 32 == 16, "tile regions");
+template <typename T>
+constexpr int kBmcGroupOf = sizeof(T) == 8 ? 4 : 8;  // FP64: 4 blocks, so that the tile fits beside others
+// slot of the in-block coefficient of (row r, column j), local to the block;
+// j < r forward, j > r backward
+__host__ __device__ constexpr int bmc_slot(bool backward, int j, int r) {
+  const int m = backward ? kBmcRows - 1 - r : r, jm = backward ? kBmcRows - 1 - j : j;
+  return m < kBmcHalf ? jm * kBmcHalf + m : kBmcSecond + jm * kBmcHalf + (m - kBmcHalf);
 }
 
+// Target of the sweep's stores from lanes that own no row, so that every
+// iteration issues the same memory operations (its contents are never read).
+__device__ double g_bmc_discard[kBmcGroupMax * kBmcRows];
+
 template <bool BACKWARD, typename TV, typename T>
-__global__ __launch_bounds__(kBmcThreads) void k_bmc_sweep(int b0, int b1, const int *__restrict__ bstart,
-                                                           const int *__restrict__ rp, const int *__restrict__ ci,
-                                                           const TV *__restrict__ v, const TV *__restrict__ dinv,
-                                                           const T *__restrict__ in, T *z) {
-  __shared__ T tile[(kBmcThreads / kBmcRows) * kBmcSlots];
-  const int sub = (int)threadIdx.x / kBmcRows, hl = (int)threadIdx.x % kBmcRows;
-  T *tl = tile + sub * kBmcSlots;
-  const int blk = b0 + (int)blockIdx.x * (kBmcThreads / kBmcRows) + sub;
-  int r0 = 0, nr = 0;
-  if (blk < b1) {
-    r0 = bstart[blk];
-    nr = bstart[blk + 1] - r0;
-  }
-  // zero the lane's own slots (the zero slot where the lane owns none)
+__global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
+    int b0, int b1, const int *__restrict__ bstart, const int *__restrict__ bK, const int *__restrict__ orp,
+    const int *__restrict__ oci, const TV *__restrict__ ov, const int *__restrict__ iptr,
+    const unsigned short *__restrict__ icode, const TV *__restrict__ iv, const TV *__restrict__ dinv,
+    const T *__restrict__ in, T *z) {
+  constexpr int G = kBmcGroupOf<T>, NT = G * kBmcRows, CAP = kBmcItems * NT, ICAP = kBmcIn * NT;
+  __shared__ __align__(16) T tile[G * kBmcTile];
+  __shared__ T prod[CAP];
+  // persistent: the groups of XCD x (workgroup id mod 8) are a contiguous
+  // range, and the P workgroups of that XCD take every P-th group of it
+  const int ngroups = (b1 - b0 + G - 1) / G;
+  const int P = (int)gridDim.x / kXcds, x = (int)blockIdx.x % kXcds, wi = (int)blockIdx.x / kXcds;
+  const int per = (ngroups + kXcds - 1) / kXcds;
+  const int xe = min(ngroups, (x + 1) * per), first = x * per + wi;
+  if (first >= xe) return;  // whole workgroup
+  const int count = (xe - first + P - 1) / P, last = first + (count - 1) * P;
+  const int tid = (int)threadIdx.x, sub = tid / kBmcRows, hl = tid % kBmcRows;
+
+  // Three load stages per group, issued in consecutive iterations so that each
+  // stage's latency overlaps the elimination of an earlier group.  Indices are
+  // clamped rather than branched on (past the last group, the last group again).
+  struct StA {  // the lane's block start and end; the group's coupling and in-block bounds (lane parity picks which)
+    int g0, gn, rlo, rhi, kb, ip;
+  };
+  struct StB {  // the lane's row, the group's entries (up to CAP / ICAP)
+    int row, nr, K0, K1, I0, I1, ks, ke;
+    T acc0, d;
+    int cc[kBmcItems];
+    TV vv[kBmcItems];
+    int ic[kBmcIn];
+    TV ivv[kBmcIn];
+  };
+  auto stage_a = [&](int t, StA &a) {
+    const int g = min(first + t * P, last);
+    a.g0 = b0 + g * G;
+    a.gn = min(G, b1 - a.g0);
+    const int sb = a.g0 + min(sub, a.gn - 1), gb = (tid & 1) ? a.g0 + a.gn : a.g0;
+    a.rlo = bstart[sb];
+    a.rhi = bstart[sb + 1];
+    a.kb = bK[gb];
+    a.ip = iptr[gb];
+  };
+  auto stage_b = [&](const StA &a, StB &b) {
+    // lane 0 of every wavefront holds the group's first bound, lane 1 its last
+    b.K0 = __builtin_amdgcn_readlane(a.kb, 0);
+    b.K1 = __builtin_amdgcn_readlane(a.kb, 1);
+    b.I0 = __builtin_amdgcn_readlane(a.ip, 0);
+    b.I1 = __builtin_amdgcn_readlane(a.ip, 1);
+    b.nr = sub < a.gn ? a.rhi - a.rlo : 0;
+    b.row = a.rlo + hl;
+    const int rc = hl < b.nr ? b.row : a.rlo;
+    b.acc0 = BACKWARD ? z[rc] : in[rc];
+    b.d = BACKWARD ? (T)dinv[rc] : (T)1;
+    b.ks = orp[rc];
+    b.ke = orp[rc + 1];
 #pragma unroll
-  for (int j = 0; j < kBmcRows; ++j) tl[bmc_own(BACKWARD, j, hl)] = (T)0;
-  const int row = r0 + hl;
-  T acc = 0, d = 1;
-  if (hl < nr) {
-    acc = BACKWARD ? z[row] : in[row];
-    if (BACKWARD) d = (T)dinv[row];
-    const int k1 = rp[row + 1];
-    for (int kb = rp[row]; kb < k1; kb += kBmcLoad) {
-      int cc[kBmcLoad];
-      TV vv[kBmcLoad];
-      T zz[kBmcLoad];
-#pragma unroll
-      for (int u = 0; u < kBmcLoad; ++u)
-        if (kb + u < k1) {
-          cc[u] = ci[kb + u];
-          vv[u] = v[kb + u];
-        }
-#pragma unroll
-      for (int u = 0; u < kBmcLoad; ++u)
-        if (kb + u < k1) zz[u] = ((unsigned)(cc[u] - r0) < (unsigned)nr) ? (T)0 : z[cc[u]];
-#pragma unroll
-      for (int u = 0; u < kBmcLoad; ++u)
-        if (kb + u < k1) {
-          const int off = cc[u] - r0;
-          if ((unsigned)off < (unsigned)nr) tl[BACKWARD ? bmc_slot_bwd(off, hl) : bmc_slot_fwd(off, hl)] = (T)vv[u];
-          else acc -= (T)vv[u] * zz[u];
-        }
+    for (int u = 0; u < kBmcItems; ++u) {
+      const int k = b.K0 + tid + u * NT, i = k < b.K1 ? k : b.K0;
+      b.cc[u] = __builtin_nontemporal_load(oci + i);
+      b.vv[u] = __builtin_nontemporal_load(ov + i);
     }
-  }
-  if (!BACKWARD) {
 #pragma unroll
-    for (int j = 0; j < kBmcRows; ++j) {
-      const T zj = half_bcast(acc, j);
-      acc -= tl[bmc_own(false, j, hl)] * zj;
+    for (int u = 0; u < kBmcIn; ++u) {
+      const int k = b.I0 + tid + u * NT, i = k < b.I1 ? k : b.I0;
+      b.ic[u] = __builtin_nontemporal_load(icode + i);
+      b.ivv[u] = __builtin_nontemporal_load(iv + i);
     }
-    if (hl < nr) z[row] = acc;
-  } else {
+  };
+  // the gathered values of the coupling entries (an empty range reads the
+  // entry at K0 of an array never empty, and the value is not used)
+  auto stage_c = [&](const StB &b, T *zz) {
 #pragma unroll
-    for (int j = kBmcRows - 1; j >= 0; --j) {
-      const T zj = half_bcast(acc * d, j);
-      acc -= tl[bmc_own(true, j, hl)] * zj;
+    for (int u = 0; u < kBmcItems; ++u) zz[u] = z[b.K0 + tid + u * NT < b.K1 ? b.cc[u] : b.row - hl];
+  };
+
+  StA a;
+  StB bc, bn, bnn;
+  T zc[kBmcItems], zn[kBmcItems];
+  stage_a(0, a);
+  stage_b(a, bc);
+  stage_c(bc, zc);
+  stage_a(1, a);
+  stage_b(a, bn);
+  stage_a(2, a);
+  // one wait here, so that the loop's waits count only the loop's own operations
+  __builtin_amdgcn_s_waitcnt(0);
+  T *discard = reinterpret_cast<T *>(g_bmc_discard) + tid;
+  for (int t = 0; t < count; ++t) {
+    stage_c(bn, zn);
+    stage_b(a, bnn);
+    stage_a(t + 3, a);
+    // eliminate the group of bc / zc
+    {
+      float4 *t4 = reinterpret_cast<float4 *>(tile);
+      static_assert((G * kBmcTile * sizeof(T)) % 16 == 0, "tile cleared 16 bytes at a time");
+      for (int i = tid; i < (int)(G * kBmcTile * sizeof(T) / 16); i += NT) t4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    if (hl < nr) z[row] = acc * d;
+    __syncthreads();  // tiles cleared
+    const int icnt = bc.I1 - bc.I0, ccnt = min(bc.K1 - bc.K0, CAP);
+#pragma unroll
+    for (int u = 0; u < kBmcIn; ++u)
+      if (tid + u * NT < icnt) {
+        const int code = bc.ic[u];
+        tile[((code / kBmcTile) % G) * kBmcTile + code % kBmcTile] = (T)bc.ivv[u];
+      }
+    for (int k = bc.I0 + ICAP + tid; k < bc.I1; k += NT) {  // groups beyond the held entries
+      const int code = icode[k];
+      tile[((code / kBmcTile) % G) * kBmcTile + code % kBmcTile] = (T)iv[k];
+    }
+#pragma unroll
+    for (int u = 0; u < kBmcItems; ++u)
+      if (tid + u * NT < ccnt) prod[tid + u * NT] = (T)bc.vv[u] * zc[u];
+    __syncthreads();  // in-block values and products in place
+    const bool own = hl < bc.nr;
+    T s = 0;
+    {
+      const int lo = own ? bc.ks : bc.K0, hi = own ? min(bc.ke, bc.K0 + ccnt) : bc.K0;
+      for (int k = lo; k < hi; ++k) s += prod[k - bc.K0];
+    }
+    // coupling entries beyond the held ones, a chunk at a time (uniform)
+    for (int kc = bc.K0 + CAP; kc < bc.K1; kc += CAP) {
+      __syncthreads();
+      const int cnt = min(CAP, bc.K1 - kc);
+      for (int k = tid; k < cnt; k += NT) prod[k] = (T)ov[kc + k] * z[oci[kc + k]];
+      __syncthreads();
+      if (own)
+        for (int k = max(bc.ks, kc); k < min(bc.ke, kc + cnt); ++k) s += prod[k - kc];
+    }
+    T acc = bc.acc0 - s;
+    // elimination order: m = row (forward) or 31 - row (backward); source lane of
+    // step j inside its row of 16: j (forward) or 15 - j (backward), mod 16
+    const int m = BACKWARD ? kBmcRows - 1 - hl : hl;
+    const bool second = m >= kBmcHalf;
+    const T *tl = tile + sub * kBmcTile + (second ? kBmcSecond + (m - kBmcHalf) : m);
+    T cf[kBmcRows];
+#pragma unroll
+    for (int j = 0; j < kBmcRows; ++j) cf[j] = (second || j < kBmcHalf) ? tl[j * kBmcHalf] : (T)0;
+    __syncthreads();  // tiles and products free for the next group
+    if (!second) {
+      unroll<kBmcHalf>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        acc = fma(cf[j], row_bcast<BACKWARD ? kBmcHalf - 1 - j : j>(acc), acc);
+      });
+    }
+    const T tv = row_swap<BACKWARD>(acc);  // second-row lanes: the first row's finished values
+    if (second) {
+      unroll<kBmcHalf>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        acc = fma(cf[j], row_bcast<BACKWARD ? kBmcHalf - 1 - j : j>(tv), acc);
+      });
+      unroll<kBmcHalf>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        acc = fma(cf[kBmcHalf + j], row_bcast<BACKWARD ? kBmcHalf - 1 - j : j>(acc), acc);
+      });
+    }
+    *(own ? z + bc.row : discard) = BACKWARD ? acc * bc.d : acc;
+    bc = bn;
+    bn = bnn;
+#pragma unroll
+    for (int u = 0; u < kBmcItems; ++u) zc[u] = zn[u];
   }
 }
 
@@ -575,6 +720,20 @@ __global__ __launch_bounds__(kBlock) void k_cast(int64_t n, const TI *__restrict
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     out[i] = (TO)a[i];
 }
+// In-block coefficients of the block multicolour sweep: -src[pos], divided by
+// the column's diagonal when dinv is given (backward triangle), in FP64 and FP32.
+__global__ __launch_bounds__(kBlock) void k_bmc_inblock(int64_t n, const int *__restrict__ pos,
+                                                       const int *__restrict__ col, const double *__restrict__ src,
+                                                       const double *__restrict__ dinv, double *__restrict__ o64,
+                                                       float *__restrict__ o32) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = -src[pos[i]];
+  if (dinv) v *= dinv[col[i]];
+  o64[i] = v;
+  o32[i] = (float)v;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill(int64_t n, T *__restrict__ a, T v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) a[i] = v;
@@ -1170,6 +1329,17 @@ struct shesolve_ctx {
   // range [bmc_cptr[q], bmc_cptr[q+1]) of its blocks
   int *d_bmc_start = nullptr;
   std::vector<int> bmc_cptr;
+  // grouped sweep layout of each triangle: coupling entries (CSR by row, with
+  // their positions in the split factor) and in-block (slot, value) lists
+  struct BmcPart {
+    int *d_orp = nullptr, *d_oci = nullptr, *d_opos = nullptr, *d_iptr = nullptr, *d_ipos = nullptr;
+    int *d_bK = nullptr, *d_icol = nullptr;
+    unsigned short *d_icode = nullptr;
+    double *d_ov64 = nullptr, *d_iv64 = nullptr;
+    float *d_ov32 = nullptr, *d_iv32 = nullptr;
+    int64_t no = 0, ni = 0;
+  } bmcL, bmcU;
+  int bmc_cap[2] = {0, 0};  // persistent grid of the sweep (FP32, FP64), from the occupancy at first use
   // CSR-stream row blocks of A and the tuned launch shape; in a batch no block
   // straddles two systems, and d_blk_sys holds each block's system
   int *d_ablk = nullptr;
@@ -1387,18 +1557,39 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
   if (!c->bmc_cptr.empty() && !perm) {
     // block multicolour factor in the solve space: one launch per colour and
     // direction, forward in ascending and backward in descending colours
-    const int nq = (int)c->bmc_cptr.size() - 1, per = kBmcThreads / kBmcRows;
+    const int nq = (int)c->bmc_cptr.size() - 1;
+    constexpr int G = kBmcGroupOf<T>;
+    // persistent grid: what the device holds at once (a multiple of the XCD
+    // count), or fewer when the colour has fewer groups
+    int &cap = c->bmc_cap[sizeof(T) == 8 ? 1 : 0];
+    if (!cap) {
+      int per_cu = 0, cus = 0;
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bmc_group<false, TV, T>, G * kBmcRows, 0);
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+      cap = std::max(kXcds, std::max(per_cu, 1) * std::max(cus, 1) / kXcds * kXcds);
+    }
+    auto grid = [&](int b0, int b1) {
+      const int ng = (b1 - b0 + G - 1) / G;
+      return std::min(cap, (ng + kXcds - 1) / kXcds * kXcds);
+    };
+    const auto &PL = c->bmcL, &PU = c->bmcU;
+    const TV *oL, *iL, *oU, *iU;
+    if constexpr (sizeof(TV) == 8) {
+      oL = (const TV *)PL.d_ov64; iL = (const TV *)PL.d_iv64; oU = (const TV *)PU.d_ov64; iU = (const TV *)PU.d_iv64;
+    } else {
+      oL = (const TV *)PL.d_ov32; iL = (const TV *)PL.d_iv32; oU = (const TV *)PU.d_ov32; iU = (const TV *)PU.d_iv32;
+    }
     for (int q = 0; q < nq; ++q) {
       KTimer kt(c, 1);
       const int b0 = c->bmc_cptr[q], b1 = c->bmc_cptr[q + 1];
-      hipLaunchKernelGGL((k_bmc_sweep<false, TV, T>), dim3((b1 - b0 + per - 1) / per), dim3(kBmcThreads), 0,
-                         c->stream, b0, b1, c->d_bmc_start, c->d_Lrp, c->d_Lci, Lv, dinv, in, out);
+      hipLaunchKernelGGL((k_bmc_group<false, TV, T>), dim3(grid(b0, b1)), dim3(G * kBmcRows), 0, c->stream, b0, b1,
+                         c->d_bmc_start, PL.d_bK, PL.d_orp, PL.d_oci, oL, PL.d_iptr, PL.d_icode, iL, dinv, in, out);
     }
     for (int q = nq - 1; q >= 0; --q) {
       KTimer kt(c, 2);
       const int b0 = c->bmc_cptr[q], b1 = c->bmc_cptr[q + 1];
-      hipLaunchKernelGGL((k_bmc_sweep<true, TV, T>), dim3((b1 - b0 + per - 1) / per), dim3(kBmcThreads), 0,
-                         c->stream, b0, b1, c->d_bmc_start, c->d_Urp, c->d_Uci, Uv, dinv, in, out);
+      hipLaunchKernelGGL((k_bmc_group<true, TV, T>), dim3(grid(b0, b1)), dim3(G * kBmcRows), 0, c->stream, b0, b1,
+                         c->d_bmc_start, PU.d_bK, PU.d_orp, PU.d_oci, oU, PU.d_iptr, PU.d_icode, iU, dinv, in, out);
     }
     return;
   }
@@ -1511,6 +1702,12 @@ void free_matrix(shesolve_ctx *c) {
   c->xref_cur = nullptr;
   dfree(c->d_bmc_start);
   c->bmc_cptr.clear();
+  for (auto *P : {&c->bmcL, &c->bmcU}) {
+    dfree(P->d_orp); dfree(P->d_oci); dfree(P->d_opos); dfree(P->d_iptr); dfree(P->d_ipos); dfree(P->d_icode);
+    dfree(P->d_bK); dfree(P->d_icol);
+    dfree(P->d_ov64); dfree(P->d_iv64); dfree(P->d_ov32); dfree(P->d_iv32);
+    P->no = P->ni = 0;
+  }
   c->solve_perm = false;
   c->s_rp = c->s_ci = c->s_blk = c->s_pairs = nullptr;
   c->s_nblk = 0;
@@ -1808,6 +2005,94 @@ int iluk_pattern(shesolve_ctx *c, std::vector<int> &pos, bool drop_outside) {
   return 0;
 }
 
+// Grouped sweep layout of one triangle of a block multicolour factor (split
+// pattern rp/ci in the solve space, block starts bst): per row its entries
+// outside the row's block (CSR, and per block the CSR position of its first
+// row), per block the tile slots of its own entries (code = block index
+// within an aligned run of kBmcGroupMax blocks of the colour, times kBmcTile,
+// plus the slot), each with its position in the triangle's value array and
+// its column, for the value gathers after a refactor.
+int build_bmc_part(shesolve_ctx *c, const std::vector<int> &bst, bool upper, const std::vector<int> &rp,
+                   const std::vector<int> &ci, shesolve_ctx::BmcPart &P) {
+  const int n = c->n, nb = (int)bst.size() - 1;
+  if (nb < 1 || bst[nb] != n) {
+    set_err("block multicolour layout: blocks do not cover the rows");
+    return -1;
+  }
+  std::vector<int> orp(n + 1, 0), oci, opos, iptr(nb + 1, 0), ipos, icol, bK(nb + 1, 0);
+  std::vector<unsigned short> icode;
+  oci.reserve(rp[n]);
+  opos.reserve(rp[n]);
+  for (int q = 0; q + 1 < (int)c->bmc_cptr.size(); ++q)
+    for (int blk = c->bmc_cptr[q]; blk < c->bmc_cptr[q + 1]; ++blk) {
+      const int sub = (blk - c->bmc_cptr[q]) % kBmcGroupMax, r0 = bst[blk], r1 = bst[blk + 1];
+      bK[blk] = (int)oci.size();
+      for (int r = r0; r < r1; ++r) {
+        for (int k = rp[r]; k < rp[r + 1]; ++k) {
+          const int j = ci[k];
+          if (j >= r0 && j < r1) {
+            if (upper ? j <= r : j >= r) {
+              set_err("block multicolour layout: entry (%d, %d) is not in its triangle", r, j);
+              return -1;
+            }
+            icode.push_back((unsigned short)(sub * kBmcTile + bmc_slot(upper, j - r0, r - r0)));
+            ipos.push_back(k);
+            icol.push_back(j);
+          } else {
+            oci.push_back(j);
+            opos.push_back(k);
+          }
+        }
+        orp[r + 1] = (int)oci.size();
+      }
+      iptr[blk + 1] = (int)ipos.size();
+    }
+  bK[nb] = (int)oci.size();
+  P.no = (int64_t)oci.size();
+  P.ni = (int64_t)ipos.size();
+  const size_t no = std::max<size_t>(oci.size(), 1), ni = std::max<size_t>(ipos.size(), 1);
+  HIPCHK(hipMalloc(&P.d_orp, sizeof(int) * (n + 1)));
+  HIPCHK(hipMalloc(&P.d_bK, sizeof(int) * (nb + 1)));
+  HIPCHK(hipMalloc(&P.d_oci, sizeof(int) * no));
+  HIPCHK(hipMalloc(&P.d_opos, sizeof(int) * no));
+  HIPCHK(hipMalloc(&P.d_ov64, sizeof(double) * no));
+  HIPCHK(hipMalloc(&P.d_ov32, sizeof(float) * no));
+  HIPCHK(hipMalloc(&P.d_iptr, sizeof(int) * (nb + 1)));
+  HIPCHK(hipMalloc(&P.d_ipos, sizeof(int) * ni));
+  HIPCHK(hipMalloc(&P.d_icol, sizeof(int) * ni));
+  HIPCHK(hipMalloc(&P.d_icode, sizeof(unsigned short) * ni));
+  HIPCHK(hipMalloc(&P.d_iv64, sizeof(double) * ni));
+  HIPCHK(hipMalloc(&P.d_iv32, sizeof(float) * ni));
+  HIPCHK(hipMemcpy(P.d_orp, orp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P.d_bK, bK.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+  if (P.no) {
+    HIPCHK(hipMemcpy(P.d_oci, oci.data(), sizeof(int) * P.no, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P.d_opos, opos.data(), sizeof(int) * P.no, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(P.d_iptr, iptr.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+  if (P.ni) {
+    HIPCHK(hipMemcpy(P.d_ipos, ipos.data(), sizeof(int) * P.ni, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P.d_icol, icol.data(), sizeof(int) * P.ni, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P.d_icode, icode.data(), sizeof(unsigned short) * P.ni, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+// Values of the grouped sweep layout from the split factor's triangle values:
+// coupling entries as they are, in-block entries negated and, for the backward
+// triangle (dinv given), divided by their column's diagonal.
+void bmc_part_values(shesolve_ctx *c, const shesolve_ctx::BmcPart &P, const double *src, const double *dinv) {
+  if (P.no) {
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(P.no, kBlock * 4)), dim3(kBlock), 0, c->stream, P.no, P.d_opos, src,
+                       P.d_ov64);
+    hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(P.no, kBlock * 4)), dim3(kBlock), 0, c->stream, P.no,
+                       (const double *)P.d_ov64, P.d_ov32);
+  }
+  if (P.ni)
+    hipLaunchKernelGGL(k_bmc_inblock, dim3(grid_for(P.ni, kBlock)), dim3(kBlock), 0, c->stream, P.ni, P.d_ipos,
+                       P.d_icol, src, dinv, P.d_iv64, P.d_iv32);
+}
+
 int analyse_and_factor(shesolve_ctx *c) {
   const int n = c->n;
   if (n <= 0) return 0;
@@ -2008,10 +2293,10 @@ int analyse_and_factor(shesolve_ctx *c) {
       }
     }
   }
+  std::vector<int> bst;
   if (!perm.empty() && c->prm.precond == SHESOLVE_PRECOND_BMC_ILU0) {
     // blocks in the solve space: a block's rows stay consecutive (one colour,
     // natural order), so a block starts where the natural block changes
-    std::vector<int> bst;
     c->bmc_cptr.assign(1, 0);
     for (int p = 0; p < n; ++p) {
       if (p > 0 && perm[p] / kBmcRows == perm[p - 1] / kBmcRows) continue;
@@ -2175,7 +2460,10 @@ int analyse_and_factor(shesolve_ctx *c) {
   if (!lblk.empty()) HIPCHK(hipMemcpyAsync(c->d_lblk, lblk.data(), sizeof(int) * lblk.size(), hipMemcpyHostToDevice, c->stream));
   if (!ublk.empty()) HIPCHK(hipMemcpyAsync(c->d_ublk, ublk.data(), sizeof(int) * ublk.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!c->bmc_cptr.empty()) {
+    if (int e = build_bmc_part(c, bst, false, Lrp, Lci, c->bmcL)) return e;
+    if (int e = build_bmc_part(c, bst, true, Urp, Uci, c->bmcU)) return e;
+  }
   return 0;
 }
 
@@ -2212,6 +2500,10 @@ int refactor(shesolve_ctx *c) {
                        c->d_U64, c->d_U32);
   hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->n, kBlock * 4)), dim3(kBlock), 0, c->stream,
                      (int64_t)c->n, c->d_dinv64, c->d_dinv32);
+  if (!c->bmc_cptr.empty()) {
+    bmc_part_values(c, c->bmcL, c->d_L64, nullptr);
+    bmc_part_values(c, c->bmcU, c->d_U64, c->d_dinv64);
+  }
   if (c->nseg > 0) {
     // 1/diagonal in the backward schedule's order
     const int g = grid_for(c->n, kBlock);
