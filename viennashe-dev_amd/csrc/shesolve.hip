@@ -151,6 +151,12 @@ struct BSys {
 // then each thread reduces whole rows from LDS.  Used for the SpMV and for the
 // sweeps of every contiguous level of the triangular factors.
 constexpr int kStreamThreads = 256;
+// The pipelined SpMV loads column indices and values four entries at a time
+// from the block's first entry rounded down to a multiple of four: the
+// matrix arrays carry kVecPad entries past the end, and a block holds at most
+// spmv_cap(items) entries so that the rounded range fits items * 256.
+constexpr int kVecPad = 4;
+constexpr int spmv_cap(int items) { return items * kStreamThreads - 3; }
 constexpr int kStreamRows = 256;  // rows per block (rows without entries must not pile up)
 // Row blocks of the colour sweeps: entries per thread and rows per block
 // (8 and 256 measured fastest on the 2D L=3 device of 16.1 M rows against
@@ -303,6 +309,201 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
         }
       }
       __syncthreads();
+    }
+  }
+  if (part) {
+    double vv[2] = {a0, a1};
+    block_sum<2>(vv);
+    if (threadIdx.x == 0) {
+      part[blockIdx.x] = vv[0];
+      part[kPartStride + blockIdx.x] = vv[1];
+    }
+  }
+}
+
+// Target of the stores of lanes that own no row (contents never read), so that
+// every iteration of a pipelined loop issues the same memory operations.
+__device__ double g_discard[kStreamThreads];
+
+// Row block descriptors of the pipelined SpMV: {first row, end row, first
+// entry, end entry} per block, from consecutive boundaries or (first, end)
+// pairs.
+__global__ __launch_bounds__(kBlock) void k_block_desc(int nblk, const int *__restrict__ blocks,
+                                                      const int *__restrict__ pairs, const int *__restrict__ rp,
+                                                      int4 *__restrict__ desc) {
+  const int bk = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (bk >= nblk) return;
+  const int rs = pairs ? pairs[2 * bk] : blocks[bk], re = pairs ? pairs[2 * bk + 1] : blocks[bk + 1];
+  desc[bk] = make_int4(rs, re, rp[rs], rp[re]);
+}
+
+// CSR-stream SpMV, y = (b ? b - A x : A x), with the dot products of y with d0
+// and d1 per workgroup (as k_stream MODE_SPMV), software-pipelined over the
+// workgroup's row blocks.  Each iteration issues, in this order, the
+// descriptor of block t+3, the row and entry loads of block t+2 and the x
+// gathers of block t+1 (each from data that arrived during the previous
+// iteration), then reduces block t; the in-order memory counter then lets
+// every wait leave the younger stages in flight.  With FP32 values, column
+// indices and values are loaded 16 bytes per lane (four entries, from the
+// block's first entry rounded down to a multiple of four); loads use clamped
+// indices and fallback
+// pointers instead of branches.  Every block spans at most ITEMS * 256 entries
+// from that rounded start (the host launches this kernel only then).
+typedef int vint4 __attribute__((ext_vector_type(4)));
+typedef float vfloat4 __attribute__((ext_vector_type(4)));
+__device__ inline void load4(const float *p, float (&o)[4]) {
+  const vfloat4 t = __builtin_nontemporal_load(reinterpret_cast<const vfloat4 *>(p));
+  o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+}
+__device__ inline void load4(const int *p, int (&o)[4]) {
+  const vint4 t = __builtin_nontemporal_load(reinterpret_cast<const vint4 *>(p));
+  o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+}
+__device__ inline void store4(float *p, const float (&v)[4]) {
+  *reinterpret_cast<vfloat4 *>(p) = vfloat4{v[0], v[1], v[2], v[3]};
+}
+
+template <typename TV, typename T, int ITEMS>
+__global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd, const int *__restrict__ desc,
+                                                              const int *__restrict__ rp, const int *__restrict__ ci,
+                                                              const TV *__restrict__ v, const T *x, T *y,
+                                                              const T *__restrict__ b, const T *d0, const T *d1,
+                                                              double *__restrict__ part) {
+  // W consecutive entries per lane and load: four with FP32 values (16-byte
+  // loads), one with FP64 values (measured faster there: 563 against 594 us
+  // on the 161x81 L=3 device)
+  constexpr int NT = kStreamThreads, W = sizeof(TV) == 4 ? 4 : 1, NV = ITEMS / W;
+  static_assert(ITEMS % 4 == 0, "four entries per load");
+  __shared__ __align__(16) T prod[ITEMS * NT];
+  double a0 = 0.0, a1 = 0.0;
+  int bk0 = (int)blockIdx.x, bk1 = nblk, bstep = (int)gridDim.x;
+  if (xcd) {
+    const int xx = (int)blockIdx.x % kXcds;
+    bk0 = (int)((int64_t)nblk * xx / kXcds) + (int)blockIdx.x / kXcds;
+    bk1 = (int)((int64_t)nblk * (xx + 1) / kXcds);
+    bstep = (int)gridDim.x / kXcds;
+  }
+  const int count = bk0 < bk1 ? (bk1 - bk0 + bstep - 1) / bstep : 0;
+  const int tid = (int)threadIdx.x;
+  // a dot product with y itself takes the new value, not a prefetched one
+  const bool d0y = (const void *)d0 == (const void *)y, d1y = (const void *)d1 == (const void *)y;
+  const T *pb = b ? b : x, *p0 = d0 && !d0y ? d0 : x, *p1 = d1 && !d1y ? d1 : x;
+  if (count > 0) {
+    const int last = bk0 + (count - 1) * bstep;
+    struct StB {
+      int rs, re, k0, k1, r0, r1;
+      T bb, e0, e1;
+      int cc[ITEMS];
+      TV vv[ITEMS];
+    };
+    // lane l holds component l % 4 of the block's descriptor
+    auto stage_a = [&](int t) { return desc[4 * min(bk0 + t * bstep, last) + (tid & 3)]; };
+    // entry j of vector q of this lane: base + W (tid + q NT) + j, base = k0 rounded down
+    auto stage_b = [&](int dv, StB &s) {
+      s.rs = __builtin_amdgcn_readlane(dv, 0);
+      s.re = __builtin_amdgcn_readlane(dv, 1);
+      s.k0 = __builtin_amdgcn_readlane(dv, 2);
+      s.k1 = __builtin_amdgcn_readlane(dv, 3);
+      const int rc = s.rs + tid < s.re ? s.rs + tid : s.rs;
+      s.r0 = rp[rc];
+      s.r1 = rp[rc + 1];
+      s.bb = pb[rc];
+      s.e0 = p0[rc];
+      s.e1 = p1[rc];
+      const int base = s.k0 & ~3;
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        const int e = base + W * (tid + q * NT), i = e < s.k1 ? e : 0;
+        if constexpr (W == 4) {
+          int c4[4];
+          TV v4[4];
+          load4(ci + i, c4);
+          load4(v + i, v4);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            s.cc[4 * q + j] = c4[j];
+            s.vv[4 * q + j] = v4[j];
+          }
+        } else {
+          s.cc[q] = __builtin_nontemporal_load(ci + i);
+          s.vv[q] = __builtin_nontemporal_load(v + i);
+        }
+      }
+    };
+    auto stage_c = [&](const StB &s, T *xg) {
+      const int base = s.k0 & ~3;
+#pragma unroll
+      for (int q = 0; q < NV; ++q)
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          const int e = base + W * (tid + q * NT) + j;
+          xg[W * q + j] = x[e >= s.k0 && e < s.k1 ? s.cc[W * q + j] : s.rs];
+        }
+    };
+    // three sets of row/entry loads and two of gathers and of descriptors, their
+    // roles rotated by unrolling (copying a register whose load is in flight
+    // would wait for it)
+    StB s0, s1, s2;
+    T x0[ITEMS], x1[ITEMS];
+    int dA, dB;
+    dA = stage_a(0);
+    stage_b(dA, s0);
+    stage_c(s0, x0);
+    dA = stage_a(1);
+    stage_b(dA, s1);
+    dA = stage_a(2);
+    // one wait here, so that the loop's waits count only the loop's own operations
+    __builtin_amdgcn_s_waitcnt(0);
+    T *discard = reinterpret_cast<T *>(g_discard) + tid;
+    // iteration t: descriptor of t+3 into dn, rows and entries of t+2 (from dc)
+    // into nn, gathers of t+1 (from nx) into xn, then block t (cur, xc)
+    auto body = [&](int t, int &dc, int &dn, StB &cur, StB &nx, StB &nn, T(&xc)[ITEMS], T(&xn)[ITEMS]) {
+      // the stages are issued in this order (the scheduler may not interleave them)
+      dn = stage_a(t + 3);
+      __builtin_amdgcn_sched_barrier(0);
+      stage_b(dc, nn);
+      __builtin_amdgcn_sched_barrier(0);
+      stage_c(nx, xn);
+      __builtin_amdgcn_sched_barrier(0);
+      const int base = cur.k0 & ~3;
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        T pw[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          const int e = base + W * (tid + q * NT) + j;
+          pw[j] = e >= cur.k0 && e < cur.k1 ? (T)cur.vv[W * q + j] * xc[W * q + j] : (T)0;
+        }
+        if constexpr (W == 4) store4(prod + 4 * (tid + q * NT), pw);
+        else prod[tid + q * NT] = pw[0];
+      }
+      __syncthreads();
+      const bool own = cur.rs + tid < cur.re;
+      T acc = 0;
+      {
+        const int lo = own ? cur.r0 : cur.k0, hi = own ? cur.r1 : cur.k0;
+        for (int k = lo; k < hi; ++k) acc += prod[k - base];
+      }
+      const T out = b ? (T)(cur.bb - acc) : acc;
+      *(own ? y + cur.rs + tid : discard) = out;
+      if (own) {
+        a0 += d0 ? (double)(d0y ? out : cur.e0) * (double)out : 0.0;
+        a1 += d1 ? (double)(d1y ? out : cur.e1) * (double)out : 0.0;
+      }
+      __syncthreads();
+    };
+    for (int t = 0; t < count; t += 6) {
+      body(t, dA, dB, s0, s1, s2, x0, x1);
+      if (t + 1 >= count) break;
+      body(t + 1, dB, dA, s1, s2, s0, x1, x0);
+      if (t + 2 >= count) break;
+      body(t + 2, dA, dB, s2, s0, s1, x0, x1);
+      if (t + 3 >= count) break;
+      body(t + 3, dB, dA, s0, s1, s2, x1, x0);
+      if (t + 4 >= count) break;
+      body(t + 4, dA, dB, s1, s2, s0, x0, x1);
+      if (t + 5 >= count) break;
+      body(t + 5, dB, dA, s2, s0, s1, x1, x0);
     }
   }
   if (part) {
@@ -1351,6 +1552,12 @@ struct shesolve_ctx {
   int spmv_grid = kGrid;  // workgroups of the last SpMV with fused dot products
   double tune_us[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   int spmv_lpr = 0;  // 0: CSR-stream SpMV; else lanes per row of the vector-CSR SpMV
+  // descriptors of the pipelined SpMV, for the row blocks at desc_src (blocks
+  // or pairs) of the solve-space matrix
+  int4 *d_desc = nullptr;
+  const int *desc_src = nullptr;
+  int desc_nblk = 0;
+  int desc_maxrow = 0;  // longest row of that matrix (a block holds more entries only as a single row)
   // split factor: strict lower / strict upper parts and 1/diagonal (factor space)
   int *d_Lrp = nullptr, *d_Lci = nullptr, *d_Urp = nullptr, *d_Uci = nullptr, *d_Lsrc = nullptr;
   double *d_L64 = nullptr, *d_U64 = nullptr, *d_dinv64 = nullptr;
@@ -1520,6 +1727,18 @@ void launch_spmv(shesolve_ctx *c, int /*lpr*/, const TV *v, const T *x, T *y, co
   // (a grid of up to 4 kGrid measured no faster)
   const int g = stream_grid(nblk, part != nullptr);
   if (part) c->spmv_grid = g;
+  const int *pairs = c->s_rp ? c->s_pairs : nullptr;
+  if (!c->spmv_skip && c->d_desc && c->desc_src == (pairs ? pairs : blk) && c->desc_nblk == nblk &&
+      c->desc_maxrow <= spmv_cap(c->items)) {
+#define SP_CASE(IT)                                                                                               \
+  case IT:                                                                                                        \
+    hipLaunchKernelGGL((k_spmv_pipe<TV, T, IT>), dim3(g), dim3(kStreamThreads), 0, c->stream, nblk, c->xcd,        \
+                       (const int *)c->d_desc, rp, ci, v, x, y, b, d0, d1, part);                                \
+    break;
+    switch (c->items) { SP_CASE(4) SP_CASE(16) default: SP_CASE(8) }
+#undef SP_CASE
+    return;
+  }
   launch_stream<TV, T, MODE_SPMV>(c, g, nblk, blk, rp, ci, v, x, y, b,
                                   (const int *)nullptr, (const T *)nullptr, (T *)nullptr, d0, d1, part,
                                   c->s_rp ? c->s_pairs : nullptr);
@@ -1712,6 +1931,7 @@ void free_matrix(shesolve_ctx *c) {
   c->s_rp = c->s_ci = c->s_blk = c->s_pairs = nullptr;
   c->s_nblk = 0;
   dfree(c->d_ablk); c->nablk = 0;
+  dfree(c->d_desc); c->desc_src = nullptr; c->desc_nblk = 0;
   dfree(c->d_blk_sys);
   c->spmv_skip = nullptr;
   dfree(c->d_Lrp); dfree(c->d_Lci); dfree(c->d_Urp); dfree(c->d_Uci); dfree(c->d_Lsrc);
@@ -2324,7 +2544,7 @@ int analyse_and_factor(shesolve_ctx *c) {
     c->d_fci = c->d_ci;
   } else {
     HIPCHK(hipMalloc(&c->d_frp, sizeof(int) * (n + 1)));
-    HIPCHK(hipMalloc(&c->d_fci, sizeof(int) * std::max<int64_t>(c->fnnz, 1)));
+    HIPCHK(hipMalloc(&c->d_fci, sizeof(int) * (std::max<int64_t>(c->fnnz, 1) + kVecPad)));
     HIPCHK(hipMemcpyAsync(c->d_frp, c->h_frp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_fci, c->h_fci.data(), sizeof(int) * c->fnnz, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMalloc(&c->d_fpos, sizeof(int) * std::max<int64_t>(c->fnnz, 1)));
@@ -2531,6 +2751,25 @@ int upload_values(shesolve_ctx *c) {
   return refactor(c);
 }
 
+// Descriptors of the pipelined SpMV for the row blocks `blocks` (boundaries)
+// or `pairs` ((first, end) per block) of the CSR with row pointer rp.
+int build_desc(shesolve_ctx *c, const int *blocks, const int *pairs, int nblk, const int *rp,
+               const std::vector<int> &hrp) {
+  dfree(c->d_desc);
+  c->desc_src = nullptr;
+  c->desc_nblk = 0;
+  c->desc_maxrow = 0;
+  for (size_t i = 0; i + 1 < hrp.size(); ++i) c->desc_maxrow = std::max(c->desc_maxrow, hrp[i + 1] - hrp[i]);
+  if (nblk <= 0) return 0;
+  HIPCHK(hipMalloc(&c->d_desc, sizeof(int4) * nblk));
+  hipLaunchKernelGGL(k_block_desc, dim3(grid_for(nblk, kBlock)), dim3(kBlock), 0, c->stream, nblk, blocks, pairs, rp,
+                     c->d_desc);
+  HIPCHK(hipGetLastError());
+  c->desc_src = pairs ? pairs : blocks;
+  c->desc_nblk = nblk;
+  return 0;
+}
+
 // Solve space of a multicolour factor: the permuted matrix (the factor's
 // pattern), its CSR-stream row blocks, and the permuted b / x of a solve.
 int setup_solve_space(shesolve_ctx *c) {
@@ -2540,11 +2779,11 @@ int setup_solve_space(shesolve_ctx *c) {
     c->s_ci = c->d_ci;
     c->s_blk = c->d_ablk;
     c->s_nblk = c->nablk;
-    return 0;
+    return build_desc(c, c->d_ablk, nullptr, c->nablk, c->d_rp, c->h_rp);
   }
   dfree(c->d_v32);  // the natural-order FP32 copy served the launch autotuner only
-  HIPCHK(hipMalloc(&c->d_sv64, sizeof(double) * std::max<int64_t>(c->fnnz, 1)));
-  HIPCHK(hipMalloc(&c->d_sv32, sizeof(float) * std::max<int64_t>(c->fnnz, 1)));
+  HIPCHK(hipMalloc(&c->d_sv64, sizeof(double) * (std::max<int64_t>(c->fnnz, 1) + kVecPad)));
+  HIPCHK(hipMalloc(&c->d_sv32, sizeof(float) * (std::max<int64_t>(c->fnnz, 1) + kVecPad)));
   HIPCHK(hipMalloc(&c->pb64, sizeof(double) * std::max(c->n, 1)));
   HIPCHK(hipMalloc(&c->px64, sizeof(double) * std::max(c->n, 1)));
   // Row blocks of the permuted matrix, inside each colour, dealt in the order
@@ -2558,7 +2797,7 @@ int setup_solve_space(shesolve_ctx *c) {
     for (int q = 1; q <= c->n; ++q) {
       if (q < c->n && c->h_colour[c->h_perm[q]] == c->h_colour[c->h_perm[start]]) continue;
       std::vector<int> blk;
-      make_blocks(c->h_frp.data(), start, q, items * kStreamThreads, blk);
+      make_blocks(c->h_frp.data(), start, q, spmv_cap(items), blk);
       for (size_t k = 0; k + 1 < blk.size(); ++k) pr.emplace_back(blk[k], blk[k + 1]);
       start = q;
     }
@@ -2578,7 +2817,7 @@ int setup_solve_space(shesolve_ctx *c) {
     c->s_blk = c->d_sblk;
     c->s_pairs = c->d_sblk;
     c->s_nblk = (int)(flat.size() / 2);
-    return 0;
+    return build_desc(c, nullptr, c->d_sblk, c->s_nblk, c->d_frp, c->h_frp);
   };
   c->s_rp = c->d_frp;
   c->s_ci = c->d_fci;
@@ -3029,7 +3268,7 @@ int tune_stream(shesolve_ctx *c) {
   std::vector<int> best_blocks;
   double best = 1e300;
   int best_items = 8, best_xcd = 1;
-  HIPCHK(hipMalloc(&c->d_v32, sizeof(float) * std::max<int64_t>(c->nnz, 1)));
+  HIPCHK(hipMalloc(&c->d_v32, sizeof(float) * (std::max<int64_t>(c->nnz, 1) + kVecPad)));
   hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(c->nnz, kBlock * 4)), dim3(kBlock), 0, c->stream, c->nnz,
                      c->d_v64, c->d_v32);
   if (int e = ensure_vectors(c)) return e;
@@ -3040,7 +3279,7 @@ int tune_stream(shesolve_ctx *c) {
   for (int ii = 0; ii < 3; ++ii) {
     if (!c->prm.autotune && cand_items[ii] != 8) continue;
     std::vector<int> ab;
-    matrix_blocks(c, cand_items[ii] * kStreamThreads, ab);
+    matrix_blocks(c, spmv_cap(cand_items[ii]), ab);
     int *d_ab = nullptr;
     HIPCHK(hipMalloc(&d_ab, sizeof(int) * ab.size()));
     HIPCHK(hipMemcpyAsync(d_ab, ab.data(), sizeof(int) * ab.size(), hipMemcpyHostToDevice, c->stream));
@@ -3150,8 +3389,8 @@ int set_matrix_body(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, co
   }
   if (int e = set_pattern_host(c, n, nnz, rp, ci, nblocks, ranges)) return e;
   HIPCHK(hipMalloc(&c->d_rp, sizeof(int) * (n + 1)));
-  HIPCHK(hipMalloc(&c->d_ci, sizeof(int) * std::max<int64_t>(nnz, 1)));
-  HIPCHK(hipMalloc(&c->d_v64, sizeof(double) * std::max<int64_t>(nnz, 1)));
+  HIPCHK(hipMalloc(&c->d_ci, sizeof(int) * (std::max<int64_t>(nnz, 1) + kVecPad)));
+  HIPCHK(hipMalloc(&c->d_v64, sizeof(double) * (std::max<int64_t>(nnz, 1) + kVecPad)));
   HIPCHK(hipMemcpyAsync(c->d_rp, c->h_rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_ci, c->h_ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_v64, val, sizeof(double) * nnz, val_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
