@@ -1,10 +1,7 @@
-// viennashe/math/sparse_matrix.hpp -- the matrix type at the solver boundary.
-//
-// Same layout and access semantics as viennashe::math::sparse_matrix
-// (viennashe/math/linalg_util.hpp:67-196): square, one std::map per row,
-// non-const operator() inserts a zero entry when absent.  Provided so that
-// the solver front end builds standalone; inside ViennaSHE the reference's
-// own type is used and this header is not included.
+// Test declarations under ViennaSHE's own path and names: the matrix type of
+// viennashe/math/linalg_util.hpp:67-196 (square, one std::map per row, the
+// non-const operator() inserting a zero entry when absent).  Used only to
+// compile the GPU solver overload standalone.
 #pragma once
 
 #include <cassert>
