@@ -63,7 +63,12 @@ struct Scalars {
   double rho, alpha, omega, beta;
   double r0v, ts, tt, rr, r0r, ss;
   double bnorm, res;
-  int breakdown, pad;
+  // halt: set by the iteration's last finalize on convergence (res <= tol),
+  // breakdown or a non-finite residual; the update kernels and finalizes of a
+  // later iteration then leave x, r, p and the scalars alone, so that the host
+  // may enqueue an iteration before it has read the previous one's result
+  int breakdown, halt;
+  double tol;
 };
 
 // ---------------------------------------------------------------- reductions
@@ -853,6 +858,7 @@ __global__ __launch_bounds__(kBlock) void k_err_partials(int n, const double *__
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_update_p(int n, T *__restrict__ p, const T *__restrict__ r,
                                                      const T *__restrict__ v, const Scalars *__restrict__ sc) {
+  if (sc->halt) return;
   const T beta = (T)sc->beta, omega = (T)sc->omega;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     p[i] = r[i] + beta * (p[i] - omega * v[i]);
@@ -861,6 +867,7 @@ __global__ __launch_bounds__(kBlock) void k_update_p(int n, T *__restrict__ p, c
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_update_s(int n, T *__restrict__ s, const T *__restrict__ r,
                                                      const T *__restrict__ v, const Scalars *__restrict__ sc) {
+  if (sc->halt) return;
   const T alpha = (T)sc->alpha;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s[i] = r[i] - alpha * v[i];
 }
@@ -872,6 +879,7 @@ __global__ __launch_bounds__(kBlock) void k_update_xr(int n, T *__restrict__ x, 
                                                       const T *__restrict__ s, const T *__restrict__ t,
                                                       const T *__restrict__ r0, const Scalars *__restrict__ sc,
                                                       double *__restrict__ part) {
+  if (sc->halt) return;
   const T alpha = (T)sc->alpha, omega = (T)sc->omega;
   double a0 = 0, a1 = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -961,6 +969,7 @@ __device__ inline double reduce_part(const double *part, int count) {
 //     4 = store norm of partial sum (sqrt) into slot
 __global__ __launch_bounds__(kBlock) void k_finalize(int op, int cnt, const double *__restrict__ part, Scalars *sc,
                                                      double *__restrict__ out_norm) {
+  if (op >= 1 && op <= 3 && sc->halt) return;  // an iteration enqueued behind a halt
   if (op == 0) {
     double rr = reduce_part(part, cnt);
     if (threadIdx.x == 0) {
@@ -971,6 +980,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int op, int cnt, const doub
       sc->r0r = rr;
       sc->beta = 0.0;
       sc->breakdown = 0;
+      sc->halt = 0;
       // first direction: p = r (beta = 0 with p = v = 0)
       sc->beta = (sc->r0r / sc->rho) * (sc->alpha / sc->omega);
       sc->rho = sc->r0r;
@@ -1003,6 +1013,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int op, int cnt, const doub
         sc->beta = (r0r / sc->rho) * (sc->alpha / sc->omega);
         sc->rho = r0r;
       }
+      if (sc->breakdown || !(sc->res > sc->tol)) sc->halt = 1;  // also a NaN residual
     }
   } else if (op == 4) {
     double s = reduce_part(part, cnt);
@@ -1596,6 +1607,9 @@ struct shesolve_ctx {
   float *v32[9] = {};
   double *d_part = nullptr;
   Scalars *d_sc = nullptr;
+  Scalars *h_ring = nullptr;         // pinned: the scalars of the last two enqueued iterations
+  hipEvent_t iter_ev[2] = {nullptr, nullptr};
+  bool graph_iters = true;           // replay one BiCGStab iteration as a HIP graph (profiling off)
   double *d_norm = nullptr;
   int vec_n = 0;
   double last_setup_ms = 0;
@@ -1905,6 +1919,9 @@ int ensure_vectors(shesolve_ctx *c) {
   for (auto &p : c->v32) HIPCHK(hipMalloc(&p, n * sizeof(float)));
   if (!c->d_part) HIPCHK(hipMalloc(&c->d_part, sizeof(double) * 2 * kPartStride));
   if (!c->d_sc) HIPCHK(hipMalloc(&c->d_sc, sizeof(Scalars)));
+  if (!c->h_ring) HIPCHK(hipHostMalloc(&c->h_ring, 2 * sizeof(Scalars), hipHostMallocDefault));
+  for (auto &e : c->iter_ev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   if (!c->d_norm) HIPCHK(hipMalloc(&c->d_norm, sizeof(double) * 4));
   c->vec_n = c->n;
   return 0;
@@ -2906,6 +2923,8 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
     Scalars tmp = hs;
     tmp.bnorm = bn;
     tmp.res = bn > 0 ? r_init / bn : 0.0;
+    tmp.tol = tol;
+    tmp.halt = 0;
     HIPCHK(hipMemcpyAsync(c->d_sc, &tmp, sizeof(Scalars), hipMemcpyHostToDevice, c->stream));
     hs = tmp;
   }
@@ -2922,10 +2941,12 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
     *res_out = hs.res;
     return 0;
   }
-  int it = 0, best_it = 0;
-  double best_res = hs.res;
-  while (it < max_iters) {
-    ++it;
+  // Iterations are enqueued one ahead of the host's check of their scalars (a
+  // pinned ring of two, each copy followed by an event), so the device does
+  // not wait for the host between iterations; an iteration enqueued behind a
+  // halt changes nothing (Scalars::halt).  With profiling off the launches of
+  // one iteration are replayed from a HIP graph.
+  auto iter_launches = [&]() -> int {
     {
       KTimer kt(c, 3);
       hipLaunchKernelGGL((k_update_p<T>), dim3(vg), dim3(kBlock), 0, c->stream, n, p, r, v, c->d_sc);
@@ -2946,8 +2967,43 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
                          part);
     }
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, 3, kGrid, part, c->d_sc, c->d_norm);
-    HIPCHK(hipMemcpyAsync(&hs, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipGetLastError());
+    return 0;
+  };
+  hipGraphExec_t gexec = nullptr;
+  if (c->graph_iters && !c->prof) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int e1 = iter_launches();
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e1 || e2 != hipSuccess || hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0) != hipSuccess) gexec = nullptr;
+    if (g) hipGraphDestroy(g);
+    (void)hipGetLastError();
+  }
+  struct GraphGuard {
+    hipGraphExec_t &g;
+    ~GraphGuard() {
+      if (g) hipGraphExecDestroy(g);
+    }
+  } guard{gexec};
+  auto enqueue = [&](int k) -> int {
+    if (gexec) HIPCHK(hipGraphLaunch(gexec, c->stream));
+    else if (int e = iter_launches()) return e;
+    HIPCHK(hipMemcpyAsync(&c->h_ring[k & 1], c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(c->iter_ev[k & 1], c->stream));
+    return 0;
+  };
+  int it = 0, enq = 0, best_it = 0;
+  double best_res = hs.res;
+  while (true) {
+    while (enq < max_iters && enq < it + 2) {
+      if (int e = enqueue(enq + 1)) return e;
+      ++enq;
+    }
+    if (it >= enq) break;  // max_iters reached and checked
+    HIPCHK(hipEventSynchronize(c->iter_ev[(it + 1) & 1]));
+    ++it;
+    hs = c->h_ring[it & 1];
     if (!std::isfinite(hs.res)) {
       *st = 2;
       break;
@@ -2970,6 +3026,11 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
         *st = 2;
         break;
       }
+    }
+    if (hs.res <= tol || hs.breakdown) {
+      // the device halted: the iteration enqueued behind this one changed nothing
+      HIPCHK(hipStreamSynchronize(c->stream));
+      enq = it;
     }
     if (hs.res <= tol) {
       // confirm with the explicit residual r = b - A x
@@ -2997,9 +3058,18 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
       HIPCHK(hipStreamSynchronize(c->stream));
       tmp.bnorm = bn;
       tmp.res = std::sqrt(tmp.rr) / bn;
+      tmp.tol = tol;
       HIPCHK(hipMemcpyAsync(c->d_sc, &tmp, sizeof(Scalars), hipMemcpyHostToDevice, c->stream));
       if (tmp.rr == 0.0) { *st = 0; hs.res = 0; break; }
     }
+  }
+  // an iteration still enqueued behind a host-side stop (FP32 stall or
+  // divergence; the device had not halted) ran its updates: report its state,
+  // which x now holds
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (enq > it && !c->h_ring[it & 1].halt) {
+    it = enq;
+    hs = c->h_ring[it & 1];
   }
   *res_out = hs.res;
   return it;
@@ -4005,6 +4075,9 @@ int shesolve_destroy(shesolve_handle h) {
   dfree(h->d_part);
   dfree(h->d_sc);
   dfree(h->d_norm);
+  if (h->h_ring) hipHostFree(h->h_ring);
+  for (auto &e : h->iter_ev)
+    if (e) hipEventDestroy(e);
   hipEventDestroy(h->ev0);
   hipEventDestroy(h->ev1);
   for (auto &e : h->evpool) hipEventDestroy(e);
