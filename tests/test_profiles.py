@@ -12,7 +12,7 @@ PROF = os.path.join(ROOT, "profiles")
 
 
 def test_traffic_files_match_their_pmc_summaries():
-    files = sorted(glob.glob(os.path.join(PROF, "k_stream_spmv_*_traffic.json")))
+    files = sorted(glob.glob(os.path.join(PROF, "*_traffic.json")))
     assert files
     for f in files:
         tr = json.load(open(f))
