@@ -1609,7 +1609,6 @@ struct shesolve_ctx {
   Scalars *d_sc = nullptr;
   Scalars *h_ring = nullptr;         // pinned: the scalars of the last two enqueued iterations
   hipEvent_t iter_ev[2] = {nullptr, nullptr};
-  bool graph_iters = true;           // replay one BiCGStab iteration as a HIP graph (profiling off)
   double *d_norm = nullptr;
   int vec_n = 0;
   double last_setup_ms = 0;
@@ -2944,8 +2943,10 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
   // Iterations are enqueued one ahead of the host's check of their scalars (a
   // pinned ring of two, each copy followed by an event), so the device does
   // not wait for the host between iterations; an iteration enqueued behind a
-  // halt changes nothing (Scalars::halt).  With profiling off the launches of
-  // one iteration are replayed from a HIP graph.
+  // halt changes nothing (Scalars::halt).  The launches are not captured into
+  // a graph: replaying one measured no faster (334 ms per solve either way on
+  // the 161x81 L=3 device), and a capture in progress makes the synchronous
+  // copies of other handles on other host threads fail.
   auto iter_launches = [&]() -> int {
     {
       KTimer kt(c, 3);
@@ -2970,25 +2971,8 @@ int bicgstab_dev(shesolve_ctx *c, const TV *Av, const TV *Mv, const T *b, T *x, 
     HIPCHK(hipGetLastError());
     return 0;
   };
-  hipGraphExec_t gexec = nullptr;
-  if (c->graph_iters && !c->prof) {
-    hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const int e1 = iter_launches();
-    const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
-    if (e1 || e2 != hipSuccess || hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0) != hipSuccess) gexec = nullptr;
-    if (g) hipGraphDestroy(g);
-    (void)hipGetLastError();
-  }
-  struct GraphGuard {
-    hipGraphExec_t &g;
-    ~GraphGuard() {
-      if (g) hipGraphExecDestroy(g);
-    }
-  } guard{gexec};
   auto enqueue = [&](int k) -> int {
-    if (gexec) HIPCHK(hipGraphLaunch(gexec, c->stream));
-    else if (int e = iter_launches()) return e;
+    if (int e = iter_launches()) return e;
     HIPCHK(hipMemcpyAsync(&c->h_ring[k & 1], c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipEventRecord(c->iter_ev[k & 1], c->stream));
     return 0;
