@@ -326,10 +326,6 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(int nblk, int xcd, co
   }
 }
 
-// Target of the stores of lanes that own no row (contents never read), so that
-// every iteration of a pipelined loop issues the same memory operations.
-__device__ double g_discard[kStreamThreads];
-
 // Row block descriptors of the pipelined SpMV: {first row, end row, first
 // entry, end entry} per block, from consecutive boundaries or (first, end)
 // pairs.
@@ -354,22 +350,39 @@ __global__ __launch_bounds__(kBlock) void k_block_desc(int nblk, const int *__re
 // indices and fallback
 // pointers instead of branches.  Every block spans at most ITEMS * 256 entries
 // from that rounded start (the host launches this kernel only then).
-typedef int vint4 __attribute__((ext_vector_type(4)));
 typedef float vfloat4 __attribute__((ext_vector_type(4)));
-__device__ inline void load4(const float *p, float (&o)[4]) {
-  const vfloat4 t = __builtin_nontemporal_load(reinterpret_cast<const vfloat4 *>(p));
-  o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
-}
-__device__ inline void load4(const int *p, int (&o)[4]) {
-  const vint4 t = __builtin_nontemporal_load(reinterpret_cast<const vint4 *>(p));
-  o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
-}
+typedef int vint4 __attribute__((ext_vector_type(4)));
 __device__ inline void store4(float *p, const float (&v)[4]) {
   *reinterpret_cast<vfloat4 *>(p) = vfloat4{v[0], v[1], v[2], v[3]};
 }
 
+// Raw buffer access (MUBUF): a resource of `bytes` from `p` (a null pointer
+// gives an empty resource); loads past the end return zero and stores past it
+// are dropped, so the pipelined kernels need neither index clamps nor branches.
+constexpr int kBufferDword3 = 0x00020000;  // gfx9-family buffer resource word 3
+typedef unsigned vuint2 __attribute__((ext_vector_type(2)));
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, p ? (int)bytes : 0, kBufferDword3);
+}
+template <typename T>
+__device__ inline T buf_load(__amdgpu_buffer_rsrc_t r, int off) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <typename T>
+__device__ inline void buf_store(T v, __amdgpu_buffer_rsrc_t r, int off) {
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(vuint2, v), r, off, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+constexpr int kBufferDrop = 0x7ffffff0;  // an offset past every resource: the store is dropped
+
 template <typename TV, typename T, int ITEMS>
-__global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd, const int *__restrict__ desc,
+__global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd, int n, int64_t nnz_alloc,
+                                                              const int *__restrict__ desc,
                                                               const int *__restrict__ rp, const int *__restrict__ ci,
                                                               const TV *__restrict__ v, const T *x, T *y,
                                                               const T *__restrict__ b, const T *d0, const T *d1,
@@ -392,58 +405,60 @@ __global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd,
   const int tid = (int)threadIdx.x;
   // a dot product with y itself takes the new value, not a prefetched one
   const bool d0y = (const void *)d0 == (const void *)y, d1y = (const void *)d1 == (const void *)y;
-  const T *pb = b ? b : x, *p0 = d0 && !d0y ? d0 : x, *p1 = d1 && !d1y ? d1 : x;
+  const auto rdesc = buf_rsrc(desc, (int64_t)nblk * 16), rrp = buf_rsrc(rp, (int64_t)(n + 1) * 4);
+  const auto rci = buf_rsrc(ci, nnz_alloc * 4), rv = buf_rsrc(v, nnz_alloc * (int64_t)sizeof(TV));
+  const auto rx = buf_rsrc(x, (int64_t)n * sizeof(T)), ry = buf_rsrc(y, (int64_t)n * sizeof(T));
+  const auto rb = buf_rsrc(b, (int64_t)n * sizeof(T));
+  const auto r0 = buf_rsrc(d0 && !d0y ? d0 : nullptr, (int64_t)n * sizeof(T));
+  const auto r1 = buf_rsrc(d1 && !d1y ? d1 : nullptr, (int64_t)n * sizeof(T));
   if (count > 0) {
     const int last = bk0 + (count - 1) * bstep;
     struct StB {
-      int rs, re, k0, k1, r0, r1;
+      int rs, re, k0, r0, r1;
       T bb, e0, e1;
       int cc[ITEMS];
       TV vv[ITEMS];
     };
     // lane l holds component l % 4 of the block's descriptor
-    auto stage_a = [&](int t) { return desc[4 * min(bk0 + t * bstep, last) + (tid & 3)]; };
-    // entry j of vector q of this lane: base + W (tid + q NT) + j, base = k0 rounded down
+    auto stage_a = [&](int t) {
+      return __builtin_amdgcn_raw_buffer_load_b32(rdesc, (4 * min(bk0 + t * bstep, last) + (tid & 3)) * 4, 0, 0);
+    };
+    // entry j of vector q of this lane: base + W (tid + q NT) + j, base = k0
+    // rounded down to a multiple of four; entries outside the block are loaded
+    // (or read as zero past the arrays) and never summed
     auto stage_b = [&](int dv, StB &s) {
       s.rs = __builtin_amdgcn_readlane(dv, 0);
       s.re = __builtin_amdgcn_readlane(dv, 1);
       s.k0 = __builtin_amdgcn_readlane(dv, 2);
-      s.k1 = __builtin_amdgcn_readlane(dv, 3);
-      const int rc = s.rs + tid < s.re ? s.rs + tid : s.rs;
-      s.r0 = rp[rc];
-      s.r1 = rp[rc + 1];
-      s.bb = pb[rc];
-      s.e0 = p0[rc];
-      s.e1 = p1[rc];
+      const int rc = s.rs + tid;
+      s.r0 = (int)__builtin_amdgcn_raw_buffer_load_b32(rrp, rc * 4, 0, 0);
+      s.r1 = (int)__builtin_amdgcn_raw_buffer_load_b32(rrp, rc * 4 + 4, 0, 0);
+      s.bb = buf_load<T>(rb, rc * (int)sizeof(T));
+      s.e0 = buf_load<T>(r0, rc * (int)sizeof(T));
+      s.e1 = buf_load<T>(r1, rc * (int)sizeof(T));
       const int base = s.k0 & ~3;
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
-        const int e = base + W * (tid + q * NT), i = e < s.k1 ? e : 0;
+        const int e = base + W * (tid + q * NT);
         if constexpr (W == 4) {
-          int c4[4];
-          TV v4[4];
-          load4(ci + i, c4);
-          load4(v + i, v4);
+          // the whole 16 bytes cast at once (a cast of one element of the
+          // builtin's result compiled to a one-dword load reused for all four)
+          const vint4 c4 = __builtin_bit_cast(vint4, __builtin_amdgcn_raw_buffer_load_b128(rci, e * 4, 0, 0));
+          const vfloat4 v4 = __builtin_bit_cast(vfloat4, __builtin_amdgcn_raw_buffer_load_b128(rv, e * 4, 0, 0));
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             s.cc[4 * q + j] = c4[j];
             s.vv[4 * q + j] = v4[j];
           }
         } else {
-          s.cc[q] = __builtin_nontemporal_load(ci + i);
-          s.vv[q] = __builtin_nontemporal_load(v + i);
+          s.cc[q] = (int)__builtin_amdgcn_raw_buffer_load_b32(rci, e * 4, 0, 0);
+          s.vv[q] = buf_load<TV>(rv, e * (int)sizeof(TV));
         }
       }
     };
     auto stage_c = [&](const StB &s, T *xg) {
-      const int base = s.k0 & ~3;
 #pragma unroll
-      for (int q = 0; q < NV; ++q)
-#pragma unroll
-        for (int j = 0; j < W; ++j) {
-          const int e = base + W * (tid + q * NT) + j;
-          xg[W * q + j] = x[e >= s.k0 && e < s.k1 ? s.cc[W * q + j] : s.rs];
-        }
+      for (int u = 0; u < ITEMS; ++u) xg[u] = buf_load<T>(rx, s.cc[u] * (int)sizeof(T));
     };
     // three sets of row/entry loads and two of gathers and of descriptors, their
     // roles rotated by unrolling (copying a register whose load is in flight
@@ -459,7 +474,6 @@ __global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd,
     dA = stage_a(2);
     // one wait here, so that the loop's waits count only the loop's own operations
     __builtin_amdgcn_s_waitcnt(0);
-    T *discard = reinterpret_cast<T *>(g_discard) + tid;
     // iteration t: descriptor of t+3 into dn, rows and entries of t+2 (from dc)
     // into nn, gathers of t+1 (from nx) into xn, then block t (cur, xc)
     auto body = [&](int t, int &dc, int &dn, StB &cur, StB &nx, StB &nn, T(&xc)[ITEMS], T(&xn)[ITEMS]) {
@@ -475,10 +489,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd,
       for (int q = 0; q < NV; ++q) {
         T pw[W];
 #pragma unroll
-        for (int j = 0; j < W; ++j) {
-          const int e = base + W * (tid + q * NT) + j;
-          pw[j] = e >= cur.k0 && e < cur.k1 ? (T)cur.vv[W * q + j] * xc[W * q + j] : (T)0;
-        }
+        for (int j = 0; j < W; ++j) pw[j] = (T)cur.vv[W * q + j] * xc[W * q + j];
         if constexpr (W == 4) store4(prod + 4 * (tid + q * NT), pw);
         else prod[tid + q * NT] = pw[0];
       }
@@ -486,11 +497,11 @@ __global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd,
       const bool own = cur.rs + tid < cur.re;
       T acc = 0;
       {
-        const int lo = own ? cur.r0 : cur.k0, hi = own ? cur.r1 : cur.k0;
+        const int lo = own ? cur.r0 : base, hi = own ? cur.r1 : base;
         for (int k = lo; k < hi; ++k) acc += prod[k - base];
       }
       const T out = b ? (T)(cur.bb - acc) : acc;
-      *(own ? y + cur.rs + tid : discard) = out;
+      buf_store<T>(out, ry, own ? (cur.rs + tid) * (int)sizeof(T) : kBufferDrop);
       if (own) {
         a0 += d0 ? (double)(d0y ? out : cur.e0) * (double)out : 0.0;
         a1 += d1 ? (double)(d1y ? out : cur.e1) * (double)out : 0.0;
@@ -1742,11 +1753,13 @@ void launch_spmv(shesolve_ctx *c, int /*lpr*/, const TV *v, const T *x, T *y, co
   if (part) c->spmv_grid = g;
   const int *pairs = c->s_rp ? c->s_pairs : nullptr;
   if (!c->spmv_skip && c->d_desc && c->desc_src == (pairs ? pairs : blk) && c->desc_nblk == nblk &&
-      c->desc_maxrow <= spmv_cap(c->items)) {
+      c->desc_maxrow <= spmv_cap(c->items) &&
+      ((int64_t)(c->solve_perm ? c->fnnz : c->nnz) + kVecPad) * (int64_t)sizeof(TV) < (int64_t)1 << 31) {
 #define SP_CASE(IT)                                                                                               \
   case IT:                                                                                                        \
-    hipLaunchKernelGGL((k_spmv_pipe<TV, T, IT>), dim3(g), dim3(kStreamThreads), 0, c->stream, nblk, c->xcd,        \
-                       (const int *)c->d_desc, rp, ci, v, x, y, b, d0, d1, part);                                \
+    hipLaunchKernelGGL((k_spmv_pipe<TV, T, IT>), dim3(g), dim3(kStreamThreads), 0, c->stream, nblk, c->xcd, c->n,  \
+                       (int64_t)(c->solve_perm ? c->fnnz : c->nnz) + kVecPad, (const int *)c->d_desc, rp, ci, v, x, y, b,  \
+                       d0, d1, part);                                                                             \
     break;
     switch (c->items) { SP_CASE(4) SP_CASE(16) default: SP_CASE(8) }
 #undef SP_CASE
