@@ -617,14 +617,10 @@ __host__ __device__ constexpr int bmc_slot(bool backward, int j, int r) {
   return m < kBmcHalf ? jm * kBmcHalf + m : kBmcSecond + jm * kBmcHalf + (m - kBmcHalf);
 }
 
-// Target of the sweep's stores from lanes that own no row, so that every
-// iteration issues the same memory operations (its contents are never read).
-__device__ double g_bmc_discard[kBmcGroupMax * kBmcRows];
-
 template <bool BACKWARD, typename TV, typename T>
 __global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
-    int b0, int b1, const int *__restrict__ bstart, const int *__restrict__ bK, const int *__restrict__ orp,
-    const int *__restrict__ oci, const TV *__restrict__ ov, const int *__restrict__ iptr,
+    int b0, int b1, int n, int nb, int64_t no, int64_t ni, const int *__restrict__ bstart, const int *__restrict__ bK,
+    const int *__restrict__ orp, const int *__restrict__ oci, const TV *__restrict__ ov, const int *__restrict__ iptr,
     const unsigned short *__restrict__ icode, const TV *__restrict__ iv, const TV *__restrict__ dinv,
     const T *__restrict__ in, T *z) {
   constexpr int G = kBmcGroupOf<T>, NT = G * kBmcRows, CAP = kBmcItems * NT, ICAP = kBmcIn * NT;
@@ -639,10 +635,18 @@ __global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
   if (first >= xe) return;  // whole workgroup
   const int count = (xe - first + P - 1) / P, last = first + (count - 1) * P;
   const int tid = (int)threadIdx.x, sub = tid / kBmcRows, hl = tid % kBmcRows;
+  // buffer resources: loads past an array read zero, stores past it are dropped
+  const auto rbst = buf_rsrc(bstart, (int64_t)(nb + 1) * 4), rbK = buf_rsrc(bK, (int64_t)(nb + 1) * 4);
+  const auto rip = buf_rsrc(iptr, (int64_t)(nb + 1) * 4), rorp = buf_rsrc(orp, (int64_t)(n + 1) * 4);
+  const auto roci = buf_rsrc(oci, no * 4), rov = buf_rsrc(ov, no * (int64_t)sizeof(TV));
+  const auto ricode = buf_rsrc(icode, ni * 2), riv = buf_rsrc(iv, ni * (int64_t)sizeof(TV));
+  const auto rdinv = buf_rsrc(BACKWARD ? (const void *)dinv : nullptr, (int64_t)n * sizeof(TV));
+  const auto rin = buf_rsrc(BACKWARD ? (const void *)z : (const void *)in, (int64_t)n * sizeof(T));
+  const auto rz = buf_rsrc(z, (int64_t)n * sizeof(T));
 
   // Three load stages per group, issued in consecutive iterations so that each
-  // stage's latency overlaps the elimination of an earlier group.  Indices are
-  // clamped rather than branched on (past the last group, the last group again).
+  // stage's latency overlaps the elimination of an earlier group.  Past the
+  // last group the stages load the last group again.
   struct StA {  // the lane's block start and end; the group's coupling and in-block bounds (lane parity picks which)
     int g0, gn, rlo, rhi, kb, ip;
   };
@@ -659,10 +663,10 @@ __global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
     a.g0 = b0 + g * G;
     a.gn = min(G, b1 - a.g0);
     const int sb = a.g0 + min(sub, a.gn - 1), gb = (tid & 1) ? a.g0 + a.gn : a.g0;
-    a.rlo = bstart[sb];
-    a.rhi = bstart[sb + 1];
-    a.kb = bK[gb];
-    a.ip = iptr[gb];
+    a.rlo = (int)__builtin_amdgcn_raw_buffer_load_b32(rbst, sb * 4, 0, 0);
+    a.rhi = (int)__builtin_amdgcn_raw_buffer_load_b32(rbst, sb * 4 + 4, 0, 0);
+    a.kb = (int)__builtin_amdgcn_raw_buffer_load_b32(rbK, gb * 4, 0, 0);
+    a.ip = (int)__builtin_amdgcn_raw_buffer_load_b32(rip, gb * 4, 0, 0);
   };
   auto stage_b = [&](const StA &a, StB &b) {
     // lane 0 of every wavefront holds the group's first bound, lane 1 its last
@@ -672,29 +676,28 @@ __global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
     b.I1 = __builtin_amdgcn_readlane(a.ip, 1);
     b.nr = sub < a.gn ? a.rhi - a.rlo : 0;
     b.row = a.rlo + hl;
-    const int rc = hl < b.nr ? b.row : a.rlo;
-    b.acc0 = BACKWARD ? z[rc] : in[rc];
-    b.d = BACKWARD ? (T)dinv[rc] : (T)1;
-    b.ks = orp[rc];
-    b.ke = orp[rc + 1];
+    b.acc0 = buf_load<T>(rin, b.row * (int)sizeof(T));
+    b.d = BACKWARD ? (T)buf_load<TV>(rdinv, b.row * (int)sizeof(TV)) : (T)1;
+    b.ks = (int)__builtin_amdgcn_raw_buffer_load_b32(rorp, b.row * 4, 0, 0);
+    b.ke = (int)__builtin_amdgcn_raw_buffer_load_b32(rorp, b.row * 4 + 4, 0, 0);
 #pragma unroll
     for (int u = 0; u < kBmcItems; ++u) {
-      const int k = b.K0 + tid + u * NT, i = k < b.K1 ? k : b.K0;
-      b.cc[u] = __builtin_nontemporal_load(oci + i);
-      b.vv[u] = __builtin_nontemporal_load(ov + i);
+      const int k = b.K0 + tid + u * NT;  // past K1: the next group's entries or zero, never used
+      b.cc[u] = (int)__builtin_amdgcn_raw_buffer_load_b32(roci, k * 4, 0, 0);
+      b.vv[u] = buf_load<TV>(rov, k * (int)sizeof(TV));
     }
 #pragma unroll
     for (int u = 0; u < kBmcIn; ++u) {
-      const int k = b.I0 + tid + u * NT, i = k < b.I1 ? k : b.I0;
-      b.ic[u] = __builtin_nontemporal_load(icode + i);
-      b.ivv[u] = __builtin_nontemporal_load(iv + i);
+      const int k = b.I0 + tid + u * NT;
+      b.ic[u] = (int)__builtin_amdgcn_raw_buffer_load_b16(ricode, k * 2, 0, 0) & 0xffff;
+      b.ivv[u] = buf_load<TV>(riv, k * (int)sizeof(TV));
     }
   };
-  // the gathered values of the coupling entries (an empty range reads the
-  // entry at K0 of an array never empty, and the value is not used)
+  // the gathered values of the coupling entries (entries past the group's
+  // range gather some value that is never summed)
   auto stage_c = [&](const StB &b, T *zz) {
 #pragma unroll
-    for (int u = 0; u < kBmcItems; ++u) zz[u] = z[b.K0 + tid + u * NT < b.K1 ? b.cc[u] : b.row - hl];
+    for (int u = 0; u < kBmcItems; ++u) zz[u] = buf_load<T>(rz, b.cc[u] * (int)sizeof(T));
   };
 
   StA a;
@@ -708,7 +711,6 @@ __global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
   stage_a(2, a);
   // one wait here, so that the loop's waits count only the loop's own operations
   __builtin_amdgcn_s_waitcnt(0);
-  T *discard = reinterpret_cast<T *>(g_bmc_discard) + tid;
   for (int t = 0; t < count; ++t) {
     stage_c(bn, zn);
     stage_b(a, bnn);
@@ -732,8 +734,7 @@ __global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
       tile[((code / kBmcTile) % G) * kBmcTile + code % kBmcTile] = (T)iv[k];
     }
 #pragma unroll
-    for (int u = 0; u < kBmcItems; ++u)
-      if (tid + u * NT < ccnt) prod[tid + u * NT] = (T)bc.vv[u] * zc[u];
+    for (int u = 0; u < kBmcItems; ++u) prod[tid + u * NT] = (T)bc.vv[u] * zc[u];  // past ccnt: never summed
     __syncthreads();  // in-block values and products in place
     const bool own = hl < bc.nr;
     T s = 0;
@@ -777,7 +778,7 @@ __global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
         acc = fma(cf[kBmcHalf + j], row_bcast<BACKWARD ? kBmcHalf - 1 - j : j>(acc), acc);
       });
     }
-    *(own ? z + bc.row : discard) = BACKWARD ? acc * bc.d : acc;
+    buf_store<T>(BACKWARD ? acc * bc.d : acc, rz, own ? bc.row * (int)sizeof(T) : kBufferDrop);
     bc = bn;
     bn = bnn;
 #pragma unroll
@@ -1802,7 +1803,7 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
   if (!c->bmc_cptr.empty() && !perm) {
     // block multicolour factor in the solve space: one launch per colour and
     // direction, forward in ascending and backward in descending colours
-    const int nq = (int)c->bmc_cptr.size() - 1;
+    const int nq = (int)c->bmc_cptr.size() - 1, nbk = c->bmc_cptr.back();
     constexpr int G = kBmcGroupOf<T>;
     // persistent grid: what the device holds at once (a multiple of the XCD
     // count), or fewer when the colour has fewer groups
@@ -1828,13 +1829,15 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
       KTimer kt(c, 1);
       const int b0 = c->bmc_cptr[q], b1 = c->bmc_cptr[q + 1];
       hipLaunchKernelGGL((k_bmc_group<false, TV, T>), dim3(grid(b0, b1)), dim3(G * kBmcRows), 0, c->stream, b0, b1,
-                         c->d_bmc_start, PL.d_bK, PL.d_orp, PL.d_oci, oL, PL.d_iptr, PL.d_icode, iL, dinv, in, out);
+                         n, nbk, PL.no, PL.ni, c->d_bmc_start, PL.d_bK, PL.d_orp, PL.d_oci, oL, PL.d_iptr, PL.d_icode,
+                         iL, dinv, in, out);
     }
     for (int q = nq - 1; q >= 0; --q) {
       KTimer kt(c, 2);
       const int b0 = c->bmc_cptr[q], b1 = c->bmc_cptr[q + 1];
       hipLaunchKernelGGL((k_bmc_group<true, TV, T>), dim3(grid(b0, b1)), dim3(G * kBmcRows), 0, c->stream, b0, b1,
-                         c->d_bmc_start, PU.d_bK, PU.d_orp, PU.d_oci, oU, PU.d_iptr, PU.d_icode, iU, dinv, in, out);
+                         n, nbk, PU.no, PU.ni, c->d_bmc_start, PU.d_bK, PU.d_orp, PU.d_oci, oU, PU.d_iptr, PU.d_icode,
+                         iU, dinv, in, out);
     }
     return;
   }
@@ -2299,6 +2302,13 @@ int build_bmc_part(shesolve_ctx *c, const std::vector<int> &bst, bool upper, con
   bK[nb] = (int)oci.size();
   P.no = (int64_t)oci.size();
   P.ni = (int64_t)ipos.size();
+  // the sweep addresses these arrays through 32-bit buffer offsets
+  if (std::max<int64_t>(P.no, P.ni) * (int64_t)sizeof(double) >= ((int64_t)1 << 31) ||
+      (int64_t)(n + 1) * (int64_t)sizeof(double) >= ((int64_t)1 << 31)) {
+    set_err("block multicolour layout: %lld coupling and %lld in-block entries exceed the sweep's 2 GiB arrays",
+            (long long)P.no, (long long)P.ni);
+    return -1;
+  }
   const size_t no = std::max<size_t>(oci.size(), 1), ni = std::max<size_t>(ipos.size(), 1);
   HIPCHK(hipMalloc(&P.d_orp, sizeof(int) * (n + 1)));
   HIPCHK(hipMalloc(&P.d_bK, sizeof(int) * (nb + 1)));
