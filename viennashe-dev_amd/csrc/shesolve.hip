@@ -414,7 +414,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd,
   if (count > 0) {
     const int last = bk0 + (count - 1) * bstep;
     struct StB {
-      int rs, re, k0, r0, r1;
+      int rs, re, k0, k1, r0, r1;
       T bb, e0, e1;
       int cc[ITEMS];
       TV vv[ITEMS];
@@ -424,12 +424,14 @@ __global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd,
       return __builtin_amdgcn_raw_buffer_load_b32(rdesc, (4 * min(bk0 + t * bstep, last) + (tid & 3)) * 4, 0, 0);
     };
     // entry j of vector q of this lane: base + W (tid + q NT) + j, base = k0
-    // rounded down to a multiple of four; entries outside the block are loaded
-    // (or read as zero past the arrays) and never summed
+    // rounded down to a multiple of four; a load that starts past the block's
+    // last entry is not made (its offset lies past the buffer, which reads
+    // zero), and entries outside the block are never summed
     auto stage_b = [&](int dv, StB &s) {
       s.rs = __builtin_amdgcn_readlane(dv, 0);
       s.re = __builtin_amdgcn_readlane(dv, 1);
       s.k0 = __builtin_amdgcn_readlane(dv, 2);
+      s.k1 = __builtin_amdgcn_readlane(dv, 3);
       const int rc = s.rs + tid;
       s.r0 = (int)__builtin_amdgcn_raw_buffer_load_b32(rrp, rc * 4, 0, 0);
       s.r1 = (int)__builtin_amdgcn_raw_buffer_load_b32(rrp, rc * 4 + 4, 0, 0);
@@ -440,19 +442,20 @@ __global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd,
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
         const int e = base + W * (tid + q * NT);
+        const int eo = e < s.k1 ? e * 4 : kBufferDrop;
         if constexpr (W == 4) {
           // the whole 16 bytes cast at once (a cast of one element of the
           // builtin's result compiled to a one-dword load reused for all four)
-          const vint4 c4 = __builtin_bit_cast(vint4, __builtin_amdgcn_raw_buffer_load_b128(rci, e * 4, 0, 0));
-          const vfloat4 v4 = __builtin_bit_cast(vfloat4, __builtin_amdgcn_raw_buffer_load_b128(rv, e * 4, 0, 0));
+          const vint4 c4 = __builtin_bit_cast(vint4, __builtin_amdgcn_raw_buffer_load_b128(rci, eo, 0, 0));
+          const vfloat4 v4 = __builtin_bit_cast(vfloat4, __builtin_amdgcn_raw_buffer_load_b128(rv, eo, 0, 0));
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             s.cc[4 * q + j] = c4[j];
             s.vv[4 * q + j] = v4[j];
           }
         } else {
-          s.cc[q] = (int)__builtin_amdgcn_raw_buffer_load_b32(rci, e * 4, 0, 0);
-          s.vv[q] = buf_load<TV>(rv, e * (int)sizeof(TV));
+          s.cc[q] = (int)__builtin_amdgcn_raw_buffer_load_b32(rci, eo, 0, 0);
+          s.vv[q] = buf_load<TV>(rv, e < s.k1 ? e * (int)sizeof(TV) : kBufferDrop);
         }
       }
     };
@@ -546,14 +549,34 @@ __device__ inline void unroll(F &&f) {
 // Value of `x` in lane L of this lane's row of 16 lanes (DPP row_newbcast).
 template <int L>
 __device__ inline float row_bcast(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x150 + L, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x150 + L, 0xF, 0xF, true));
 }
 template <int L>
 __device__ inline double row_bcast(double x) {
   const long long bits = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)bits, 0x150 + L, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), 0x150 + L, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)bits, 0x150 + L, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(bits >> 32), 0x150 + L, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// acc += c * (src of lane L of this lane's row of 16).  FP32: one
+// v_fmac_f32_dpp (the broadcast applied to the multiplicand), after the two
+// wait states a DPP read of a VGPR written by the previous VALU op needs.
+template <int L>
+__device__ inline void fma_bcast(float &acc, float src, float c) {
+  asm("s_nop 1\n\tv_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+      : "+v"(acc) : "v"(src), "v"(c), "n"(L));
+}
+template <int L>
+__device__ inline void fma_bcast_self(float &acc, float c) {
+  asm("s_nop 1\n\tv_fmac_f32_dpp %0, %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf" : "+v"(acc) : "v"(c), "n"(L));
+}
+template <int L>
+__device__ inline void fma_bcast(double &acc, double src, double c) {
+  acc = fma(c, row_bcast<L>(src), acc);
+}
+template <int L>
+__device__ inline void fma_bcast_self(double &acc, double c) {
+  acc = fma(c, row_bcast<L>(acc), acc);
 }
 // Rows of 16 lanes exchanged in pairs (v_permlane16_swap): lanes 16-31 (and
 // 48-63) receive the value of lanes 0-15 (32-47) when `upper` is false, and
@@ -682,15 +705,15 @@ __global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
     b.ke = (int)__builtin_amdgcn_raw_buffer_load_b32(rorp, b.row * 4 + 4, 0, 0);
 #pragma unroll
     for (int u = 0; u < kBmcItems; ++u) {
-      const int k = b.K0 + tid + u * NT;  // past K1: the next group's entries or zero, never used
-      b.cc[u] = (int)__builtin_amdgcn_raw_buffer_load_b32(roci, k * 4, 0, 0);
-      b.vv[u] = buf_load<TV>(rov, k * (int)sizeof(TV));
+      const int k = b.K0 + tid + u * NT;  // past K1: not loaded (zero), never used
+      b.cc[u] = (int)__builtin_amdgcn_raw_buffer_load_b32(roci, k < b.K1 ? k * 4 : kBufferDrop, 0, 0);
+      b.vv[u] = buf_load<TV>(rov, k < b.K1 ? k * (int)sizeof(TV) : kBufferDrop);
     }
 #pragma unroll
     for (int u = 0; u < kBmcIn; ++u) {
       const int k = b.I0 + tid + u * NT;
-      b.ic[u] = (int)__builtin_amdgcn_raw_buffer_load_b16(ricode, k * 2, 0, 0) & 0xffff;
-      b.ivv[u] = buf_load<TV>(riv, k * (int)sizeof(TV));
+      b.ic[u] = (int)__builtin_amdgcn_raw_buffer_load_b16(ricode, k < b.I1 ? k * 2 : kBufferDrop, 0, 0) & 0xffff;
+      b.ivv[u] = buf_load<TV>(riv, k < b.I1 ? k * (int)sizeof(TV) : kBufferDrop);
     }
   };
   // the gathered values of the coupling entries (entries past the group's
@@ -764,18 +787,18 @@ __global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
     if (!second) {
       unroll<kBmcHalf>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        acc = fma(cf[j], row_bcast<BACKWARD ? kBmcHalf - 1 - j : j>(acc), acc);
+        fma_bcast_self<BACKWARD ? kBmcHalf - 1 - j : j>(acc, cf[j]);
       });
     }
     const T tv = row_swap<BACKWARD>(acc);  // second-row lanes: the first row's finished values
     if (second) {
       unroll<kBmcHalf>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        acc = fma(cf[j], row_bcast<BACKWARD ? kBmcHalf - 1 - j : j>(tv), acc);
+        fma_bcast<BACKWARD ? kBmcHalf - 1 - j : j>(acc, tv, cf[j]);
       });
       unroll<kBmcHalf>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        acc = fma(cf[kBmcHalf + j], row_bcast<BACKWARD ? kBmcHalf - 1 - j : j>(acc), acc);
+        fma_bcast_self<BACKWARD ? kBmcHalf - 1 - j : j>(acc, cf[kBmcHalf + j]);
       });
     }
     buf_store<T>(BACKWARD ? acc * bc.d : acc, rz, own ? bc.row * (int)sizeof(T) : kBufferDrop);
@@ -1563,7 +1586,7 @@ struct shesolve_ctx {
     float *d_ov32 = nullptr, *d_iv32 = nullptr;
     int64_t no = 0, ni = 0;
   } bmcL, bmcU;
-  int bmc_cap[2] = {0, 0};  // persistent grid of the sweep (FP32, FP64), from the occupancy at first use
+  int bmc_cap[4] = {0, 0, 0, 0};  // persistent grid of the sweep (FP32 forward, backward, FP64 forward, backward), from the occupancy at first use
   // CSR-stream row blocks of A and the tuned launch shape; in a batch no block
   // straddles two systems, and d_blk_sys holds each block's system
   int *d_ablk = nullptr;
@@ -1807,16 +1830,20 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     constexpr int G = kBmcGroupOf<T>;
     // persistent grid: what the device holds at once (a multiple of the XCD
     // count), or fewer when the colour has fewer groups
-    int &cap = c->bmc_cap[sizeof(T) == 8 ? 1 : 0];
-    if (!cap) {
-      int per_cu = 0, cus = 0;
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bmc_group<false, TV, T>, G * kBmcRows, 0);
+    int *cap = c->bmc_cap + (sizeof(T) == 8 ? 2 : 0);
+    if (!cap[0]) {
+      int cus = 0;
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
-      cap = std::max(kXcds, std::max(per_cu, 1) * std::max(cus, 1) / kXcds * kXcds);
+      const void *fn[2] = {(const void *)k_bmc_group<false, TV, T>, (const void *)k_bmc_group<true, TV, T>};
+      for (int d = 0; d < 2; ++d) {
+        int per_cu = 0;
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn[d], G * kBmcRows, 0);
+        cap[d] = std::max(kXcds, std::max(per_cu, 1) * std::max(cus, 1) / kXcds * kXcds);
+      }
     }
-    auto grid = [&](int b0, int b1) {
+    auto grid = [&](int b0, int b1, bool backward) {
       const int ng = (b1 - b0 + G - 1) / G;
-      return std::min(cap, (ng + kXcds - 1) / kXcds * kXcds);
+      return std::min(cap[backward ? 1 : 0], (ng + kXcds - 1) / kXcds * kXcds);
     };
     const auto &PL = c->bmcL, &PU = c->bmcU;
     const TV *oL, *iL, *oU, *iU;
@@ -1828,14 +1855,14 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     for (int q = 0; q < nq; ++q) {
       KTimer kt(c, 1);
       const int b0 = c->bmc_cptr[q], b1 = c->bmc_cptr[q + 1];
-      hipLaunchKernelGGL((k_bmc_group<false, TV, T>), dim3(grid(b0, b1)), dim3(G * kBmcRows), 0, c->stream, b0, b1,
+      hipLaunchKernelGGL((k_bmc_group<false, TV, T>), dim3(grid(b0, b1, false)), dim3(G * kBmcRows), 0, c->stream, b0, b1,
                          n, nbk, PL.no, PL.ni, c->d_bmc_start, PL.d_bK, PL.d_orp, PL.d_oci, oL, PL.d_iptr, PL.d_icode,
                          iL, dinv, in, out);
     }
     for (int q = nq - 1; q >= 0; --q) {
       KTimer kt(c, 2);
       const int b0 = c->bmc_cptr[q], b1 = c->bmc_cptr[q + 1];
-      hipLaunchKernelGGL((k_bmc_group<true, TV, T>), dim3(grid(b0, b1)), dim3(G * kBmcRows), 0, c->stream, b0, b1,
+      hipLaunchKernelGGL((k_bmc_group<true, TV, T>), dim3(grid(b0, b1, true)), dim3(G * kBmcRows), 0, c->stream, b0, b1,
                          n, nbk, PU.no, PU.ni, c->d_bmc_start, PU.d_bK, PU.d_orp, PU.d_oci, oU, PU.d_iptr, PU.d_icode,
                          iU, dinv, in, out);
     }
