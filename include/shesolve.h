@@ -191,6 +191,12 @@ int shesolve_kernel_stats(shesolve_handle h, int kind, double *avg_us, int64_t *
  * HIP events on the solver stream; returns the average per launch in
  * microseconds. */
 int shesolve_time_spmv(shesolve_handle h, int fp32, int reps, double *avg_us);
+/* Algorithmic bytes of one SpMV in the format the solver launches (FP32 values
+ * and vectors if fp32 != 0, else FP64): the values, the 4-byte column indices
+ * or, in the block multicolour solve space, the 16-bit column codes and their
+ * per-row-block tables, the row pointer, x once and y.  Measurement helper for
+ * the roofline of the SpMV (no counterpart in the reference). */
+int shesolve_spmv_bytes(shesolve_handle h, int fp32, int64_t *bytes);
 
 /* ---------------- batched solves ----------------
  * Many independent systems of uneven size (bias points, devices, expansion

@@ -65,6 +65,34 @@ def test_spmv_fp64_fp32(name):
     assert np.all(np.abs(y32.astype(np.float64) - y_ref) <= 2e-6 * bound + 1e-30)
 
 
+@pytest.mark.parametrize("name", ["nin2d_L3", "mos2d_L1", "mos2d_L3"])
+def test_spmv_with_column_codes(name):
+    """In the block multicolour solve space the pipelined SpMV decodes 16-bit
+    column codes (a row block's table of 32-row block starts plus the row
+    inside the block) instead of reading 4-byte column indices: the codes are
+    in use (the launched format's bytes are nnz x 2 B fewer, plus the tables),
+    and the product is the oracle's within the SpMV bars, in FP64 and FP32."""
+    import shesolve
+    s = oracle_system(name)
+    A, b, _, _ = s.reduced()
+    sv = _solver(precond=shesolve.PRECOND_BMC_ILU0, precision=shesolve.FP64)
+    sv.set_matrix(A.row_ptr, A.col, A.val)
+    csr = A.nnz * 12 + (A.n + 1) * 4 + 2 * A.n * 8
+    coded = sv.spmv_bytes(fp32=False)
+    assert csr - 2 * A.nnz < coded < csr - A.nnz, (coded, csr)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(A.n)
+    y_ref = A.matvec(x)
+    absA = A.to_scipy().copy()
+    absA.data = np.abs(absA.data)
+    bound = absA @ np.abs(x)
+    y = sv.spmv(x)
+    assert np.all(np.abs(y - y_ref) <= 1e-13 * bound + 1e-300)
+    y32 = sv.spmv_f32(x.astype(np.float32))
+    assert np.all(np.abs(y32.astype(np.float64) - y_ref) <= 2e-6 * bound + 1e-30)
+    sv.close()
+
+
 @pytest.mark.parametrize("name", ["diode_L1", "mos2d_L3"])
 @pytest.mark.parametrize("kind", ["ilu0", "block_ilu0"])
 def test_ilu0_factors_match_oracle(name, kind):

@@ -380,13 +380,24 @@ __device__ inline void buf_store(T v, __amdgpu_buffer_rsrc_t r, int off) {
 }
 constexpr int kBufferDrop = 0x7ffffff0;  // an offset past every resource: the store is dropped
 
-template <typename TV, typename T, int ITEMS>
+// CODED: the column of an entry is not read from ci but decoded from a 16-bit
+// code, table[block's table start + (code >> 5)] + (code & 31): the table
+// lists the first solve-space row of each 32-row block of the block
+// multicolour ordering that the row block's columns fall in (at most
+// kCodeSlots), and the low five bits are the column's row inside that block.
+// The descriptor then has eight ints per block, {first row, end row, first
+// entry, end entry, table start, table end, 0, 0}, and every wavefront holds
+// the row block's table in two registers per lane, read with ds_bpermute.
+constexpr int kCodeSlots = 128;
+template <typename TV, typename T, int ITEMS, bool CODED = false>
 __global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd, int n, int64_t nnz_alloc,
                                                               const int *__restrict__ desc,
                                                               const int *__restrict__ rp, const int *__restrict__ ci,
                                                               const TV *__restrict__ v, const T *x, T *y,
                                                               const T *__restrict__ b, const T *d0, const T *d1,
-                                                              double *__restrict__ part) {
+                                                              double *__restrict__ part,
+                                                              const unsigned short *__restrict__ code = nullptr,
+                                                              const int *__restrict__ tab = nullptr, int64_t ntab = 0) {
   // W consecutive entries per lane and load: four with FP32 values (16-byte
   // loads), one with FP64 values (measured faster there: 563 against 594 us
   // on the 161x81 L=3 device)
@@ -405,8 +416,11 @@ __global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd,
   const int tid = (int)threadIdx.x;
   // a dot product with y itself takes the new value, not a prefetched one
   const bool d0y = (const void *)d0 == (const void *)y, d1y = (const void *)d1 == (const void *)y;
-  const auto rdesc = buf_rsrc(desc, (int64_t)nblk * 16), rrp = buf_rsrc(rp, (int64_t)(n + 1) * 4);
-  const auto rci = buf_rsrc(ci, nnz_alloc * 4), rv = buf_rsrc(v, nnz_alloc * (int64_t)sizeof(TV));
+  constexpr int DW = CODED ? 8 : 4;  // descriptor ints per block
+  const auto rdesc = buf_rsrc(desc, (int64_t)nblk * DW * 4), rrp = buf_rsrc(rp, (int64_t)(n + 1) * 4);
+  const auto rci = buf_rsrc(CODED ? nullptr : ci, nnz_alloc * 4), rv = buf_rsrc(v, nnz_alloc * (int64_t)sizeof(TV));
+  const auto rcode = buf_rsrc(CODED ? code : nullptr, nnz_alloc * 2), rtab = buf_rsrc(CODED ? tab : nullptr, ntab * 4);
+  const int lane = tid & 63;
   const auto rx = buf_rsrc(x, (int64_t)n * sizeof(T)), ry = buf_rsrc(y, (int64_t)n * sizeof(T));
   const auto rb = buf_rsrc(b, (int64_t)n * sizeof(T));
   const auto r0 = buf_rsrc(d0 && !d0y ? d0 : nullptr, (int64_t)n * sizeof(T));
@@ -416,12 +430,14 @@ __global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd,
     struct StB {
       int rs, re, k0, k1, r0, r1;
       T bb, e0, e1;
-      int cc[ITEMS];
+      int cc[ITEMS];  // columns, or (CODED) codes
       TV vv[ITEMS];
+      int tb0, tb1, wide;  // CODED: the block's table (slots lane and 64 + lane), more than 64 slots
     };
-    // lane l holds component l % 4 of the block's descriptor
+    // lane l holds component l % DW of the block's descriptor
     auto stage_a = [&](int t) {
-      return __builtin_amdgcn_raw_buffer_load_b32(rdesc, (4 * min(bk0 + t * bstep, last) + (tid & 3)) * 4, 0, 0);
+      return __builtin_amdgcn_raw_buffer_load_b32(rdesc, (DW * min(bk0 + t * bstep, last) + (tid & (DW - 1))) * 4,
+                                                 0, 0);
     };
     // entry j of vector q of this lane: base + W (tid + q NT) + j, base = k0
     // rounded down to a multiple of four; a load that starts past the block's
@@ -432,6 +448,13 @@ __global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd,
       s.re = __builtin_amdgcn_readlane(dv, 1);
       s.k0 = __builtin_amdgcn_readlane(dv, 2);
       s.k1 = __builtin_amdgcn_readlane(dv, 3);
+      if constexpr (CODED) {
+        const int t0 = __builtin_amdgcn_readlane(dv, 4), t1 = __builtin_amdgcn_readlane(dv, 5);
+        s.tb0 = (int)__builtin_amdgcn_raw_buffer_load_b32(rtab, t0 + lane < t1 ? (t0 + lane) * 4 : kBufferDrop, 0, 0);
+        s.tb1 = (int)__builtin_amdgcn_raw_buffer_load_b32(rtab, t0 + 64 + lane < t1 ? (t0 + 64 + lane) * 4 : kBufferDrop,
+                                                         0, 0);
+        s.wide = t1 - t0 > 64;
+      }
       const int rc = s.rs + tid;
       s.r0 = (int)__builtin_amdgcn_raw_buffer_load_b32(rrp, rc * 4, 0, 0);
       s.r1 = (int)__builtin_amdgcn_raw_buffer_load_b32(rrp, rc * 4 + 4, 0, 0);
@@ -443,7 +466,15 @@ __global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd,
       for (int q = 0; q < NV; ++q) {
         const int e = base + W * (tid + q * NT);
         const int eo = e < s.k1 ? e * 4 : kBufferDrop;
-        if constexpr (W == 4) {
+        if constexpr (W == 4 && CODED) {
+          // four 16-bit codes in one 8-byte load, split when the gathers are issued
+          const vuint2 c2 = __builtin_amdgcn_raw_buffer_load_b64(rcode, e < s.k1 ? e * 2 : kBufferDrop, 0, 0);
+          const vfloat4 v4 = __builtin_bit_cast(vfloat4, __builtin_amdgcn_raw_buffer_load_b128(rv, eo, 0, 0));
+#pragma unroll
+          for (int j = 0; j < 4; ++j) s.vv[4 * q + j] = v4[j];
+          s.cc[4 * q] = (int)c2[0];
+          s.cc[4 * q + 1] = (int)c2[1];
+        } else if constexpr (W == 4) {
           // the whole 16 bytes cast at once (a cast of one element of the
           // builtin's result compiled to a one-dword load reused for all four)
           const vint4 c4 = __builtin_bit_cast(vint4, __builtin_amdgcn_raw_buffer_load_b128(rci, eo, 0, 0));
@@ -454,14 +485,36 @@ __global__ __launch_bounds__(kStreamThreads) void k_spmv_pipe(int nblk, int xcd,
             s.vv[4 * q + j] = v4[j];
           }
         } else {
-          s.cc[q] = (int)__builtin_amdgcn_raw_buffer_load_b32(rci, eo, 0, 0);
+          if constexpr (CODED)
+            s.cc[q] = (int)__builtin_amdgcn_raw_buffer_load_b16(rcode, e < s.k1 ? e * 2 : kBufferDrop, 0, 0) & 0xffff;
+          else
+            s.cc[q] = (int)__builtin_amdgcn_raw_buffer_load_b32(rci, eo, 0, 0);
           s.vv[q] = buf_load<TV>(rv, e < s.k1 ? e * (int)sizeof(TV) : kBufferDrop);
         }
       }
     };
+    // column of a code: the table slot through ds_bpermute (slot j < 64 from
+    // lane j of the first half, 64 + j from lane j of the second)
+    auto decode = [&](const StB &s, int cd) {
+      const int sl = cd >> 5, lo = __builtin_amdgcn_ds_bpermute((sl & 63) * 4, s.tb0);
+      const int hi = s.wide ? __builtin_amdgcn_ds_bpermute((sl & 63) * 4, s.tb1) : 0;
+      return (sl < 64 ? lo : hi) + (cd & 31);
+    };
     auto stage_c = [&](const StB &s, T *xg) {
+      if constexpr (CODED && W == 4) {
 #pragma unroll
-      for (int u = 0; u < ITEMS; ++u) xg[u] = buf_load<T>(rx, s.cc[u] * (int)sizeof(T));
+        for (int u = 0; u < ITEMS / 2; ++u) {
+          const unsigned pk = (unsigned)s.cc[(u / 2) * 4 + (u % 2)];
+          xg[2 * u] = buf_load<T>(rx, decode(s, (int)(pk & 0xffff)) * (int)sizeof(T));
+          xg[2 * u + 1] = buf_load<T>(rx, decode(s, (int)(pk >> 16)) * (int)sizeof(T));
+        }
+      } else if constexpr (CODED) {
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u) xg[u] = buf_load<T>(rx, decode(s, s.cc[u]) * (int)sizeof(T));
+      } else {
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u) xg[u] = buf_load<T>(rx, s.cc[u] * (int)sizeof(T));
+      }
     };
     // three sets of row/entry loads and two of gathers and of descriptors, their
     // roles rotated by unrolling (copying a register whose load is in flight
@@ -1601,6 +1654,14 @@ struct shesolve_ctx {
   // descriptors of the pipelined SpMV, for the row blocks at desc_src (blocks
   // or pairs) of the solve-space matrix
   int4 *d_desc = nullptr;
+  // coded columns of the pipelined SpMV (block multicolour solve space): 16-bit
+  // codes per entry, the row blocks' tables and eight-int descriptors, valid
+  // for the row blocks at code_src
+  unsigned short *d_code = nullptr;
+  int *d_tab = nullptr, *d_cdesc = nullptr;
+  int64_t ntab = 0;
+  const int *code_src = nullptr;
+  int code_nblk = 0;
   const int *desc_src = nullptr;
   int desc_nblk = 0;
   int desc_maxrow = 0;  // longest row of that matrix (a block holds more entries only as a single row)
@@ -1779,11 +1840,22 @@ void launch_spmv(shesolve_ctx *c, int /*lpr*/, const TV *v, const T *x, T *y, co
   if (!c->spmv_skip && c->d_desc && c->desc_src == (pairs ? pairs : blk) && c->desc_nblk == nblk &&
       c->desc_maxrow <= spmv_cap(c->items) &&
       ((int64_t)(c->solve_perm ? c->fnnz : c->nnz) + kVecPad) * (int64_t)sizeof(TV) < (int64_t)1 << 31) {
+    const int64_t alloc = (int64_t)(c->solve_perm ? c->fnnz : c->nnz) + kVecPad;
+    if (c->d_cdesc && pairs && c->code_src == pairs && c->code_nblk == nblk) {
+#define SP_CASE(IT)                                                                                               \
+  case IT:                                                                                                        \
+    hipLaunchKernelGGL((k_spmv_pipe<TV, T, IT, true>), dim3(g), dim3(kStreamThreads), 0, c->stream, nblk, c->xcd, \
+                       c->n, alloc, (const int *)c->d_cdesc, rp, ci, v, x, y, b, d0, d1, part, c->d_code, c->d_tab, \
+                       c->ntab);                                                                                  \
+    break;
+      switch (c->items) { SP_CASE(4) SP_CASE(16) default: SP_CASE(8) }
+#undef SP_CASE
+      return;
+    }
 #define SP_CASE(IT)                                                                                               \
   case IT:                                                                                                        \
     hipLaunchKernelGGL((k_spmv_pipe<TV, T, IT>), dim3(g), dim3(kStreamThreads), 0, c->stream, nblk, c->xcd, c->n,  \
-                       (int64_t)(c->solve_perm ? c->fnnz : c->nnz) + kVecPad, (const int *)c->d_desc, rp, ci, v, x, y, b,  \
-                       d0, d1, part);                                                                             \
+                       alloc, (const int *)c->d_desc, rp, ci, v, x, y, b, d0, d1, part);                         \
     break;
     switch (c->items) { SP_CASE(4) SP_CASE(16) default: SP_CASE(8) }
 #undef SP_CASE
@@ -1969,6 +2041,13 @@ int ensure_vectors(shesolve_ctx *c) {
   return 0;
 }
 
+void free_codes(shesolve_ctx *c) {
+  dfree(c->d_code); dfree(c->d_tab); dfree(c->d_cdesc);
+  c->ntab = 0;
+  c->code_src = nullptr;
+  c->code_nblk = 0;
+}
+
 void free_matrix(shesolve_ctx *c) {
   dfree(c->d_rp); dfree(c->d_ci); dfree(c->d_v64); dfree(c->d_v32);
   if (!c->factor_is_A) { dfree(c->d_frp); dfree(c->d_fci); }
@@ -1991,6 +2070,7 @@ void free_matrix(shesolve_ctx *c) {
   c->s_nblk = 0;
   dfree(c->d_ablk); c->nablk = 0;
   dfree(c->d_desc); c->desc_src = nullptr; c->desc_nblk = 0;
+  free_codes(c);
   dfree(c->d_blk_sys);
   c->spmv_skip = nullptr;
   dfree(c->d_Lrp); dfree(c->d_Lci); dfree(c->d_Urp); dfree(c->d_Uci); dfree(c->d_Lsrc);
@@ -2836,6 +2916,63 @@ int build_desc(shesolve_ctx *c, const int *blocks, const int *pairs, int nblk, c
   return 0;
 }
 
+// 16-bit column codes of the solve-space matrix for the coded pipelined SpMV,
+// for the row blocks `flat` ((first, end) row pairs).  Block multicolour
+// ordering only: its 32-row blocks keep consecutive natural rows together and in
+// order, so column col lies in the block starting at col - perm[col] % 32.  Per
+// row block, the sorted distinct block starts of its columns form its table
+// (when a row block needs more than kCodeSlots, no codes are built and the SpMV
+// reads the column indices); an entry's code is slot << 5 | perm[col] % 32.
+int build_codes(shesolve_ctx *c, const std::vector<int> &flat) {
+  free_codes(c);
+  if (c->prm.precond != SHESOLVE_PRECOND_BMC_ILU0 || !c->solve_perm || c->h_perm.size() != (size_t)c->n) return 0;
+  const int nb = (int)(flat.size() / 2);
+  const std::vector<int> &rp = c->h_frp, &ci = c->h_fci, &perm = c->h_perm;
+  std::vector<std::vector<int>> tabs(nb);
+  std::vector<unsigned short> code((size_t)c->fnnz + kVecPad, 0);
+  std::atomic<bool> ok{true};
+  auto work = [&](int k0, int k1) {
+    std::vector<int> T;
+    for (int k = k0; k < k1 && ok.load(std::memory_order_relaxed); ++k) {
+      const int e0 = rp[flat[2 * k]], e1 = rp[flat[2 * k + 1]];
+      T.clear();
+      for (int e = e0; e < e1; ++e) T.push_back(ci[e] - (perm[ci[e]] & 31));
+      std::sort(T.begin(), T.end());
+      T.erase(std::unique(T.begin(), T.end()), T.end());
+      if ((int)T.size() > kCodeSlots) { ok = false; break; }
+      for (int e = e0; e < e1; ++e) {
+        const int off = perm[ci[e]] & 31;
+        const int slot = (int)(std::lower_bound(T.begin(), T.end(), ci[e] - off) - T.begin());
+        code[e] = (unsigned short)(slot << 5 | off);
+      }
+      tabs[k] = T;
+    }
+  };
+  const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (int w = 0; w < nth; ++w) th.emplace_back(work, (int)((int64_t)nb * w / nth), (int)((int64_t)nb * (w + 1) / nth));
+  for (auto &x : th) x.join();
+  if (!ok) return 0;
+  std::vector<int> tab, desc((size_t)nb * 8, 0);
+  for (int k = 0; k < nb; ++k) {
+    const int r0 = flat[2 * k], r1 = flat[2 * k + 1];
+    const int t0 = (int)tab.size();
+    tab.insert(tab.end(), tabs[k].begin(), tabs[k].end());
+    const int v[8] = {r0, r1, rp[r0], rp[r1], t0, (int)tab.size(), 0, 0};
+    std::copy(v, v + 8, desc.begin() + (size_t)k * 8);
+  }
+  HIPCHK(hipMalloc(&c->d_code, sizeof(unsigned short) * code.size()));
+  HIPCHK(hipMalloc(&c->d_tab, sizeof(int) * std::max<size_t>(tab.size(), 1)));
+  HIPCHK(hipMalloc(&c->d_cdesc, sizeof(int) * std::max<size_t>(desc.size(), 8)));
+  HIPCHK(hipMemcpy(c->d_code, code.data(), sizeof(unsigned short) * code.size(), hipMemcpyHostToDevice));
+  if (!tab.empty()) HIPCHK(hipMemcpy(c->d_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_cdesc, desc.data(), sizeof(int) * desc.size(), hipMemcpyHostToDevice));
+  c->ntab = (int64_t)tab.size();
+  c->code_src = c->s_pairs;
+  c->code_nblk = nb;
+  return 0;
+}
+
 // Solve space of a multicolour factor: the permuted matrix (the factor's
 // pattern), its CSR-stream row blocks, and the permuted b / x of a solve.
 int setup_solve_space(shesolve_ctx *c) {
@@ -2877,6 +3014,7 @@ int setup_solve_space(shesolve_ctx *c) {
     }
   };
   auto install = [&](const std::vector<int> &flat) -> int {
+    free_codes(c);
     dfree(c->d_sblk);
     HIPCHK(hipMalloc(&c->d_sblk, sizeof(int) * std::max<size_t>(flat.size(), 2)));
     HIPCHK(hipMemcpy(c->d_sblk, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
@@ -2891,7 +3029,8 @@ int setup_solve_space(shesolve_ctx *c) {
   std::vector<int> flat;
   if (!c->prm.autotune) {
     build_pairs(c->items, flat);
-    return install(flat);
+    if (int e = install(flat)) return e;
+    return build_codes(c, flat);
   }
   // Entries per thread and block-to-XCD mapping of the permuted matrix, timed
   // on it as the solve launches it (fused dot product, persistent grid).  The
@@ -2931,7 +3070,7 @@ int setup_solve_space(shesolve_ctx *c) {
   c->items = best_items;
   c->xcd = best_xcd;
   if (int e = install(best_flat)) return e;
-  return 0;
+  return build_codes(c, best_flat);
 }
 
 // One BiCGStab solve (right preconditioned) on the device, precision T.
@@ -4370,6 +4509,16 @@ int shesolve_kernel_stats(shesolve_handle h, int kind, double *avg_us, int64_t *
   resolve_events(h);
   if (avg_us) *avg_us = h->kcount[kind] ? h->ktime[kind] / h->kcount[kind] : 0.0;
   if (launches) *launches = h->kcount[kind];
+  return 0;
+}
+
+int shesolve_spmv_bytes(shesolve_handle h, int fp32, int64_t *bytes) {
+  if (!h) return 1;
+  if (!bytes) return 3;
+  if (!h->d_rp) { set_err("no matrix set"); return -1; }
+  const int64_t nz = h->solve_perm ? h->fnnz : h->nnz, sv = fp32 ? 4 : 8;
+  const bool coded = h->d_cdesc && h->s_pairs && h->code_src == h->s_pairs && h->code_nblk == h->s_nblk;
+  *bytes = nz * (sv + (coded ? 2 : 4)) + (coded ? h->ntab * 4 : 0) + (int64_t)(h->n + 1) * 4 + 2 * (int64_t)h->n * sv;
   return 0;
 }
 

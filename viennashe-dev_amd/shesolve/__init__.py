@@ -24,7 +24,7 @@ EXPORTS = (
     "shesolve_set_params", "shesolve_set_matrix", "shesolve_set_matrix_device", "shesolve_update_values",
     "shesolve_update_values_device", "shesolve_solve", "shesolve_solve_device", "shesolve_spmv",
     "shesolve_spmv_f32", "shesolve_precond_apply", "shesolve_factor_nnz", "shesolve_get_factors",
-    "shesolve_levels", "shesolve_set_profiling", "shesolve_kernel_stats", "shesolve_time_spmv",
+    "shesolve_levels", "shesolve_set_profiling", "shesolve_kernel_stats", "shesolve_time_spmv", "shesolve_spmv_bytes",
     "shesolve_get_ordering", "shesolve_stream_config", "shesolve_set_reference_solution", "shesolve_ir_history",
     "shesolve_batch_create", "shesolve_batch_destroy", "shesolve_batch_set_params", "shesolve_batch_set_systems",
     "shesolve_batch_set_systems_device", "shesolve_batch_update_values", "shesolve_batch_update_values_device",
@@ -99,6 +99,7 @@ def lib():
     L.shesolve_set_profiling.argtypes = [C.c_void_p, C.c_int]
     L.shesolve_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.shesolve_time_spmv.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.shesolve_spmv_bytes.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64)]
     L.shesolve_set_reference_solution.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int]
     L.shesolve_ir_history.argtypes = [C.c_void_p, C.c_int32, C.POINTER(IRStep), C.POINTER(C.c_int32)]
     VP = C.POINTER(C.c_void_p)
@@ -278,6 +279,13 @@ class Solver:
         a = C.c_double()
         _check(lib().shesolve_time_spmv(self._h, 1 if fp32 else 0, reps, C.byref(a)), "shesolve_time_spmv")
         return a.value
+
+    def spmv_bytes(self, fp32=True):
+        """Algorithmic bytes of one SpMV in the launched format (16-bit column
+        codes in the block multicolour solve space, 4-byte indices otherwise)."""
+        b = C.c_int64()
+        _check(lib().shesolve_spmv_bytes(self._h, 1 if fp32 else 0, C.byref(b)), "shesolve_spmv_bytes")
+        return b.value
 
 
 def _ptrs(items):
