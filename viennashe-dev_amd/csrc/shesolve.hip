@@ -3047,6 +3047,7 @@ int setup_solve_space(shesolve_ctx *c) {
   for (int ii = 0; ii < 3; ++ii) {
     build_pairs(cand[ii], flat);
     if (int e = install(flat)) return e;
+    if (int e = build_codes(c, flat)) return e;  // timed as launched: with column codes where they apply
     c->items = cand[ii];
     for (int xm = 0; xm < 2; ++xm) {
       c->xcd = xm;
@@ -3059,6 +3060,7 @@ int setup_solve_space(shesolve_ctx *c) {
       HIPCHK(hipEventSynchronize(c->ev1));
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      c->tune_us[ii * 2 + xm] = ms * 1e3 / 5;  // the solve space's candidates replace the natural matrix's
       if (ms < best) {
         best = ms;
         best_items = cand[ii];
