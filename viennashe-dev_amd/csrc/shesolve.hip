@@ -661,7 +661,8 @@ __device__ inline double row_swap(double x) {
 // counted from the colour's first block; with xcd set each XCD, workgroup id
 // mod 8, takes a contiguous eighth of them), half a wavefront per block and
 // one lane per row.  It streams the group's coupling entries as one contiguous
-// range (consecutive lanes, consecutive entries), multiplies each by its
+// range (consecutive lanes, consecutive entries; with FP32 values four entries
+// per lane in one 16-byte load per array), multiplies each by its
 // gathered z into LDS and sums each row's run, while the in-block lists fill
 // each block's coefficient tile.  Then each block is eliminated in registers:
 // in the elimination order (forward: rows ascending; backward: descending,
@@ -714,7 +715,8 @@ __global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
   // buffer resources: loads past an array read zero, stores past it are dropped
   const auto rbst = buf_rsrc(bstart, (int64_t)(nb + 1) * 4), rbK = buf_rsrc(bK, (int64_t)(nb + 1) * 4);
   const auto rip = buf_rsrc(iptr, (int64_t)(nb + 1) * 4), rorp = buf_rsrc(orp, (int64_t)(n + 1) * 4);
-  const auto roci = buf_rsrc(oci, no * 4), rov = buf_rsrc(ov, no * (int64_t)sizeof(TV));
+  // the coupling arrays carry kVecPad entries past the end (16-byte loads of the last group)
+  const auto roci = buf_rsrc(oci, (no + kVecPad) * 4), rov = buf_rsrc(ov, (no + (sizeof(TV) == 4 ? kVecPad : 0)) * (int64_t)sizeof(TV));
   const auto ricode = buf_rsrc(icode, ni * 2), riv = buf_rsrc(iv, ni * (int64_t)sizeof(TV));
   const auto rdinv = buf_rsrc(BACKWARD ? (const void *)dinv : nullptr, (int64_t)n * sizeof(TV));
   const auto rin = buf_rsrc(BACKWARD ? (const void *)z : (const void *)in, (int64_t)n * sizeof(T));
@@ -756,11 +758,25 @@ __global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
     b.d = BACKWARD ? (T)buf_load<TV>(rdinv, b.row * (int)sizeof(TV)) : (T)1;
     b.ks = (int)__builtin_amdgcn_raw_buffer_load_b32(rorp, b.row * 4, 0, 0);
     b.ke = (int)__builtin_amdgcn_raw_buffer_load_b32(rorp, b.row * 4 + 4, 0, 0);
+    if constexpr (sizeof(TV) == 4 && kBmcItems == 4) {
+      // entries (K0 rounded down to four) + 4 tid + u in one 16-byte load per
+      // array; a load starting past K1 is not made
+      const int e = (b.K0 & ~3) + 4 * tid;
+      const int eo = e < b.K1 ? e * 4 : kBufferDrop;
+      const vint4 c4 = __builtin_bit_cast(vint4, __builtin_amdgcn_raw_buffer_load_b128(roci, eo, 0, 0));
+      const vfloat4 v4 = __builtin_bit_cast(vfloat4, __builtin_amdgcn_raw_buffer_load_b128(rov, eo, 0, 0));
 #pragma unroll
-    for (int u = 0; u < kBmcItems; ++u) {
-      const int k = b.K0 + tid + u * NT;  // past K1: not loaded (zero), never used
-      b.cc[u] = (int)__builtin_amdgcn_raw_buffer_load_b32(roci, k < b.K1 ? k * 4 : kBufferDrop, 0, 0);
-      b.vv[u] = buf_load<TV>(rov, k < b.K1 ? k * (int)sizeof(TV) : kBufferDrop);
+      for (int u = 0; u < 4; ++u) {
+        b.cc[u] = c4[u];
+        b.vv[u] = v4[u];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kBmcItems; ++u) {
+        const int k = b.K0 + tid + u * NT;  // past K1: not loaded (zero), never used
+        b.cc[u] = (int)__builtin_amdgcn_raw_buffer_load_b32(roci, k < b.K1 ? k * 4 : kBufferDrop, 0, 0);
+        b.vv[u] = buf_load<TV>(rov, k < b.K1 ? k * (int)sizeof(TV) : kBufferDrop);
+      }
     }
 #pragma unroll
     for (int u = 0; u < kBmcIn; ++u) {
@@ -798,7 +814,10 @@ __global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
       for (int i = tid; i < (int)(G * kBmcTile * sizeof(T) / 16); i += NT) t4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     __syncthreads();  // tiles cleared
-    const int icnt = bc.I1 - bc.I0, ccnt = min(bc.K1 - bc.K0, CAP);
+    constexpr bool V4 = sizeof(TV) == 4 && kBmcItems == 4;
+    // products are held from kb (K0, rounded down to four with 16-byte loads)
+    const int kb = V4 ? (bc.K0 & ~3) : bc.K0;
+    const int icnt = bc.I1 - bc.I0, ccnt = min(bc.K1, kb + CAP) - bc.K0;
 #pragma unroll
     for (int u = 0; u < kBmcIn; ++u)
       if (tid + u * NT < icnt) {
@@ -810,16 +829,16 @@ __global__ __launch_bounds__(kBmcGroupOf<T> * kBmcRows) void k_bmc_group(
       tile[((code / kBmcTile) % G) * kBmcTile + code % kBmcTile] = (T)iv[k];
     }
 #pragma unroll
-    for (int u = 0; u < kBmcItems; ++u) prod[tid + u * NT] = (T)bc.vv[u] * zc[u];  // past ccnt: never summed
+    for (int u = 0; u < kBmcItems; ++u) prod[V4 ? 4 * tid + u : tid + u * NT] = (T)bc.vv[u] * zc[u];  // outside [K0, K0 + ccnt): never summed
     __syncthreads();  // in-block values and products in place
     const bool own = hl < bc.nr;
     T s = 0;
     {
       const int lo = own ? bc.ks : bc.K0, hi = own ? min(bc.ke, bc.K0 + ccnt) : bc.K0;
-      for (int k = lo; k < hi; ++k) s += prod[k - bc.K0];
+      for (int k = lo; k < hi; ++k) s += prod[k - kb];
     }
     // coupling entries beyond the held ones, a chunk at a time (uniform)
-    for (int kc = bc.K0 + CAP; kc < bc.K1; kc += CAP) {
+    for (int kc = kb + CAP; kc < bc.K1; kc += CAP) {
       __syncthreads();
       const int cnt = min(CAP, bc.K1 - kc);
       for (int k = tid; k < cnt; k += NT) prod[k] = (T)ov[kc + k] * z[oci[kc + k]];
@@ -2419,10 +2438,12 @@ int build_bmc_part(shesolve_ctx *c, const std::vector<int> &bst, bool upper, con
   const size_t no = std::max<size_t>(oci.size(), 1), ni = std::max<size_t>(ipos.size(), 1);
   HIPCHK(hipMalloc(&P.d_orp, sizeof(int) * (n + 1)));
   HIPCHK(hipMalloc(&P.d_bK, sizeof(int) * (nb + 1)));
-  HIPCHK(hipMalloc(&P.d_oci, sizeof(int) * no));
+  HIPCHK(hipMalloc(&P.d_oci, sizeof(int) * (no + kVecPad)));
+  HIPCHK(hipMemset(P.d_oci, 0, sizeof(int) * (no + kVecPad)));
   HIPCHK(hipMalloc(&P.d_opos, sizeof(int) * no));
   HIPCHK(hipMalloc(&P.d_ov64, sizeof(double) * no));
-  HIPCHK(hipMalloc(&P.d_ov32, sizeof(float) * no));
+  HIPCHK(hipMalloc(&P.d_ov32, sizeof(float) * (no + kVecPad)));
+  HIPCHK(hipMemset(P.d_ov32, 0, sizeof(float) * (no + kVecPad)));
   HIPCHK(hipMalloc(&P.d_iptr, sizeof(int) * (nb + 1)));
   HIPCHK(hipMalloc(&P.d_ipos, sizeof(int) * ni));
   HIPCHK(hipMalloc(&P.d_icol, sizeof(int) * ni));
