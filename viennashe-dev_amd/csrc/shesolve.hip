@@ -1455,12 +1455,15 @@ __device__ inline void lds_barrier() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-template <int NT, int LPR, int R, int J, typename TV, typename T, bool UPPER>
+template <int NT, int LPR, int R, int J, typename TV, typename T, bool UPPER, bool WIN = false>
 __global__ __launch_bounds__(NT) void k_wgsweep(const int *__restrict__ seg_lvl, const int *__restrict__ lvl_ptr,
                                                 const int *__restrict__ lrp, const int *__restrict__ cq,
                                                 const TV *__restrict__ v, const T *__restrict__ rhs, T *zq,
                                                 const TV *__restrict__ dq, const BSys *__restrict__ bs,
                                                 const int *__restrict__ seg_sys, int W) {
+  // WIN: every column the sweep reads lies within W positions of the end of
+  // the level that reads it (the host checks the whole factor), so x comes
+  // from the LDS window only and levels are separated by LDS-only barriers.
   extern __shared__ unsigned char lds_raw[];
   T *win = reinterpret_cast<T *>(lds_raw);
   __shared__ int lp[kLvlChunk + 1];
@@ -1498,18 +1501,25 @@ __global__ __launch_bounds__(NT) void k_wgsweep(const int *__restrict__ seg_lvl,
         E.v[r][j] = kk ? v[k] : (TV)0;
       }
   };
-  // level pointers of the segment, staged in LDS kLvlChunk levels at a time
+  // x of position qc: the window for positions >= lo, zq otherwise
+  auto xval = [&](int qc, int lo) -> T {
+    if constexpr (WIN) return win[qc & Wm];
+    else return qc >= lo ? win[qc & Wm] : zq[qc];
+  };
+  // level pointers of the segment, staged in LDS kLvlChunk levels at a time;
+  // the chunk always holds levels l .. l + 3 while level l is swept (so they
+  // are read from LDS: a pointer that may be global or LDS would compile to
+  // flat loads, whose waits drain the prefetched rows of the next levels)
   int lc = l0;  // first level of the staged chunk
   for (int t = threadIdx.x; t <= kLvlChunk && lc + t <= l1; t += NT) lp[t] = lvl_ptr[lc + t];
   __syncthreads();
-  auto qp = [&](int l) { return (l - lc <= kLvlChunk) ? lp[l - lc] : lvl_ptr[l]; };
+  auto qp = [&](int l) { return lp[l - lc]; };
   int q_safe = lp[0];  // positions below q_safe are visible in zq to every wave
-  Meta cur, nxt, ahead;
-  Ent ecur, enxt;
-  meta(cur, qp(l0), qp(l0 + 1));
-  ents(ecur, cur);
-  if (l0 + 1 < l1) meta(nxt, qp(l0 + 1), qp(l0 + 2));
-  for (int l = l0; l < l1; ++l) {
+  // level l with the row data of l (cur) and l + 1 (nxt) and the entries of l
+  // (ecur) in registers; loads ahead (l + 2) and enxt (l + 1).  The register
+  // sets rotate by unrolling (a copy of a register whose load is in flight
+  // would wait for it).  Returns whether level l + 1 exists.
+  auto level = [&](int l, const Meta &cur, const Meta &nxt, Meta &ahead, const Ent &ecur, Ent &enxt) -> bool {
     const int q0 = qp(l), q1 = qp(l + 1);
     const bool more = l + 1 < l1;
     if (l + 2 < l1) meta(ahead, qp(l + 2), qp(l + 3));
@@ -1530,16 +1540,10 @@ __global__ __launch_bounds__(NT) void k_wgsweep(const int *__restrict__ seg_lvl,
 #pragma unroll
         for (int j = 0; j < J; ++j) {
           const int k = m.kb[r] + lane + j * LPR;
-          if (k < m.ke[r]) {
-            const int qc = e.c[r][j];
-            acc[r] += (T)e.v[r][j] * (qc >= lo ? win[qc & Wm] : zq[qc]);
-          }
+          if (k < m.ke[r]) acc[r] += (T)e.v[r][j] * xval(e.c[r][j], lo);
         }
         // entries beyond the prefetched J per lane
-        for (int k = m.kb[r] + lane + J * LPR; k < m.ke[r]; k += LPR) {
-          const int qc = cq[k];
-          acc[r] += (T)v[k] * (qc >= lo ? win[qc & Wm] : zq[qc]);
-        }
+        for (int k = m.kb[r] + lane + J * LPR; k < m.ke[r]; k += LPR) acc[r] += (T)v[k] * xval(cq[k], lo);
       }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -1550,29 +1554,158 @@ __global__ __launch_bounds__(NT) void k_wgsweep(const int *__restrict__ seg_lvl,
         if (lane == 0 && q < q1) {
           const T out = UPPER ? m.d[r] * (m.rh[r] - a) : m.rh[r] - a;
           zq[q] = out;
-          if (q >= lo) win[q & Wm] = out;
+          if (WIN || q >= lo) win[q & Wm] = out;
         }
       }
     }
-    if (!more) break;
+    if (!more) return false;
     const int nq1 = qp(l + 2);
-    if (l + 3 - lc > kLvlChunk) {
+    if (l + 4 - lc > kLvlChunk) {
       // stage the next chunk of level pointers (levels l + 1 ..)
       __syncthreads();
       lc = l + 1;
       for (int t = threadIdx.x; t <= kLvlChunk && lc + t <= l1; t += NT) lp[t] = lvl_ptr[lc + t];
       __syncthreads();
       q_safe = q1;
-    } else if (nq1 - W > q_safe) {
+    } else if (!WIN && nq1 - W > q_safe) {
       // the next level may read positions >= q_safe from zq
       __syncthreads();
       q_safe = q1;
     } else {
       lds_barrier();
     }
-    cur = nxt;
-    ecur = enxt;
-    nxt = ahead;
+    return true;
+  };
+  Meta m0, m1, m2;
+  Ent e0, e1;
+  meta(m0, qp(l0), qp(l0 + 1));
+  ents(e0, m0);
+  if (l0 + 1 < l1) meta(m1, qp(l0 + 1), qp(l0 + 2));
+  for (int l = l0;; l += 6) {
+    if (!level(l, m0, m1, m2, e0, e1)) break;
+    if (!level(l + 1, m1, m2, m0, e1, e0)) break;
+    if (!level(l + 2, m2, m0, m1, e0, e1)) break;
+    if (!level(l + 3, m0, m1, m2, e1, e0)) break;
+    if (!level(l + 4, m1, m2, m0, e0, e1)) break;
+    if (!level(l + 5, m2, m0, m1, e1, e0)) break;
+  }
+}
+
+// Segment sweep for narrow level schedules (the energy slabs of 1D devices are
+// chains: a level holds the few rows of one cell), one wavefront per segment.
+// At set-up each segment's levels are cut into chunks of at most kChainRows
+// rows and kChainEnts entries; a chunk's level pointers, rows (entry range,
+// right-hand side, 1/diagonal) and entries are copied into LDS while the
+// previous chunk is swept, so the walk from level to level reads LDS only: the
+// row data, the entries, and x from the window of the last W outputs (every
+// column lies within W positions of the end of the level that reads it).  One
+// global round trip per chunk instead of one per level.  chunk[8 c ...] =
+// {first level, end level, first position, end position, first entry, end
+// entry}; chunks of segment s are [cptr[s], cptr[s + 1]).
+//   lower: zq[q] = rhs[q] - sum L[k] x(cq[k])
+//   upper: zq[q] = dq[q] (rhs[q] - sum U[k] x(cq[k]))
+constexpr int kChainRows = 32;
+constexpr int kChainEnts = 256;
+template <int LPR, typename TV, typename T, bool UPPER>
+__global__ __launch_bounds__(64) void k_chainsweep(const int *__restrict__ cptr, const int *__restrict__ chunk,
+                                                   const int *__restrict__ lvl_ptr, const int *__restrict__ lrp,
+                                                   const int *__restrict__ cq, const TV *__restrict__ v,
+                                                   const T *__restrict__ rhs, T *zq, const TV *__restrict__ dq,
+                                                   const BSys *__restrict__ bs, const int *__restrict__ seg_sys,
+                                                   int W) {
+  constexpr int NE = kChainEnts / 64, G = 64 / LPR;
+  struct Buf {
+    int lv[kChainRows + 1], kr[kChainRows + 1];
+    T rh[kChainRows], d[kChainRows];
+    int c[kChainEnts];
+    TV v[kChainEnts];
+  };
+  __shared__ Buf buf[2];
+  extern __shared__ unsigned char lds_raw[];
+  T *win = reinterpret_cast<T *>(lds_raw);
+  const int s = blockIdx.x;
+  if (bs && !bs[seg_sys ? seg_sys[s] : s].active) return;
+  const int c0 = cptr[s], c1 = cptr[s + 1];
+  if (c0 >= c1) return;
+  const int t = (int)threadIdx.x, g = t / LPR, lane = t & (LPR - 1), Wm = W - 1;
+  // a chunk in flight: this lane's level pointer, row data and entries
+  struct Fly {
+    int la, lb, qa, qb, ka, kb;
+    int lv, kr, c[NE];
+    T rh, d;
+    TV v[NE];
+  };
+  auto issue = [&](int ci, Fly &f) {
+    const int *h = chunk + 8 * ci;
+    f.la = h[0]; f.lb = h[1]; f.qa = h[2]; f.qb = h[3]; f.ka = h[4]; f.kb = h[5];
+    f.lv = t <= f.lb - f.la ? lvl_ptr[f.la + t] : 0;
+    const bool row = t <= f.qb - f.qa;
+    f.kr = row ? lrp[f.qa + t] : 0;
+    f.rh = (row && t < f.qb - f.qa) ? rhs[f.qa + t] : (T)0;
+    f.d = (UPPER && row && t < f.qb - f.qa) ? (T)dq[f.qa + t] : (T)1;
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int k = f.ka + t + 64 * u;
+      const bool ok = k < f.kb;
+      f.c[u] = ok ? cq[k] : 0;
+      f.v[u] = ok ? v[k] : (TV)0;
+    }
+  };
+  auto store = [&](const Fly &f, Buf &B) {
+    if (t <= f.lb - f.la) B.lv[t] = f.lv;
+    if (t <= f.qb - f.qa) B.kr[t] = f.kr - f.ka;
+    if (t < f.qb - f.qa) {
+      B.rh[t] = f.rh;
+      B.d[t] = f.d;
+    }
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      B.c[t + 64 * u] = f.c[u];
+      B.v[t + 64 * u] = f.v[u];
+    }
+  };
+  auto fence = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  // sweep the levels of the chunk in B (positions from qa)
+  auto walk = [&](const Buf &B, int la, int lb, int qa) {
+    for (int l = la; l < lb; ++l) {
+      const int q0 = B.lv[l - la], q1 = B.lv[l - la + 1];
+      for (int base = q0 + g; base < q1; base += G) {
+        const int i = base - qa;
+        T acc = 0;
+        for (int k = B.kr[i] + lane; k < B.kr[i + 1]; k += LPR) acc += (T)B.v[k] * win[B.c[k] & Wm];
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+        if (lane == 0) {
+          const T out = UPPER ? B.d[i] * (B.rh[i] - acc) : B.rh[i] - acc;
+          win[base & Wm] = out;
+          zq[base] = out;
+        }
+      }
+      fence();  // this level's outputs in the window before the next level reads them
+    }
+  };
+  Fly f0, f1;
+  issue(c0, f0);
+  __builtin_amdgcn_s_waitcnt(0);
+  store(f0, buf[0]);
+  fence();
+  // chunk ci is in buf[ci & 1]; the loads of chunk ci + 1 fly while it is swept
+  // (static rotation of the two register sets: a copy would wait for its loads)
+  for (int ci = c0; ci < c1; ci += 2) {
+    if (ci + 1 < c1) issue(ci + 1, f1);
+    walk(buf[(ci - c0) & 1], f0.la, f0.lb, f0.qa);
+    if (ci + 1 >= c1) break;
+    store(f1, buf[(ci + 1 - c0) & 1]);
+    fence();
+    if (ci + 2 < c1) issue(ci + 2, f0);
+    walk(buf[(ci + 1 - c0) & 1], f1.la, f1.lb, f1.qa);
+    if (ci + 2 >= c1) break;
+    store(f0, buf[(ci + 2 - c0) & 1]);
+    fence();
   }
 }
 
@@ -1711,6 +1844,13 @@ struct shesolve_ctx {
   int *d_segU = nullptr, *d_lvlU = nullptr, *d_rowsU = nullptr;
   int *d_loLrp = nullptr, *d_loUrp = nullptr;  // level-ordered row pointers of the split factor
   int seg_lpr = 4;                             // lanes per row of the segment sweeps
+  // farthest back a segment sweep reads (level end - column position, lower
+  // and upper triangle): within the LDS window the sweep reads LDS only
+  int seg_reach[2] = {0, 0};
+  std::vector<int> h_lvlq[2], h_segq[2];  // level boundaries and per-segment levels of the schedules (set-up only)
+  // chain sweeps: per triangle the chunks of kChainRows rows / kChainEnts
+  // entries and each segment's first chunk (null when some level is wider)
+  int *d_chunkL = nullptr, *d_chunkU = nullptr, *d_cptrL = nullptr, *d_cptrU = nullptr;
   int *d_mapLU = nullptr;                      // U position -> L position of the same row
   double *d_dq64 = nullptr, *d_tq64 = nullptr, *d_zq64 = nullptr;  // level-ordered 1/diag, rhs, output
   float *d_dq32 = nullptr, *d_tq32 = nullptr, *d_zq32 = nullptr;
@@ -1971,16 +2111,43 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     // in (natural) -> forward schedule order; forward sweep; -> backward order;
     // backward sweep; -> natural order
     hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsL, in, tq);
+    // the window-only form where every read of both triangles lies within the window
+    const bool win = c->seg_reach[0] <= W && c->seg_reach[1] <= W;
+    if (win && c->seg_nt == 64 && c->d_chunkL && c->d_chunkU) {
+      // chain sweeps: a window of the next power of two above the reach
+      int Wc = 64;
+      while (Wc <= std::max(c->seg_reach[0], c->seg_reach[1])) Wc *= 2;
+      const size_t ldsc = sizeof(T) * (size_t)Wc;
+#define CH_CASE(LL)                                                                                                \
+  case LL:                                                                                                         \
+    hipLaunchKernelGGL((k_chainsweep<LL, TV, T, false>), dim3(c->nseg), dim3(64), ldsc, c->stream, c->d_cptrL,      \
+                       c->d_chunkL, c->d_lvlL, c->d_loLrp, c->d_Lci, Lv, (const T *)tq, zq, (const TV *)nullptr,   \
+                       c->seg_mask, c->d_fseg_sys, Wc);                                                            \
+    hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_mapLU, (const T *)zq, tq);  \
+    hipLaunchKernelGGL((k_chainsweep<LL, TV, T, true>), dim3(c->nseg), dim3(64), ldsc, c->stream, c->d_cptrU,      \
+                       c->d_chunkU, c->d_lvlU, c->d_loUrp, c->d_Uci, Uv, (const T *)tq, zq, dq, c->seg_mask,       \
+                       c->d_fseg_sys, Wc);                                                                         \
+    break;
+      switch (c->seg_lpr) { CH_CASE(2) CH_CASE(4) default: CH_CASE(8) }
+#undef CH_CASE
+      hipLaunchKernelGGL((k_qscatter<T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsU, (const T *)zq, out);
+      return;
+    }
+#define WG_LAUNCH(NT, LL, WN)                                                                                      \
+  hipLaunchKernelGGL((k_wgsweep<NT, LL, kSweepRowsL<T>, kSegJ, TV, T, false, WN>), dim3(c->nseg), dim3(NT), lds,    \
+                     c->stream, c->d_segL, c->d_lvlL, c->d_loLrp, c->d_Lci, Lv, (const T *)tq, zq,                 \
+                     (const TV *)nullptr, c->seg_mask, c->d_fseg_sys, W);                                          \
+  hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_mapLU, (const T *)zq, tq);    \
+  hipLaunchKernelGGL((k_wgsweep<NT, LL, 2, kSegJ, TV, T, true, WN>), dim3(c->nseg), dim3(NT), lds, c->stream,      \
+                     c->d_segU, c->d_lvlU, c->d_loUrp, c->d_Uci, Uv, (const T *)tq, zq, dq, c->seg_mask,           \
+                     c->d_fseg_sys, W);
 #define WG_CASE(NT, LL)                                                                                            \
   case LL:                                                                                                         \
-    hipLaunchKernelGGL((k_wgsweep<NT, LL, kSweepRowsL<T>, kSegJ, TV, T, false>), dim3(c->nseg), dim3(NT), lds,      \
-                       c->stream,                                                                                  \
-                       c->d_segL, c->d_lvlL, c->d_loLrp, c->d_Lci, Lv, (const T *)tq, zq, (const TV *)nullptr,     \
-                       c->seg_mask, c->d_fseg_sys, W);                                                             \
-    hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_mapLU, (const T *)zq, tq);  \
-    hipLaunchKernelGGL((k_wgsweep<NT, LL, 2, kSegJ, TV, T, true>), dim3(c->nseg), dim3(NT), lds, c->stream,        \
-                       c->d_segU, c->d_lvlU, c->d_loUrp, c->d_Uci, Uv, (const T *)tq, zq, dq, c->seg_mask,         \
-                       c->d_fseg_sys, W);                                                                          \
+    if (NT == 64 && win) {                                                                                         \
+      WG_LAUNCH(64, LL, true)                                                                                      \
+    } else {                                                                                                       \
+      WG_LAUNCH(NT, LL, false)                                                                                     \
+    }                                                                                                              \
     break;
     if (c->seg_nt >= 1024) {
       switch (c->seg_lpr) { WG_CASE(1024, 2) WG_CASE(1024, 4) default: WG_CASE(1024, 8) }
@@ -1990,6 +2157,7 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
       switch (c->seg_lpr) { WG_CASE(64, 2) WG_CASE(64, 4) default: WG_CASE(64, 8) }
     }
 #undef WG_CASE
+#undef WG_LAUNCH
     hipLaunchKernelGGL((k_qscatter<T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsU, (const T *)zq, out);
     return;
   }
@@ -2099,6 +2267,7 @@ void free_matrix(shesolve_ctx *c) {
   dfree(c->lvL.d_rows); dfree(c->lvU.d_rows);
   c->lvL.ptr.clear(); c->lvU.ptr.clear();
   dfree(c->d_segL); dfree(c->d_lvlL); dfree(c->d_rowsL);
+  dfree(c->d_chunkL); dfree(c->d_chunkU); dfree(c->d_cptrL); dfree(c->d_cptrU);
   dfree(c->d_segU); dfree(c->d_lvlU); dfree(c->d_rowsU);
   dfree(c->d_loLrp); dfree(c->d_loUrp); dfree(c->d_mapLU);
   dfree(c->d_dq64); dfree(c->d_tq64); dfree(c->d_zq64); dfree(c->d_dq32); dfree(c->d_tq32); dfree(c->d_zq32);
@@ -2263,6 +2432,8 @@ int segment_levels(shesolve_ctx *c) {
     HIPCHK(hipMemcpy(dlvl, lvl.data(), sizeof(int) * lvl.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(drows, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice));
     (upper ? c->h_segrowsU : c->h_segrowsL).swap(rows);
+    c->h_lvlq[upper].swap(lvl);
+    c->h_segq[upper].swap(seg);
   }
   if (!c->seg_ptr.empty()) {
     // batch: the segment's system holds its convergence mask
@@ -2767,6 +2938,53 @@ int analyse_and_factor(shesolve_ctx *c) {
     };
     reorder(Lrp, Lci, c->h_segrowsL, loLrp);
     reorder(Urp, Uci, c->h_segrowsU, loUrp);
+    for (int upper = 0; upper < 2; ++upper) {
+      const std::vector<int> &lv = c->h_lvlq[upper], &lrp = upper ? loUrp : loLrp, &lci = upper ? Uci : Lci;
+      int reach = 0;
+      for (size_t li = 0; li + 1 < lv.size(); ++li)
+        for (int q = lv[li]; q < lv[li + 1]; ++q)
+          for (int k = lrp[q]; k < lrp[q + 1]; ++k) reach = std::max(reach, lv[li + 1] - lci[k]);
+      c->seg_reach[upper] = reach;
+    }
+    {
+      // chunks of the chain sweeps: consecutive levels of a segment while they
+      // hold at most kChainRows rows and kChainEnts entries
+      bool ok = true;
+      std::vector<int> ch[2], cp[2];
+      for (int upper = 0; upper < 2 && ok; ++upper) {
+        const std::vector<int> &lv = c->h_lvlq[upper], &sg = c->h_segq[upper], &lrp = upper ? loUrp : loLrp;
+        cp[upper].assign(1, 0);
+        for (size_t s = 0; s + 1 < sg.size() && ok; ++s) {
+          int la = sg[s];
+          while (la < sg[s + 1] && ok) {
+            int lb = la;
+            while (lb < sg[s + 1] && lv[lb + 1] - lv[la] <= kChainRows && lrp[lv[lb + 1]] - lrp[lv[la]] <= kChainEnts)
+              ++lb;
+            if (lb == la) { ok = false; break; }  // a level wider than a chunk
+            const int hd[8] = {la, lb, lv[la], lv[lb], lrp[lv[la]], lrp[lv[lb]], 0, 0};
+            ch[upper].insert(ch[upper].end(), hd, hd + 8);
+            la = lb;
+          }
+          cp[upper].push_back((int)(ch[upper].size() / 8));
+        }
+      }
+      if (ok) {
+        int **dch[2] = {&c->d_chunkL, &c->d_chunkU}, **dcp[2] = {&c->d_cptrL, &c->d_cptrU};
+        for (int upper = 0; upper < 2; ++upper) {
+          HIPCHK(hipMalloc(dch[upper], sizeof(int) * std::max<size_t>(ch[upper].size(), 8)));
+          HIPCHK(hipMalloc(dcp[upper], sizeof(int) * cp[upper].size()));
+          HIPCHK(hipMemcpy(*dch[upper], ch[upper].data(), sizeof(int) * ch[upper].size(), hipMemcpyHostToDevice));
+          HIPCHK(hipMemcpy(*dcp[upper], cp[upper].data(), sizeof(int) * cp[upper].size(), hipMemcpyHostToDevice));
+        }
+      }
+      for (int upper = 0; upper < 2; ++upper) {
+        std::vector<int>().swap(c->h_lvlq[upper]);
+        std::vector<int>().swap(c->h_segq[upper]);
+      }
+    }
+    if (getenv("SHESOLVE_VERBOSE"))
+      fprintf(stderr, "[shesolve] segment sweeps: %d segments, %d threads, reach %d / %d positions\n", c->nseg,
+              c->seg_nt, c->seg_reach[0], c->seg_reach[1]);
     {
       // lanes per row: enough that kSegJ entries per lane cover the longest
       // rows, at most kSegLprMax (longer rows loop over the rest)
@@ -2941,9 +3159,10 @@ int build_desc(shesolve_ctx *c, const int *blocks, const int *pairs, int nblk, c
 // for the row blocks `flat` ((first, end) row pairs).  Block multicolour
 // ordering only: its 32-row blocks keep consecutive natural rows together and in
 // order, so column col lies in the block starting at col - perm[col] % 32.  Per
-// row block, the sorted distinct block starts of its columns form its table
-// (when a row block needs more than kCodeSlots, no codes are built and the SpMV
-// reads the column indices); an entry's code is slot << 5 | perm[col] % 32.
+// row block, the distinct block starts of its columns, in order of first
+// appearance, form its table (when a row block needs more than kCodeSlots, no
+// codes are built and the SpMV reads the column indices); an entry's code is
+// slot << 5 | perm[col] % 32.
 int build_codes(shesolve_ctx *c, const std::vector<int> &flat) {
   free_codes(c);
   if (c->prm.precond != SHESOLVE_PRECOND_BMC_ILU0 || !c->solve_perm || c->h_perm.size() != (size_t)c->n) return 0;
@@ -2952,21 +3171,26 @@ int build_codes(shesolve_ctx *c, const std::vector<int> &flat) {
   std::vector<std::vector<int>> tabs(nb);
   std::vector<unsigned short> code((size_t)c->fnnz + kVecPad, 0);
   std::atomic<bool> ok{true};
+  const int nbn = (c->n + 31) / 32;  // natural 32-row blocks
   auto work = [&](int k0, int k1) {
-    std::vector<int> T;
+    // a column's slot in the row block's table, by its natural block: seen[]
+    // holds the row block that last met that natural block (tables are in
+    // order of first appearance)
+    std::vector<int> seen(nbn, -1);
+    std::vector<unsigned char> slot_of(nbn, 0);
     for (int k = k0; k < k1 && ok.load(std::memory_order_relaxed); ++k) {
       const int e0 = rp[flat[2 * k]], e1 = rp[flat[2 * k + 1]];
-      T.clear();
-      for (int e = e0; e < e1; ++e) T.push_back(ci[e] - (perm[ci[e]] & 31));
-      std::sort(T.begin(), T.end());
-      T.erase(std::unique(T.begin(), T.end()), T.end());
-      if ((int)T.size() > kCodeSlots) { ok = false; break; }
+      std::vector<int> &T = tabs[k];
       for (int e = e0; e < e1; ++e) {
-        const int off = perm[ci[e]] & 31;
-        const int slot = (int)(std::lower_bound(T.begin(), T.end(), ci[e] - off) - T.begin());
-        code[e] = (unsigned short)(slot << 5 | off);
+        const int nat = perm[ci[e]], nbk = nat >> 5, off = nat & 31;
+        if (seen[nbk] != k) {
+          if ((int)T.size() == kCodeSlots) { ok = false; break; }
+          seen[nbk] = k;
+          slot_of[nbk] = (unsigned char)T.size();
+          T.push_back(ci[e] - off);
+        }
+        code[e] = (unsigned short)(slot_of[nbk] << 5 | off);
       }
-      tabs[k] = T;
     }
   };
   const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
@@ -3054,7 +3278,10 @@ int setup_solve_space(shesolve_ctx *c) {
     return build_codes(c, flat);
   }
   // Entries per thread and block-to-XCD mapping of the permuted matrix, timed
-  // on it as the solve launches it (fused dot product, persistent grid).  The
+  // on it as the solve launches it (fused dot product, persistent grid), with
+  // 4-byte column indices: the column codes are built once, for the winner
+  // (with codes on every candidate the 161x81 L=3 device picked the same shape,
+  // 8 entries and the strided mapping, at 320 against 329-370 us).  The
   // natural matrix's choice does not carry over (contiguous eighths of the
   // interleaved order give each XCD its own eighth of the natural index space
   // in every colour), and plain launches do not tell: on the 161x81 L=3
@@ -3068,7 +3295,6 @@ int setup_solve_space(shesolve_ctx *c) {
   for (int ii = 0; ii < 3; ++ii) {
     build_pairs(cand[ii], flat);
     if (int e = install(flat)) return e;
-    if (int e = build_codes(c, flat)) return e;  // timed as launched: with column codes where they apply
     c->items = cand[ii];
     for (int xm = 0; xm < 2; ++xm) {
       c->xcd = xm;
