@@ -1368,12 +1368,21 @@ __global__ void k_bfinalize(int op, int S, const double *__restrict__ part, BSys
 //   1 s = r - alpha v                     a = s, c1 = r, c2 = v
 //   2 x += alpha ph + omega sh; r = s - omega t    a = x, bb = r, c1 = ph, c2 = sh, c3 = s, c4 = t
 //   3 restart: r0 = r, p = 0, v = 0       a = r0, bb = p, cw = v, c1 = r
+// (the rows are taken kBlock at a time; rsys[b], the system of rows
+// [b kBlock, (b + 1) kBlock) or -1 where they span two, lets a workgroup skip
+// the blocks of converged systems without reading their rows' systems)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bupdate(int op, int n, const int *__restrict__ row_sys,
-                                                    const BSys *__restrict__ bs, T *a, T *bb, T *cw, const T *c1,
-                                                    const T *c2, const T *c3, const T *c4) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const BSys &c = bs[row_sys[i]];
+                                                    const int *__restrict__ rsys, const BSys *__restrict__ bs, T *a,
+                                                    T *bb, T *cw, const T *c1, const T *c2, const T *c3,
+                                                    const T *c4) {
+  const int nb = (n + kBlock - 1) / kBlock;
+  for (int blk = (int)blockIdx.x; blk < nb; blk += (int)gridDim.x) {
+    const int sb = rsys[blk];
+    if (sb >= 0 && !bs[sb].active) continue;
+    const int i = blk * kBlock + (int)threadIdx.x;
+    if (i >= n) continue;
+    const BSys &c = bs[sb >= 0 ? sb : row_sys[i]];
     if (!c.active) continue;
     if (op == 0) {
       a[i] = c1[i] + (T)c.beta * (a[i] - (T)c.omega * c2[i]);
@@ -1393,11 +1402,16 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(int op, int n, const int *__
 // Masked FP64 refinement helpers (mask = int per system):
 //   0 r32 = r64 / nrm[s]      1 x += nrm[s] * d      2 dst = src (FP64 copy)     3 x = 0     4 f = 0
 __global__ __launch_bounds__(kBlock) void k_bref(int op, int n, const int *__restrict__ row_sys,
-                                                 const int *__restrict__ mask, double *__restrict__ x,
-                                                 const double *__restrict__ src, float *__restrict__ f,
-                                                 const double *__restrict__ nrm) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int s = row_sys[i];
+                                                 const int *__restrict__ rsys, const int *__restrict__ mask,
+                                                 double *__restrict__ x, const double *__restrict__ src,
+                                                 float *__restrict__ f, const double *__restrict__ nrm) {
+  const int nb = (n + kBlock - 1) / kBlock;
+  for (int blk = (int)blockIdx.x; blk < nb; blk += (int)gridDim.x) {
+    const int sb = rsys[blk];
+    if (sb >= 0 && !mask[sb]) continue;
+    const int i = blk * kBlock + (int)threadIdx.x;
+    if (i >= n) continue;
+    const int s = sb >= 0 ? sb : row_sys[i];
     if (!mask[s]) continue;
     if (op == 0) f[i] = (float)(src[i] * (nrm[s] > 0 ? 1.0 / nrm[s] : 1.0));
     else if (op == 1) x[i] += (double)f[i] * nrm[s];
@@ -1710,15 +1724,22 @@ __global__ __launch_bounds__(64) void k_chainsweep(const int *__restrict__ cptr,
 }
 
 // level-order permutations of the segment sweeps: dst[i] = src[map[i]] / dst[map[i]] = src[i]
+// (batches: a block of kBlock positions whose system has converged is
+// skipped; qsys[b] is the system of positions [b kBlock, (b + 1) kBlock), -1
+// where a block spans two systems)
 template <typename TI, typename TO>
 __global__ __launch_bounds__(kBlock) void k_qgather(int n, const int *__restrict__ map, const TI *__restrict__ src,
-                                                    TO *__restrict__ dst) {
+                                                    TO *__restrict__ dst, const BSys *__restrict__ bs = nullptr,
+                                                    const int *__restrict__ qsys = nullptr) {
+  if (bs && qsys[blockIdx.x] >= 0 && !bs[qsys[blockIdx.x]].active) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = (TO)src[map[i]];
 }
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_qscatter(int n, const int *__restrict__ map, const T *__restrict__ src,
-                                                     T *__restrict__ dst) {
+                                                     T *__restrict__ dst, const BSys *__restrict__ bs = nullptr,
+                                                     const int *__restrict__ qsys = nullptr) {
+  if (bs && qsys[blockIdx.x] >= 0 && !bs[qsys[blockIdx.x]].active) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[map[i]] = src[i];
 }
@@ -1851,6 +1872,7 @@ struct shesolve_ctx {
   // chain sweeps: per triangle the chunks of kChainRows rows / kChainEnts
   // entries and each segment's first chunk (null when some level is wider)
   int *d_chunkL = nullptr, *d_chunkU = nullptr, *d_cptrL = nullptr, *d_cptrU = nullptr;
+  int *d_qsys = nullptr;  // batches: system of each kBlock positions of the schedules (-1: two systems)
   int *d_mapLU = nullptr;                      // U position -> L position of the same row
   double *d_dq64 = nullptr, *d_tq64 = nullptr, *d_zq64 = nullptr;  // level-ordered 1/diag, rhs, output
   float *d_dq32 = nullptr, *d_tq32 = nullptr, *d_zq32 = nullptr;
@@ -2110,7 +2132,9 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     if constexpr (sizeof(TV) == 8) dq = (const TV *)c->d_dq64; else dq = (const TV *)c->d_dq32;
     // in (natural) -> forward schedule order; forward sweep; -> backward order;
     // backward sweep; -> natural order
-    hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsL, in, tq);
+    // a batch skips the positions of converged systems
+    const BSys *qbs = c->d_qsys ? c->seg_mask : nullptr;
+    hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsL, in, tq, qbs, c->d_qsys);
     // the window-only form where every read of both triangles lies within the window
     const bool win = c->seg_reach[0] <= W && c->seg_reach[1] <= W;
     if (win && c->seg_nt == 64 && c->d_chunkL && c->d_chunkU) {
@@ -2123,21 +2147,24 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     hipLaunchKernelGGL((k_chainsweep<LL, TV, T, false>), dim3(c->nseg), dim3(64), ldsc, c->stream, c->d_cptrL,      \
                        c->d_chunkL, c->d_lvlL, c->d_loLrp, c->d_Lci, Lv, (const T *)tq, zq, (const TV *)nullptr,   \
                        c->seg_mask, c->d_fseg_sys, Wc);                                                            \
-    hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_mapLU, (const T *)zq, tq);  \
+    hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_mapLU, (const T *)zq, tq,    \
+                       qbs, c->d_qsys);                                                                            \
     hipLaunchKernelGGL((k_chainsweep<LL, TV, T, true>), dim3(c->nseg), dim3(64), ldsc, c->stream, c->d_cptrU,      \
                        c->d_chunkU, c->d_lvlU, c->d_loUrp, c->d_Uci, Uv, (const T *)tq, zq, dq, c->seg_mask,       \
                        c->d_fseg_sys, Wc);                                                                         \
     break;
       switch (c->seg_lpr) { CH_CASE(2) CH_CASE(4) default: CH_CASE(8) }
 #undef CH_CASE
-      hipLaunchKernelGGL((k_qscatter<T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsU, (const T *)zq, out);
+      hipLaunchKernelGGL((k_qscatter<T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsU, (const T *)zq, out, qbs,
+                         c->d_qsys);
       return;
     }
 #define WG_LAUNCH(NT, LL, WN)                                                                                      \
   hipLaunchKernelGGL((k_wgsweep<NT, LL, kSweepRowsL<T>, kSegJ, TV, T, false, WN>), dim3(c->nseg), dim3(NT), lds,    \
                      c->stream, c->d_segL, c->d_lvlL, c->d_loLrp, c->d_Lci, Lv, (const T *)tq, zq,                 \
                      (const TV *)nullptr, c->seg_mask, c->d_fseg_sys, W);                                          \
-  hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_mapLU, (const T *)zq, tq);    \
+  hipLaunchKernelGGL((k_qgather<T, T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_mapLU, (const T *)zq, tq, qbs, \
+                     c->d_qsys);                                                                                   \
   hipLaunchKernelGGL((k_wgsweep<NT, LL, 2, kSegJ, TV, T, true, WN>), dim3(c->nseg), dim3(NT), lds, c->stream,      \
                      c->d_segU, c->d_lvlU, c->d_loUrp, c->d_Uci, Uv, (const T *)tq, zq, dq, c->seg_mask,           \
                      c->d_fseg_sys, W);
@@ -2158,7 +2185,8 @@ void launch_precond(shesolve_ctx *c, const TV * /*unused*/, const T *in, T *out)
     }
 #undef WG_CASE
 #undef WG_LAUNCH
-    hipLaunchKernelGGL((k_qscatter<T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsU, (const T *)zq, out);
+    hipLaunchKernelGGL((k_qscatter<T>), dim3(g), dim3(kBlock), 0, c->stream, n, c->d_rowsU, (const T *)zq, out, qbs,
+                       c->d_qsys);
     return;
   }
   T *z = out;
@@ -2267,7 +2295,7 @@ void free_matrix(shesolve_ctx *c) {
   dfree(c->lvL.d_rows); dfree(c->lvU.d_rows);
   c->lvL.ptr.clear(); c->lvU.ptr.clear();
   dfree(c->d_segL); dfree(c->d_lvlL); dfree(c->d_rowsL);
-  dfree(c->d_chunkL); dfree(c->d_chunkU); dfree(c->d_cptrL); dfree(c->d_cptrU);
+  dfree(c->d_chunkL); dfree(c->d_chunkU); dfree(c->d_cptrL); dfree(c->d_cptrU); dfree(c->d_qsys);
   dfree(c->d_segU); dfree(c->d_lvlU); dfree(c->d_rowsU);
   dfree(c->d_loLrp); dfree(c->d_loUrp); dfree(c->d_mapLU);
   dfree(c->d_dq64); dfree(c->d_tq64); dfree(c->d_zq64); dfree(c->d_dq32); dfree(c->d_tq32); dfree(c->d_zq32);
@@ -2976,6 +3004,18 @@ int analyse_and_factor(shesolve_ctx *c) {
           HIPCHK(hipMemcpy(*dch[upper], ch[upper].data(), sizeof(int) * ch[upper].size(), hipMemcpyHostToDevice));
           HIPCHK(hipMemcpy(*dcp[upper], cp[upper].data(), sizeof(int) * cp[upper].size(), hipMemcpyHostToDevice));
         }
+      }
+      if (!c->fseg_sys.empty()) {
+        // positions of segment s: [lv[sg[s]], lv[sg[s + 1]]) in both schedules
+        const std::vector<int> &lv = c->h_lvlq[0], &sg = c->h_segq[0];
+        std::vector<int> qs((n + kBlock - 1) / kBlock, -2);
+        for (size_t s = 0; s + 1 < sg.size(); ++s) {
+          const int sy = c->fseg_sys[s], q0 = lv[sg[s]], q1 = lv[sg[s + 1]];
+          for (int b = q0 / kBlock; q0 < q1 && b <= (q1 - 1) / kBlock; ++b) qs[b] = (qs[b] == -2 || qs[b] == sy) ? sy : -1;
+        }
+        for (int &e : qs) e = e == -2 ? -1 : e;
+        HIPCHK(hipMalloc(&c->d_qsys, sizeof(int) * qs.size()));
+        HIPCHK(hipMemcpy(c->d_qsys, qs.data(), sizeof(int) * qs.size(), hipMemcpyHostToDevice));
       }
       for (int upper = 0; upper < 2; ++upper) {
         std::vector<int>().swap(c->h_lvlq[upper]);
@@ -3907,6 +3947,7 @@ struct shesolve_batch_ctx {
   std::vector<int> sys_ptr;        // row offsets of the systems
   std::vector<int64_t> val_ptr;    // value offsets of the systems
   int *d_sys_ptr = nullptr, *d_row_sys = nullptr, *d_mask = nullptr, *d_caps = nullptr;
+  int *d_rsys = nullptr;  // system of each kBlock rows (-1: two systems)
   BSys *d_bs = nullptr;
   double *d_bpart = nullptr, *d_nrm = nullptr;
   double *x64 = nullptr, *b64 = nullptr, *xprev = nullptr;
@@ -3926,7 +3967,7 @@ void batch_free(shesolve_batch_ctx *b) {
   if (b->c) b->c->seg_mask = nullptr;
   for (auto &g : b->graph)
     if (g) { hipGraphExecDestroy(g); g = nullptr; }
-  dfree(b->d_sys_ptr); dfree(b->d_row_sys); dfree(b->d_mask); dfree(b->d_caps); dfree(b->d_bs);
+  dfree(b->d_sys_ptr); dfree(b->d_row_sys); dfree(b->d_mask); dfree(b->d_caps); dfree(b->d_bs); dfree(b->d_rsys);
   dfree(b->d_bpart); dfree(b->d_nrm); dfree(b->x64); dfree(b->b64); dfree(b->xprev); dfree(b->dd);
 }
 
@@ -3949,12 +3990,12 @@ template <typename T>
 void bupd(shesolve_batch_ctx *b, int op, T *a, T *bb, T *cw, const T *c1, const T *c2, const T *c3, const T *c4) {
   shesolve_ctx *c = b->c;
   KTimer kt(c, 3);
-  hipLaunchKernelGGL((k_bupdate<T>), dim3(kGrid), dim3(kBlock), 0, c->stream, op, c->n, b->d_row_sys, b->d_bs, a, bb,
-                     cw, c1, c2, c3, c4);
+  hipLaunchKernelGGL((k_bupdate<T>), dim3(kGrid), dim3(kBlock), 0, c->stream, op, c->n, b->d_row_sys, b->d_rsys,
+                     b->d_bs, a, bb, cw, c1, c2, c3, c4);
 }
 void bref(shesolve_batch_ctx *b, int op, double *x, const double *src, float *f) {
-  hipLaunchKernelGGL(k_bref, dim3(kGrid), dim3(kBlock), 0, b->c->stream, op, b->c->n, b->d_row_sys, b->d_mask, x, src,
-                     f, b->d_nrm);
+  hipLaunchKernelGGL(k_bref, dim3(kGrid), dim3(kBlock), 0, b->c->stream, op, b->c->n, b->d_row_sys, b->d_rsys,
+                     b->d_mask, x, src, f, b->d_nrm);
 }
 int upload_mask(shesolve_batch_ctx *b, const std::vector<int> &m) {
   HIPCHK(hipMemcpyAsync(b->d_mask, m.data(), sizeof(int) * b->S, hipMemcpyHostToDevice, b->c->stream));
@@ -4363,6 +4404,15 @@ int batch_set_systems(shesolve_batch_ctx *b, bool device_src, int count, const i
   HIPCHK(hipMalloc(&b->dd, sizeof(float) * N));
   HIPCHK(hipMemcpy(b->d_sys_ptr, b->sys_ptr.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_row_sys, row_sys.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+  {
+    std::vector<int> rs((N + kBlock - 1) / kBlock);
+    for (size_t blk = 0; blk < rs.size(); ++blk) {
+      const int i0 = (int)blk * kBlock, i1 = (int)std::min<int64_t>(N, i0 + kBlock);
+      rs[blk] = row_sys[i0] == row_sys[i1 - 1] ? row_sys[i0] : -1;
+    }
+    HIPCHK(hipMalloc(&b->d_rsys, sizeof(int) * rs.size()));
+    HIPCHK(hipMemcpy(b->d_rsys, rs.data(), sizeof(int) * rs.size(), hipMemcpyHostToDevice));
+  }
   HIPCHK(hipMemset(b->d_bs, 0, sizeof(BSys) * count));
   c->seg_mask = b->d_bs;
   b->last_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
