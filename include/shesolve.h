@@ -141,8 +141,8 @@ int shesolve_get_factors(shesolve_handle h, int64_t *row_ptr, int32_t *col, doub
  * XCD-contiguous block mapping (0/1); and the nine timings it compared (FP32
  * SpMV, microseconds; items 4/8/16 x mapping strided/contiguous, then vector
  * CSR with 4/8/16 lanes per row; with a multicolour factor the first six are
- * those of the permuted matrix's SpMV with its fused dot product, as the solve
- * launches it).  Any pointer may be NULL. */
+ * those of the permuted matrix's SpMV with its fused dot product and 4-byte
+ * column indices).  Any pointer may be NULL. */
 int shesolve_stream_config(shesolve_handle h, int32_t *items, int32_t *xcd, double *tune_us9);
 /* Row ordering of the factor (new -> old); the identity unless MC_ILU0. */
 int shesolve_get_ordering(shesolve_handle h, int32_t *perm);
