@@ -2679,9 +2679,25 @@ void bmc_part_values(shesolve_ctx *c, const shesolve_ctx::BmcPart &P, const doub
                        P.d_icol, src, dinv, P.d_iv64, P.d_iv32);
 }
 
+// SHESOLVE_VERBOSE: durations of the set-up phases on stderr
+struct PhaseClock {
+  bool on = getenv("SHESOLVE_VERBOSE") != nullptr;
+  hipStream_t stream;
+  std::chrono::steady_clock::time_point tp = std::chrono::steady_clock::now();
+  explicit PhaseClock(hipStream_t s) : stream(s) {}
+  void operator()(const char *what) {
+    if (!on) return;
+    hipStreamSynchronize(stream);
+    const auto t = std::chrono::steady_clock::now();
+    fprintf(stderr, "[shesolve] set-up %s: %.1f ms\n", what, std::chrono::duration<double, std::milli>(t - tp).count());
+    tp = t;
+  }
+};
+
 int analyse_and_factor(shesolve_ctx *c) {
   const int n = c->n;
   if (n <= 0) return 0;
+  PhaseClock phase(c->stream);
   // factor pattern (in the factor's own index space) and the gather map from A
   if (c->prm.precond == SHESOLVE_PRECOND_NONE) return 0;
   std::vector<int> pos, perm;
@@ -2823,6 +2839,7 @@ int analyse_and_factor(shesolve_ctx *c) {
     c->h_frp = c->h_rp;
     c->h_fci = c->h_ci;
   }
+  phase("  factor pattern");
   c->factor_is_A = pos.empty();
   c->h_perm = perm;
   c->fnnz = c->h_frp[n];
@@ -2842,14 +2859,17 @@ int analyse_and_factor(shesolve_ctx *c) {
     set_err("ILU0: row length %d exceeds %d", maxlen, kMaxRowLDS);
     return 2;
   }
+  phase("  diagonal");
   std::vector<int> Lptr, Lrows, Uptr, Urows;
   compute_levels(c->h_frp, c->h_fci, c->h_fdg, n, Lptr, Lrows, Uptr, Urows);
+  phase("  levels");
   if (perm.empty()) {
     // factor segments (systems of a batch, per-energy slabs or the whole
     // matrix): one workgroup walks each segment's levels, unless one launch
     // per level of the whole matrix is estimated faster
     if (segment_sweeps_pay(c, Lptr, Lrows))
       if (int e = segment_levels(c)) return e;
+    phase("  segment levels");
   }
   if (!perm.empty()) {
     // multicolour ordering: colour q occupies the contiguous range
@@ -2928,6 +2948,7 @@ int analyse_and_factor(shesolve_ctx *c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMalloc(&c->d_f64, sizeof(double) * std::max<int64_t>(c->fnnz, 1)));
   c->flpr = pick_lpr((double)c->fnnz / std::max(n, 1) / 2.0);
+  phase("  uploads");
   // split pattern of the factor
   std::vector<int> Lrp(n + 1, 0), Urp(n + 1, 0), Lci, Uci;
   Lci.reserve(c->fnnz / 2 + 1);
@@ -2964,8 +2985,10 @@ int analyse_and_factor(shesolve_ctx *c) {
       rp.swap(start);
       ci.swap(lci);
     };
+    phase("  split");
     reorder(Lrp, Lci, c->h_segrowsL, loLrp);
     reorder(Urp, Uci, c->h_segrowsU, loUrp);
+    phase("  level order");
     for (int upper = 0; upper < 2; ++upper) {
       const std::vector<int> &lv = c->h_lvlq[upper], &lrp = upper ? loUrp : loLrp, &lci = upper ? Uci : Lci;
       int reach = 0;
@@ -3022,6 +3045,7 @@ int analyse_and_factor(shesolve_ctx *c) {
         std::vector<int>().swap(c->h_segq[upper]);
       }
     }
+    phase("  reach, chunks");
     if (getenv("SHESOLVE_VERBOSE"))
       fprintf(stderr, "[shesolve] segment sweeps: %d segments, %d threads, reach %d / %d positions\n", c->nseg,
               c->seg_nt, c->seg_reach[0], c->seg_reach[1]);
@@ -3053,6 +3077,7 @@ int analyse_and_factor(shesolve_ctx *c) {
     HIPCHK(hipMemcpy(c->d_loLrp, loLrp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_loUrp, loUrp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
   }
+  phase("  segment uploads");
   // contiguous levels get CSR-stream row blocks
   auto runs = [&](const Levels &lv, const std::vector<int> &rows, const std::vector<int> &rp,
                   std::vector<shesolve_ctx::LevelRun> &out, std::vector<int> &blk) {
@@ -3919,7 +3944,19 @@ int set_matrix_body(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, co
     rp = hrp.data();
     ci = hci.data();
   }
+  // SHESOLVE_VERBOSE: the duration of each set-up phase on stderr
+  static const bool verbose = getenv("SHESOLVE_VERBOSE") != nullptr;
+  auto tp = t0;
+  auto phase = [&](const char *what) {
+    if (!verbose) return;
+    hipStreamSynchronize(c->stream);
+    const auto t = std::chrono::steady_clock::now();
+    fprintf(stderr, "[shesolve] set-up %s: %.1f ms\n", what, std::chrono::duration<double, std::milli>(t - tp).count());
+    tp = t;
+  };
+  phase("pattern copy");
   if (int e = set_pattern_host(c, n, nnz, rp, ci, nblocks, ranges)) return e;
+  phase("pattern");
   HIPCHK(hipMalloc(&c->d_rp, sizeof(int) * (n + 1)));
   HIPCHK(hipMalloc(&c->d_ci, sizeof(int) * (std::max<int64_t>(nnz, 1) + kVecPad)));
   HIPCHK(hipMalloc(&c->d_v64, sizeof(double) * (std::max<int64_t>(nnz, 1) + kVecPad)));
@@ -3928,10 +3965,15 @@ int set_matrix_body(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, co
   HIPCHK(hipMemcpyAsync(c->d_v64, val, sizeof(double) * nnz, val_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         c->stream));
   c->lpr = c->prm.lanes_per_row > 0 ? c->prm.lanes_per_row : pick_lpr((double)nnz / n);
+  phase("upload");
   if (int e = tune_stream(c)) return e;
+  phase("stream tuning");
   if (int e = analyse_and_factor(c)) return e;
+  phase("analysis");
   if (int e = setup_solve_space(c)) return e;
+  phase("solve space");
   if (int e = upload_values(c)) return e;
+  phase("values");
   if (int e = ensure_vectors(c)) return e;
   c->last_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
