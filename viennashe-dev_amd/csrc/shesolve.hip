@@ -1515,11 +1515,6 @@ __global__ __launch_bounds__(NT) void k_wgsweep(const int *__restrict__ seg_lvl,
         E.v[r][j] = kk ? v[k] : (TV)0;
       }
   };
-  // x of position qc: the window for positions >= lo, zq otherwise
-  auto xval = [&](int qc, int lo) -> T {
-    if constexpr (WIN) return win[qc & Wm];
-    else return qc >= lo ? win[qc & Wm] : zq[qc];
-  };
   // level pointers of the segment, staged in LDS kLvlChunk levels at a time;
   // the chunk always holds levels l .. l + 3 while level l is swept (so they
   // are read from LDS: a pointer that may be global or LDS would compile to
@@ -1528,6 +1523,17 @@ __global__ __launch_bounds__(NT) void k_wgsweep(const int *__restrict__ seg_lvl,
   for (int t = threadIdx.x; t <= kLvlChunk && lc + t <= l1; t += NT) lp[t] = lvl_ptr[lc + t];
   __syncthreads();
   auto qp = [&](int l) { return lp[l - lc]; };
+  // x of position qc: the window for positions >= lo, zq otherwise.  zq is
+  // read through a buffer resource over the segment's positions (the host
+  // keeps a segment under 2 GiB): an LDS-or-global choice of address would
+  // compile to a flat load, whose wait drains the LDS and the global counter
+  // together, prefetched rows included.
+  const int qs0 = lp[0];
+  const __amdgpu_buffer_rsrc_t rz = buf_rsrc(zq + qs0, (int64_t)(lvl_ptr[l1] - qs0) * (int64_t)sizeof(T));
+  auto xval = [&](int qc, int lo) -> T {
+    if constexpr (WIN) return win[qc & Wm];
+    else return qc >= lo ? win[qc & Wm] : buf_load<T>(rz, (qc - qs0) * (int)sizeof(T));
+  };
   int q_safe = lp[0];  // positions below q_safe are visible in zq to every wave
   // level l with the row data of l (cur) and l + 1 (nxt) and the entries of l
   // (ecur) in registers; loads ahead (l + 2) and enxt (l + 1).  The register
@@ -2364,6 +2370,9 @@ bool segment_sweeps_pay(shesolve_ctx *c, const std::vector<int> &Lptr, const std
   const auto &sp = c->fseg_ptr;
   const int S = (int)sp.size() - 1, nl = (int)Lptr.size() - 1;
   if (S < 1 || nl < 1) return false;
+  // the sweeps address a segment's FP64 outputs through 32-bit buffer offsets
+  for (int sg = 0; sg < S; ++sg)
+    if ((int64_t)(sp[sg + 1] - sp[sg]) * (int64_t)sizeof(double) >= ((int64_t)1 << 31)) return false;
   std::vector<double> ent(S, 0.0);
   std::vector<int> lev(S, 0), rows(S, 0);
   std::vector<int> touched;
