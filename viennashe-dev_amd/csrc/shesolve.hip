@@ -27,6 +27,7 @@
 #include <queue>
 #include <thread>
 #include <atomic>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -2327,6 +2328,22 @@ void make_blocks(const int *rp, int r0, int r1, int cap, std::vector<int> &out, 
 }
 
 // Level sets of the lower (forward) and upper (backward) triangles of the factor pattern.
+// Runs f(i0, i1) over [0, n) cut into contiguous ranges, one per host thread
+// (at most 16, and none shorter than min_per_thread items).
+template <typename F>
+void host_parallel(int64_t n, F &&f, int64_t min_per_thread = 1 << 15) {
+  const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(hw, n / std::max<int64_t>(min_per_thread, 1)));
+  if (nt <= 1) {
+    if (n > 0) f((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back([&f, n, nt, t] { f(n * t / nt, n * (t + 1) / nt); });
+  f((int64_t)0, n / nt);
+  for (auto &x : th) x.join();
+}
+
 void compute_levels(const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<int> &dg, int n,
                     std::vector<int> &Lptr, std::vector<int> &Lrows, std::vector<int> &Uptr, std::vector<int> &Urows) {
   std::vector<int> lev(n);
@@ -2607,33 +2624,67 @@ int build_bmc_part(shesolve_ctx *c, const std::vector<int> &bst, bool upper, con
   }
   std::vector<int> orp(n + 1, 0), oci, opos, iptr(nb + 1, 0), ipos, icol, bK(nb + 1, 0);
   std::vector<unsigned short> icode;
-  oci.reserve(rp[n]);
-  opos.reserve(rp[n]);
+  // first block of each block's colour (the colours are consecutive block ranges)
+  std::vector<int> bq0(nb, 0);
   for (int q = 0; q + 1 < (int)c->bmc_cptr.size(); ++q)
-    for (int blk = c->bmc_cptr[q]; blk < c->bmc_cptr[q + 1]; ++blk) {
-      const int sub = (blk - c->bmc_cptr[q]) % kBmcGroupMax, r0 = bst[blk], r1 = bst[blk + 1];
-      bK[blk] = (int)oci.size();
+    for (int blk = c->bmc_cptr[q]; blk < c->bmc_cptr[q + 1]; ++blk) bq0[blk] = c->bmc_cptr[q];
+  // two passes over the blocks in parallel: count each row's coupling entries
+  // and each block's own entries, then fill at the prefix sums
+  std::atomic<int64_t> bad{-1};
+  host_parallel(nb, [&](int64_t b0, int64_t b1) {
+    for (int blk = (int)b0; blk < (int)b1; ++blk) {
+      const int r0 = bst[blk], r1 = bst[blk + 1];
+      int in = 0;
       for (int r = r0; r < r1; ++r) {
+        int out = 0;
         for (int k = rp[r]; k < rp[r + 1]; ++k) {
           const int j = ci[k];
           if (j >= r0 && j < r1) {
-            if (upper ? j <= r : j >= r) {
-              set_err("block multicolour layout: entry (%d, %d) is not in its triangle", r, j);
-              return -1;
-            }
-            icode.push_back((unsigned short)(sub * kBmcTile + bmc_slot(upper, j - r0, r - r0)));
-            ipos.push_back(k);
-            icol.push_back(j);
+            if (upper ? j <= r : j >= r) bad = ((int64_t)r << 32) | (uint32_t)j;
+            ++in;
           } else {
-            oci.push_back(j);
-            opos.push_back(k);
+            ++out;
           }
         }
-        orp[r + 1] = (int)oci.size();
+        orp[r + 1] = out;
       }
-      iptr[blk + 1] = (int)ipos.size();
+      iptr[blk + 1] = in;
     }
-  bK[nb] = (int)oci.size();
+  }, 1024);
+  if (bad >= 0) {
+    set_err("block multicolour layout: entry (%d, %d) is not in its triangle", (int)(bad >> 32), (int)(bad & 0xffffffff));
+    return -1;
+  }
+  for (int r = 0; r < n; ++r) orp[r + 1] += orp[r];
+  for (int blk = 0; blk < nb; ++blk) iptr[blk + 1] += iptr[blk];
+  for (int blk = 0; blk <= nb; ++blk) bK[blk] = orp[bst[blk]];
+  oci.resize(orp[n]);
+  opos.resize(orp[n]);
+  icode.resize(iptr[nb]);
+  ipos.resize(iptr[nb]);
+  icol.resize(iptr[nb]);
+  host_parallel(nb, [&](int64_t b0, int64_t b1) {
+    for (int blk = (int)b0; blk < (int)b1; ++blk) {
+      const int sub = (blk - bq0[blk]) % kBmcGroupMax, r0 = bst[blk], r1 = bst[blk + 1];
+      int wi = iptr[blk];
+      for (int r = r0; r < r1; ++r) {
+        int wo = orp[r];
+        for (int k = rp[r]; k < rp[r + 1]; ++k) {
+          const int j = ci[k];
+          if (j >= r0 && j < r1) {
+            icode[wi] = (unsigned short)(sub * kBmcTile + bmc_slot(upper, j - r0, r - r0));
+            ipos[wi] = k;
+            icol[wi] = j;
+            ++wi;
+          } else {
+            oci[wo] = j;
+            opos[wo] = k;
+            ++wo;
+          }
+        }
+      }
+    }
+  }, 1024);
   P.no = (int64_t)oci.size();
   P.ni = (int64_t)ipos.size();
   // the sweep addresses these arrays through 32-bit buffer offsets
@@ -2766,19 +2817,19 @@ int analyse_and_factor(shesolve_ctx *c) {
     // form colours blocks of kBmcRows consecutive rows instead (two blocks
     // conflict when an entry links them); a block keeps its rows together and
     // in natural order, and the blocks of one colour are independent.
-    std::vector<int> trp(n + 1, 0), tci(c->h_ci.size());
-    for (int i = 0; i < n; ++i)
-      for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) trp[c->h_ci[k] + 1]++;
-    for (int i = 0; i < n; ++i) trp[i + 1] += trp[i];
-    {
-      std::vector<int> fillp(trp.begin(), trp.end() - 1);
-      for (int i = 0; i < n; ++i)
-        for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) tci[fillp[c->h_ci[k]]++] = i;
-    }
     std::vector<int> colour(n, -1);
     std::vector<int> mark(1024, -1);
     int ncol = 0;
     if (c->prm.precond == SHESOLVE_PRECOND_MC_ILU0) {
+      std::vector<int> trp(n + 1, 0), tci(c->h_ci.size());
+      for (int i = 0; i < n; ++i)
+        for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) trp[c->h_ci[k] + 1]++;
+      for (int i = 0; i < n; ++i) trp[i + 1] += trp[i];
+      {
+        std::vector<int> fillp(trp.begin(), trp.end() - 1);
+        for (int i = 0; i < n; ++i)
+          for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) tci[fillp[c->h_ci[k]]++] = i;
+      }
       for (int i = 0; i < n; ++i) {
         for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) {
           const int cj = colour[c->h_ci[k]];
@@ -2794,24 +2845,49 @@ int analyse_and_factor(shesolve_ctx *c) {
         ncol = std::max(ncol, q + 1);
       }
     } else {
+      // on the graph of the blocks: the blocks that the entries of a block's
+      // rows reach (built in parallel), and for each block the earlier blocks
+      // whose entries reach it; a block takes the smallest colour that no
+      // earlier neighbour in either direction holds
       const int nb = (n + kBmcRows - 1) / kBmcRows;
-      for (int bi = 0; bi < nb; ++bi) {
-        const int i0 = bi * kBmcRows, i1 = std::min(n, i0 + kBmcRows);
-        for (int i = i0; i < i1; ++i) {
-          for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) {
-            const int j = c->h_ci[k];
-            if ((j < i0 || j >= i1) && colour[j] >= 0) mark[colour[j]] = bi;
+      std::vector<std::vector<int>> reach(nb);
+      host_parallel(nb, [&](int64_t b0, int64_t b1) {
+        std::vector<int> tmp;
+        for (int bi = (int)b0; bi < (int)b1; ++bi) {
+          tmp.clear();
+          const int i0 = bi * kBmcRows, i1 = std::min(n, i0 + kBmcRows);
+          for (int k = c->h_rp[i0]; k < c->h_rp[i1]; ++k) {
+            const int bj = c->h_ci[k] / kBmcRows;
+            if (bj != bi) tmp.push_back(bj);
           }
-          for (int k = trp[i]; k < trp[i + 1]; ++k) {
-            const int j = tci[k];
-            if ((j < i0 || j >= i1) && colour[j] >= 0) mark[colour[j]] = bi;
-          }
+          std::sort(tmp.begin(), tmp.end());
+          tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+          reach[bi] = tmp;
         }
+      }, 1024);
+      std::vector<int> tptr(nb + 1, 0), tsrc;
+      for (int bj = 0; bj < nb; ++bj)
+        for (int b : reach[bj])
+          if (b > bj) tptr[b + 1]++;
+      for (int b = 0; b < nb; ++b) tptr[b + 1] += tptr[b];
+      tsrc.resize(tptr[nb]);
+      {
+        std::vector<int> fillp(tptr.begin(), tptr.end() - 1);
+        for (int bj = 0; bj < nb; ++bj)
+          for (int b : reach[bj])
+            if (b > bj) tsrc[fillp[b]++] = bj;
+      }
+      std::vector<int> bcol(nb, -1);
+      for (int bi = 0; bi < nb; ++bi) {
+        for (int b : reach[bi])
+          if (b < bi) mark[bcol[b]] = bi;
+        for (int k = tptr[bi]; k < tptr[bi + 1]; ++k) mark[bcol[tsrc[k]]] = bi;
         int q = 0;
         while (q < 1023 && mark[q] == bi) ++q;
-        for (int i = i0; i < i1; ++i) colour[i] = q;
+        bcol[bi] = q;
         ncol = std::max(ncol, q + 1);
       }
+      for (int i = 0; i < n; ++i) colour[i] = bcol[i / kBmcRows];
     }
     c->ncolors = ncol;
     c->h_colour = colour;
@@ -2826,22 +2902,24 @@ int analyse_and_factor(shesolve_ctx *c) {
       iperm[i] = p;
     }
     c->h_frp.assign(n + 1, 0);
+    for (int p = 0; p < n; ++p) c->h_frp[p + 1] = c->h_frp[p] + (c->h_rp[perm[p] + 1] - c->h_rp[perm[p]]);
     c->h_fci.resize(c->h_ci.size());
     pos.resize(c->h_ci.size());
-    std::vector<std::pair<int, int>> tmp;
-    int w = 0;
-    for (int p = 0; p < n; ++p) {
-      const int i = perm[p];
-      tmp.clear();
-      for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) tmp.emplace_back(iperm[c->h_ci[k]], k);
-      std::sort(tmp.begin(), tmp.end());
-      for (auto &e : tmp) {
-        c->h_fci[w] = e.first;
-        pos[w] = e.second;
-        ++w;
+    host_parallel(n, [&](int64_t p0, int64_t p1) {
+      std::vector<std::pair<int, int>> tmp;
+      for (int p = (int)p0; p < (int)p1; ++p) {
+        const int i = perm[p];
+        tmp.clear();
+        for (int k = c->h_rp[i]; k < c->h_rp[i + 1]; ++k) tmp.emplace_back(iperm[c->h_ci[k]], k);
+        std::sort(tmp.begin(), tmp.end());
+        int w = c->h_frp[p];
+        for (auto &e : tmp) {
+          c->h_fci[w] = e.first;
+          pos[w] = e.second;
+          ++w;
+        }
       }
-      c->h_frp[p + 1] = w;
-    }
+    });
   } else if (c->prm.precond == SHESOLVE_PRECOND_ILUK && (c->prm.fill_level > 0 || !c->seg_fill.empty())) {
     if (int e = iluk_pattern(c, pos, false)) return e;
   } else {
@@ -2853,15 +2931,25 @@ int analyse_and_factor(shesolve_ctx *c) {
   c->h_perm = perm;
   c->fnnz = c->h_frp[n];
   c->h_fdg.assign(n, -1);
-  int maxlen = 0;
-  for (int i = 0; i < n; ++i) {
-    maxlen = std::max(maxlen, c->h_frp[i + 1] - c->h_frp[i]);
-    for (int k = c->h_frp[i]; k < c->h_frp[i + 1]; ++k)
-      if (c->h_fci[k] == i) { c->h_fdg[i] = k; break; }
-    if (c->h_fdg[i] < 0) {
-      set_err("ILU0: row %d has no diagonal entry", i);
-      return 2;
-    }
+  int maxlen = 0, nodiag = n;
+  {
+    std::mutex mu;
+    host_parallel(n, [&](int64_t i0, int64_t i1) {
+      int ml = 0, nd = n;
+      for (int i = (int)i0; i < (int)i1; ++i) {
+        ml = std::max(ml, c->h_frp[i + 1] - c->h_frp[i]);
+        for (int k = c->h_frp[i]; k < c->h_frp[i + 1]; ++k)
+          if (c->h_fci[k] == i) { c->h_fdg[i] = k; break; }
+        if (c->h_fdg[i] < 0) nd = std::min(nd, i);
+      }
+      std::lock_guard<std::mutex> g(mu);
+      maxlen = std::max(maxlen, ml);
+      nodiag = std::min(nodiag, nd);
+    });
+  }
+  if (nodiag < n) {
+    set_err("ILU0: row %d has no diagonal entry", nodiag);
+    return 2;
   }
   c->fmaxlen = maxlen;
   if (maxlen > kMaxRowLDS) {
@@ -2870,16 +2958,7 @@ int analyse_and_factor(shesolve_ctx *c) {
   }
   phase("  diagonal");
   std::vector<int> Lptr, Lrows, Uptr, Urows;
-  compute_levels(c->h_frp, c->h_fci, c->h_fdg, n, Lptr, Lrows, Uptr, Urows);
-  phase("  levels");
-  if (perm.empty()) {
-    // factor segments (systems of a batch, per-energy slabs or the whole
-    // matrix): one workgroup walks each segment's levels, unless one launch
-    // per level of the whole matrix is estimated faster
-    if (segment_sweeps_pay(c, Lptr, Lrows))
-      if (int e = segment_levels(c)) return e;
-    phase("  segment levels");
-  }
+  bool colour_levels = false;
   if (!perm.empty()) {
     // multicolour ordering: colour q occupies the contiguous range
     // [cstart[q], cstart[q+1]) of the factor; use whole colours as levels
@@ -2891,10 +2970,13 @@ int analyse_and_factor(shesolve_ctx *c) {
       colour_of[p] = (int)cstart.size() - 1;
     }
     cstart.push_back(n);
-    bool valid = true;
-    for (int p = 0; p < n && valid; ++p)
-      for (int k = c->h_frp[p]; k < c->h_frp[p + 1]; ++k)
-        if (c->h_fci[k] != p && colour_of[c->h_fci[k]] == colour_of[p]) { valid = false; break; }
+    std::atomic<bool> valid{true};
+    host_parallel(n, [&](int64_t p0, int64_t p1) {
+      for (int p = (int)p0; p < (int)p1 && valid.load(std::memory_order_relaxed); ++p)
+        for (int k = c->h_frp[p]; k < c->h_frp[p + 1]; ++k)
+          if (c->h_fci[k] != p && colour_of[c->h_fci[k]] == colour_of[p]) { valid = false; break; }
+    });
+    colour_levels = valid;
     if (valid) {
       const int nq = (int)cstart.size() - 1;
       Lptr.assign(cstart.begin(), cstart.end());
@@ -2907,6 +2989,16 @@ int analyse_and_factor(shesolve_ctx *c) {
         Uptr.push_back((int)Urows.size());
       }
     }
+  }
+  if (!colour_levels) compute_levels(c->h_frp, c->h_fci, c->h_fdg, n, Lptr, Lrows, Uptr, Urows);
+  phase("  levels");
+  if (perm.empty()) {
+    // factor segments (systems of a batch, per-energy slabs or the whole
+    // matrix): one workgroup walks each segment's levels, unless one launch
+    // per level of the whole matrix is estimated faster
+    if (segment_sweeps_pay(c, Lptr, Lrows))
+      if (int e = segment_levels(c)) return e;
+    phase("  segment levels");
   }
   std::vector<int> bst;
   if (!perm.empty() && c->prm.precond == SHESOLVE_PRECOND_BMC_ILU0) {
@@ -2960,15 +3052,19 @@ int analyse_and_factor(shesolve_ctx *c) {
   phase("  uploads");
   // split pattern of the factor
   std::vector<int> Lrp(n + 1, 0), Urp(n + 1, 0), Lci, Uci;
-  Lci.reserve(c->fnnz / 2 + 1);
-  Uci.reserve(c->fnnz / 2 + 1);
   for (int i = 0; i < n; ++i) {
-    for (int k = c->h_frp[i]; k < c->h_fdg[i]; ++k) Lci.push_back(c->h_fci[k]);
-    for (int k = c->h_fdg[i] + 1; k < c->h_frp[i + 1]; ++k) Uci.push_back(c->h_fci[k]);
-    Lrp[i + 1] = (int)Lci.size();
-    Uci.size();
-    Urp[i + 1] = (int)Uci.size();
+    Lrp[i + 1] = Lrp[i] + (c->h_fdg[i] - c->h_frp[i]);
+    Urp[i + 1] = Urp[i] + (c->h_frp[i + 1] - c->h_fdg[i] - 1);
   }
+  Lci.resize(Lrp[n]);
+  Uci.resize(Urp[n]);
+  host_parallel(n, [&](int64_t i0, int64_t i1) {
+    const auto f = c->h_fci.begin();
+    for (int i = (int)i0; i < (int)i1; ++i) {
+      std::copy(f + c->h_frp[i], f + c->h_fdg[i], Lci.begin() + Lrp[i]);
+      std::copy(f + c->h_fdg[i] + 1, f + c->h_frp[i + 1], Uci.begin() + Urp[i]);
+    }
+  });
   c->nL = Lci.size();
   c->nU = Uci.size();
   std::vector<int> loLrp, loUrp;
@@ -3782,15 +3878,27 @@ int set_pattern_host(shesolve_ctx *c, int64_t n, int64_t nnz, const int64_t *rp,
   for (int64_t i = 0; i < n; ++i)
     if (rp[i + 1] < rp[i]) { set_err("row_ptr decreases at row %lld", (long long)i); return 4; }
   c->h_rp.resize(n + 1);
-  for (int64_t i = 0; i <= n; ++i) c->h_rp[i] = (int)rp[i];
-  c->h_ci.assign(ci, ci + nnz);
-  for (int64_t i = 0; i < n; ++i)
-    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
-      if (ci[k] < 0 || ci[k] >= n || (k > rp[i] && ci[k] <= ci[k - 1])) {
-        set_err("column indices must be in range and strictly ascending per row (row %lld)", (long long)i);
-        return 5;
-      }
+  if (ci != c->h_ci.data()) c->h_ci.resize(nnz);
+  // copies and checks in parallel; the first row with a bad column is reported
+  std::atomic<int64_t> bad{n};
+  host_parallel(n, [&](int64_t i0, int64_t i1) {
+    if (ci != c->h_ci.data()) std::copy(ci + rp[i0], ci + rp[i1], c->h_ci.begin() + rp[i0]);
+    for (int64_t i = i0; i < i1; ++i) {
+      c->h_rp[i] = (int)rp[i];
+      for (int64_t k = rp[i]; k < rp[i + 1]; ++k)
+        if (ci[k] < 0 || ci[k] >= n || (k > rp[i] && ci[k] <= ci[k - 1])) {
+          int64_t cur = bad.load();
+          while (i < cur && !bad.compare_exchange_weak(cur, i)) {
+          }
+          break;
+        }
     }
+  });
+  c->h_rp[n] = (int)rp[n];
+  if (bad < n) {
+    set_err("column indices must be in range and strictly ascending per row (row %lld)", (long long)bad.load());
+    return 5;
+  }
   c->blk_of_row.clear();
   if (c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILU0 || c->prm.precond == SHESOLVE_PRECOND_BLOCK_ILUK) {
     if (nblocks > 0 && !ranges) { set_err("block ranges missing"); return 7; }
@@ -3942,16 +4050,16 @@ int set_matrix_body(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, co
                     const int32_t *ci, const double *val, int nblocks, const int64_t *ranges, int val_device) {
   auto t0 = std::chrono::steady_clock::now();
   std::vector<int64_t> hrp;
-  std::vector<int32_t> hci;
   if (device_src) {
     hrp.resize(n + 1);
     HIPCHK(hipMemcpy(hrp.data(), rp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
     nnz = hrp[n];
     if (hrp[0] != 0 || nnz < 0 || nnz > 0x7fffffff) { set_err("row_ptr inconsistent: nnz %lld", (long long)nnz); return 4; }
-    hci.resize(nnz);
-    HIPCHK(hipMemcpy(hci.data(), ci, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    // the column indices go straight into the analysis copy
+    c->h_ci.resize(nnz);
+    HIPCHK(hipMemcpy(c->h_ci.data(), ci, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
     rp = hrp.data();
-    ci = hci.data();
+    ci = c->h_ci.data();
   }
   // SHESOLVE_VERBOSE: the duration of each set-up phase on stderr
   static const bool verbose = getenv("SHESOLVE_VERBOSE") != nullptr;
