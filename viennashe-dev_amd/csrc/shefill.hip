@@ -946,7 +946,20 @@ struct Builder {
   }
 };
 
+// SHESOLVE_VERBOSE: durations of the build phases on stderr
+struct FillClock {
+  bool on = getenv("SHESOLVE_VERBOSE") != nullptr;
+  std::chrono::steady_clock::time_point tp = std::chrono::steady_clock::now();
+  void operator()(const char *what) {
+    if (!on) return;
+    const auto t = std::chrono::steady_clock::now();
+    fprintf(stderr, "[shefill] build %s: %.1f ms\n", what, std::chrono::duration<double, std::milli>(t - tp).count());
+    tp = t;
+  }
+};
+
 int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
+  FillClock phase;
   const size_t NH = c->NH;
   const double hw = optical_phonon_energy(c->cfg.energy_spacing);
   if (c->cfg.optical && !(hw > 0)) { ferr("optical phonon energy snaps to zero on this grid"); return 2; }
@@ -1024,6 +1037,7 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
   for (int k = 0; k < nn; ++k)
     for (int s = 0; s < 2; ++s)
       if (c->nd_in[2 * k + s] >= 0) c->nd_inn[2 * k + s] = c->node_of[(size_t)c->nd_c[k] * NH + c->nd_in[2 * k + s]];
+  phase("  nodes");
   // rows: pattern per node
   const int64_t n = c->n;
   c->rp.assign(n + 1, 0);
@@ -1031,10 +1045,12 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
   c->diag_pos.assign(n, 0);
   c->diag_struct.assign(n, 0);
   // row lists of one node, computed twice (count, fill) in parallel over nodes
-  auto node_rows = [&](int k, std::vector<std::vector<int>> &cols, std::vector<char> &dstruct) {
+  // (cols keeps its row vectors between nodes, so their storage is reused)
+  auto node_rows = [&](int k, std::vector<std::vector<int>> &cols, std::vector<char> &dstruct) -> int {
     const int cell = c->nd_c[k], Lc = c->nd_L[k];
     const int e = (int)B.ev[Lc].size();
-    cols.assign(e, {});
+    if ((int)cols.size() < e) cols.resize(e);
+    for (int r = 0; r < e; ++r) cols[r].clear();
     dstruct.assign(e, 0);
     // blocks in ascending column order: lower partner, bottom, left, self, right, top, upper partner
     struct Blk { int base; int kind; int slot; };
@@ -1051,11 +1067,13 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
     // 1D cells reorders the left and right neighbours)
     std::sort(blks, blks + nb, [](const Blk &a, const Blk &b) { return a.base < b.base; });
     // self block pattern
-    std::vector<std::vector<char>> selfm(e, std::vector<char>(e, 0));
+    thread_local std::vector<char> selfv;
+    selfv.assign((size_t)e * e, 0);
+    auto selfm = [&](int r, int q) -> char & { return selfv[(size_t)r * e + q]; };
     const bool optical_out = c->nd_out[2 * k] >= 0 || c->nd_out[2 * k + 1] >= 0;
     for (int r = 0; r < e; ++r) {
-      if (c->contact[cell] || optical_out) selfm[r][r] = 1;
-      if (c->cfg.acoustic && r > 0) selfm[r][r] = 1;
+      if (c->contact[cell] || optical_out) selfm(r, r) = 1;
+      if (c->cfg.acoustic && r > 0) selfm(r, r) = 1;
     }
     for (int s = 0; s < 4; ++s) {
       const int fn = c->nd_fn[(size_t)k * 4 + s];
@@ -1063,7 +1081,7 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
       const int d = kSlotDir[s];
       const Pattern &P = B.elim(d, Lc, c->fn_L[fn], Lc, rev_dir(d));
       for (int r = 0; r < e; ++r)
-        for (int q : P[r]) selfm[r][q] = 1;
+        for (int q : P[r]) selfm(r, q) = 1;
     }
     for (int bi = 0; bi < nb; ++bi) {
       const Blk &bk = blks[bi];
@@ -1071,9 +1089,9 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
         cols[0].push_back(bk.base);
       } else if (bk.kind == 0) {
         for (int r = 0; r < e; ++r) {
-          dstruct[r] = selfm[r][r];
+          dstruct[r] = selfm(r, r);
           for (int q = 0; q < e; ++q)
-            if (selfm[r][q] || q == r) cols[r].push_back(bk.base + q);
+            if (selfm(r, q) || q == r) cols[r].push_back(bk.base + q);
         }
       } else {
         const int s = bk.slot;
@@ -1084,6 +1102,7 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
           for (int q : P[r]) cols[r].push_back(bk.base + q);
       }
     }
+    return e;
   };
   // pattern cache is filled serially first (all key combinations of this device)
   {
@@ -1114,9 +1133,9 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
     for (auto &x : th) x.join();
   };
   parallel([&](int k, std::vector<std::vector<int>> &cols, std::vector<char> &ds) {
-    node_rows(k, cols, ds);
+    const int e = node_rows(k, cols, ds);
     const int base = c->nd_base[k];
-    for (size_t r = 0; r < cols.size(); ++r) {
+    for (int r = 0; r < e; ++r) {
       c->rp[base + r + 1] = (int)cols[r].size();
       c->row_node[base + r] = k;
       c->diag_struct[base + r] = ds[r];
@@ -1126,12 +1145,13 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
     if ((int64_t)c->rp[i] + c->rp[i + 1] > 0x7fffffffLL) { ferr("more than 2^31 nonzeros"); return -3; }
     c->rp[i + 1] += c->rp[i];
   }
+  phase("  row lengths");
   c->nnz = c->rp[n];
   c->ci.assign(c->nnz, 0);
   parallel([&](int k, std::vector<std::vector<int>> &cols, std::vector<char> &ds) {
-    node_rows(k, cols, ds);
+    const int e = node_rows(k, cols, ds);
     const int base = c->nd_base[k];
-    for (size_t r = 0; r < cols.size(); ++r) {
+    for (int r = 0; r < e; ++r) {
       int p = c->rp[base + r];
       for (int col : cols[r]) {
         if (col == base + (int)r) c->diag_pos[base + r] = p;
@@ -1139,21 +1159,38 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
       }
     }
   });
+  phase("  columns");
   // structural entries per column (the diagonal inserted by the normalisation is not counted)
+  // (counted and placed with atomic increments on host threads, each column's
+  // positions then sorted, which orders them by row)
+  auto ranges = [&](int64_t m, auto fn) {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) th.emplace_back([&, t] { fn(m * t / nthreads, m * (t + 1) / nthreads); });
+    for (auto &x : th) x.join();
+  };
   c->colcount.assign(n, 0);
-  for (int64_t i = 0; i < n; ++i)
-    for (int j = c->rp[i]; j < c->rp[i + 1]; ++j)
-      if (j != c->diag_pos[i] || c->diag_struct[i]) c->colcount[c->ci[j]]++;
+  ranges(n, [&](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i)
+      for (int j = c->rp[i]; j < c->rp[i + 1]; ++j)
+        if (j != c->diag_pos[i] || c->diag_struct[i]) __atomic_fetch_add(&c->colcount[c->ci[j]], 1, __ATOMIC_RELAXED);
+  });
   // column-ordered positions of the structural entries (rows ascending)
   c->cptr.assign(n + 1, 0);
   for (int64_t i = 0; i < n; ++i) c->cptr[i + 1] = c->cptr[i] + c->colcount[i];
   c->cpos.assign(c->cptr[n], 0);
   {
     std::vector<int> fillp(c->cptr.begin(), c->cptr.end() - 1);
-    for (int64_t i = 0; i < n; ++i)
-      for (int j = c->rp[i]; j < c->rp[i + 1]; ++j)
-        if (j != c->diag_pos[i] || c->diag_struct[i]) c->cpos[fillp[c->ci[j]]++] = j;
+    ranges(n, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i)
+        for (int j = c->rp[i]; j < c->rp[i + 1]; ++j)
+          if (j != c->diag_pos[i] || c->diag_struct[i])
+            c->cpos[__atomic_fetch_add(&fillp[c->ci[j]], 1, __ATOMIC_RELAXED)] = j;
+    });
+    ranges(n, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) std::sort(c->cpos.begin() + c->cptr[i], c->cpos.begin() + c->cptr[i + 1]);
+    });
   }
+  phase("  column positions");
   // block ranges (she/linear_solver.hpp:42-83)
   c->blocks.assign((size_t)2 * NH, 0);
   int64_t last = 0;
@@ -1415,12 +1452,20 @@ int shefill_build(shefill_handle h, const shefill_device *d, const shefill_confi
   }
   h->built = false;
   h->cfg = *cfg;
+  FillClock phase;
   if (int e = build_geometry(h, d)) return e;
+  phase("geometry");
   if (int e = build_energies_orders_mapping(h)) return e;
+  phase("energies and mapping");
   Builder B(h);
   if (int e = B.tables()) return e == 2 ? 3 : e;
+  phase("tables");
   if (int e = build_nodes_and_pattern(h, B)) return e;
-  return fill_values(h, B);
+  phase("nodes and pattern");
+  const int e = fill_values(h, B);
+  if (phase.on) hipStreamSynchronize(h->stream);
+  phase("uploads and fill");
+  return e;
 }
 
 int shefill_refill(shefill_handle h) {
@@ -1441,17 +1486,34 @@ int shefill_pattern_hash(shefill_handle h, uint64_t *out) {
   if (!h) { ferr("handle is NULL"); return 1; }
   if (!out) { ferr("out is NULL"); return 2; }
   if (!h->built) { ferr("nothing built"); return -1; }
-  // FNV-1a over n, the row pointer and the column indices
-  uint64_t x = 1469598103934665603ULL;
-  auto mix = [&](uint64_t v) {
-    for (int b = 0; b < 8; ++b) {
-      x ^= (v >> (8 * b)) & 0xff;
-      x *= 1099511628211ULL;
+  // FNV-1a over 32-bit words: n, then the row pointer and the column indices
+  // cut into 16 slices hashed in parallel, the slice hashes mixed in order
+  constexpr uint64_t kBasis = 1469598103934665603ULL, kPrime = 1099511628211ULL;
+  auto words = [&](const std::vector<int> &v, size_t i0, size_t i1) {
+    uint64_t x = kBasis;
+    for (size_t i = i0; i < i1; ++i) {
+      x ^= (uint32_t)v[i];
+      x *= kPrime;
     }
+    return x;
+  };
+  constexpr int kSlices = 16;
+  uint64_t part[2][kSlices];
+  std::vector<std::thread> th;
+  for (int t = 0; t < kSlices; ++t)
+    th.emplace_back([&, t] {
+      part[0][t] = words(h->rp, h->rp.size() * t / kSlices, h->rp.size() * (t + 1) / kSlices);
+      part[1][t] = words(h->ci, h->ci.size() * t / kSlices, h->ci.size() * (t + 1) / kSlices);
+    });
+  for (auto &x : th) x.join();
+  uint64_t x = kBasis;
+  auto mix = [&](uint64_t v) {
+    x ^= v;
+    x *= kPrime;
   };
   mix((uint64_t)h->n);
-  for (int v : h->rp) mix((uint64_t)(uint32_t)v);
-  for (int v : h->ci) mix((uint64_t)(uint32_t)v);
+  for (int a = 0; a < 2; ++a)
+    for (int t = 0; t < kSlices; ++t) mix(part[a][t]);
   *out = x;
   return 0;
 }

@@ -2608,6 +2608,21 @@ int iluk_pattern(shesolve_ctx *c, std::vector<int> &pos, bool drop_outside) {
   return 0;
 }
 
+// SHESOLVE_VERBOSE: durations of the set-up phases on stderr
+struct PhaseClock {
+  bool on = getenv("SHESOLVE_VERBOSE") != nullptr;
+  hipStream_t stream;
+  std::chrono::steady_clock::time_point tp = std::chrono::steady_clock::now();
+  explicit PhaseClock(hipStream_t s) : stream(s) {}
+  void operator()(const char *what) {
+    if (!on) return;
+    hipStreamSynchronize(stream);
+    const auto t = std::chrono::steady_clock::now();
+    fprintf(stderr, "[shesolve] set-up %s: %.1f ms\n", what, std::chrono::duration<double, std::milli>(t - tp).count());
+    tp = t;
+  }
+};
+
 // Grouped sweep layout of one triangle of a block multicolour factor (split
 // pattern rp/ci in the solve space, block starts bst): per row its entries
 // outside the row's block (CSR, and per block the CSR position of its first
@@ -2622,6 +2637,7 @@ int build_bmc_part(shesolve_ctx *c, const std::vector<int> &bst, bool upper, con
     set_err("block multicolour layout: blocks do not cover the rows");
     return -1;
   }
+  PhaseClock phase(c->stream);
   std::vector<int> orp(n + 1, 0), oci, opos, iptr(nb + 1, 0), ipos, icol, bK(nb + 1, 0);
   std::vector<unsigned short> icode;
   // first block of each block's colour (the colours are consecutive block ranges)
@@ -2655,6 +2671,7 @@ int build_bmc_part(shesolve_ctx *c, const std::vector<int> &bst, bool upper, con
     set_err("block multicolour layout: entry (%d, %d) is not in its triangle", (int)(bad >> 32), (int)(bad & 0xffffffff));
     return -1;
   }
+  phase("    layout counts");
   for (int r = 0; r < n; ++r) orp[r + 1] += orp[r];
   for (int blk = 0; blk < nb; ++blk) iptr[blk + 1] += iptr[blk];
   for (int blk = 0; blk <= nb; ++blk) bK[blk] = orp[bst[blk]];
@@ -2685,6 +2702,7 @@ int build_bmc_part(shesolve_ctx *c, const std::vector<int> &bst, bool upper, con
       }
     }
   }, 1024);
+  phase("    layout fill");
   P.no = (int64_t)oci.size();
   P.ni = (int64_t)ipos.size();
   // the sweep addresses these arrays through 32-bit buffer offsets
@@ -2738,21 +2756,6 @@ void bmc_part_values(shesolve_ctx *c, const shesolve_ctx::BmcPart &P, const doub
     hipLaunchKernelGGL(k_bmc_inblock, dim3(grid_for(P.ni, kBlock)), dim3(kBlock), 0, c->stream, P.ni, P.d_ipos,
                        P.d_icol, src, dinv, P.d_iv64, P.d_iv32);
 }
-
-// SHESOLVE_VERBOSE: durations of the set-up phases on stderr
-struct PhaseClock {
-  bool on = getenv("SHESOLVE_VERBOSE") != nullptr;
-  hipStream_t stream;
-  std::chrono::steady_clock::time_point tp = std::chrono::steady_clock::now();
-  explicit PhaseClock(hipStream_t s) : stream(s) {}
-  void operator()(const char *what) {
-    if (!on) return;
-    hipStreamSynchronize(stream);
-    const auto t = std::chrono::steady_clock::now();
-    fprintf(stderr, "[shesolve] set-up %s: %.1f ms\n", what, std::chrono::duration<double, std::milli>(t - tp).count());
-    tp = t;
-  }
-};
 
 int analyse_and_factor(shesolve_ctx *c) {
   const int n = c->n;
@@ -2889,6 +2892,7 @@ int analyse_and_factor(shesolve_ctx *c) {
       }
       for (int i = 0; i < n; ++i) colour[i] = bcol[i / kBmcRows];
     }
+    phase("  colouring");
     c->ncolors = ncol;
     c->h_colour = colour;
     std::vector<int> start(ncol + 1, 0);
@@ -2901,6 +2905,7 @@ int analyse_and_factor(shesolve_ctx *c) {
       perm[p] = i;
       iperm[i] = p;
     }
+    phase("  order");
     c->h_frp.assign(n + 1, 0);
     for (int p = 0; p < n; ++p) c->h_frp[p + 1] = c->h_frp[p] + (c->h_rp[perm[p] + 1] - c->h_rp[perm[p]]);
     c->h_fci.resize(c->h_ci.size());
@@ -2990,6 +2995,7 @@ int analyse_and_factor(shesolve_ctx *c) {
       }
     }
   }
+  phase("  colour levels");
   if (!colour_levels) compute_levels(c->h_frp, c->h_fci, c->h_fdg, n, Lptr, Lrows, Uptr, Urows);
   phase("  levels");
   if (perm.empty()) {
@@ -3216,6 +3222,7 @@ int analyse_and_factor(shesolve_ctx *c) {
     for (const auto &r : c->runU) all_runs = all_runs && r.blk_cnt > 0;
     if (all_runs && c->runL.size() > 1) c->c0 = c->runL[0].r1;
   }
+  phase("  level runs");
   HIPCHK(hipMalloc(&c->d_Lrp, sizeof(int) * (n + 1)));
   HIPCHK(hipMalloc(&c->d_Urp, sizeof(int) * (n + 1)));
   HIPCHK(hipMalloc(&c->d_Lci, sizeof(int) * std::max<int64_t>(c->nL, 1)));
@@ -3235,9 +3242,11 @@ int analyse_and_factor(shesolve_ctx *c) {
   if (!lblk.empty()) HIPCHK(hipMemcpyAsync(c->d_lblk, lblk.data(), sizeof(int) * lblk.size(), hipMemcpyHostToDevice, c->stream));
   if (!ublk.empty()) HIPCHK(hipMemcpyAsync(c->d_ublk, ublk.data(), sizeof(int) * ublk.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  phase("  triangle uploads");
   if (!c->bmc_cptr.empty()) {
     if (int e = build_bmc_part(c, bst, false, Lrp, Lci, c->bmcL)) return e;
     if (int e = build_bmc_part(c, bst, true, Urp, Uci, c->bmcU)) return e;
+    phase("  grouped layout");
   }
   return 0;
 }
