@@ -1830,6 +1830,14 @@ struct shesolve_ctx {
   int xcd = 1;     // XCD-contiguous block mapping
   int spmv_grid = kGrid;  // workgroups of the last SpMV with fused dot products
   double tune_us[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // the last tuned launch shapes (natural matrix: items, mapping, vector-CSR
+  // lanes; solve space: items, mapping) and the row count and preconditioner
+  // they were tuned for: a later matrix of that preconditioner within 10 % of
+  // that size takes them over instead of timing the candidates again (the
+  // successive systems of a Gummel sequence on one handle)
+  int tuned_n = 0, tuned_precond = -1, tuned_items = 8, tuned_xcd = 0, tuned_lpr = 0;
+  int tuned_sitems = 8, tuned_sxcd = 0;
+  bool tune_reuse = false;
   int spmv_lpr = 0;  // 0: CSR-stream SpMV; else lanes per row of the vector-CSR SpMV
   // descriptors of the pipelined SpMV, for the row blocks at desc_src (blocks
   // or pairs) of the solve-space matrix
@@ -1901,6 +1909,9 @@ struct shesolve_ctx {
   double ktime[5] = {0, 0, 0, 0, 0};
   int64_t kcount[5] = {0, 0, 0, 0, 0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // pinned chunks of the staged device -> host copies (kept until destroy)
+  void *stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
   // live SpMV timing (profiling mode 2): event pairs recorded around SpMV
   // launches without waiting, resolved when the statistics are read
   std::vector<hipEvent_t> evpool;
@@ -2328,6 +2339,27 @@ void make_blocks(const int *rp, int r0, int r1, int cap, std::vector<int> &out, 
 }
 
 // Level sets of the lower (forward) and upper (backward) triangles of the factor pattern.
+// Host arrays of the analysis that are written in full after they are sized:
+// no value initialisation, so resizing costs no serial zero fill and the pages
+// are first touched by the (parallel) writers.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <typename U>
+  NoInitAlloc(const NoInitAlloc<U> &) {}
+  template <typename U, typename... A>
+  void construct(U *p, A &&...a) {
+    if constexpr (sizeof...(A) == 0) ::new ((void *)p) U;
+    else ::new ((void *)p) U(std::forward<A>(a)...);
+  }
+};
+template <typename T>
+using hvec = std::vector<T, NoInitAlloc<T>>;
+
 // Runs f(i0, i1) over [0, n) cut into contiguous ranges, one per host thread
 // (at most 16, and none shorter than min_per_thread items).
 template <typename F>
@@ -2344,6 +2376,16 @@ void host_parallel(int64_t n, F &&f, int64_t min_per_thread = 1 << 15) {
   for (auto &x : th) x.join();
 }
 
+// Rows bucketed by level: ptr[l] .. ptr[l + 1] in rows, ascending row within a level.
+void bucket_levels(const std::vector<int> &lev, int maxl, int n, std::vector<int> &ptr, std::vector<int> &rows) {
+  ptr.assign(maxl + 2, 0);
+  for (int i = 0; i < n; ++i) ptr[lev[i] + 1]++;
+  for (int l = 0; l <= maxl; ++l) ptr[l + 1] += ptr[l];
+  rows.assign(n, 0);
+  std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+  for (int i = 0; i < n; ++i) rows[pos[lev[i]]++] = i;
+}
+
 void compute_levels(const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<int> &dg, int n,
                     std::vector<int> &Lptr, std::vector<int> &Lrows, std::vector<int> &Uptr, std::vector<int> &Urows) {
   std::vector<int> lev(n);
@@ -2354,16 +2396,7 @@ void compute_levels(const std::vector<int> &rp, const std::vector<int> &ci, cons
     lev[i] = l;
     maxl = std::max(maxl, l);
   }
-  auto bucket = [&](int nl, std::vector<int> &ptr, std::vector<int> &rows) {
-    ptr.assign(nl + 2, 0);
-    for (int i = 0; i < n; ++i) ptr[lev[i] + 1]++;
-    for (int l = 0; l <= nl; ++l) ptr[l + 1] += ptr[l];
-    rows.assign(n, 0);
-    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
-    for (int i = 0; i < n; ++i) rows[pos[lev[i]]++] = i;
-    ptr.resize(nl + 2);
-  };
-  bucket(maxl, Lptr, Lrows);
+  bucket_levels(lev, maxl, n, Lptr, Lrows);
   maxl = 0;
   for (int i = n - 1; i >= 0; --i) {
     int l = 0;
@@ -2371,7 +2404,49 @@ void compute_levels(const std::vector<int> &rp, const std::vector<int> &ci, cons
     lev[i] = l;
     maxl = std::max(maxl, l);
   }
-  bucket(maxl, Uptr, Urows);
+  bucket_levels(lev, maxl, n, Uptr, Urows);
+}
+
+// The same levels for a block multicolour factor, computed on host threads:
+// blocks (rows bst[b] .. bst[b + 1]) of one colour (blocks cptr[q] .. cptr[q + 1])
+// share no entry, so a colour's blocks are independent once the earlier
+// colours (forward) or the later ones (backward) are done.
+void compute_levels_blocked(const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<int> &dg, int n,
+                            const std::vector<int> &bst, const std::vector<int> &cptr, std::vector<int> &Lptr,
+                            std::vector<int> &Lrows, std::vector<int> &Uptr, std::vector<int> &Urows) {
+  std::vector<int> lev(n);
+  const int nq = (int)cptr.size() - 1;
+  std::mutex mu;
+  for (int upper = 0; upper < 2; ++upper) {
+    int maxl = 0;
+    for (int qq = 0; qq < nq; ++qq) {
+      const int q = upper ? nq - 1 - qq : qq;
+      host_parallel(cptr[q + 1] - cptr[q], [&](int64_t b0, int64_t b1) {
+        int ml = 0;
+        for (int b = cptr[q] + (int)b0; b < cptr[q] + (int)b1; ++b) {
+          if (!upper) {
+            for (int i = bst[b]; i < bst[b + 1]; ++i) {
+              int l = 0;
+              for (int k = rp[i]; k < dg[i]; ++k) l = std::max(l, lev[ci[k]] + 1);
+              lev[i] = l;
+              ml = std::max(ml, l);
+            }
+          } else {
+            for (int i = bst[b + 1] - 1; i >= bst[b]; --i) {
+              int l = 0;
+              for (int k = dg[i] + 1; k < rp[i + 1]; ++k) l = std::max(l, lev[ci[k]] + 1);
+              lev[i] = l;
+              ml = std::max(ml, l);
+            }
+          }
+        }
+        std::lock_guard<std::mutex> g(mu);
+        maxl = std::max(maxl, ml);
+      }, 256);
+    }
+    if (!upper) bucket_levels(lev, maxl, n, Lptr, Lrows);
+    else bucket_levels(lev, maxl, n, Uptr, Urows);
+  }
 }
 
 // Whether the segment sweeps (one workgroup per segment, the segments side by
@@ -2564,7 +2639,7 @@ int iluk_rows(const std::vector<int> &rp, const std::vector<int> &ci, int r0, in
 // (systems of a batch, or per-energy slabs for BLOCK_ILUK, built in parallel,
 // each with its own level), entries outside a segment dropped when
 // drop_outside.  pos maps factor entries to A's entries, -1 for fill.
-int iluk_pattern(shesolve_ctx *c, std::vector<int> &pos, bool drop_outside) {
+int iluk_pattern(shesolve_ctx *c, hvec<int> &pos, bool drop_outside) {
   const int n = c->n;
   const std::vector<int> &seg = c->fseg_ptr;
   const int S = (int)seg.size() - 1;
@@ -2630,16 +2705,17 @@ struct PhaseClock {
 // within an aligned run of kBmcGroupMax blocks of the colour, times kBmcTile,
 // plus the slot), each with its position in the triangle's value array and
 // its column, for the value gathers after a refactor.
-int build_bmc_part(shesolve_ctx *c, const std::vector<int> &bst, bool upper, const std::vector<int> &rp,
-                   const std::vector<int> &ci, shesolve_ctx::BmcPart &P) {
+int build_bmc_part(shesolve_ctx *c, const std::vector<int> &bst, bool upper, const hvec<int> &rp,
+                   const hvec<int> &ci, shesolve_ctx::BmcPart &P) {
   const int n = c->n, nb = (int)bst.size() - 1;
   if (nb < 1 || bst[nb] != n) {
     set_err("block multicolour layout: blocks do not cover the rows");
     return -1;
   }
   PhaseClock phase(c->stream);
-  std::vector<int> orp(n + 1, 0), oci, opos, iptr(nb + 1, 0), ipos, icol, bK(nb + 1, 0);
-  std::vector<unsigned short> icode;
+  std::vector<int> orp(n + 1, 0), iptr(nb + 1, 0), bK(nb + 1, 0);
+  hvec<int> oci, opos, ipos, icol;
+  hvec<unsigned short> icode;
   // first block of each block's colour (the colours are consecutive block ranges)
   std::vector<int> bq0(nb, 0);
   for (int q = 0; q + 1 < (int)c->bmc_cptr.size(); ++q)
@@ -2763,7 +2839,8 @@ int analyse_and_factor(shesolve_ctx *c) {
   PhaseClock phase(c->stream);
   // factor pattern (in the factor's own index space) and the gather map from A
   if (c->prm.precond == SHESOLVE_PRECOND_NONE) return 0;
-  std::vector<int> pos, perm;
+  hvec<int> pos;
+  std::vector<int> perm;
   // Segments of the factor: the systems of a batch (or the whole matrix), cut
   // further into the blocks of the block preconditioners when every block is
   // one contiguous row range inside one system (the per-energy slabs): the
@@ -2964,7 +3041,28 @@ int analyse_and_factor(shesolve_ctx *c) {
   phase("  diagonal");
   std::vector<int> Lptr, Lrows, Uptr, Urows;
   bool colour_levels = false;
-  if (!perm.empty()) {
+  std::vector<int> bst;
+  if (!perm.empty() && c->prm.precond == SHESOLVE_PRECOND_BMC_ILU0) {
+    // blocks in the solve space: a block's rows stay consecutive (one colour,
+    // natural order), so a block starts where the natural block changes
+    c->bmc_cptr.assign(1, 0);
+    for (int p = 0; p < n; ++p) {
+      if (p > 0 && perm[p] / kBmcRows == perm[p - 1] / kBmcRows) continue;
+      if (p > 0 && c->h_colour[perm[p]] != c->h_colour[perm[p - 1]]) c->bmc_cptr.push_back((int)bst.size());
+      bst.push_back(p);
+    }
+    c->bmc_cptr.push_back((int)bst.size());
+    bst.push_back(n);
+    for (size_t k = 0; k + 1 < bst.size(); ++k)
+      if (bst[k + 1] - bst[k] > kBmcRows) {
+        set_err("block multicolour ordering: block %zu holds %d rows", k, bst[k + 1] - bst[k]);
+        return -1;
+      }
+    HIPCHK(hipMalloc(&c->d_bmc_start, sizeof(int) * bst.size()));
+    HIPCHK(hipMemcpyAsync(c->d_bmc_start, bst.data(), sizeof(int) * bst.size(), hipMemcpyHostToDevice, c->stream));
+  }
+  const bool bmc = !bst.empty();
+  if (!perm.empty() && !bmc) {
     // multicolour ordering: colour q occupies the contiguous range
     // [cstart[q], cstart[q+1]) of the factor; use whole colours as levels
     // (forward: ascending colours, backward: descending) when every factor
@@ -2996,7 +3094,8 @@ int analyse_and_factor(shesolve_ctx *c) {
     }
   }
   phase("  colour levels");
-  if (!colour_levels) compute_levels(c->h_frp, c->h_fci, c->h_fdg, n, Lptr, Lrows, Uptr, Urows);
+  if (bmc) compute_levels_blocked(c->h_frp, c->h_fci, c->h_fdg, n, bst, c->bmc_cptr, Lptr, Lrows, Uptr, Urows);
+  else if (!colour_levels) compute_levels(c->h_frp, c->h_fci, c->h_fdg, n, Lptr, Lrows, Uptr, Urows);
   phase("  levels");
   if (perm.empty()) {
     // factor segments (systems of a batch, per-energy slabs or the whole
@@ -3005,26 +3104,6 @@ int analyse_and_factor(shesolve_ctx *c) {
     if (segment_sweeps_pay(c, Lptr, Lrows))
       if (int e = segment_levels(c)) return e;
     phase("  segment levels");
-  }
-  std::vector<int> bst;
-  if (!perm.empty() && c->prm.precond == SHESOLVE_PRECOND_BMC_ILU0) {
-    // blocks in the solve space: a block's rows stay consecutive (one colour,
-    // natural order), so a block starts where the natural block changes
-    c->bmc_cptr.assign(1, 0);
-    for (int p = 0; p < n; ++p) {
-      if (p > 0 && perm[p] / kBmcRows == perm[p - 1] / kBmcRows) continue;
-      if (p > 0 && c->h_colour[perm[p]] != c->h_colour[perm[p - 1]]) c->bmc_cptr.push_back((int)bst.size());
-      bst.push_back(p);
-    }
-    c->bmc_cptr.push_back((int)bst.size());
-    bst.push_back(n);
-    for (size_t k = 0; k + 1 < bst.size(); ++k)
-      if (bst[k + 1] - bst[k] > kBmcRows) {
-        set_err("block multicolour ordering: block %zu holds %d rows", k, bst[k + 1] - bst[k]);
-        return -1;
-      }
-    HIPCHK(hipMalloc(&c->d_bmc_start, sizeof(int) * bst.size()));
-    HIPCHK(hipMemcpyAsync(c->d_bmc_start, bst.data(), sizeof(int) * bst.size(), hipMemcpyHostToDevice, c->stream));
   }
   c->lvL.ptr = Lptr;
   c->lvU.ptr = Uptr;
@@ -3057,7 +3136,7 @@ int analyse_and_factor(shesolve_ctx *c) {
   c->flpr = pick_lpr((double)c->fnnz / std::max(n, 1) / 2.0);
   phase("  uploads");
   // split pattern of the factor
-  std::vector<int> Lrp(n + 1, 0), Urp(n + 1, 0), Lci, Uci;
+  hvec<int> Lrp(n + 1, 0), Urp(n + 1, 0), Lci, Uci;
   for (int i = 0; i < n; ++i) {
     Lrp[i + 1] = Lrp[i] + (c->h_fdg[i] - c->h_frp[i]);
     Urp[i + 1] = Urp[i] + (c->h_frp[i + 1] - c->h_fdg[i] - 1);
@@ -3079,9 +3158,10 @@ int analyse_and_factor(shesolve_ctx *c) {
     // "row pointer" becomes the start of each row in that storage (k_split
     // writes through it), lrp[q] the start of the q-th level-ordered row
     // columns become positions in the same schedule (the sweep's window index)
-    auto reorder = [&](std::vector<int> &rp, std::vector<int> &ci, const std::vector<int> &rows,
+    auto reorder = [&](hvec<int> &rp, hvec<int> &ci, const std::vector<int> &rows,
                        std::vector<int> &lrp) {
-      std::vector<int> start(n + 1, 0), lci(ci.size()), qpos(n);
+      hvec<int> start(n + 1, 0), lci(ci.size());
+      std::vector<int> qpos(n);
       for (int q = 0; q < n; ++q) qpos[rows[q]] = q;
       lrp.assign(n + 1, 0);
       int w = 0;
@@ -3101,7 +3181,8 @@ int analyse_and_factor(shesolve_ctx *c) {
     reorder(Urp, Uci, c->h_segrowsU, loUrp);
     phase("  level order");
     for (int upper = 0; upper < 2; ++upper) {
-      const std::vector<int> &lv = c->h_lvlq[upper], &lrp = upper ? loUrp : loLrp, &lci = upper ? Uci : Lci;
+      const std::vector<int> &lv = c->h_lvlq[upper], &lrp = upper ? loUrp : loLrp;
+      const hvec<int> &lci = upper ? Uci : Lci;
       int reach = 0;
       for (size_t li = 0; li + 1 < lv.size(); ++li)
         for (int q = lv[li]; q < lv[li + 1]; ++q)
@@ -3190,7 +3271,7 @@ int analyse_and_factor(shesolve_ctx *c) {
   }
   phase("  segment uploads");
   // contiguous levels get CSR-stream row blocks
-  auto runs = [&](const Levels &lv, const std::vector<int> &rows, const std::vector<int> &rp,
+  auto runs = [&](const Levels &lv, const std::vector<int> &rows, const hvec<int> &rp,
                   std::vector<shesolve_ctx::LevelRun> &out, std::vector<int> &blk) {
     out.clear();
     blk.clear();
@@ -3451,7 +3532,11 @@ int setup_solve_space(shesolve_ctx *c) {
   c->s_ci = c->d_fci;
   c->spmv_lpr = 0;  // the interleaved block order needs the CSR-stream kernel
   std::vector<int> flat;
-  if (!c->prm.autotune) {
+  if (!c->prm.autotune || c->tune_reuse) {
+    if (c->tune_reuse) {
+      c->items = c->tuned_sitems;
+      c->xcd = c->tuned_sxcd;
+    }
     build_pairs(c->items, flat);
     if (int e = install(flat)) return e;
     return build_codes(c, flat);
@@ -3465,6 +3550,7 @@ int setup_solve_space(shesolve_ctx *c) {
   // interleaved order give each XCD its own eighth of the natural index space
   // in every colour), and plain launches do not tell: on the 161x81 L=3
   // device two mappings ran 362 and 375 us plain but 414 and 560 us fused.
+  PhaseClock phase(c->stream);
   hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)c->fnnz, c->d_sv32, 1.0f);
   hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)c->n, c->v32[2], 1.0f);
   float best = 1e30f;
@@ -3497,8 +3583,13 @@ int setup_solve_space(shesolve_ctx *c) {
   }
   c->items = best_items;
   c->xcd = best_xcd;
+  c->tuned_sitems = best_items;
+  c->tuned_sxcd = best_xcd;
   if (int e = install(best_flat)) return e;
-  return build_codes(c, best_flat);
+  phase("  solve-space tuning");
+  const int e = build_codes(c, best_flat);
+  phase("  column codes");
+  return e;
 }
 
 // One BiCGStab solve (right preconditioned) on the device, precision T.
@@ -3959,8 +4050,10 @@ int tune_stream(shesolve_ctx *c) {
   hipLaunchKernelGGL((k_fill<float>), dim3(kGrid), dim3(kBlock), 0, c->stream, (int64_t)n, c->v32[2], 1.0f);
   // every candidate is timed as the solve launches it: with the fused dot
   // product on the persistent grid (plain launches rank the mappings differently)
+  const bool reuse = c->tune_reuse;
   for (int ii = 0; ii < 3; ++ii) {
     if (!c->prm.autotune && cand_items[ii] != 8) continue;
+    if (reuse && cand_items[ii] != c->tuned_items) continue;
     std::vector<int> ab;
     matrix_blocks(c, spmv_cap(cand_items[ii]), ab);
     int *d_ab = nullptr;
@@ -3968,6 +4061,7 @@ int tune_stream(shesolve_ctx *c) {
     HIPCHK(hipMemcpyAsync(d_ab, ab.data(), sizeof(int) * ab.size(), hipMemcpyHostToDevice, c->stream));
     for (int xm = 0; xm < 2; ++xm) {
       if (!c->prm.autotune && xm != 0) continue;
+      if (reuse && xm != c->tuned_xcd) continue;
       c->items = cand_items[ii];
       c->xcd = xm;
       c->d_ablk = d_ab;
@@ -3993,8 +4087,8 @@ int tune_stream(shesolve_ctx *c) {
     HIPCHK(hipFree(d_ab));
   }
   // vector-CSR candidates (lanes per row 4, 8, 16)
-  int best_lpr = 0;
-  if (c->prm.autotune) {
+  int best_lpr = reuse ? c->tuned_lpr : 0;
+  if (c->prm.autotune && !reuse) {
     c->items = best_items;
     c->xcd = best_xcd;
     const int cand_lpr[3] = {4, 8, 16};
@@ -4019,6 +4113,13 @@ int tune_stream(shesolve_ctx *c) {
   c->spmv_lpr = best_lpr;
   c->items = best_items;
   c->xcd = best_xcd;
+  if (c->prm.autotune && !reuse) {
+    c->tuned_n = n;
+    c->tuned_precond = c->prm.precond;
+    c->tuned_items = best_items;
+    c->tuned_xcd = best_xcd;
+    c->tuned_lpr = best_lpr;
+  }
   c->nablk = (int)best_blocks.size() - 1;
   HIPCHK(hipMalloc(&c->d_ablk, sizeof(int) * best_blocks.size()));
   HIPCHK(hipMemcpyAsync(c->d_ablk, best_blocks.data(), sizeof(int) * best_blocks.size(), hipMemcpyHostToDevice,
@@ -4055,18 +4156,43 @@ int set_matrix_common(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, 
   return e;
 }
 
+// Device -> host copy through the context's two pinned chunks: the copy of one
+// chunk into the (pageable) destination overlaps the transfer of the next,
+// where a copy straight into pageable memory moves a few GB/s.
+int staged_d2h(shesolve_ctx *c, void *dst, const void *src, size_t bytes) {
+  constexpr size_t kChunk = (size_t)8 << 20;
+  for (int k = 0; k < 2; ++k) {
+    if (!c->stage[k]) HIPCHK(hipHostMalloc(&c->stage[k], kChunk));
+    if (!c->stage_ev[k]) HIPCHK(hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming));
+  }
+  const size_t nch = (bytes + kChunk - 1) / kChunk;
+  for (size_t k = 0; k <= nch; ++k) {
+    if (k < nch) {
+      const size_t off = k * kChunk, len = std::min(kChunk, bytes - off);
+      HIPCHK(hipMemcpyAsync(c->stage[k & 1], (const char *)src + off, len, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipEventRecord(c->stage_ev[k & 1], c->stream));
+    }
+    if (k > 0) {
+      const size_t j = k - 1, off = j * kChunk, len = std::min(kChunk, bytes - off);
+      HIPCHK(hipEventSynchronize(c->stage_ev[j & 1]));
+      std::memcpy((char *)dst + off, c->stage[j & 1], len);
+    }
+  }
+  return 0;
+}
+
 int set_matrix_body(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, const int64_t *rp,
                     const int32_t *ci, const double *val, int nblocks, const int64_t *ranges, int val_device) {
   auto t0 = std::chrono::steady_clock::now();
   std::vector<int64_t> hrp;
   if (device_src) {
     hrp.resize(n + 1);
-    HIPCHK(hipMemcpy(hrp.data(), rp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+    if (int e = staged_d2h(c, hrp.data(), rp, sizeof(int64_t) * (n + 1))) return e;
     nnz = hrp[n];
     if (hrp[0] != 0 || nnz < 0 || nnz > 0x7fffffff) { set_err("row_ptr inconsistent: nnz %lld", (long long)nnz); return 4; }
     // the column indices go straight into the analysis copy
     c->h_ci.resize(nnz);
-    HIPCHK(hipMemcpy(c->h_ci.data(), ci, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    if (int e = staged_d2h(c, c->h_ci.data(), ci, sizeof(int32_t) * nnz)) return e;
     rp = hrp.data();
     ci = c->h_ci.data();
   }
@@ -4092,6 +4218,8 @@ int set_matrix_body(shesolve_ctx *c, bool device_src, int64_t n, int64_t nnz, co
                         c->stream));
   c->lpr = c->prm.lanes_per_row > 0 ? c->prm.lanes_per_row : pick_lpr((double)nnz / n);
   phase("upload");
+  c->tune_reuse = c->prm.autotune && c->tuned_n > 0 && c->tuned_precond == c->prm.precond && c->seg_ptr.empty() &&
+                  std::llabs((long long)n - (long long)c->tuned_n) * 10 <= (long long)c->tuned_n;
   if (int e = tune_stream(c)) return e;
   phase("stream tuning");
   if (int e = analyse_and_factor(c)) return e;
@@ -4720,6 +4848,10 @@ int shesolve_destroy(shesolve_handle h) {
     if (e) hipEventDestroy(e);
   hipEventDestroy(h->ev0);
   hipEventDestroy(h->ev1);
+  for (int k = 0; k < 2; ++k) {
+    if (h->stage[k]) hipHostFree(h->stage[k]);
+    if (h->stage_ev[k]) hipEventDestroy(h->stage_ev[k]);
+  }
   for (auto &e : h->evpool) hipEventDestroy(e);
   hipStreamDestroy(h->stream);
   delete h;
