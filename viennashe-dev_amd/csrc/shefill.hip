@@ -1177,7 +1177,7 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
   // column-ordered positions of the structural entries (rows ascending)
   c->cptr.assign(n + 1, 0);
   for (int64_t i = 0; i < n; ++i) c->cptr[i + 1] = c->cptr[i] + c->colcount[i];
-  c->cpos.assign(c->cptr[n], 0);
+  c->cpos.resize(c->cptr[n]);
   {
     std::vector<int> fillp(c->cptr.begin(), c->cptr.end() - 1);
     ranges(n, [&](int64_t i0, int64_t i1) {
