@@ -1751,6 +1751,27 @@ __global__ __launch_bounds__(kBlock) void k_qscatter(int n, const int *__restric
   if (i < n) dst[map[i]] = src[i];
 }
 
+// Host arrays of the analysis that are written in full after they are sized:
+// no value initialisation, so resizing costs no serial zero fill and the pages
+// are first touched by the (parallel) writers.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <typename U>
+  NoInitAlloc(const NoInitAlloc<U> &) {}
+  template <typename U, typename... A>
+  void construct(U *p, A &&...a) {
+    if constexpr (sizeof...(A) == 0) ::new ((void *)p) U;
+    else ::new ((void *)p) U(std::forward<A>(a)...);
+  }
+};
+template <typename T>
+using hvec = std::vector<T, NoInitAlloc<T>>;
+
 // ---------------------------------------------------------------- host side
 inline int grid_for(int64_t items, int per_block) {
   int64_t g = (items + per_block - 1) / per_block;
@@ -1775,14 +1796,16 @@ struct shesolve_ctx {
   int *d_rp = nullptr, *d_ci = nullptr;
   double *d_v64 = nullptr;
   float *d_v32 = nullptr;
-  std::vector<int> h_rp, h_ci;  // pattern copy for analysis
+  std::vector<int> h_rp;  // pattern copy for analysis
+  hvec<int> h_ci;
   // factor (pattern = A or its block-diagonal part)
   bool factor_is_A = true;
   int64_t fnnz = 0;
   int *d_frp = nullptr, *d_fci = nullptr, *d_fdg = nullptr, *d_fpos = nullptr;
   double *d_f64 = nullptr;
   int fmaxlen = 0;  // longest factor row
-  std::vector<int> h_frp, h_fci, h_fdg;
+  std::vector<int> h_frp, h_fdg;
+  hvec<int> h_fci;
   std::vector<int> blk_of_row;
   int *d_perm = nullptr;            // multicolour ordering: new -> old row
   // solve space: the matrix the Krylov iteration multiplies by -- the stored
@@ -2339,27 +2362,6 @@ void make_blocks(const int *rp, int r0, int r1, int cap, std::vector<int> &out, 
 }
 
 // Level sets of the lower (forward) and upper (backward) triangles of the factor pattern.
-// Host arrays of the analysis that are written in full after they are sized:
-// no value initialisation, so resizing costs no serial zero fill and the pages
-// are first touched by the (parallel) writers.
-template <typename T>
-struct NoInitAlloc : std::allocator<T> {
-  template <typename U>
-  struct rebind {
-    using other = NoInitAlloc<U>;
-  };
-  NoInitAlloc() = default;
-  template <typename U>
-  NoInitAlloc(const NoInitAlloc<U> &) {}
-  template <typename U, typename... A>
-  void construct(U *p, A &&...a) {
-    if constexpr (sizeof...(A) == 0) ::new ((void *)p) U;
-    else ::new ((void *)p) U(std::forward<A>(a)...);
-  }
-};
-template <typename T>
-using hvec = std::vector<T, NoInitAlloc<T>>;
-
 // Runs f(i0, i1) over [0, n) cut into contiguous ranges, one per host thread
 // (at most 16, and none shorter than min_per_thread items).
 template <typename F>
@@ -2376,6 +2378,21 @@ void host_parallel(int64_t n, F &&f, int64_t min_per_thread = 1 << 15) {
   for (auto &x : th) x.join();
 }
 
+// SHESOLVE_VERBOSE: durations of the set-up phases on stderr
+struct PhaseClock {
+  bool on = getenv("SHESOLVE_VERBOSE") != nullptr;
+  hipStream_t stream;
+  std::chrono::steady_clock::time_point tp = std::chrono::steady_clock::now();
+  explicit PhaseClock(hipStream_t s) : stream(s) {}
+  void operator()(const char *what) {
+    if (!on) return;
+    hipStreamSynchronize(stream);
+    const auto t = std::chrono::steady_clock::now();
+    fprintf(stderr, "[shesolve] set-up %s: %.1f ms\n", what, std::chrono::duration<double, std::milli>(t - tp).count());
+    tp = t;
+  }
+};
+
 // Rows bucketed by level: ptr[l] .. ptr[l + 1] in rows, ascending row within a level.
 void bucket_levels(const std::vector<int> &lev, int maxl, int n, std::vector<int> &ptr, std::vector<int> &rows) {
   ptr.assign(maxl + 2, 0);
@@ -2386,7 +2403,7 @@ void bucket_levels(const std::vector<int> &lev, int maxl, int n, std::vector<int
   for (int i = 0; i < n; ++i) rows[pos[lev[i]]++] = i;
 }
 
-void compute_levels(const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<int> &dg, int n,
+void compute_levels(const std::vector<int> &rp, const hvec<int> &ci, const std::vector<int> &dg, int n,
                     std::vector<int> &Lptr, std::vector<int> &Lrows, std::vector<int> &Uptr, std::vector<int> &Urows) {
   std::vector<int> lev(n);
   int maxl = 0;
@@ -2407,11 +2424,51 @@ void compute_levels(const std::vector<int> &rp, const std::vector<int> &ci, cons
   bucket_levels(lev, maxl, n, Uptr, Urows);
 }
 
+// The same levels when the factor is block diagonal over segments (row ranges
+// sp[s] .. sp[s + 1] that no entry leaves): each segment on its own, the
+// segments on host threads.  Returns false (nothing computed) when an entry
+// leaves its segment.
+bool compute_levels_segments(const std::vector<int> &rp, const hvec<int> &ci, const std::vector<int> &dg, int n,
+                             const std::vector<int> &sp, std::vector<int> &Lptr, std::vector<int> &Lrows,
+                             std::vector<int> &Uptr, std::vector<int> &Urows) {
+  const int S = (int)sp.size() - 1;
+  std::vector<int> lev(n);
+  std::atomic<bool> ok{true};
+  std::mutex mu;
+  for (int upper = 0; upper < 2; ++upper) {
+    int maxl = 0;
+    host_parallel(S, [&](int64_t s0, int64_t s1) {
+      int ml = 0;
+      for (int s = (int)s0; s < (int)s1; ++s) {
+        const int r0 = sp[s], r1 = sp[s + 1];
+        for (int ii = r0; ii < r1; ++ii) {
+          const int i = upper ? r1 - 1 - (ii - r0) : ii;
+          const int k0 = upper ? dg[i] + 1 : rp[i], k1 = upper ? rp[i + 1] : dg[i];
+          int l = 0;
+          for (int k = k0; k < k1; ++k) {
+            const int j = ci[k];
+            if (j < r0 || j >= r1) ok = false;
+            else l = std::max(l, lev[j] + 1);
+          }
+          lev[i] = l;
+          ml = std::max(ml, l);
+        }
+      }
+      std::lock_guard<std::mutex> g(mu);
+      maxl = std::max(maxl, ml);
+    }, 1);
+    if (!ok) return false;
+    if (!upper) bucket_levels(lev, maxl, n, Lptr, Lrows);
+    else bucket_levels(lev, maxl, n, Uptr, Urows);
+  }
+  return true;
+}
+
 // The same levels for a block multicolour factor, computed on host threads:
 // blocks (rows bst[b] .. bst[b + 1]) of one colour (blocks cptr[q] .. cptr[q + 1])
 // share no entry, so a colour's blocks are independent once the earlier
 // colours (forward) or the later ones (backward) are done.
-void compute_levels_blocked(const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<int> &dg, int n,
+void compute_levels_blocked(const std::vector<int> &rp, const hvec<int> &ci, const std::vector<int> &dg, int n,
                             const std::vector<int> &bst, const std::vector<int> &cptr, std::vector<int> &Lptr,
                             std::vector<int> &Lrows, std::vector<int> &Uptr, std::vector<int> &Urows) {
   std::vector<int> lev(n);
@@ -2465,26 +2522,31 @@ bool segment_sweeps_pay(shesolve_ctx *c, const std::vector<int> &Lptr, const std
   // the sweeps address a segment's FP64 outputs through 32-bit buffer offsets
   for (int sg = 0; sg < S; ++sg)
     if ((int64_t)(sp[sg + 1] - sp[sg]) * (int64_t)sizeof(double) >= ((int64_t)1 << 31)) return false;
-  std::vector<double> ent(S, 0.0);
-  std::vector<int> lev(S, 0), rows(S, 0);
-  std::vector<int> touched;
-  std::vector<int64_t> cur(S, 0);
+  // per segment: its rows, the number of distinct levels holding them, and
+  // its forward entries (each row counted with its diagonal); on host threads
+  const int n = c->n;
+  std::vector<int> lev(S, 0), rows(S, 0), level_of(n);
   std::vector<double> entries_cost(S, 0.0);
-  for (int l = 0; l < nl; ++l) {
-    touched.clear();
-    for (int q = Lptr[l]; q < Lptr[l + 1]; ++q) {
-      const int i = Lrows[q];
-      const int sg = (int)(std::upper_bound(sp.begin(), sp.end(), i) - sp.begin()) - 1;
-      if (cur[sg] == 0) touched.push_back(sg);
-      cur[sg] += 1 + (c->h_fdg[i] - c->h_frp[i]);
-      rows[sg] += 1;
+  host_parallel(nl, [&](int64_t l0, int64_t l1) {
+    for (int64_t l = l0; l < l1; ++l)
+      for (int q = Lptr[l]; q < Lptr[l + 1]; ++q) level_of[Lrows[q]] = (int)l;
+  }, 64);
+  host_parallel(S, [&](int64_t s0, int64_t s1) {
+    std::vector<int> stamp(nl, -1);
+    for (int sg = (int)s0; sg < (int)s1; ++sg) {
+      int cnt = 0;
+      for (int i = sp[sg]; i < sp[sg + 1]; ++i)
+        if (stamp[level_of[i]] != sg) {
+          stamp[level_of[i]] = sg;
+          ++cnt;
+        }
+      lev[sg] = cnt;
+      rows[sg] = sp[sg + 1] - sp[sg];
+      int64_t ents = 0;
+      for (int i = sp[sg]; i < sp[sg + 1]; ++i) ents += 1 + (c->h_fdg[i] - c->h_frp[i]);
+      entries_cost[sg] = kSegEntryUs * (double)ents;
     }
-    for (int sg : touched) {
-      lev[sg] += 1;
-      entries_cost[sg] += kSegEntryUs * (double)cur[sg];
-      cur[sg] = 0;
-    }
-  }
+  }, 1);
   // workgroup size from the mean level width (rows per level of a segment)
   int64_t tot_rows = 0, tot_lev = 0;
   for (int sg = 0; sg < S; ++sg) { tot_rows += rows[sg]; tot_lev += lev[sg]; }
@@ -2509,48 +2571,65 @@ int segment_levels(shesolve_ctx *c) {
   const int n = c->n;
   const std::vector<int> &sp = c->fseg_ptr;
   const int S = (int)sp.size() - 1;
-  const auto &rp = c->h_frp, &ci = c->h_fci, &dg = c->h_fdg;
+  const auto &rp = c->h_frp, &dg = c->h_fdg;
+  const auto &ci = c->h_fci;
   std::vector<int> lev(n);
-  for (int s = 0; s < S; ++s)
-    for (int i = sp[s]; i < sp[s + 1]; ++i)
-      for (int k = rp[i]; k < rp[i + 1]; ++k)
-        if (ci[k] < sp[s] || ci[k] >= sp[s + 1]) {
-          set_err("factor entry (%d, %d) couples two segments", i, ci[k]);
-          return 2;
-        }
+  // (segments on host threads: a check, then per triangle a pass that levels
+  // each segment's rows and one that places them after the prefix sums)
+  std::atomic<int64_t> bad{-1};
+  host_parallel(S, [&](int64_t s0, int64_t s1) {
+    for (int s = (int)s0; s < (int)s1; ++s)
+      for (int i = sp[s]; i < sp[s + 1]; ++i)
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+          if (ci[k] < sp[s] || ci[k] >= sp[s + 1]) bad = ((int64_t)i << 32) | (uint32_t)ci[k];
+  }, 1);
+  if (bad >= 0) {
+    set_err("factor entry (%d, %d) couples two segments", (int)(bad >> 32), (int)(bad & 0xffffffff));
+    return 2;
+  }
   c->seg_levels_max = 0;
   for (int upper = 0; upper < 2; ++upper) {
-    std::vector<int> seg(S + 1, 0), lvl(1, 0), rows;
-    rows.reserve(n);
-    for (int s = 0; s < S; ++s) {
-      const int r0 = sp[s], r1 = sp[s + 1];
-      int maxl = -1;
-      if (!upper) {
-        for (int i = r0; i < r1; ++i) {
-          int l = 0;
-          for (int k = rp[i]; k < dg[i]; ++k) l = std::max(l, lev[ci[k]] + 1);
-          lev[i] = l;
-          maxl = std::max(maxl, l);
+    std::vector<int> nl(S, 0);
+    host_parallel(S, [&](int64_t s0, int64_t s1) {
+      for (int s = (int)s0; s < (int)s1; ++s) {
+        const int r0 = sp[s], r1 = sp[s + 1];
+        int maxl = -1;
+        if (!upper) {
+          for (int i = r0; i < r1; ++i) {
+            int l = 0;
+            for (int k = rp[i]; k < dg[i]; ++k) l = std::max(l, lev[ci[k]] + 1);
+            lev[i] = l;
+            maxl = std::max(maxl, l);
+          }
+        } else {
+          for (int i = r1 - 1; i >= r0; --i) {
+            int l = 0;
+            for (int k = dg[i] + 1; k < rp[i + 1]; ++k) l = std::max(l, lev[ci[k]] + 1);
+            lev[i] = l;
+            maxl = std::max(maxl, l);
+          }
         }
-      } else {
-        for (int i = r1 - 1; i >= r0; --i) {
-          int l = 0;
-          for (int k = dg[i] + 1; k < rp[i + 1]; ++k) l = std::max(l, lev[ci[k]] + 1);
-          lev[i] = l;
-          maxl = std::max(maxl, l);
-        }
+        nl[s] = maxl + 1;
       }
-      std::vector<int> cnt(maxl + 2, 0);
-      for (int i = r0; i < r1; ++i) cnt[lev[i] + 1]++;
-      for (int l = 0; l <= maxl; ++l) cnt[l + 1] += cnt[l];
-      const int base = (int)rows.size();
-      rows.resize(base + (r1 - r0));
-      std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
-      for (int i = r0; i < r1; ++i) rows[base + fillp[lev[i]]++] = i;
-      for (int l = 0; l <= maxl; ++l) lvl.push_back(base + cnt[l + 1]);
-      seg[s + 1] = (int)lvl.size() - 1;
-      c->seg_levels_max = std::max(c->seg_levels_max, maxl + 1);
+    }, 1);
+    std::vector<int> seg(S + 1, 0), lvl, rows(n);
+    for (int s = 0; s < S; ++s) {
+      seg[s + 1] = seg[s] + nl[s];
+      c->seg_levels_max = std::max(c->seg_levels_max, nl[s]);
     }
+    lvl.assign(seg[S] + 1, 0);
+    host_parallel(S, [&](int64_t s0, int64_t s1) {
+      std::vector<int> cnt;
+      for (int s = (int)s0; s < (int)s1; ++s) {
+        const int r0 = sp[s], r1 = sp[s + 1], m = nl[s];
+        cnt.assign(m + 1, 0);
+        for (int i = r0; i < r1; ++i) cnt[lev[i] + 1]++;
+        for (int l = 0; l < m; ++l) cnt[l + 1] += cnt[l];
+        // the segment's rows occupy positions r0 .. r1 of the schedule
+        for (int l = 0; l < m; ++l) lvl[seg[s] + l + 1] = r0 + cnt[l + 1];
+        for (int i = r0; i < r1; ++i) rows[r0 + cnt[lev[i]]++] = i;
+      }
+    }, 1);
     int *&dseg = upper ? c->d_segU : c->d_segL;
     int *&dlvl = upper ? c->d_lvlU : c->d_lvlL;
     int *&drows = upper ? c->d_rowsU : c->d_rowsL;
@@ -2580,7 +2659,7 @@ int segment_levels(shesolve_ctx *c) {
 // columns).  The rows may only reach columns in [r0, r1) (a diagonal block).
 // Output row lengths in len[i - r0], columns and the A-entry of each factor
 // entry (-1 for fill) appended to fci / pos.
-int iluk_rows(const std::vector<int> &rp, const std::vector<int> &ci, int r0, int r1, int k, std::vector<int> &len,
+int iluk_rows(const std::vector<int> &rp, const hvec<int> &ci, int r0, int r1, int k, std::vector<int> &len,
               std::vector<int> &fci, std::vector<int> &pos, bool drop_outside = false) {
   const int m = r1 - r0;
   std::vector<int> frp(m + 1, 0), udg(m);
@@ -2588,6 +2667,49 @@ int iluk_rows(const std::vector<int> &rp, const std::vector<int> &ci, int r0, in
   std::vector<int> lev(m, -1), src(m, -1), touched;
   std::priority_queue<int, std::vector<int>, std::greater<int>> heap;
   len.assign(m, 0);
+  if (k <= 1) {
+    // Levels 0 and 1 without the heap: a level-1 entry (i, c) needs lev(i, j) =
+    // lev(j, c) = 0, so row i is the block's part of A's row i plus, for each j < i
+    // in it, the block's part of A's row j right of j (level 1 where A has no entry)
+    for (int i = 0; i < m; ++i) {
+      touched.clear();
+      const int gi = r0 + i;
+      for (int q = rp[gi]; q < rp[gi + 1]; ++q) {
+        const int cl = ci[q] - r0;
+        if (cl < 0 || cl >= m) {
+          if (drop_outside) continue;
+          return -1;
+        }
+        lev[cl] = 0;
+        src[cl] = q;
+        touched.push_back(cl);
+      }
+      if (k == 1) {
+        const size_t na = touched.size();
+        for (size_t t = 0; t < na && touched[t] < i; ++t) {
+          const int j = touched[t], gj = r0 + j;
+          for (int q = rp[gj]; q < rp[gj + 1]; ++q) {
+            const int col = ci[q] - r0;
+            if (col <= j || col >= m || lev[col] >= 0) continue;
+            lev[col] = 1;
+            touched.push_back(col);
+          }
+        }
+        std::sort(touched.begin(), touched.end());
+      }
+      int dg = -1;
+      for (int col : touched) {
+        if (col == i) dg = (int)fci.size();
+        fci.push_back(col + r0);
+        pos.push_back(src[col]);
+        lev[col] = -1;
+        src[col] = -1;
+      }
+      if (dg < 0) return -2 - gi;
+      len[i] = (int)touched.size();
+    }
+    return 0;
+  }
   for (int i = 0; i < m; ++i) {
     touched.clear();
     const int gi = r0 + i;
@@ -2647,6 +2769,83 @@ int iluk_pattern(shesolve_ctx *c, hvec<int> &pos, bool drop_outside) {
   const int nsys = c->seg_ptr.empty() ? 1 : (int)c->seg_ptr.size() - 1;
   if ((int)c->seg_fill.size() == nsys)
     for (int q = 0; q < S; ++q) kseg[q] = c->seg_fill[c->fseg_sys[q]];
+  if (*std::max_element(kseg.begin(), kseg.end()) <= 1) {
+    // ILU(0) and ILU(1): two passes over the segments on host threads (row
+    // sizes, then the rows written in place), no per-segment buffers.  A
+    // level-1 entry (i, c) needs lev(i, j) = lev(j, c) = 0, so row i is the
+    // segment's part of A's row i plus, for each j < i in it, the segment's part
+    // of A's row j right of j (level 1 where A has no entry).
+    PhaseClock phase(c->stream);
+    int mmax = 0;
+    for (int q = 0; q < S; ++q) mmax = std::max(mmax, seg[q + 1] - seg[q]);
+    std::atomic<int64_t> out_seg{-1}, no_diag{-1};
+    c->h_frp.assign(n + 1, 0);
+    auto pass = [&](bool fill) {
+      host_parallel(S, [&](int64_t s0, int64_t s1) {
+        std::vector<int> lev(mmax, -1), src(mmax, -1), touched;
+        for (int q = (int)s0; q < (int)s1; ++q) {
+          const int r0 = seg[q], m = seg[q + 1] - r0, k = kseg[q];
+          for (int gi = r0; gi < r0 + m; ++gi) {
+            const int i = gi - r0;
+            touched.clear();
+            bool outside = false;
+            for (int e = c->h_rp[gi]; e < c->h_rp[gi + 1]; ++e) {
+              const int cl = c->h_ci[e] - r0;
+              if (cl < 0 || cl >= m) {
+                outside = true;
+                continue;
+              }
+              lev[cl] = 0;
+              src[cl] = e;
+              touched.push_back(cl);
+            }
+            if (outside && !drop_outside) out_seg = q;
+            if (k == 1) {
+              const size_t na = touched.size();
+              for (size_t t = 0; t < na && touched[t] < i; ++t) {
+                const int j = touched[t], gj = r0 + j;
+                for (int e = c->h_rp[gj]; e < c->h_rp[gj + 1]; ++e) {
+                  const int col = c->h_ci[e] - r0;
+                  if (col <= j || col >= m || lev[col] >= 0) continue;
+                  lev[col] = 1;
+                  touched.push_back(col);
+                }
+              }
+              std::sort(touched.begin(), touched.end());
+            }
+            bool dg = false;
+            int w = fill ? c->h_frp[gi] : 0;
+            for (int col : touched) {
+              dg = dg || col == i;
+              if (fill) {
+                c->h_fci[w] = col + r0;
+                pos[w] = src[col];
+                ++w;
+              }
+              lev[col] = -1;
+              src[col] = -1;
+            }
+            if (!dg) no_diag = gi;
+            if (!fill) c->h_frp[gi + 1] = (int)touched.size();
+          }
+        }
+      }, 1);
+    };
+    pass(false);
+    if (out_seg >= 0) { set_err("ILU(k): an entry of segment %d leaves its diagonal block", (int)out_seg.load()); return 2; }
+    if (no_diag >= 0) { set_err("ILU(k): row %d has no diagonal entry", (int)no_diag.load()); return 2; }
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+      total += c->h_frp[i + 1];
+      if (total > 0x7fffffff) { set_err("ILU(k) factor too large for 32-bit indices"); return 2; }
+      c->h_frp[i + 1] = (int)total;
+    }
+    c->h_fci.resize(total);
+    pos.resize(total);
+    pass(true);
+    phase("    level 0/1 patterns");
+    return 0;
+  }
   std::vector<std::vector<int>> lens(S), fcis(S), poss(S);
   std::vector<int> err(S, 0);
   std::atomic<int> next{0};
@@ -2658,10 +2857,12 @@ int iluk_pattern(shesolve_ctx *c, hvec<int> &pos, bool drop_outside) {
     }
   };
   const int nt = std::max(1, std::min<int>(S, (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+  PhaseClock phase(c->stream);
   std::vector<std::thread> th;
   for (int t = 1; t < nt; ++t) th.emplace_back(work);
   work();
   for (auto &t : th) t.join();
+  phase("    segment patterns");
   int64_t total = 0;
   for (int s = 0; s < S; ++s) {
     if (err[s] == -1) { set_err("ILU(k): an entry of segment %d leaves its diagonal block", s); return 2; }
@@ -2672,31 +2873,22 @@ int iluk_pattern(shesolve_ctx *c, hvec<int> &pos, bool drop_outside) {
   c->h_frp.assign(n + 1, 0);
   c->h_fci.resize(total);
   pos.resize(total);
-  int64_t w = 0;
-  for (int s = 0; s < S; ++s) {
-    for (int i = 0; i < seg[s + 1] - seg[s]; ++i) c->h_frp[seg[s] + i + 1] = (int)(w += lens[s][i]) ;
-    std::copy(fcis[s].begin(), fcis[s].end(), c->h_fci.begin() + (w - (int64_t)fcis[s].size()));
-    std::copy(poss[s].begin(), poss[s].end(), pos.begin() + (w - (int64_t)poss[s].size()));
-    std::vector<int>().swap(fcis[s]);
-    std::vector<int>().swap(poss[s]);
-  }
+  // each segment's rows go to their place in parallel
+  std::vector<int64_t> off(S + 1, 0);
+  for (int s = 0; s < S; ++s) off[s + 1] = off[s] + (int64_t)fcis[s].size();
+  host_parallel(S, [&](int64_t s0, int64_t s1) {
+    for (int s = (int)s0; s < (int)s1; ++s) {
+      int64_t w = off[s];
+      for (int i = 0; i < seg[s + 1] - seg[s]; ++i) c->h_frp[seg[s] + i + 1] = (int)(w += lens[s][i]);
+      std::copy(fcis[s].begin(), fcis[s].end(), c->h_fci.begin() + off[s]);
+      std::copy(poss[s].begin(), poss[s].end(), pos.begin() + off[s]);
+      std::vector<int>().swap(fcis[s]);
+      std::vector<int>().swap(poss[s]);
+    }
+  }, 1);
+  phase("    merge");
   return 0;
 }
-
-// SHESOLVE_VERBOSE: durations of the set-up phases on stderr
-struct PhaseClock {
-  bool on = getenv("SHESOLVE_VERBOSE") != nullptr;
-  hipStream_t stream;
-  std::chrono::steady_clock::time_point tp = std::chrono::steady_clock::now();
-  explicit PhaseClock(hipStream_t s) : stream(s) {}
-  void operator()(const char *what) {
-    if (!on) return;
-    hipStreamSynchronize(stream);
-    const auto t = std::chrono::steady_clock::now();
-    fprintf(stderr, "[shesolve] set-up %s: %.1f ms\n", what, std::chrono::duration<double, std::milli>(t - tp).count());
-    tp = t;
-  }
-};
 
 // Grouped sweep layout of one triangle of a block multicolour factor (split
 // pattern rp/ci in the solve space, block starts bst): per row its entries
@@ -3095,7 +3287,10 @@ int analyse_and_factor(shesolve_ctx *c) {
   }
   phase("  colour levels");
   if (bmc) compute_levels_blocked(c->h_frp, c->h_fci, c->h_fdg, n, bst, c->bmc_cptr, Lptr, Lrows, Uptr, Urows);
-  else if (!colour_levels) compute_levels(c->h_frp, c->h_fci, c->h_fdg, n, Lptr, Lrows, Uptr, Urows);
+  else if (!colour_levels &&
+           !(perm.empty() && c->fseg_ptr.size() > 2 &&
+             compute_levels_segments(c->h_frp, c->h_fci, c->h_fdg, n, c->fseg_ptr, Lptr, Lrows, Uptr, Urows)))
+    compute_levels(c->h_frp, c->h_fci, c->h_fdg, n, Lptr, Lrows, Uptr, Urows);
   phase("  levels");
   if (perm.empty()) {
     // factor segments (systems of a batch, per-energy slabs or the whole
@@ -3160,17 +3355,38 @@ int analyse_and_factor(shesolve_ctx *c) {
     // columns become positions in the same schedule (the sweep's window index)
     auto reorder = [&](hvec<int> &rp, hvec<int> &ci, const std::vector<int> &rows,
                        std::vector<int> &lrp) {
-      hvec<int> start(n + 1, 0), lci(ci.size());
+      hvec<int> start(n + 1), lci(ci.size());
       std::vector<int> qpos(n);
-      for (int q = 0; q < n; ++q) qpos[rows[q]] = q;
       lrp.assign(n + 1, 0);
-      int w = 0;
-      for (int q = 0; q < n; ++q) {
-        const int row = rows[q];
-        start[row] = w;
-        lrp[q] = w;
-        for (int k = rp[row]; k < rp[row + 1]; ++k) lci[w++] = qpos[ci[k]];
-      }
+      // row lengths in schedule order, their prefix sums in 16 slices (slice
+      // sums first), then the columns mapped and placed, all on host threads
+      constexpr int kSlices = 16;
+      int64_t part[kSlices + 1] = {0};
+      std::vector<std::thread> th;
+      for (int t = 0; t < kSlices; ++t)
+        th.emplace_back([&, t] {
+          int64_t acc = 0;
+          for (int q = (int)((int64_t)n * t / kSlices); q < (int)((int64_t)n * (t + 1) / kSlices); ++q) {
+            qpos[rows[q]] = q;
+            acc += rp[rows[q] + 1] - rp[rows[q]];
+          }
+          part[t + 1] = acc;
+        });
+      for (auto &x : th) x.join();
+      th.clear();
+      for (int t = 0; t < kSlices; ++t) part[t + 1] += part[t];
+      for (int t = 0; t < kSlices; ++t)
+        th.emplace_back([&, t] {
+          int w = (int)part[t];
+          for (int q = (int)((int64_t)n * t / kSlices); q < (int)((int64_t)n * (t + 1) / kSlices); ++q) {
+            const int row = rows[q];
+            start[row] = w;
+            lrp[q] = w;
+            for (int k = rp[row]; k < rp[row + 1]; ++k) lci[w++] = qpos[ci[k]];
+          }
+        });
+      for (auto &x : th) x.join();
+      const int w = (int)part[kSlices];
       lrp[n] = w;
       start[n] = w;
       rp.swap(start);
@@ -3184,32 +3400,49 @@ int analyse_and_factor(shesolve_ctx *c) {
       const std::vector<int> &lv = c->h_lvlq[upper], &lrp = upper ? loUrp : loLrp;
       const hvec<int> &lci = upper ? Uci : Lci;
       int reach = 0;
-      for (size_t li = 0; li + 1 < lv.size(); ++li)
-        for (int q = lv[li]; q < lv[li + 1]; ++q)
-          for (int k = lrp[q]; k < lrp[q + 1]; ++k) reach = std::max(reach, lv[li + 1] - lci[k]);
+      std::mutex mu;
+      host_parallel((int64_t)lv.size() - 1, [&](int64_t l0, int64_t l1) {
+        int r = 0;
+        for (int64_t li = l0; li < l1; ++li)
+          for (int q = lv[li]; q < lv[li + 1]; ++q)
+            for (int k = lrp[q]; k < lrp[q + 1]; ++k) r = std::max(r, lv[li + 1] - lci[k]);
+        std::lock_guard<std::mutex> g(mu);
+        reach = std::max(reach, r);
+      }, 1024);
       c->seg_reach[upper] = reach;
     }
     {
       // chunks of the chain sweeps: consecutive levels of a segment while they
       // hold at most kChainRows rows and kChainEnts entries
-      bool ok = true;
+      // (each segment's chunks on host threads, then concatenated)
+      std::atomic<bool> ok{true};
       std::vector<int> ch[2], cp[2];
       for (int upper = 0; upper < 2 && ok; ++upper) {
         const std::vector<int> &lv = c->h_lvlq[upper], &sg = c->h_segq[upper], &lrp = upper ? loUrp : loLrp;
-        cp[upper].assign(1, 0);
-        for (size_t s = 0; s + 1 < sg.size() && ok; ++s) {
-          int la = sg[s];
-          while (la < sg[s + 1] && ok) {
-            int lb = la;
-            while (lb < sg[s + 1] && lv[lb + 1] - lv[la] <= kChainRows && lrp[lv[lb + 1]] - lrp[lv[la]] <= kChainEnts)
-              ++lb;
-            if (lb == la) { ok = false; break; }  // a level wider than a chunk
-            const int hd[8] = {la, lb, lv[la], lv[lb], lrp[lv[la]], lrp[lv[lb]], 0, 0};
-            ch[upper].insert(ch[upper].end(), hd, hd + 8);
-            la = lb;
+        const int S = (int)sg.size() - 1;
+        std::vector<std::vector<int>> sch(std::max(S, 0));
+        host_parallel(S, [&](int64_t s0, int64_t s1) {
+          for (int s = (int)s0; s < (int)s1 && ok; ++s) {
+            int la = sg[s];
+            while (la < sg[s + 1]) {
+              int lb = la;
+              while (lb < sg[s + 1] && lv[lb + 1] - lv[la] <= kChainRows && lrp[lv[lb + 1]] - lrp[lv[la]] <= kChainEnts)
+                ++lb;
+              if (lb == la) { ok = false; break; }  // a level wider than a chunk
+              const int hd[8] = {la, lb, lv[la], lv[lb], lrp[lv[la]], lrp[lv[lb]], 0, 0};
+              sch[s].insert(sch[s].end(), hd, hd + 8);
+              la = lb;
+            }
           }
-          cp[upper].push_back((int)(ch[upper].size() / 8));
-        }
+        }, 1);
+        if (!ok) break;
+        cp[upper].assign(S + 1, 0);
+        for (int s = 0; s < S; ++s) cp[upper][s + 1] = cp[upper][s] + (int)(sch[s].size() / 8);
+        ch[upper].resize((size_t)cp[upper][S] * 8);
+        host_parallel(S, [&](int64_t s0, int64_t s1) {
+          for (int s = (int)s0; s < (int)s1; ++s)
+            std::copy(sch[s].begin(), sch[s].end(), ch[upper].begin() + (size_t)cp[upper][s] * 8);
+        }, 1);
       }
       if (ok) {
         int **dch[2] = {&c->d_chunkL, &c->d_chunkU}, **dcp[2] = {&c->d_cptrL, &c->d_cptrU};
@@ -3427,7 +3660,8 @@ int build_codes(shesolve_ctx *c, const std::vector<int> &flat) {
   free_codes(c);
   if (c->prm.precond != SHESOLVE_PRECOND_BMC_ILU0 || !c->solve_perm || c->h_perm.size() != (size_t)c->n) return 0;
   const int nb = (int)(flat.size() / 2);
-  const std::vector<int> &rp = c->h_frp, &ci = c->h_fci, &perm = c->h_perm;
+  const std::vector<int> &rp = c->h_frp, &perm = c->h_perm;
+  const hvec<int> &ci = c->h_fci;
   std::vector<std::vector<int>> tabs(nb);
   std::vector<unsigned short> code((size_t)c->fnnz + kVecPad, 0);
   std::atomic<bool> ok{true};
