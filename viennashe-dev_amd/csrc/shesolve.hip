@@ -612,6 +612,30 @@ __device__ inline double row_bcast(double x) {
   const int hi = __builtin_amdgcn_mov_dpp((int)(bits >> 32), 0x150 + L, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
+// Sum over aligned groups of LPR lanes (1, 2, 4, 8 or 16), every lane of a
+// group receiving it, by DPP moves inside rows of 16 lanes (no LDS round trip,
+// as a lane shuffle would take): quad_perm [1,0,3,2] and [2,3,0,1], then
+// row_half_mirror and row_mirror.  Every lane of a group must be active.
+template <int CTRL>
+__device__ inline float dpp_mov(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ inline double dpp_mov(double x) {
+  const long long bits = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)bits, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(bits >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int LPR, typename T>
+__device__ inline T group_sum(T a) {
+  static_assert(LPR == 1 || LPR == 2 || LPR == 4 || LPR == 8 || LPR == 16, "groups of 1 to 16 lanes");
+  if constexpr (LPR >= 2) a += dpp_mov<0xB1>(a);
+  if constexpr (LPR >= 4) a += dpp_mov<0x4E>(a);
+  if constexpr (LPR >= 8) a += dpp_mov<0x141>(a);
+  if constexpr (LPR >= 16) a += dpp_mov<0x140>(a);
+  return a;
+}
 // acc += c * (src of lane L of this lane's row of 16).  FP32: one
 // v_fmac_f32_dpp (the broadcast applied to the multiplicand), after the two
 // wait states a DPP read of a VGPR written by the previous VALU op needs.
@@ -1569,8 +1593,7 @@ __global__ __launch_bounds__(NT) void k_wgsweep(const int *__restrict__ seg_lvl,
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         T a = acc[r];
-#pragma unroll
-        for (int off = LPR / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, LPR);
+        a = group_sum<LPR>(a);
         const int q = base + r * G;
         if (lane == 0 && q < q1) {
           const T out = UPPER ? m.d[r] * (m.rh[r] - a) : m.rh[r] - a;
@@ -1698,8 +1721,7 @@ __global__ __launch_bounds__(64) void k_chainsweep(const int *__restrict__ cptr,
         const int i = base - qa;
         T acc = 0;
         for (int k = B.kr[i] + lane; k < B.kr[i + 1]; k += LPR) acc += (T)B.v[k] * win[B.c[k] & Wm];
-#pragma unroll
-        for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+        acc = group_sum<LPR>(acc);
         if (lane == 0) {
           const T out = UPPER ? B.d[i] * (B.rh[i] - acc) : B.rh[i] - acc;
           win[base & Wm] = out;
