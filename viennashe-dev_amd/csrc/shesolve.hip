@@ -72,6 +72,31 @@ struct Scalars {
   double tol;
 };
 
+// Sum over aligned groups of LPR lanes (1, 2, 4, 8 or 16), every lane of a
+// group receiving it, by DPP moves inside rows of 16 lanes (no LDS round trip,
+// as a lane shuffle would take): quad_perm [1,0,3,2] and [2,3,0,1], then
+// row_half_mirror and row_mirror.  Every lane of a group must be active.
+template <int CTRL>
+__device__ inline float dpp_mov(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ inline double dpp_mov(double x) {
+  const long long bits = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)bits, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(bits >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int LPR, typename T>
+__device__ inline T group_sum(T a) {
+  static_assert(LPR == 1 || LPR == 2 || LPR == 4 || LPR == 8 || LPR == 16, "groups of 1 to 16 lanes");
+  if constexpr (LPR >= 2) a += dpp_mov<0xB1>(a);
+  if constexpr (LPR >= 4) a += dpp_mov<0x4E>(a);
+  if constexpr (LPR >= 8) a += dpp_mov<0x141>(a);
+  if constexpr (LPR >= 16) a += dpp_mov<0x140>(a);
+  return a;
+}
+
 // ---------------------------------------------------------------- reductions
 __device__ inline double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -117,8 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ 
     TA s = 0;
     const int beg = rp[row], end = rp[row + 1];
     for (int k = beg + lane; k < end; k += LPR) s += (TA)v[k] * (TA)x[ci[k]];
-#pragma unroll
-    for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
+    s = group_sum<LPR>(s);
     if (lane == 0) {
       T out = b ? (T)(b[row] - s) : (T)s;
       y[row] = out;
@@ -611,30 +635,6 @@ __device__ inline double row_bcast(double x) {
   const int lo = __builtin_amdgcn_mov_dpp((int)bits, 0x150 + L, 0xF, 0xF, true);
   const int hi = __builtin_amdgcn_mov_dpp((int)(bits >> 32), 0x150 + L, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-// Sum over aligned groups of LPR lanes (1, 2, 4, 8 or 16), every lane of a
-// group receiving it, by DPP moves inside rows of 16 lanes (no LDS round trip,
-// as a lane shuffle would take): quad_perm [1,0,3,2] and [2,3,0,1], then
-// row_half_mirror and row_mirror.  Every lane of a group must be active.
-template <int CTRL>
-__device__ inline float dpp_mov(float x) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, true));
-}
-template <int CTRL>
-__device__ inline double dpp_mov(double x) {
-  const long long bits = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_mov_dpp((int)bits, CTRL, 0xF, 0xF, true);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(bits >> 32), CTRL, 0xF, 0xF, true);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-template <int LPR, typename T>
-__device__ inline T group_sum(T a) {
-  static_assert(LPR == 1 || LPR == 2 || LPR == 4 || LPR == 8 || LPR == 16, "groups of 1 to 16 lanes");
-  if constexpr (LPR >= 2) a += dpp_mov<0xB1>(a);
-  if constexpr (LPR >= 4) a += dpp_mov<0x4E>(a);
-  if constexpr (LPR >= 8) a += dpp_mov<0x141>(a);
-  if constexpr (LPR >= 16) a += dpp_mov<0x140>(a);
-  return a;
 }
 // acc += c * (src of lane L of this lane's row of 16).  FP32: one
 // v_fmac_f32_dpp (the broadcast applied to the multiplicand), after the two
