@@ -4506,7 +4506,8 @@ struct shesolve_batch_ctx {
   float *dd = nullptr;
   std::vector<BSys> h_bs;
   std::vector<shesolve_report> rep;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};  // one iteration: [0] FP64, [1] FP32
+  // one iteration: [0] FP64, [1] FP32, [2] FP64 with the FP32 factor
+  hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
   bool use_graph = true;
   int poll = 8;                    // iterations between host checks of the masks
   bool verbose = getenv("SHESOLVE_VERBOSE") != nullptr;  // progress lines on stderr
@@ -4565,9 +4566,10 @@ int bnorms(shesolve_batch_ctx *b, const T *u, std::vector<double> &out) {
 }
 
 // One iteration of the batched right-preconditioned BiCGStab (kernels only,
-// so that it can be captured into a graph).
+// so that it can be captured into a graph).  m32: an FP64 iteration that
+// applies the FP32 copy of the factor (FP64 vectors in and out).
 template <typename TV, typename T>
-void bbicg_iter(shesolve_batch_ctx *b, T *x) {
+void bbicg_iter(shesolve_batch_ctx *b, T *x, bool m32 = false) {
   shesolve_ctx *c = b->c;
   T **V;
   const TV *Av, *Mv;
@@ -4577,12 +4579,14 @@ void bbicg_iter(shesolve_batch_ctx *b, T *x) {
   // the SpMVs of the iteration skip the row blocks of converged systems
   c->spmv_skip = c->d_blk_sys ? b->d_bs : nullptr;
   bupd<T>(b, 0, p, nullptr, nullptr, r, v, nullptr, nullptr);
-  launch_precond<TV, T>(c, Mv, p, ph);
+  if (sizeof(T) == 8 && m32) launch_precond<float, T>(c, nullptr, p, ph);
+  else launch_precond<TV, T>(c, Mv, p, ph);
   launch_spmv<TV, T, T>(c, c->lpr, Av, ph, v, nullptr, nullptr, nullptr, nullptr);
   bdot<T>(b, nullptr, r0, v, (const T *)nullptr, (const T *)nullptr);
   bfin(b, 1);
   bupd<T>(b, 1, s, nullptr, nullptr, r, v, nullptr, nullptr);
-  launch_precond<TV, T>(c, Mv, s, sh);
+  if (sizeof(T) == 8 && m32) launch_precond<float, T>(c, nullptr, s, sh);
+  else launch_precond<TV, T>(c, Mv, s, sh);
   launch_spmv<TV, T, T>(c, c->lpr, Av, sh, t, nullptr, nullptr, nullptr, nullptr);
   bdot<T>(b, nullptr, t, s, t, t);
   bfin(b, 2);
@@ -4613,12 +4617,16 @@ int bbicg_run(shesolve_batch_ctx *b, const T *rhs, T *x, double tol, int max_ite
   bdot<T>(b, nullptr, r, r, rhs, rhs);
   bfin(b, 0);
   HIPCHK(hipGetLastError());
-  const int gi = sizeof(T) == 8 ? 0 : 1;
+  // the FP64 solve of the systems an FP32+IR batch hands over applies the FP32
+  // factor: the Krylov recurrence is what has to be FP64 for them, the factor's
+  // rounding is far below its ILU error, and the sweeps read half the bytes
+  const bool m32 = sizeof(T) == 8 && c->prm.precision == SHESOLVE_FP32_IR;
+  const int gi = sizeof(T) == 8 ? (m32 ? 2 : 0) : 1;
   const bool graph = b->use_graph && !c->prof;
   if (graph && !b->graph[gi]) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    bbicg_iter<TV, T>(b, x);
+    bbicg_iter<TV, T>(b, x, m32);
     HIPCHK(hipStreamEndCapture(c->stream, &g));
     HIPCHK(hipGraphInstantiate(&b->graph[gi], g, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(g));
@@ -4636,7 +4644,7 @@ int bbicg_run(shesolve_batch_ctx *b, const T *rhs, T *x, double tol, int max_ite
     const int k = std::min(b->poll, max_iters - done);
     for (int i = 0; i < k; ++i) {
       if (graph) HIPCHK(hipGraphLaunch(b->graph[gi], c->stream));
-      else bbicg_iter<TV, T>(b, x);
+      else bbicg_iter<TV, T>(b, x, m32);
     }
     done += k;
   }
