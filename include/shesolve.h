@@ -80,8 +80,10 @@ typedef struct {
   int fill_level;        /* ILUK: level of fill k, default 1 */
   double drop_tol;       /* ILUK: relative drop tolerance (0 = plain ILU(k)), default 0 */
   int autotune;          /* 1: time the CSR-stream launch shapes at set-up and keep the fastest
-                            (default); 0: fixed shape (2048-entry row blocks, strided), so that
-                            results do not depend on timings */
+                            (default; a handle re-set with a matrix of the same preconditioner
+                            within 10 % of the row count it tuned for keeps those shapes);
+                            0: fixed shape (2048-entry row blocks, strided), so that results do
+                            not depend on timings */
 } shesolve_params;
 
 typedef struct {
