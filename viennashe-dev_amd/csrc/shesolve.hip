@@ -28,6 +28,7 @@
 #include <thread>
 #include <atomic>
 #include <mutex>
+#include <sys/mman.h>
 #include <utility>
 #include <vector>
 
@@ -1776,6 +1777,8 @@ __global__ __launch_bounds__(kBlock) void k_qscatter(int n, const int *__restric
 // Host arrays of the analysis that are written in full after they are sized:
 // no value initialisation, so resizing costs no serial zero fill and the pages
 // are first touched by the (parallel) writers.
+// Large ones ask for transparent huge pages, so that first touches fault a
+// 2 MiB page at a time (where the kernel has them enabled on request).
 template <typename T>
 struct NoInitAlloc : std::allocator<T> {
   template <typename U>
@@ -1785,6 +1788,15 @@ struct NoInitAlloc : std::allocator<T> {
   NoInitAlloc() = default;
   template <typename U>
   NoInitAlloc(const NoInitAlloc<U> &) {}
+  T *allocate(size_t m) {
+    T *p = std::allocator<T>::allocate(m);
+    const size_t bytes = m * sizeof(T);
+    if (bytes >= ((size_t)64 << 20)) {
+      const uintptr_t a = ((uintptr_t)p + 4095) & ~(uintptr_t)4095, e = ((uintptr_t)p + bytes) & ~(uintptr_t)4095;
+      if (e > a) madvise((void *)a, e - a, MADV_HUGEPAGE);
+    }
+    return p;
+  }
   template <typename U, typename... A>
   void construct(U *p, A &&...a) {
     if constexpr (sizeof...(A) == 0) ::new ((void *)p) U;
@@ -2416,13 +2428,38 @@ struct PhaseClock {
 };
 
 // Rows bucketed by level: ptr[l] .. ptr[l + 1] in rows, ascending row within a level.
+// (on host threads over row slices: per-slice counts, then each slice places
+// its rows after the earlier slices' rows of the same level)
 void bucket_levels(const std::vector<int> &lev, int maxl, int n, std::vector<int> &ptr, std::vector<int> &rows) {
-  ptr.assign(maxl + 2, 0);
-  for (int i = 0; i < n; ++i) ptr[lev[i] + 1]++;
-  for (int l = 0; l <= maxl; ++l) ptr[l + 1] += ptr[l];
-  rows.assign(n, 0);
-  std::vector<int> pos(ptr.begin(), ptr.end() - 1);
-  for (int i = 0; i < n; ++i) rows[pos[lev[i]]++] = i;
+  const int L = maxl + 1;
+  constexpr int kSlices = 16;
+  const int ns = n >= (1 << 20) ? kSlices : 1;
+  std::vector<std::vector<int>> cnt(ns, std::vector<int>(L, 0));
+  auto slices = [&](auto fn) {
+    std::vector<std::thread> th;
+    for (int t = 1; t < ns; ++t) th.emplace_back([&, t] { fn(t); });
+    fn(0);
+    for (auto &x : th) x.join();
+  };
+  auto lo = [&](int t) { return (int)((int64_t)n * t / ns); };
+  slices([&](int t) {
+    for (int i = lo(t); i < lo(t + 1); ++i) cnt[t][lev[i]]++;
+  });
+  ptr.assign(L + 1, 0);
+  for (int l = 0; l < L; ++l) {
+    int acc = ptr[l];
+    for (int t = 0; t < ns; ++t) {
+      const int c = cnt[t][l];
+      cnt[t][l] = acc;  // this slice's first place in level l
+      acc += c;
+    }
+    ptr[l + 1] = acc;
+  }
+  rows.resize(n);
+  slices([&](int t) {
+    std::vector<int> &pos = cnt[t];
+    for (int i = lo(t); i < lo(t + 1); ++i) rows[pos[lev[i]]++] = i;
+  });
 }
 
 void compute_levels(const std::vector<int> &rp, const hvec<int> &ci, const std::vector<int> &dg, int n,
@@ -4882,10 +4919,11 @@ int batch_set_systems(shesolve_batch_ctx *b, bool device_src, int count, const i
       return 3;
     }
     hrp[s].resize(n[s] + 1);
-    if (device_src)
-      HIPCHK(hipMemcpy(hrp[s].data(), rp[s], sizeof(int64_t) * (n[s] + 1), hipMemcpyDeviceToHost));
-    else
+    if (device_src) {
+      if (int e = staged_d2h(c, hrp[s].data(), rp[s], sizeof(int64_t) * (n[s] + 1))) return e;
+    } else {
       std::copy(rp[s], rp[s] + n[s] + 1, hrp[s].begin());
+    }
     nnz[s] = hrp[s][n[s]] - hrp[s][0];
     if (hrp[s][0] != 0 || nnz[s] < 0) { set_err("system %d: row_ptr must start at 0", s); return 4; }
     N += n[s];
@@ -4895,26 +4933,22 @@ int batch_set_systems(shesolve_batch_ctx *b, bool device_src, int count, const i
     set_err("batch too large for 32-bit indices: %lld rows, %lld entries", (long long)N, (long long)NNZ);
     return 2;
   }
-  std::vector<int64_t> crp(N + 1);
-  std::vector<int32_t> cci(NNZ);
+  // the concatenated pattern: each system's columns straight into place (the
+  // columns then shifted and checked per system on host threads), the column
+  // array becoming the analysis copy without another copy
+  hvec<int64_t> crp(N + 1);
+  hvec<int32_t> cci(NNZ);
   std::vector<int64_t> cranges;
   b->S = count;
   b->sys_ptr.assign(count + 1, 0);
   b->val_ptr.assign(count + 1, 0);
-  std::vector<int32_t> tmpc;
   for (int s = 0; s < count; ++s) {
-    const int64_t r0 = b->sys_ptr[s], v0 = b->val_ptr[s];
-    for (int64_t i = 0; i <= n[s]; ++i) crp[r0 + i] = v0 + hrp[s][i];
-    const int32_t *src = ci[s];
+    b->sys_ptr[s + 1] = (int)(b->sys_ptr[s] + n[s]);
+    b->val_ptr[s + 1] = b->val_ptr[s] + nnz[s];
     if (device_src) {
-      tmpc.resize(nnz[s]);
-      HIPCHK(hipMemcpy(tmpc.data(), ci[s], sizeof(int32_t) * nnz[s], hipMemcpyDeviceToHost));
-      src = tmpc.data();
+      if (int e = staged_d2h(c, cci.data() + b->val_ptr[s], ci[s], sizeof(int32_t) * nnz[s])) return e;
     }
-    for (int64_t k = 0; k < nnz[s]; ++k) {
-      if (src[k] < 0 || src[k] >= n[s]) { set_err("system %d: column index out of range", s); return 5; }
-      cci[v0 + k] = (int32_t)(src[k] + r0);
-    }
+    const int64_t r0 = b->sys_ptr[s];
     if (nblocks && ranges && nblocks[s] > 0 && ranges[s]) {
       for (int q = 0; q < nblocks[s]; ++q) {
         cranges.push_back(r0 + std::max<int64_t>(0, std::min<int64_t>(n[s], ranges[s][2 * q])));
@@ -4924,9 +4958,21 @@ int batch_set_systems(shesolve_batch_ctx *b, bool device_src, int count, const i
       cranges.push_back(r0);
       cranges.push_back(r0 + n[s]);
     }
-    b->sys_ptr[s + 1] = (int)(r0 + n[s]);
-    b->val_ptr[s + 1] = v0 + nnz[s];
   }
+  std::atomic<int> bad_sys{-1};
+  host_parallel(count, [&](int64_t s0, int64_t s1) {
+    for (int s = (int)s0; s < (int)s1; ++s) {
+      const int64_t r0 = b->sys_ptr[s], v0 = b->val_ptr[s];
+      for (int64_t i = 0; i <= n[s]; ++i) crp[r0 + i] = v0 + hrp[s][i];
+      const int32_t *src = device_src ? cci.data() + v0 : ci[s];
+      for (int64_t k = 0; k < nnz[s]; ++k) {
+        const int32_t j = src[k];
+        if (j < 0 || j >= n[s]) bad_sys = s;
+        cci[v0 + k] = (int32_t)(j + r0);
+      }
+    }
+  }, 1);
+  if (bad_sys >= 0) { set_err("system %d: column index out of range", bad_sys.load()); return 5; }
   // values: one concatenated buffer (device staging for device sources)
   double *dval = nullptr;
   std::vector<double> hval;
@@ -4944,13 +4990,15 @@ int batch_set_systems(shesolve_batch_ctx *b, bool device_src, int count, const i
     set_err("%d fill levels set for %d systems", (int)c->seg_fill.size(), count);
     return 2;
   }
-  int e = set_matrix_common(c, false, N, NNZ, crp.data(), cci.data(), device_src ? dval : hval.data(),
+  c->h_ci.swap(cci);
+  int e = set_matrix_common(c, false, N, NNZ, crp.data(), c->h_ci.data(), device_src ? dval : hval.data(),
                             (int)(cranges.size() / 2), cranges.data(), device_src ? 1 : 0);
   if (dval) hipFree(dval);
   if (e) return e;
   std::vector<int> row_sys(N);
-  for (int s = 0; s < count; ++s)
-    for (int i = b->sys_ptr[s]; i < b->sys_ptr[s + 1]; ++i) row_sys[i] = s;
+  host_parallel(count, [&](int64_t s0, int64_t s1) {
+    for (int s = (int)s0; s < (int)s1; ++s) std::fill(row_sys.begin() + b->sys_ptr[s], row_sys.begin() + b->sys_ptr[s + 1], s);
+  }, 1);
   HIPCHK(hipMalloc(&b->d_sys_ptr, sizeof(int) * (count + 1)));
   HIPCHK(hipMalloc(&b->d_row_sys, sizeof(int) * N));
   HIPCHK(hipMalloc(&b->d_mask, sizeof(int) * count));
