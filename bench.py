@@ -110,7 +110,7 @@ def cpu_baseline_parallel(rp, ci, val, b, ranges, iters, budget_s):
     }
 
 
-def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0, ordering=0):
+def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0, ordering=0, graph=True):
     """The batched workload: `count` 1D diodes drawn from a seed (length
     U[100,1000] nm, n+ 20 % each side, L in {1,3,5}, dE U[2,10] meV to 1.0 eV,
     bias U[0,1.5] V), solved together by the batched FP32+IR BiCGStab (one
@@ -118,8 +118,10 @@ def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0, ordering=0):
     iterations).  Preconditioner: ILU(k) of each per-energy slab (the block
     ranges of she::solve, she/linear_solver.hpp:42-83), k = 0, and k = 1 for
     L = 5 (ILU(0) diverges there); the slabs of all systems are swept side by
-    side, one workgroup each.  A step = GPU refill of every system +
-    refactorisation + batched solve to 1e-10."""
+    side, one wavefront each.  A step = GPU refill of every system +
+    refactorisation + batched solve to 1e-10.  graph=False launches each
+    iteration's kernels directly instead of replaying a captured graph (for
+    kernel traces)."""
     if count <= 0:
         return {}
     import ctypes
@@ -170,6 +172,8 @@ def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0, ordering=0):
         return lat, reps
 
     bs, t_setup = make(shesolve.FP32_IR)
+    if not graph:
+        bs.options(use_graph=False)
     info = bs.info()
     log(f"[bench] batch of {count}: {info}, build {t_build:.1f} s, setup {t_setup:.1f} s")
     if warmup:
@@ -189,7 +193,7 @@ def bench_batch(steps, count=96, seed=1234, fp64=False, warmup=0, ordering=0):
                      "ir_steps": reps[i]["ir_steps"], "rel_residual": reps[i]["rel_residual"]}
                     for i in sorted(range(count), key=lambda i: -reps[i]["iterations"])[:6]],
         "setup_s": t_setup, "build_s": t_build,
-        "precond": "per-energy block ILU(k): k=0, k=1 for L=5; one workgroup per slab",
+        "precond": "per-energy block ILU(k): k=0, k=1 for L=5; one wavefront per slab",
     }
     if fp64:
         bs.set_params(precision=shesolve.FP64)
@@ -247,6 +251,8 @@ def main():
     ap.add_argument("--batch-count", type=int, default=96,
                     help="systems of the batched workload (the seeded family of 96; fewer takes the first N)")
     ap.add_argument("--batch-fp64", action="store_true", help="also time the batch in plain FP64")
+    ap.add_argument("--batch-no-graph", action="store_true",
+                    help="launch the batched iterations directly instead of replaying a graph (for kernel traces)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=40)
     args = ap.parse_args()
@@ -448,7 +454,8 @@ def main():
     hip.hipFree(d_x)
     if not args.no_batch:
         os.environ.setdefault("SHESOLVE_VERBOSE", "1")  # refinement-step progress lines on stderr
-        result["batch"] = bench_batch(args.batch_steps, count=args.batch_count, fp64=args.batch_fp64)
+        result["batch"] = bench_batch(args.batch_steps, count=args.batch_count, fp64=args.batch_fp64,
+                                      graph=not args.batch_no_graph)
         log(f"[bench] batch: {result['batch']}")
     if not args.no_serve:
         result["serving"] = bench_serving(args.serve_requests)
