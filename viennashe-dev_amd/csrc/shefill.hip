@@ -963,27 +963,56 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
   const size_t NH = c->NH;
   const double hw = optical_phonon_energy(c->cfg.energy_spacing);
   if (c->cfg.optical && !(hw > 0)) { ferr("optical phonon energy snaps to zero on this grid"); return 2; }
+  // (facet and cell nodes are numbered energy level by energy level, in mesh
+  // order inside a level: per-level counts, prefix sums, then every level
+  // filled on host threads; the node data after that, node by node in parallel)
+  const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  auto par = [&](int64_t m, auto fn) {
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(nthr, m));
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back([&, t] { fn(m * t / nt, m * (t + 1) / nt); });
+    fn(0, m / nt);
+    for (auto &x : th) x.join();
+  };
   // facet nodes
   c->fnode_of.assign((size_t)c->nf * NH, -1);
-  c->fn_f.clear(); c->fn_H.clear(); c->fn_L.clear(); c->fn_out.clear();
-  for (size_t H = 0; H < NH; ++H)
-    for (int f = 0; f < c->nf; ++f) {
-      const size_t q = (size_t)f * NH + H;
-      if (c->idxf[q] < 0) continue;
-      c->fnode_of[q] = (int)c->fn_f.size();
-      c->fn_f.push_back(f);
-      c->fn_H.push_back((int)H);
-      c->fn_L.push_back(c->ordf[q]);
-      const double E = kin_f(c, f, (int)H);
-      for (int s = 0; s < 2; ++s) {
-        int Hf = -1;
-        if (c->cfg.optical) {
-          long h = energy_index_host(c, true, f, (long)H, s == 0 ? E - hw : E + hw);
-          if (h >= 0 && c->idxf[(size_t)f * NH + h] >= 0) Hf = (int)h;
+  std::vector<int> fcount(NH + 1, 0);
+  par((int64_t)NH, [&](int64_t h0, int64_t h1) {
+    for (int64_t H = h0; H < h1; ++H) {
+      int cnt = 0;
+      for (int f = 0; f < c->nf; ++f) cnt += c->idxf[(size_t)f * NH + H] >= 0;
+      fcount[H + 1] = cnt;
+    }
+  });
+  for (size_t H = 0; H < NH; ++H) fcount[H + 1] += fcount[H];
+  const int nfn_all = fcount[NH];
+  c->fn_f.assign(nfn_all, 0);
+  c->fn_H.assign(nfn_all, 0);
+  c->fn_L.assign(nfn_all, 0);
+  c->fn_out.assign((size_t)2 * nfn_all, -1);
+  par((int64_t)NH, [&](int64_t h0, int64_t h1) {
+    for (int64_t H = h0; H < h1; ++H) {
+      int w = fcount[H];
+      for (int f = 0; f < c->nf; ++f) {
+        const size_t q = (size_t)f * NH + H;
+        if (c->idxf[q] < 0) continue;
+        c->fnode_of[q] = w;
+        c->fn_f[w] = f;
+        c->fn_H[w] = (int)H;
+        c->fn_L[w] = c->ordf[q];
+        const double E = kin_f(c, f, (int)H);
+        for (int s2 = 0; s2 < 2; ++s2) {
+          int Hf = -1;
+          if (c->cfg.optical) {
+            long h = energy_index_host(c, true, f, (long)H, s2 == 0 ? E - hw : E + hw);
+            if (h >= 0 && c->idxf[(size_t)f * NH + h] >= 0) Hf = (int)h;
+          }
+          c->fn_out[(size_t)2 * w + s2] = Hf;
         }
-        c->fn_out.push_back(Hf);
+        ++w;
       }
     }
+  });
   c->fn_base.assign(c->fn_f.size(), 0);
   {
     int acc = 0;
@@ -994,49 +1023,67 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
   }
   // cell nodes in unknown order
   c->node_of.assign((size_t)c->nc * NH, -1);
-  std::vector<int> order;
-  for (size_t H = 0; H < NH; ++H)
-    for (int k = 0; k < c->nc; ++k)
-      if (c->idxc[(size_t)k * NH + H] >= 0) {
-        c->node_of[(size_t)k * NH + H] = (int)order.size();
-        order.push_back(k * (int)NH + (int)H);
-      }
-  const int nn = (int)order.size();
+  std::vector<int> ccount(NH + 1, 0);
+  par((int64_t)NH, [&](int64_t h0, int64_t h1) {
+    for (int64_t H = h0; H < h1; ++H) {
+      int cnt = 0;
+      for (int k = 0; k < c->nc; ++k) cnt += c->idxc[(size_t)k * NH + H] >= 0;
+      ccount[H + 1] = cnt;
+    }
+  });
+  for (size_t H = 0; H < NH; ++H) ccount[H + 1] += ccount[H];
+  const int nn = ccount[NH];
+  std::vector<int> order(nn);
+  par((int64_t)NH, [&](int64_t h0, int64_t h1) {
+    for (int64_t H = h0; H < h1; ++H) {
+      int w = ccount[H];
+      for (int k = 0; k < c->nc; ++k)
+        if (c->idxc[(size_t)k * NH + H] >= 0) {
+          c->node_of[(size_t)k * NH + H] = w;
+          order[w++] = k * (int)NH + (int)H;
+        }
+    }
+  });
   c->nd_c.resize(nn); c->nd_H.resize(nn); c->nd_base.resize(nn); c->nd_L.resize(nn);
   c->nd_fn.assign((size_t)nn * 4, -1); c->nd_nb.assign((size_t)nn * 4, -1);
   c->nd_inn.assign((size_t)nn * 2, -1); c->nd_in.assign((size_t)nn * 2, -1); c->nd_out.assign((size_t)nn * 2, -1);
-  for (int k = 0; k < nn; ++k) {
-    const int cell = order[k] / (int)NH, H = order[k] % (int)NH;
-    const size_t q = (size_t)cell * NH + H;
-    c->nd_c[k] = cell;
-    c->nd_H[k] = H;
-    c->nd_base[k] = c->idxc[q];
-    c->nd_L[k] = c->ordc[q];
-    for (int s = 0; s < 4; ++s) {
-      const int f = c->cf[(size_t)cell * 4 + s];
-      if (f < 0) continue;
-      const int fn = c->fnode_of[(size_t)f * NH + H];
-      if (fn < 0) continue;
-      const int o = c->fc1[f] == cell ? c->fc2[f] : c->fc1[f];
-      c->nd_fn[(size_t)k * 4 + s] = fn;
-      c->nd_nb[(size_t)k * 4 + s] = c->node_of[(size_t)o * NH + H];
-    }
-    if (c->cfg.optical) {
-      const double E = kin_c(c, cell, H);
-      // out: to E - hw (slot 0) and E + hw (slot 1); in: from E - hw (slot 0) and E + hw (slot 1)
-      for (int s = 0; s < 2; ++s) {
-        long h = energy_index_host(c, false, cell, H, s == 0 ? E - hw : E + hw);
-        if (h >= 0 && c->idxc[(size_t)cell * NH + h] >= 0) {
-          c->nd_out[2 * k + s] = (int)h;
-          c->nd_in[2 * k + s] = (int)h;
+  par(nn, [&](int64_t k0, int64_t k1) {
+    for (int k = (int)k0; k < (int)k1; ++k) {
+      const int cell = order[k] / (int)NH, H = order[k] % (int)NH;
+      const size_t q = (size_t)cell * NH + H;
+      c->nd_c[k] = cell;
+      c->nd_H[k] = H;
+      c->nd_base[k] = c->idxc[q];
+      c->nd_L[k] = c->ordc[q];
+      for (int s2 = 0; s2 < 4; ++s2) {
+        const int f = c->cf[(size_t)cell * 4 + s2];
+        if (f < 0) continue;
+        const int fn = c->fnode_of[(size_t)f * NH + H];
+        if (fn < 0) continue;
+        const int o = c->fc1[f] == cell ? c->fc2[f] : c->fc1[f];
+        c->nd_fn[(size_t)k * 4 + s2] = fn;
+        c->nd_nb[(size_t)k * 4 + s2] = c->node_of[(size_t)o * NH + H];
+      }
+      if (c->cfg.optical) {
+        const double E = kin_c(c, cell, H);
+        // out: to E - hw (slot 0) and E + hw (slot 1); in: from E - hw (slot 0) and E + hw (slot 1)
+        for (int s2 = 0; s2 < 2; ++s2) {
+          long h = energy_index_host(c, false, cell, H, s2 == 0 ? E - hw : E + hw);
+          if (h >= 0 && c->idxc[(size_t)cell * NH + h] >= 0) {
+            c->nd_out[2 * k + s2] = (int)h;
+            c->nd_in[2 * k + s2] = (int)h;
+          }
         }
       }
     }
-  }
+  });
   // in-scattering partner nodes (need node ids of other energies: second pass)
-  for (int k = 0; k < nn; ++k)
-    for (int s = 0; s < 2; ++s)
-      if (c->nd_in[2 * k + s] >= 0) c->nd_inn[2 * k + s] = c->node_of[(size_t)c->nd_c[k] * NH + c->nd_in[2 * k + s]];
+  par(nn, [&](int64_t k0, int64_t k1) {
+    for (int k = (int)k0; k < (int)k1; ++k)
+      for (int s2 = 0; s2 < 2; ++s2)
+        if (c->nd_in[2 * k + s2] >= 0)
+          c->nd_inn[2 * k + s2] = c->node_of[(size_t)c->nd_c[k] * NH + c->nd_in[2 * k + s2]];
+  });
   phase("  nodes");
   // rows: pattern per node
   const int64_t n = c->n;
