@@ -852,13 +852,35 @@ int build_energies_orders_mapping(shefill_ctx *c) {
   for (int k = 0; k < c->nc; ++k) cells[k] = k;
   if (cf.ordering == 1 && c->dim == 1)
     for (int k = 0; k < c->nc; ++k) cells[k] = (k % 2 == 0) ? k / 2 : c->nc - 1 - k / 2;
+  // which (cell, level) and (facet, level) pairs carry unknowns, evaluated on
+  // host threads; the numbering below is sequential
+  std::vector<char> livec((size_t)c->nc * NH, 0), livef((size_t)c->nf * NH, 0);
+  {
+    const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto par = [&](int64_t m, auto fn) {
+      const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(nthr, m));
+      std::vector<std::thread> th;
+      for (int t = 1; t < nt; ++t) th.emplace_back([&, t] { fn(m * t / nt, m * (t + 1) / nt); });
+      fn(0, m / nt);
+      for (auto &x : th) x.join();
+    };
+    par(c->nc, [&](int64_t k0, int64_t k1) {
+      for (int64_t k = k0; k < k1; ++k)
+        for (size_t H = 1; H + 1 < NH; ++H)
+          livec[(size_t)k * NH + H] = avg_dos_host(c, false, (int)k, (int)H) > 0 && kin_c(c, (int)k, (int)H) < cf.max_kin_range;
+    });
+    par(c->nf, [&](int64_t f0, int64_t f1) {
+      for (int64_t f = f0; f < f1; ++f)
+        for (size_t H = 0; H < NH; ++H) livef[(size_t)f * NH + H] = avg_dos_host(c, true, (int)f, (int)H) > 0;
+    });
+  }
   int64_t u = 0;
   for (size_t H = 0; H < NH; ++H)
     for (int kk = 0; kk < c->nc; ++kk) {
       const int k = cells[kk];
       const size_t q = (size_t)k * NH + H;
       if (H == 0 || H == NH - 1) { c->idxc[q] = -1; continue; }
-      if (avg_dos_host(c, false, k, (int)H) > 0 && kin_c(c, k, (int)H) < cf.max_kin_range) {
+      if (livec[q]) {
         c->idxc[q] = (int)u;
         u += even_count(c->ordc[q]);
       } else {
@@ -872,8 +894,7 @@ int build_energies_orders_mapping(shefill_ctx *c) {
   for (size_t H = 0; H < NH; ++H)
     for (int f = 0; f < c->nf; ++f) {
       const size_t q = (size_t)f * NH + H;
-      if (c->idxc[(size_t)c->fc1[f] * NH + H] > -1 && c->idxc[(size_t)c->fc2[f] * NH + H] > -1 &&
-          avg_dos_host(c, true, f, (int)H) > 0) {
+      if (c->idxc[(size_t)c->fc1[f] * NH + H] > -1 && c->idxc[(size_t)c->fc2[f] * NH + H] > -1 && livef[q]) {
         c->idxf[q] = 1;
         uo += odd_count(c->ordf[q]);
       } else {
