@@ -592,6 +592,38 @@ __global__ void k_collog(int n, const int *cptr, const int *cpos, const double *
   for (int k = cptr[col]; k < cptr[col + 1]; ++k) s += log(fabs(val[cpos[k]]));
   logsum[col] = s;
 }
+// Column-ordered positions of the structural entries (the diagonal that the
+// normalisation inserts is not one), built on the device from the stored
+// pattern: counts per column, then each entry takes a slot in its column, then
+// each column's slots are sorted, which orders them by row.
+__global__ void k_colcount(int n, const int *rp, const int *ci, const int *diag_pos, const unsigned char *dstruct,
+                           int *colcount) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int j = rp[i]; j < rp[i + 1]; ++j)
+    if (j != diag_pos[i] || dstruct[i]) atomicAdd(&colcount[ci[j]], 1);
+}
+__global__ void k_colfill(int n, const int *rp, const int *ci, const int *diag_pos, const unsigned char *dstruct,
+                          int *fillp, int *cpos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int j = rp[i]; j < rp[i + 1]; ++j)
+    if (j != diag_pos[i] || dstruct[i]) cpos[atomicAdd(&fillp[ci[j]], 1)] = j;
+}
+__global__ void k_colsort(int n, const int *cptr, int *cpos) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n) return;
+  const int b = cptr[col], e = cptr[col + 1];
+  for (int k = b + 1; k < e; ++k) {  // insertion sort: a column holds a few entries
+    const int v = cpos[k];
+    int m = k - 1;
+    while (m >= b && cpos[m] > v) {
+      cpos[m + 1] = cpos[m];
+      --m;
+    }
+    cpos[m + 1] = v;
+  }
+}
 __global__ void k_colnorm(int n, const int *colcount, const int *diag_pos, const double *val, double *logsum,
                           double *colfac) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -677,7 +709,7 @@ struct shefill_ctx {
   std::vector<int> fn_base;  // first odd unknown of each facet node, relative to n
   double a_l01[2][3] = {{0, 0, 0}, {0, 0, 0}};  // raw a_x, a_y entries (0, 1..3)
   int64_t n = 0, n_odd = 0, nnz = 0;
-  std::vector<int> rp, ci, row_node, diag_pos, colcount, cptr, cpos;
+  std::vector<int> rp, ci, row_node, diag_pos;
   std::vector<unsigned char> diag_struct;
   std::vector<int64_t> blocks;
   // device
@@ -1228,37 +1260,6 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
     }
   });
   phase("  columns");
-  // structural entries per column (the diagonal inserted by the normalisation is not counted)
-  // (counted and placed with atomic increments on host threads, each column's
-  // positions then sorted, which orders them by row)
-  auto ranges = [&](int64_t m, auto fn) {
-    std::vector<std::thread> th;
-    for (int t = 0; t < nthreads; ++t) th.emplace_back([&, t] { fn(m * t / nthreads, m * (t + 1) / nthreads); });
-    for (auto &x : th) x.join();
-  };
-  c->colcount.assign(n, 0);
-  ranges(n, [&](int64_t i0, int64_t i1) {
-    for (int64_t i = i0; i < i1; ++i)
-      for (int j = c->rp[i]; j < c->rp[i + 1]; ++j)
-        if (j != c->diag_pos[i] || c->diag_struct[i]) __atomic_fetch_add(&c->colcount[c->ci[j]], 1, __ATOMIC_RELAXED);
-  });
-  // column-ordered positions of the structural entries (rows ascending)
-  c->cptr.assign(n + 1, 0);
-  for (int64_t i = 0; i < n; ++i) c->cptr[i + 1] = c->cptr[i] + c->colcount[i];
-  c->cpos.resize(c->cptr[n]);
-  {
-    std::vector<int> fillp(c->cptr.begin(), c->cptr.end() - 1);
-    ranges(n, [&](int64_t i0, int64_t i1) {
-      for (int64_t i = i0; i < i1; ++i)
-        for (int j = c->rp[i]; j < c->rp[i + 1]; ++j)
-          if (j != c->diag_pos[i] || c->diag_struct[i])
-            c->cpos[__atomic_fetch_add(&fillp[c->ci[j]], 1, __ATOMIC_RELAXED)] = j;
-    });
-    ranges(n, [&](int64_t i0, int64_t i1) {
-      for (int64_t i = i0; i < i1; ++i) std::sort(c->cpos.begin() + c->cptr[i], c->cpos.begin() + c->cptr[i + 1]);
-    });
-  }
-  phase("  column positions");
   // block ranges (she/linear_solver.hpp:42-83)
   c->blocks.assign((size_t)2 * NH, 0);
   int64_t last = 0;
@@ -1281,6 +1282,33 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
 
 int grid(int64_t n) { return (int)std::max<int64_t>(1, (n + kB - 1) / kB); }
 int launch_fill(shefill_ctx *c);
+
+// The column-ordered positions of the structural entries on the device
+// (k_colcount, k_colfill, k_colsort); the prefix sums of the counts on the host.
+int build_column_positions(shefill_ctx *c) {
+  const int n = (int)c->n;
+  if (c->dcolcount.alloc(n) || c->dcptr.alloc((size_t)n + 1)) return -11;
+  FHIP(hipMemsetAsync(c->dcolcount.p, 0, sizeof(int) * std::max(n, 1), c->stream));
+  if (n > 0)
+    hipLaunchKernelGGL(k_colcount, dim3(grid(n)), dim3(kB), 0, c->stream, n, c->drp.p, c->dci.p, c->ddiag_pos.p,
+                       c->ddiag_struct.p, c->dcolcount.p);
+  std::vector<int> cnt(n), ptr((size_t)n + 1, 0);
+  if (n > 0) FHIP(hipMemcpyAsync(cnt.data(), c->dcolcount.p, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+  FHIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + cnt[i];
+  DBuf<int> fillp;
+  if (c->dcpos.alloc(std::max(ptr[n], 1)) || fillp.alloc((size_t)n + 1)) return -11;
+  FHIP(hipMemcpyAsync(c->dcptr.p, ptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
+  FHIP(hipMemcpyAsync(fillp.p, ptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_colfill, dim3(grid(n)), dim3(kB), 0, c->stream, n, c->drp.p, c->dci.p, c->ddiag_pos.p,
+                       c->ddiag_struct.p, fillp.p, c->dcpos.p);
+    hipLaunchKernelGGL(k_colsort, dim3(grid(n)), dim3(kB), 0, c->stream, n, c->dcptr.p, c->dcpos.p);
+  }
+  FHIP(hipGetLastError());
+  FHIP(hipStreamSynchronize(c->stream));  // ptr and fillp live until the copies and kernels are done
+  return 0;
+}
 
 int fill_values(shefill_ctx *c, Builder &B) {
   const int K = c->K;
@@ -1346,10 +1374,8 @@ int fill_values(shefill_ctx *c, Builder &B) {
   e = e ? e : upload(c, c->dci, c->ci);
   e = e ? e : upload(c, c->drow_node, c->row_node);
   e = e ? e : upload(c, c->ddiag_pos, c->diag_pos);
-  e = e ? e : upload(c, c->dcolcount, c->colcount);
-  e = e ? e : upload(c, c->dcptr, c->cptr);
-  e = e ? e : upload(c, c->dcpos, c->cpos);
   e = e ? e : upload(c, c->ddiag_struct, c->diag_struct);
+  e = e ? e : build_column_positions(c);
   const int nn = (int)c->nd_c.size(), nfn = (int)c->fn_f.size();
   const int64_t n = c->n;
   e = e ? e : c->dcorr.alloc(c->nc);
