@@ -1145,6 +1145,18 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
   c->diag_pos.assign(n, 0);
   c->diag_struct.assign(n, 0);
   // row lists of one node, computed twice (count, fill) in parallel over nodes
+  // elimination patterns by direct index once the cache holds every key this
+  // device uses (a std::map lookup per facet and node cost as much as the rows)
+  const int LM = c->cfg.L_max + 1;
+  std::vector<const Pattern *> etab;
+  auto elim = [&](int d, int La, int Lf, int Lb, int dd) -> const Pattern & {
+    if (!etab.empty() && d >= 0 && d < 4 && dd >= 0 && dd < 4 && La >= 0 && La < LM && Lf >= 0 && Lf < LM &&
+        Lb >= 0 && Lb < LM) {
+      const Pattern *q = etab[(size_t)(((d * LM + La) * LM + Lf) * LM + Lb) * 4 + dd];
+      if (q) return *q;
+    }
+    return B.elim(d, La, Lf, Lb, dd);
+  };
   // (cols keeps its row vectors between nodes, so their storage is reused)
   auto node_rows = [&](int k, std::vector<std::vector<int>> &cols, std::vector<char> &dstruct) -> int {
     const int cell = c->nd_c[k], Lc = c->nd_L[k];
@@ -1179,7 +1191,7 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
       const int fn = c->nd_fn[(size_t)k * 4 + s];
       if (fn < 0) continue;
       const int d = kSlotDir[s];
-      const Pattern &P = B.elim(d, Lc, c->fn_L[fn], Lc, rev_dir(d));
+      const Pattern &P = elim(d, Lc, c->fn_L[fn], Lc, rev_dir(d));
       for (int r = 0; r < e; ++r)
         for (int q : P[r]) selfm(r, q) = 1;
     }
@@ -1197,7 +1209,7 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
         const int s = bk.slot;
         const int fn = c->nd_fn[(size_t)k * 4 + s], nbn = c->nd_nb[(size_t)k * 4 + s];
         const int d = kSlotDir[s];
-        const Pattern &P = B.elim(d, Lc, c->fn_L[fn], c->nd_L[nbn], d);
+        const Pattern &P = elim(d, Lc, c->fn_L[fn], c->nd_L[nbn], d);
         for (int r = 0; r < e; ++r)
           for (int q : P[r]) cols[r].push_back(bk.base + q);
       }
@@ -1219,6 +1231,13 @@ int build_nodes_and_pattern(shefill_ctx *c, Builder &B) {
       }
       if (needed) node_rows(k, cols, ds);
     }
+  }
+  etab.assign((size_t)4 * LM * LM * LM * 4, nullptr);
+  for (const auto &kv : B.cache) {
+    const int d = std::get<0>(kv.first), La = std::get<1>(kv.first), Lf = std::get<2>(kv.first),
+              Lb = std::get<3>(kv.first), dd = std::get<4>(kv.first);
+    if (d >= 0 && d < 4 && dd >= 0 && dd < 4 && La >= 0 && La < LM && Lf >= 0 && Lf < LM && Lb >= 0 && Lb < LM)
+      etab[(size_t)(((d * LM + La) * LM + Lf) * LM + Lb) * 4 + dd] = &kv.second;
   }
   const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   auto parallel = [&](auto fn) {
