@@ -3617,8 +3617,21 @@ int analyse_and_factor(shesolve_ctx *c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   phase("  triangle uploads");
   if (!c->bmc_cptr.empty()) {
-    if (int e = build_bmc_part(c, bst, false, Lrp, Lci, c->bmcL)) return e;
-    if (int e = build_bmc_part(c, bst, true, Urp, Uci, c->bmcU)) return e;
+    // the two triangles' layouts are independent: the backward one on a thread of its own
+    // (its error message, kept per thread, is carried back to this one)
+    int eu = 0;
+    std::string eu_msg;
+    std::thread tu([&] {
+      eu = build_bmc_part(c, bst, true, Urp, Uci, c->bmcU);
+      if (eu) eu_msg = g_err;
+    });
+    const int el = build_bmc_part(c, bst, false, Lrp, Lci, c->bmcL);
+    tu.join();
+    if (el) return el;
+    if (eu) {
+      g_err = eu_msg;
+      return eu;
+    }
     phase("  grouped layout");
   }
   return 0;
